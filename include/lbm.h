@@ -1,0 +1,265 @@
+/*
+ * lbm.h -- C ABI of the MI355X-native D3Q19 LBM time step (liblbm.so).
+ *
+ * The reference solver (LBM_3D_LES_air_over_waves_CODEX.py) has no function
+ * API: its hot path is a fixed sequence of Taichi kernels on module-global
+ * fields, driven by main() (CODEX:3757-3959).  Each entry point below replaces
+ * one part of that surface; the comment on each names the reference code it
+ * stands in for.  All pointers are plain host pointers; no device pointer
+ * escapes the library.  Every function returns 0 on success and a negative
+ * LBM_E_* code on failure; lbm_last_error() then describes the failure.
+ *
+ * Threading: one handle owns one HIP stream and is not thread-safe; distinct
+ * handles are independent.
+ */
+#ifndef LBM_H
+#define LBM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LBM_CONFIG_VERSION 3
+
+/* ---- error codes ---------------------------------------------------- */
+#define LBM_OK 0
+#define LBM_E_ARG (-1)      /* bad argument (size, field id, null)       */
+#define LBM_E_CONFIG (-2)   /* config rejected (version, dims, modes)     */
+#define LBM_E_HIP (-3)      /* HIP runtime error                          */
+#define LBM_E_NOMEM (-4)    /* device allocation failed                   */
+#define LBM_E_STATE (-5)    /* call not valid in the current state        */
+#define LBM_E_UNSUPPORTED (-6)
+
+/* ---- enumerations (values mirror the reference's option strings) ---- */
+/* physics_mode: LBM_PHYSICS_MODE (CODEX:48) */
+#define LBM_PHYS_AIR 0
+#define LBM_PHYS_WATER 1
+/* collision: LBM_COLLISION (CODEX:491) */
+#define LBM_COLL_KBC 0
+#define LBM_COLL_REGULARIZED 1
+/* x_boundary: LBM_X_BOUNDARY (CODEX:200) + inflow_outflow (new) */
+#define LBM_XB_PERIODIC 0
+#define LBM_XB_SOLID 1
+#define LBM_XB_INFLOW_OUTFLOW 2
+/* y_boundary: periodic as in the reference; solid is new */
+#define LBM_YB_PERIODIC 0
+#define LBM_YB_SOLID 1
+/* top_boundary: "keep" is the reference's out-of-domain rule (CODEX:3549-3551);
+   free_slip (specular reflection) is new */
+#define LBM_TOP_KEEP 0
+#define LBM_TOP_FREE_SLIP 1
+/* bed_profile: LBM_BED_PROFILE (CODEX:205) + moving_sine1 (single harmonic, new) */
+#define LBM_BED_MOVING_SINE 0
+#define LBM_BED_FLAT 1
+#define LBM_BED_PIECEWISE 2
+#define LBM_BED_MOVING_SINE1 3
+/* storage of the populations in HBM (new) */
+#define LBM_STORE_FP32 0
+#define LBM_STORE_FP16 1 /* (f_i - w_i) * 2^shift as IEEE half */
+#define LBM_STORE_BF16 2 /* (f_i - w_i) * 2^shift as bfloat16   */
+#define LBM_STORE_FP64 3 /* FP64 storage and FP64 arithmetic    */
+/* kernel path */
+#define LBM_PATH_AUTO 0
+#define LBM_PATH_STAGED 1 /* one kernel per reference stage */
+#define LBM_PATH_FUSED 2  /* single fused z-marching pass (air mode) */
+/* free_surface_tracking: LBM_FREE_SURFACE_TRACKING (CODEX:87) */
+#define LBM_FST_HEIGHT 0
+#define LBM_FST_VOF 1
+/* pressure_formulation: LBM_PRESSURE_FORMULATION (CODEX:79) */
+#define LBM_PF_TOTAL 0
+#define LBM_PF_HYDROSTATIC_BALANCED 1
+#define LBM_PF_VOF_COMPONENT_BALANCED 2
+/* free_surface_boundary: LBM_FREE_SURFACE_BOUNDARY (CODEX:93) */
+#define LBM_FSB_CELL 0
+#define LBM_FSB_HYDROSTATIC 1
+/* refill: LBM_FREE_SURFACE_REFILL (CODEX:99) */
+#define LBM_REFILL_EQUILIBRIUM 0
+#define LBM_REFILL_NONEQ 1
+#define LBM_REFILL_DIRECTIONAL 2
+
+/* wall types (CODEX:441-445) and free-surface cell types (CODEX:447-449) */
+#define LBM_WALL_NONE 0
+#define LBM_WALL_BED 1
+#define LBM_WALL_CUBE 2
+#define LBM_WALL_XMIN 3
+#define LBM_WALL_XMAX 4
+#define LBM_WALL_YMIN 5 /* new: solid y walls */
+#define LBM_WALL_YMAX 6
+#define LBM_FS_GAS 0
+#define LBM_FS_LIQUID 1
+#define LBM_FS_INTERFACE 2
+
+/*
+ * Run configuration: the derived constants of CODEX:35-529 in the precision
+ * the reference's kernels see them (Taichi f32), plus the options that are new
+ * in this build.  Built by the Python config layer from LBM_* variables.
+ */
+typedef struct lbm_config {
+    int32_t version; /* = LBM_CONFIG_VERSION */
+    int32_t nx, ny, nz;
+
+    int32_t physics_mode, collision, use_les;
+    int32_t x_boundary, y_boundary, top_boundary;
+    int32_t bed_profile, include_bed, wall_velocity_link;
+    int32_t enable_top_drive, enable_x_fringe, enable_top_sponge;
+    int32_t storage, storage_shift, kernel_path;
+
+    /* water / free surface */
+    int32_t fs_enabled, fs_tracking, pressure_formulation, fs_boundary_mode;
+    int32_t refill_mode, vof_link_aperture, vof_thin_gap_bridge;
+    int32_t vof_detached_advection, vof_collapse_trapped_gas;
+    int32_t vof_orphan_search_radius, vof_orphan_max_weak_neighbors;
+    int32_t vof_detached_max_wet_neighbors, vof_detached_max_resolved_neighbors;
+    int32_t vof_collapse_interval, vof_collapse_flood_sweeps;
+    uint32_t seed; /* backscatter counter-based RNG seed */
+
+    float dx, rho0, tau0, nu0;
+    float cs_les, delta_les, nu_t_max, kappa, z0_phys;
+    float u_top_lb, u_current_lb, g_lb, g_body_lb;
+    float c_backscatter;
+    float offset_m, bed_amp_m, bed_wavelength_m, wave_speed, wave_period;
+    float depth_swl_m, vel_scale, bed_phase_speed_mps;
+    float free_surface_base_m;
+    float vof_empty_fill_threshold, vof_orphan_empty_threshold;
+    float vof_detached_residual_fill_threshold, vof_thin_gap_bridge_strength;
+    float vof_collapse_max_volume_m3;
+    float pad0;
+    double dt_phys; /* per-step time, the host forms t=(step+1)*dt_phys */
+} lbm_config_t;
+
+/* ---- field identifiers for upload / download ---------------------------
+ * Shapes are C order with x fastest: 3-D (nz,ny,nx); per-link (19,nz,ny,nx);
+ * 2-D (ny,nx).  "R" fields are float32 (float64 when storage==FP64),
+ * "I" fields are int32.                                                     */
+#define LBM_FIELD_F 0                /* R (19,nz,ny,nx) populations after streaming */
+#define LBM_FIELD_RHO 1              /* R */
+#define LBM_FIELD_UX 2               /* R */
+#define LBM_FIELD_UY 3               /* R */
+#define LBM_FIELD_UZ 4               /* R */
+#define LBM_FIELD_OMEGA 5            /* R omegaLoc */
+#define LBM_FIELD_PHI 6              /* R */
+#define LBM_FIELD_LATTICE_OPEN 7     /* I (19,...) */
+#define LBM_FIELD_LATTICE_OPEN_FRAC 8 /* R (19,...) */
+#define LBM_FIELD_WALL_TYPE 9        /* I (19,...) */
+#define LBM_FIELD_NEAR_OBSTACLE 10   /* I */
+#define LBM_FIELD_OBSTACLE 11        /* I */
+#define LBM_FIELD_WATER_H 12         /* R 2-D */
+#define LBM_FIELD_U_WAVE 13          /* R 2-D */
+#define LBM_FIELD_V_WAVE 14          /* R 2-D */
+#define LBM_FIELD_W_WAVE 15          /* R 2-D */
+#define LBM_FIELD_FS_FILL 16         /* R */
+#define LBM_FIELD_FS_TYPE 17         /* I */
+#define LBM_FIELD_FS_TYPE_PREV 18    /* I */
+#define LBM_FIELD_FS_MASS 19         /* R */
+#define LBM_FIELD_FS_MASS_EXCESS 20  /* R */
+#define LBM_FIELD_FS_H 21            /* R 2-D */
+#define LBM_FIELD_PRESSURE_REF_HEAD 22 /* R */
+#define LBM_FIELD_GRAVITY_FRACTION 23 /* R */
+#define LBM_FIELD_U_INLET_PROFILE 24 /* R (nz) */
+#define LBM_FIELD_INFLOW_UX 25       /* R (nz,ny) inflow plane (new) */
+#define LBM_FIELD_PIECEWISE_BED_Z 26 /* R (nx) */
+#define LBM_FIELD_NU_T 27            /* R */
+#define LBM_FIELD_SMAG 28            /* R */
+#define LBM_FIELD_VOF_EXTERIOR_GAS 29 /* I */
+#define LBM_FIELD_COUNT 30
+
+/* ---- counters: the reference's shape-() fields (CODEX:600-617) --------- */
+typedef struct lbm_counters {
+    int32_t boundary_cut_link_count;
+    int32_t boundary_refill_count;
+    int32_t free_surface_refill_count;
+    int32_t free_surface_coalescence_repair_count;
+    int32_t free_surface_thin_gap_bridge_count;
+    int32_t free_surface_cut_link_count;
+    int32_t free_surface_vof_excess_count;
+    int32_t free_surface_vof_orphan_count;
+    int32_t free_surface_vof_collapse_cell_count;
+    int32_t free_surface_vof_detached_advect_count;
+    int32_t free_surface_vof_detached_compress_count;
+    int32_t pad0;
+    double free_surface_vof_collapse_candidate_volume_m3;
+    double free_surface_vof_collapse_applied_volume_m3;
+    double free_surface_vof_collapse_removal_scale;
+    double free_surface_vof_collapse_candidate_mass;
+    double free_surface_vof_collapse_exterior_capacity;
+    double free_surface_vof_detached_advect_mass;
+    double free_surface_vof_detached_compress_mass;
+} lbm_counters_t;
+
+/* reductions (FP64 on the device) */
+#define LBM_REDUCE_MASS 0        /* sum of populations over open cells      */
+#define LBM_REDUCE_VOF_MASS 1    /* sum of free_surface_mass                */
+#define LBM_REDUCE_VOF_VOLUME 2  /* sum of free_surface_fill (cells)        */
+#define LBM_REDUCE_MAX_SPEED 3   /* max |u| over active cells (CODEX:3930-3940) */
+#define LBM_REDUCE_NONFINITE 4   /* count of non-finite populations          */
+
+typedef struct lbm_handle lbm_handle_t;
+
+/* Module import + field allocation (CODEX:35-620): validate the config and
+   allocate every device buffer of one case. */
+int lbm_create(const lbm_config_t *cfg, lbm_handle_t **out);
+
+/* Release the case (Taichi frees fields at process exit; no counterpart). */
+void lbm_destroy(lbm_handle_t *h);
+
+/* Geometry at time t: update_wave_bed_and_velocities(t),
+   compute_phi_slope_corrected(), build_lattice_open_from_free_surface_periodic()
+   (CODEX:1082-1162, 2432-2490, 2492-2686).  Used by initialisation. */
+int lbm_geometry(lbm_handle_t *h, float t);
+
+/* init_fields() populations (CODEX:2808-2818) from the uploaded rho/ux/uy/uz:
+   f = feq(rho, u + 0.5 g) with the Guo shift on wet cells; f_new = f
+   (initialize_free_surface_boundary_populations + copy_post_and_swap,
+   CODEX:2829-2875, 3714-3727).  Water/VOF: also
+   initialize_vof_mass_from_fill + enforce_vof_interface_layer +
+   update_free_surface_height_from_vof (CODEX:1290-1303, 1627-1656, 2146-2161). */
+int lbm_init_populations(lbm_handle_t *h);
+
+/* nsteps iterations of the step loop body (CODEX:3873-3925); the first step
+   advanced is number step_index+1 of the run. */
+int lbm_step(lbm_handle_t *h, int32_t nsteps);
+
+/* Host <-> device copies of one field: `.from_numpy()` / `.to_numpy()`. */
+int lbm_upload_field(lbm_handle_t *h, int32_t field, const void *src, size_t bytes);
+int lbm_download_field(lbm_handle_t *h, int32_t field, void *dst, size_t bytes);
+
+/* `counter[None]` reads of the shape-() fields. */
+int lbm_get_counters(lbm_handle_t *h, lbm_counters_t *out);
+
+/* FP64 device reductions (the health print CODEX:3930-3955 and the mass /
+   volume diagnostics of run_ab_case.py:286-679). */
+int lbm_reduce(lbm_handle_t *h, int32_t what, double *out);
+
+/* Number of steps advanced so far. */
+int lbm_get_step(lbm_handle_t *h, int64_t *out);
+
+/* Block until the handle's stream is idle (Taichi syncs on .to_numpy()). */
+int lbm_sync(lbm_handle_t *h);
+
+/* Description of the last failure on this handle (or of the last failed
+   lbm_create when h is NULL). */
+const char *lbm_last_error(lbm_handle_t *h);
+
+/* Average duration (ms) of the dominant step kernel over the last lbm_step
+   call, measured with HIP events on the handle's stream (no counterpart). */
+int lbm_last_step_timing(lbm_handle_t *h, double *ms_per_step, double *ms_main_kernel);
+
+/* Bytes of device memory the case holds (admission control; no counterpart). */
+int lbm_device_bytes(const lbm_config_t *cfg, size_t *out);
+
+/* Tuning and instrumentation knobs (no counterpart in the reference):
+   "timing" (0/1: record HIP events around every step kernel),
+   "kchunk" (z planes per block of the fused kernel, >= 1). */
+int lbm_set_option(lbm_handle_t *h, const char *key, int64_t value);
+
+/* Resolved execution path of the handle: LBM_PATH_STAGED or LBM_PATH_FUSED. */
+int lbm_get_path(lbm_handle_t *h, int32_t *out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* LBM_H */

@@ -1,0 +1,668 @@
+// lbm_capi.hip -- the extern "C" ABI of liblbm.so (declared in include/lbm.h).
+//
+// One handle = one case = one HIP stream.  The handle owns every device
+// buffer; hosts pass plain pointers.  The population state is either the
+// post-stream set f(n) of the reference (after initialisation, uploads and
+// every step of the staged path) or the post-collision set f*(n) whose
+// stream is still pending (after a step of the fused path); downloads and
+// reductions of the populations apply the pending stream first, so callers
+// always see the reference's f(n).
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/lbm.h"
+#include "lbm_internal.h"
+
+using namespace lbm;
+
+namespace {
+
+std::string g_create_error;
+
+struct Buf {
+    void* p = nullptr;
+};
+
+}  // namespace
+
+struct lbm_handle {
+    lbm_config_t cfg{};
+    DevBufs b{};
+    hipStream_t stream = nullptr;
+    int cur = 0;
+    int post_stream = 1;  // 1: f[cur] = f(step), 0: f[cur] = f*(step), stream pending
+    int64_t step = 0;
+    int path = LBM_PATH_STAGED;
+    int omega_band_top = -1;
+    int kchunk = 16;
+    int initialized = 0;
+    int geometry_done = 0;
+    int timing = 0;
+    double last_ms_step = 0.0, last_ms_kernel = 0.0;
+    std::vector<hipEvent_t> ev;  // timing events: 2 per step + 2 around the loop
+    std::string err;
+    std::vector<void*> allocs;
+};
+
+#define CHECK(expr)                                                                                     \
+    do {                                                                                                \
+        hipError_t e_ = (expr);                                                                         \
+        if (e_ != hipSuccess) throw std::runtime_error(std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+static size_t real_size(const lbm_config_t& c) { return c.storage == LBM_STORE_FP64 ? 8 : 4; }
+static size_t store_size(const lbm_config_t& c) {
+    switch (c.storage) {
+        case LBM_STORE_FP64: return 8;
+        case LBM_STORE_FP16:
+        case LBM_STORE_BF16: return 2;
+        default: return 4;
+    }
+}
+
+static const char* validate(const lbm_config_t* c) {
+    if (!c) return "config is NULL";
+    if (c->version != LBM_CONFIG_VERSION) return "config version mismatch";
+    if (c->nx < 2 || c->ny < 2 || c->nz < 3) return "grid must be at least 2 x 2 x 3";
+    if ((size_t)c->nx * c->ny * c->nz > (size_t)1 << 31) return "grid larger than 2^31 cells";
+    if (c->storage < 0 || c->storage > 3) return "unknown storage";
+    if (c->storage_shift < 0 || c->storage_shift > 24) return "storage_shift must be in [0, 24]";
+    if (c->physics_mode == LBM_PHYS_WATER) {
+        if (!c->fs_enabled) return "water mode without a free surface is not supported";
+        if (c->fs_tracking != LBM_FST_VOF) return "height-function tracking is not supported";
+        if (c->storage == LBM_STORE_FP16 || c->storage == LBM_STORE_BF16) return "water/VOF runs need fp32 or fp64 storage";
+        if (c->kernel_path == LBM_PATH_FUSED) return "the fused path covers air mode only";
+    }
+    if (!(c->tau0 > 0.5f)) return "tau0 must exceed 0.5";
+    if (!(c->dx > 0.0f)) return "dx must be positive";
+    if (c->vof_collapse_interval < 1) return "vof_collapse_interval must be positive";
+    if ((c->storage == LBM_STORE_FP16 || c->storage == LBM_STORE_BF16) && !fused_supported(*c))
+        return "fp16/bf16 storage runs on the fused path: air mode with nx % 32 == 0 and ny % 16 == 0";
+    if (c->kernel_path == LBM_PATH_FUSED && !fused_supported(*c)) return "the fused path needs air mode, nx % 32 == 0, ny % 16 == 0";
+    return nullptr;
+}
+
+static int resolve_path(const lbm_config_t& c) {
+    if (c.kernel_path == LBM_PATH_STAGED) return LBM_PATH_STAGED;
+    if (c.kernel_path == LBM_PATH_FUSED) return LBM_PATH_FUSED;
+    if (c.storage == LBM_STORE_FP16 || c.storage == LBM_STORE_BF16) return LBM_PATH_FUSED;
+    return fused_supported(c) ? LBM_PATH_FUSED : LBM_PATH_STAGED;
+}
+
+// highest k at which a cell can switch between solid and fluid (moving beds)
+static int band_top(const lbm_config_t& c) {
+    if (!(c.bed_profile == LBM_BED_MOVING_SINE || c.bed_profile == LBM_BED_MOVING_SINE1) || !c.include_bed) return -1;
+    double amp = c.bed_profile == LBM_BED_MOVING_SINE ? 2.0 * fabs((double)c.bed_amp_m) : fabs((double)c.bed_amp_m);
+    double hmax = (double)c.offset_m + amp;
+    int k = (int)ceil(hmax / (double)c.dx) + 2;
+    return k >= c.nz ? c.nz - 1 : k;
+}
+
+static size_t device_bytes(const lbm_config_t& c, int path) {
+    size_t n = (size_t)c.nx * c.ny * c.nz, n2 = (size_t)c.nx * c.ny;
+    size_t R = real_size(c);
+    size_t bytes = 2 * 19 * n * store_size(c);  // ping-pong populations
+    bytes += 5 * n * R + n;                      // rho, u, omega, cell flags
+    if (path == LBM_PATH_STAGED) bytes += 2 * n * R + 2 * n * 4;  // nu_t, smag, obstacle masks
+    bytes += 2 * n2 * (13 * R + 4) + (c.nx + c.nz + (size_t)c.nz * c.ny) * R;
+    if (c.fs_enabled) bytes += n * (11 * R + 7 * 4) + 6 * n * R + n2 * R;
+    return bytes + (1 << 20);
+}
+
+static void* dalloc(lbm_handle* h, size_t bytes) {
+    void* p = nullptr;
+    if (bytes == 0) bytes = 4;
+    hipError_t e = hipMalloc(&p, bytes);
+    if (e != hipSuccess) throw std::runtime_error(std::string("hipMalloc(") + std::to_string(bytes) + "): " + hipGetErrorString(e));
+    h->allocs.push_back(p);
+    return p;
+}
+
+static void free_all(lbm_handle* h) {
+    for (void* p : h->allocs) (void)hipFree(p);
+    h->allocs.clear();
+}
+
+static void fill_real(lbm_handle* h, void* dst, size_t count, double value) {
+    size_t R = real_size(h->cfg);
+    std::vector<char> tmp(count * R);
+    if (R == 8)
+        for (size_t i = 0; i < count; i++) ((double*)tmp.data())[i] = value;
+    else
+        for (size_t i = 0; i < count; i++) ((float*)tmp.data())[i] = (float)value;
+    CHECK(hipMemcpyAsync(dst, tmp.data(), tmp.size(), hipMemcpyHostToDevice, h->stream));
+    CHECK(hipStreamSynchronize(h->stream));
+}
+
+static void allocate(lbm_handle* h) {
+    const lbm_config_t& c = h->cfg;
+    DevBufs& b = h->b;
+    size_t n = (size_t)c.nx * c.ny * c.nz, n2 = (size_t)c.nx * c.ny;
+    size_t R = real_size(c), Sz = store_size(c);
+    b.storage = c.storage;
+    b.n = n;
+    b.n2 = n2;
+    b.f[0] = dalloc(h, 19 * n * Sz);
+    b.f[1] = dalloc(h, 19 * n * Sz);
+    CHECK(hipMemsetAsync(b.f[0], 0, 19 * n * Sz, h->stream));
+    CHECK(hipMemsetAsync(b.f[1], 0, 19 * n * Sz, h->stream));
+    b.ftmp = nullptr;  // allocated on first download
+    b.rho = dalloc(h, n * R);
+    b.ux = dalloc(h, n * R);
+    b.uy = dalloc(h, n * R);
+    b.uz = dalloc(h, n * R);
+    b.omega = dalloc(h, n * R);
+    b.cellflags = (uint8_t*)dalloc(h, n);
+    if (h->path == LBM_PATH_STAGED) {
+        b.nu_t = dalloc(h, n * R);
+        b.smag = dalloc(h, n * R);
+        b.obst = (int*)dalloc(h, n * 4);
+        b.obst_prev = (int*)dalloc(h, n * 4);
+        CHECK(hipMemsetAsync(b.nu_t, 0, n * R, h->stream));
+        CHECK(hipMemsetAsync(b.smag, 0, n * R, h->stream));
+        CHECK(hipMemsetAsync(b.obst, 0, n * 4, h->stream));
+        CHECK(hipMemsetAsync(b.obst_prev, 0, n * 4, h->stream));
+    } else {
+        b.obst = (int*)dalloc(h, n * 4);  // init writes the t=0 mask here
+        b.obst_prev = nullptr;
+    }
+    for (int s = 0; s < 2; s++) {
+        b.h[s] = dalloc(h, n2 * R);
+        b.uw[s] = dalloc(h, n2 * R);
+        b.vw[s] = dalloc(h, n2 * R);
+        b.ww[s] = dalloc(h, n2 * R);
+        b.eta9[s] = dalloc(h, 9 * n2 * R);
+        b.denom[s] = dalloc(h, n2 * R);
+        b.kband[s] = (int*)dalloc(h, n2 * 4);
+        CHECK(hipMemsetAsync(b.uw[s], 0, n2 * R, h->stream));
+        CHECK(hipMemsetAsync(b.vw[s], 0, n2 * R, h->stream));
+        CHECK(hipMemsetAsync(b.ww[s], 0, n2 * R, h->stream));
+    }
+    b.piecewise = dalloc(h, c.nx * R);
+    b.u_inlet = dalloc(h, c.nz * R);
+    b.inflow = dalloc(h, (size_t)c.nz * c.ny * R);
+    CHECK(hipMemsetAsync(b.u_inlet, 0, c.nz * R, h->stream));
+    CHECK(hipMemsetAsync(b.inflow, 0, (size_t)c.nz * c.ny * R, h->stream));
+    b.icnt = (int*)dalloc(h, IC_COUNT * sizeof(int));
+    b.dcnt = (double*)dalloc(h, DC_COUNT * sizeof(double));
+    CHECK(hipMemsetAsync(b.icnt, 0, IC_COUNT * sizeof(int), h->stream));
+    CHECK(hipMemsetAsync(b.dcnt, 0, DC_COUNT * sizeof(double), h->stream));
+    if (c.fs_enabled) {
+        b.fill = dalloc(h, n * R);
+        b.mass = dalloc(h, n * R);
+        b.mass2 = dalloc(h, n * R);
+        b.mdelta = dalloc(h, n * R);
+        b.excess = dalloc(h, n * R);
+        b.sa = dalloc(h, n * R);
+        b.sb = dalloc(h, n * R);
+        b.sc = dalloc(h, n * R);
+        b.sd = dalloc(h, n * R);
+        b.se = dalloc(h, n * R);
+        b.smv = dalloc(h, 6 * n * R);
+        b.fs_h = dalloc(h, n2 * R);
+        b.type = (int*)dalloc(h, n * 4);
+        b.type_prev = (int*)dalloc(h, n * 4);
+        b.excess_dir = (int*)dalloc(h, n * 4);
+        b.ext_gas = (int*)dalloc(h, n * 4);
+        b.si = (int*)dalloc(h, n * 4);
+        CHECK(hipMemsetAsync(b.mdelta, 0, n * R, h->stream));
+        CHECK(hipMemsetAsync(b.excess, 0, n * R, h->stream));
+        CHECK(hipMemsetAsync(b.excess_dir, 0, n * 4, h->stream));
+        CHECK(hipMemsetAsync(b.ext_gas, 0, n * 4, h->stream));
+        fill_real(h, b.fill, n, 1.0);
+        fill_real(h, b.mass, n, (double)c.rho0);
+        fill_real(h, b.fs_h, n2, (double)c.free_surface_base_m);
+        std::vector<int> liquid(n, LBM_FS_LIQUID);
+        CHECK(hipMemcpyAsync(b.type, liquid.data(), n * 4, hipMemcpyHostToDevice, h->stream));
+        CHECK(hipMemcpyAsync(b.type_prev, liquid.data(), n * 4, hipMemcpyHostToDevice, h->stream));
+    }
+    // init_constants (CODEX:1022-1063): rho = rho0, u = 0, omega = 1/tau0,
+    // bed height = the bed level
+    fill_real(h, b.rho, n, (double)c.rho0);
+    CHECK(hipMemsetAsync(b.ux, 0, n * R, h->stream));
+    CHECK(hipMemsetAsync(b.uy, 0, n * R, h->stream));
+    CHECK(hipMemsetAsync(b.uz, 0, n * R, h->stream));
+    double om0 = R == 8 ? 1.0 / (double)c.tau0 : (double)(1.0f / c.tau0);
+    fill_real(h, b.omega, n, om0);
+    fill_real(h, b.h[0], n2, (double)c.offset_m);
+    fill_real(h, b.h[1], n2, (double)c.offset_m);
+    fill_real(h, b.piecewise, c.nx, (double)c.offset_m);
+    CHECK(hipStreamSynchronize(h->stream));
+}
+
+static void ensure_ftmp(lbm_handle* h) {
+    if (!h->b.ftmp) h->b.ftmp = dalloc(h, 19 * h->b.n * real_size(h->cfg));
+}
+
+static int slot_of(int64_t step) { return (int)(step & 1); }
+
+// the reference's post-stream populations f(step) as an arithmetic-type
+// device buffer (either the state itself or ftmp)
+static const void* post_stream_populations(lbm_handle* h) {
+    if (h->post_stream) {
+        if (h->cfg.storage == LBM_STORE_FP32 || h->cfg.storage == LBM_STORE_FP64) return h->b.f[h->cur];
+        ensure_ftmp(h);
+        launch_load_populations(h->cfg, h->b, h->cur, h->b.ftmp, h->stream);
+        return h->b.ftmp;
+    }
+    ensure_ftmp(h);
+    launch_materialize(h->cfg, h->b, h->cur, slot_of(h->step), h->b.ftmp, h->stream);
+    return h->b.ftmp;
+}
+
+#define API_BEGIN(h)                      \
+    if (!(h)) return LBM_E_ARG;           \
+    try {
+#define API_END(h)                                   \
+    }                                                \
+    catch (const std::exception& e) {                \
+        (h)->err = e.what();                         \
+        return LBM_E_HIP;                            \
+    }                                                \
+    return LBM_OK;
+
+static int fail(lbm_handle* h, int code, const std::string& msg) {
+    h->err = msg;
+    return code;
+}
+
+extern "C" {
+
+int lbm_create(const lbm_config_t* cfg, lbm_handle_t** out) {
+    if (!out) {
+        g_create_error = "out is NULL";
+        return LBM_E_ARG;
+    }
+    *out = nullptr;
+    const char* why = validate(cfg);
+    if (why) {
+        g_create_error = why;
+        return LBM_E_CONFIG;
+    }
+    lbm_handle* h = new lbm_handle();
+    h->cfg = *cfg;
+    h->path = resolve_path(*cfg);
+    h->omega_band_top = band_top(*cfg);
+    h->kchunk = cfg->nz >= 64 ? 16 : 8;
+    try {
+        int ndev = 0;
+        if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) throw std::runtime_error("no HIP device visible");
+        CHECK(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+        allocate(h);
+    } catch (const std::exception& e) {
+        g_create_error = e.what();
+        free_all(h);
+        if (h->stream) (void)hipStreamDestroy(h->stream);
+        delete h;
+        return LBM_E_NOMEM;
+    }
+    *out = h;
+    return LBM_OK;
+}
+
+void lbm_destroy(lbm_handle_t* h) {
+    if (!h) return;
+    if (h->stream) (void)hipStreamSynchronize(h->stream);
+    for (hipEvent_t e : h->ev) (void)hipEventDestroy(e);
+    free_all(h);
+    if (h->stream) (void)hipStreamDestroy(h->stream);
+    delete h;
+}
+
+const char* lbm_last_error(lbm_handle_t* h) { return h ? h->err.c_str() : g_create_error.c_str(); }
+
+int lbm_geometry(lbm_handle_t* h, float t) {
+    API_BEGIN(h)
+    launch_bed2d(h->cfg, h->b, t, 0, h->stream);
+    launch_obstacle(h->cfg, h->b, 0, h->b.obst, h->stream);
+    h->geometry_done = 1;
+    h->step = 0;
+    CHECK(hipStreamSynchronize(h->stream));
+    API_END(h)
+}
+
+int lbm_init_populations(lbm_handle_t* h) {
+    API_BEGIN(h)
+    if (!h->geometry_done) return fail(h, LBM_E_STATE, "lbm_geometry must run before lbm_init_populations");
+    h->cur = 0;
+    launch_init_populations(h->cfg, h->b, 0, h->stream);
+    h->post_stream = 1;
+    h->step = 0;
+    h->initialized = 1;
+    CHECK(hipStreamSynchronize(h->stream));
+    API_END(h)
+}
+
+int lbm_step(lbm_handle_t* h, int32_t nsteps) {
+    API_BEGIN(h)
+    if (!h->initialized) return fail(h, LBM_E_STATE, "lbm_init_populations (or an upload of F) must run first");
+    if (nsteps < 0) return fail(h, LBM_E_ARG, "nsteps must be >= 0");
+    if (nsteps == 0) return LBM_OK;
+    const lbm_config_t& c = h->cfg;
+    size_t need = h->timing ? (size_t)2 * nsteps + 2 : 0;
+    while (h->ev.size() < need) {
+        hipEvent_t e;
+        CHECK(hipEventCreate(&e));
+        h->ev.push_back(e);
+    }
+    if (h->timing) CHECK(hipEventRecord(h->ev[0], h->stream));
+    for (int s = 0; s < nsteps; s++) {
+        int64_t step = h->step + 1;
+        float t = (float)((double)(step + 1) * c.dt_phys);  // CODEX:3874
+        int slot = slot_of(step);
+        launch_bed2d(c, h->b, t, slot, h->stream);
+        if (h->timing) CHECK(hipEventRecord(h->ev[2 + 2 * s], h->stream));
+        if (h->path == LBM_PATH_STAGED) {
+            if (!h->post_stream) throw std::runtime_error("internal: staged step on a post-collision state");
+            launch_staged_step(c, h->b, h->cur, slot, step, h->omega_band_top, h->stream);
+        } else {
+            int last = s == nsteps - 1;
+            launch_fused_step(c, h->b, h->cur, h->post_stream ? 0 : 1, slot_of(step - 1), slot, step, last, h->omega_band_top,
+                              h->kchunk, h->stream);
+            h->post_stream = 0;
+        }
+        if (h->timing) CHECK(hipEventRecord(h->ev[3 + 2 * s], h->stream));
+        h->step = step;
+    }
+    if (h->timing) {
+        CHECK(hipEventRecord(h->ev[1], h->stream));
+        CHECK(hipEventSynchronize(h->ev[1]));
+        float ms = 0.0f;
+        CHECK(hipEventElapsedTime(&ms, h->ev[0], h->ev[1]));
+        h->last_ms_step = ms / nsteps;
+        double k = 0.0;
+        for (int s = 0; s < nsteps; s++) {
+            float m = 0.0f;
+            CHECK(hipEventElapsedTime(&m, h->ev[2 + 2 * s], h->ev[3 + 2 * s]));
+            k += m;
+        }
+        h->last_ms_kernel = k / nsteps;
+    }
+    API_END(h)
+}
+
+int lbm_upload_field(lbm_handle_t* h, int32_t field, const void* src, size_t bytes) {
+    API_BEGIN(h)
+    if (!src) return fail(h, LBM_E_ARG, "src is NULL");
+    const lbm_config_t& c = h->cfg;
+    DevBufs& b = h->b;
+    size_t n = b.n, n2 = b.n2, R = real_size(c);
+    void* dst = nullptr;
+    size_t need = 0;
+    bool water_only = false;
+    switch (field) {
+        case LBM_FIELD_F: need = 19 * n * R; break;
+        case LBM_FIELD_RHO: dst = b.rho; need = n * R; break;
+        case LBM_FIELD_UX: dst = b.ux; need = n * R; break;
+        case LBM_FIELD_UY: dst = b.uy; need = n * R; break;
+        case LBM_FIELD_UZ: dst = b.uz; need = n * R; break;
+        case LBM_FIELD_OMEGA: dst = b.omega; need = n * R; break;
+        case LBM_FIELD_U_INLET_PROFILE: dst = b.u_inlet; need = c.nz * R; break;
+        case LBM_FIELD_INFLOW_UX: dst = b.inflow; need = (size_t)c.nz * c.ny * R; break;
+        case LBM_FIELD_PIECEWISE_BED_Z: dst = b.piecewise; need = c.nx * R; break;
+        case LBM_FIELD_FS_FILL: dst = b.fill; need = n * R; water_only = true; break;
+        case LBM_FIELD_FS_MASS: dst = b.mass; need = n * R; water_only = true; break;
+        case LBM_FIELD_FS_TYPE: dst = b.type; need = n * 4; water_only = true; break;
+        case LBM_FIELD_FS_TYPE_PREV: dst = b.type_prev; need = n * 4; water_only = true; break;
+        case LBM_FIELD_FS_H: dst = b.fs_h; need = n2 * R; water_only = true; break;
+        default: return fail(h, LBM_E_ARG, "field " + std::to_string(field) + " cannot be uploaded");
+    }
+    if (water_only && !c.fs_enabled) return fail(h, LBM_E_ARG, "free-surface fields exist only in water runs");
+    if (bytes != need)
+        return fail(h, LBM_E_ARG, "field " + std::to_string(field) + ": expected " + std::to_string(need) + " bytes, got " +
+                                      std::to_string(bytes));
+    if (field == LBM_FIELD_F) {
+        if (c.storage == LBM_STORE_FP32 || c.storage == LBM_STORE_FP64) {
+            CHECK(hipMemcpyAsync(b.f[h->cur], src, need, hipMemcpyHostToDevice, h->stream));
+        } else {
+            ensure_ftmp(h);
+            CHECK(hipMemcpyAsync(b.ftmp, src, need, hipMemcpyHostToDevice, h->stream));
+            launch_store_populations(c, b, h->cur, b.ftmp, h->stream);
+        }
+        h->post_stream = 1;
+        h->initialized = 1;
+    } else {
+        CHECK(hipMemcpyAsync(dst, src, need, hipMemcpyHostToDevice, h->stream));
+        if (field == LBM_FIELD_PIECEWISE_BED_Z && c.bed_profile == LBM_BED_PIECEWISE) {
+            // init_constants seeds the bed height with the profile (CODEX:1050-1054)
+            std::vector<char> plane(n2 * R);
+            for (size_t j = 0; j < (size_t)c.ny; j++) memcpy(plane.data() + j * c.nx * R, src, c.nx * R);
+            CHECK(hipMemcpyAsync(b.h[0], plane.data(), n2 * R, hipMemcpyHostToDevice, h->stream));
+            CHECK(hipMemcpyAsync(b.h[1], plane.data(), n2 * R, hipMemcpyHostToDevice, h->stream));
+        }
+    }
+    CHECK(hipStreamSynchronize(h->stream));
+    API_END(h)
+}
+
+int lbm_download_field(lbm_handle_t* h, int32_t field, void* dst, size_t bytes) {
+    API_BEGIN(h)
+    if (!dst) return fail(h, LBM_E_ARG, "dst is NULL");
+    const lbm_config_t& c = h->cfg;
+    DevBufs& b = h->b;
+    size_t n = b.n, n2 = b.n2, R = real_size(c);
+    int slot = slot_of(h->step);
+    const void* src = nullptr;
+    size_t need = 0;
+    std::vector<void*> tmp;
+    auto scratch = [&](size_t sz) {
+        void* p = nullptr;
+        CHECK(hipMalloc(&p, sz));
+        tmp.push_back(p);
+        return p;
+    };
+    struct Cleanup {
+        std::vector<void*>* v;
+        ~Cleanup() {
+            for (void* p : *v) (void)hipFree(p);
+        }
+    } cleanup{&tmp};
+    bool water_only = false;
+    switch (field) {
+        case LBM_FIELD_F: need = 19 * n * R; break;
+        case LBM_FIELD_RHO: src = b.rho; need = n * R; break;
+        case LBM_FIELD_UX: src = b.ux; need = n * R; break;
+        case LBM_FIELD_UY: src = b.uy; need = n * R; break;
+        case LBM_FIELD_UZ: src = b.uz; need = n * R; break;
+        case LBM_FIELD_OMEGA: src = b.omega; need = n * R; break;
+        case LBM_FIELD_NU_T: src = b.nu_t; need = n * R; break;
+        case LBM_FIELD_SMAG: src = b.smag; need = n * R; break;
+        case LBM_FIELD_PHI:
+        case LBM_FIELD_LATTICE_OPEN:
+        case LBM_FIELD_LATTICE_OPEN_FRAC:
+        case LBM_FIELD_WALL_TYPE:
+        case LBM_FIELD_NEAR_OBSTACLE: {
+            int32_t* lo = (int32_t*)scratch(19 * n * 4);
+            void* lf = scratch(19 * n * R);
+            int32_t* wt = (int32_t*)scratch(19 * n * 4);
+            int32_t* nr = (int32_t*)scratch(n * 4);
+            void* phi = scratch(n * R);
+            launch_geometry_fields(c, b, slot, lo, lf, wt, nr, phi, h->stream);
+            if (field == LBM_FIELD_PHI) src = phi, need = n * R;
+            if (field == LBM_FIELD_LATTICE_OPEN) src = lo, need = 19 * n * 4;
+            if (field == LBM_FIELD_LATTICE_OPEN_FRAC) src = lf, need = 19 * n * R;
+            if (field == LBM_FIELD_WALL_TYPE) src = wt, need = 19 * n * 4;
+            if (field == LBM_FIELD_NEAR_OBSTACLE) src = nr, need = n * 4;
+            break;
+        }
+        case LBM_FIELD_OBSTACLE: {
+            int* o = (int*)scratch(n * 4);
+            launch_obstacle(c, b, slot, o, h->stream);
+            src = o;
+            need = n * 4;
+            break;
+        }
+        case LBM_FIELD_WATER_H: src = b.h[slot]; need = n2 * R; break;
+        case LBM_FIELD_U_WAVE: src = b.uw[slot]; need = n2 * R; break;
+        case LBM_FIELD_V_WAVE: src = b.vw[slot]; need = n2 * R; break;
+        case LBM_FIELD_W_WAVE: src = b.ww[slot]; need = n2 * R; break;
+        case LBM_FIELD_U_INLET_PROFILE: src = b.u_inlet; need = c.nz * R; break;
+        case LBM_FIELD_INFLOW_UX: src = b.inflow; need = (size_t)c.nz * c.ny * R; break;
+        case LBM_FIELD_PIECEWISE_BED_Z: src = b.piecewise; need = c.nx * R; break;
+        case LBM_FIELD_FS_FILL: src = b.fill; need = n * R; water_only = true; break;
+        case LBM_FIELD_FS_TYPE: src = b.type; need = n * 4; water_only = true; break;
+        case LBM_FIELD_FS_TYPE_PREV: src = b.type_prev; need = n * 4; water_only = true; break;
+        case LBM_FIELD_FS_MASS: src = b.mass; need = n * R; water_only = true; break;
+        case LBM_FIELD_FS_MASS_EXCESS: src = b.excess; need = n * R; water_only = true; break;
+        case LBM_FIELD_FS_H: src = b.fs_h; need = n2 * R; water_only = true; break;
+        case LBM_FIELD_VOF_EXTERIOR_GAS: src = b.ext_gas; need = n * 4; water_only = true; break;
+        case LBM_FIELD_PRESSURE_REF_HEAD:
+        case LBM_FIELD_GRAVITY_FRACTION: need = n * R; break;
+        default: return fail(h, LBM_E_ARG, "unknown field " + std::to_string(field));
+    }
+    if (bytes != need)
+        return fail(h, LBM_E_ARG, "field " + std::to_string(field) + ": expected " + std::to_string(need) + " bytes, got " +
+                                      std::to_string(bytes));
+    if (field == LBM_FIELD_NU_T || field == LBM_FIELD_SMAG) {
+        if (!src) {  // the fused path keeps no LES diagnostics
+            memset(dst, 0, need);
+            return LBM_OK;
+        }
+    }
+    if (field == LBM_FIELD_PRESSURE_REF_HEAD || field == LBM_FIELD_GRAVITY_FRACTION) {
+        // update_pressure_reference_fields (CODEX:710-740) on the host from
+        // the open mask and cell types
+        std::vector<int> lo0(n, 1), ty(n, LBM_FS_LIQUID);
+        int* o = (int*)scratch(n * 4);
+        launch_obstacle(c, b, slot, o, h->stream);
+        CHECK(hipMemcpyAsync(lo0.data(), o, n * 4, hipMemcpyDeviceToHost, h->stream));
+        if (c.fs_enabled) CHECK(hipMemcpyAsync(ty.data(), b.type, n * 4, hipMemcpyDeviceToHost, h->stream));
+        CHECK(hipStreamSynchronize(h->stream));
+        for (size_t id = 0; id < n; id++) {
+            size_t k = id / n2;
+            float zc = ((float)k + 0.5f) * c.dx;
+            double head = 0.0, gf = 0.0;
+            bool open = lo0[id] == 0;  // obstacle mask: 0 = open
+            if (c.fs_enabled && open && ty[id] != LBM_FS_GAS) {
+                if (c.pressure_formulation == LBM_PF_HYDROSTATIC_BALANCED) {
+                    head = (float)(c.free_surface_base_m - zc);
+                } else if (c.pressure_formulation == LBM_PF_VOF_COMPONENT_BALANCED) {
+                    if (zc < c.free_surface_base_m)
+                        head = (float)(c.free_surface_base_m - zc);
+                    else
+                        gf = 1.0;
+                } else {
+                    gf = 1.0;
+                }
+            }
+            double v = field == LBM_FIELD_PRESSURE_REF_HEAD ? head : gf;
+            if (R == 8)
+                ((double*)dst)[id] = v;
+            else
+                ((float*)dst)[id] = (float)v;
+        }
+        return LBM_OK;
+    }
+    if (water_only && !c.fs_enabled) return fail(h, LBM_E_ARG, "free-surface fields exist only in water runs");
+    if (field == LBM_FIELD_F) src = post_stream_populations(h);
+    CHECK(hipMemcpyAsync(dst, src, need, hipMemcpyDeviceToHost, h->stream));
+    CHECK(hipStreamSynchronize(h->stream));
+    API_END(h)
+}
+
+int lbm_get_counters(lbm_handle_t* h, lbm_counters_t* out) {
+    API_BEGIN(h)
+    if (!out) return fail(h, LBM_E_ARG, "out is NULL");
+    if (h->path == LBM_PATH_FUSED && h->step > 0) launch_cut_link_count(h->cfg, h->b, slot_of(h->step), h->stream);
+    int ic[IC_COUNT];
+    double dc[DC_COUNT];
+    CHECK(hipMemcpyAsync(ic, h->b.icnt, sizeof(ic), hipMemcpyDeviceToHost, h->stream));
+    CHECK(hipMemcpyAsync(dc, h->b.dcnt, sizeof(dc), hipMemcpyDeviceToHost, h->stream));
+    CHECK(hipStreamSynchronize(h->stream));
+    memset(out, 0, sizeof(*out));
+    out->boundary_cut_link_count = ic[IC_BOUNDARY_CUT];
+    out->boundary_refill_count = ic[IC_BOUNDARY_REFILL];
+    out->free_surface_refill_count = ic[IC_FS_REFILL];
+    out->free_surface_coalescence_repair_count = ic[IC_FS_REPAIR];
+    out->free_surface_thin_gap_bridge_count = ic[IC_FS_BRIDGE];
+    out->free_surface_cut_link_count = ic[IC_FS_CUT];
+    out->free_surface_vof_excess_count = ic[IC_VOF_EXCESS];
+    out->free_surface_vof_orphan_count = ic[IC_VOF_ORPHAN];
+    out->free_surface_vof_collapse_cell_count = ic[IC_VOF_COLLAPSE_CELLS];
+    out->free_surface_vof_detached_advect_count = ic[IC_VOF_ADVECT];
+    out->free_surface_vof_detached_compress_count = ic[IC_VOF_COMPRESS];
+    out->pad0 = ic[IC_FLOOD_UNCONVERGED];
+    out->free_surface_vof_collapse_candidate_volume_m3 = dc[DC_CAND_VOLUME];
+    out->free_surface_vof_collapse_applied_volume_m3 = dc[DC_APPLIED_VOLUME];
+    out->free_surface_vof_collapse_removal_scale = dc[DC_REMOVAL_SCALE];
+    out->free_surface_vof_collapse_candidate_mass = dc[DC_CAND_MASS];
+    out->free_surface_vof_collapse_exterior_capacity = dc[DC_EXT_CAPACITY];
+    out->free_surface_vof_detached_advect_mass = dc[DC_ADVECT_MASS];
+    out->free_surface_vof_detached_compress_mass = dc[DC_COMPRESS_MASS];
+    API_END(h)
+}
+
+int lbm_reduce(lbm_handle_t* h, int32_t what, double* out) {
+    API_BEGIN(h)
+    if (!out) return fail(h, LBM_E_ARG, "out is NULL");
+    if (what < 0 || what > LBM_REDUCE_NONFINITE) return fail(h, LBM_E_ARG, "unknown reduction");
+    if ((what == LBM_REDUCE_VOF_MASS || what == LBM_REDUCE_VOF_VOLUME) && !h->cfg.fs_enabled)
+        return fail(h, LBM_E_ARG, "VOF reductions need a water run");
+    launch_cellflags(h->cfg, h->b, slot_of(h->step), h->stream);
+    const void* f = h->b.f[h->cur];
+    int is_storage = 1;
+    if (what == LBM_REDUCE_MASS || what == LBM_REDUCE_NONFINITE) {
+        f = post_stream_populations(h);
+        is_storage = f == h->b.f[h->cur];
+    }
+    launch_reduce(h->cfg, h->b, f, is_storage, what, h->stream);
+    double v = 0.0;
+    CHECK(hipMemcpyAsync(&v, h->b.dcnt + DC_REDUCE, sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    CHECK(hipStreamSynchronize(h->stream));
+    *out = v;
+    API_END(h)
+}
+
+int lbm_get_step(lbm_handle_t* h, int64_t* out) {
+    if (!h || !out) return LBM_E_ARG;
+    *out = h->step;
+    return LBM_OK;
+}
+
+int lbm_sync(lbm_handle_t* h) {
+    API_BEGIN(h)
+    CHECK(hipStreamSynchronize(h->stream));
+    API_END(h)
+}
+
+int lbm_last_step_timing(lbm_handle_t* h, double* ms_per_step, double* ms_main_kernel) {
+    if (!h) return LBM_E_ARG;
+    if (!h->timing) return fail(h, LBM_E_STATE, "timing is off (lbm_set_option(h, \"timing\", 1))");
+    if (ms_per_step) *ms_per_step = h->last_ms_step;
+    if (ms_main_kernel) *ms_main_kernel = h->last_ms_kernel;
+    return LBM_OK;
+}
+
+int lbm_device_bytes(const lbm_config_t* cfg, size_t* out) {
+    const char* why = validate(cfg);
+    if (why || !out) return LBM_E_CONFIG;
+    *out = device_bytes(*cfg, resolve_path(*cfg));
+    return LBM_OK;
+}
+
+int lbm_set_option(lbm_handle_t* h, const char* key, int64_t value) {
+    if (!h || !key) return LBM_E_ARG;
+    if (!strcmp(key, "timing")) {
+        h->timing = value ? 1 : 0;
+        return LBM_OK;
+    }
+    if (!strcmp(key, "kchunk")) {
+        if (value < 1 || value > h->cfg.nz) return fail(h, LBM_E_ARG, "kchunk must be in [1, nz]");
+        h->kchunk = (int)value;
+        return LBM_OK;
+    }
+    return fail(h, LBM_E_ARG, std::string("unknown option ") + key);
+}
+
+int lbm_get_path(lbm_handle_t* h, int32_t* out) {
+    if (!h || !out) return LBM_E_ARG;
+    *out = h->path;
+    return LBM_OK;
+}
+
+}  // extern "C"

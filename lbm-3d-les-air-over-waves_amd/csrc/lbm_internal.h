@@ -1,0 +1,87 @@
+// lbm_internal.h -- interface between the C ABI (lbm_capi.hip) and the
+// kernel launchers (lbm_kernels.hip).  Device buffers are passed as untyped
+// pointers; the launchers cast them to the arithmetic type of the run
+// (double when storage == LBM_STORE_FP64, float otherwise).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#include "../../include/lbm.h"
+
+namespace lbm {
+
+// integer counters kept on the device (indices into DevBufs::icnt)
+enum {
+    IC_BOUNDARY_CUT = 0,
+    IC_FS_CUT,
+    IC_FS_BRIDGE,
+    IC_BOUNDARY_REFILL,
+    IC_FS_REFILL,
+    IC_FS_REPAIR,
+    IC_VOF_EXCESS,
+    IC_VOF_ORPHAN,
+    IC_VOF_COLLAPSE_CELLS,
+    IC_VOF_ADVECT,
+    IC_VOF_COMPRESS,
+    IC_FLOOD_CHANGED,      // set by a flood-fill sweep that marked a cell
+    IC_FLOOD_UNCONVERGED,  // sweeps exhausted before the fixed point
+    IC_COUNT
+};
+// FP64 counters (indices into DevBufs::dcnt)
+enum {
+    DC_CAND_MASS = 0,
+    DC_CAND_VOLUME,
+    DC_EXT_CAPACITY,
+    DC_APPLIED_VOLUME,
+    DC_REMOVAL_SCALE,
+    DC_ADVECT_MASS,
+    DC_COMPRESS_MASS,
+    DC_REDUCE,  // scratch for lbm_reduce
+    DC_COUNT = 12
+};
+
+struct DevBufs {
+    int storage;
+    size_t n, n2;
+    void* f[2];  // populations, storage type, 19*n each; f[cur] is the state
+    void* ftmp;  // arithmetic-type scratch of 19*n for materialised downloads
+    void *rho, *ux, *uy, *uz, *omega, *nu_t, *smag;
+    void *h[2], *uw[2], *vw[2], *ww[2], *eta9[2], *denom[2];
+    int* kband[2];
+    void *piecewise, *u_inlet, *inflow;
+    uint8_t* cellflags;  // bit0 lattice_open[0], bit1 near_obstacle (staged path, current slot)
+    // water / VOF
+    void *fill, *mass, *mass2, *mdelta, *excess, *sa, *sb, *sc, *sd, *se, *smv, *fs_h;
+    int *type, *type_prev, *excess_dir, *ext_gas, *obst, *obst_prev, *si;
+    int* icnt;      // [IC_COUNT]
+    double* dcnt;   // [DC_COUNT]
+};
+
+// host-side launchers (lbm_kernels.hip)
+void launch_bed2d(const lbm_config_t& c, DevBufs& b, float t, int slot, hipStream_t s);
+void launch_geometry_fields(const lbm_config_t& c, DevBufs& b, int slot, int32_t* lo, void* lf, int32_t* wt, int32_t* near,
+                            void* phi, hipStream_t s);
+void launch_cellflags(const lbm_config_t& c, DevBufs& b, int slot, hipStream_t s);
+void launch_init_populations(const lbm_config_t& c, DevBufs& b, int slot, hipStream_t s);
+// one staged step: bed arrays for slot must be current; advances b.f[cur]
+void launch_staged_step(const lbm_config_t& c, DevBufs& b, int& cur, int slot, int64_t step, int omega_band_top,
+                        hipStream_t s);
+// fused step: mode 0 reads post-stream populations, mode 1 post-collision
+// ones; slot_prev is the time level of the pending stream, slot the new one
+void launch_fused_step(const lbm_config_t& c, DevBufs& b, int& cur, int mode, int slot_prev, int slot, int64_t step,
+                       int write_macro, int omega_band_top, int kchunk, hipStream_t s);
+// post-collision state -> post-stream populations (arithmetic type) in dst
+void launch_materialize(const lbm_config_t& c, DevBufs& b, int cur, int slot, void* dst, hipStream_t s);
+// convert arithmetic-type populations in src into the storage buffer f[cur]
+void launch_store_populations(const lbm_config_t& c, DevBufs& b, int cur, const void* src, hipStream_t s);
+void launch_reduce(const lbm_config_t& c, DevBufs& b, const void* f, int f_is_storage, int what, hipStream_t s);
+void launch_cut_link_count(const lbm_config_t& c, DevBufs& b, int slot, hipStream_t s);
+bool fused_supported(const lbm_config_t& c);
+// obstacle mask of a time level (init_fields, CODEX:2711-2712) into dst
+void launch_obstacle(const lbm_config_t& c, DevBufs& b, int slot, int* dst, hipStream_t s);
+// storage-type populations f[cur] -> arithmetic-type dst
+void launch_load_populations(const lbm_config_t& c, DevBufs& b, int cur, void* dst, hipStream_t s);
+
+}  // namespace lbm

@@ -1,0 +1,2183 @@
+// lbm_kernels.hip -- gfx950 kernels of the D3Q19 step and their launchers.
+//
+// Two execution paths compute the same step:
+//  * staged: one kernel per stage of the reference step loop
+//    (LBM_3D_LES_air_over_waves_CODEX.py:3873-3925), whole fields in HBM,
+//    used for water/VOF runs and as the cross-check of the fused path;
+//  * fused: one z-marching pass per step for air runs.  The state between
+//    steps is the post-collision population set f*; one pass streams it
+//    (with every boundary rule), conditions the wall links of the next
+//    step, forms the moments, evaluates the LES viscosity from velocity
+//    planes held in LDS, collides and writes f* of the next step.  Each
+//    population is read once and written once per step.
+//
+// Arithmetic follows the CPU restatement operation for operation (see
+// lbm_device.h), so results agree bit for bit in FP32 and FP64 storage.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <stdexcept>
+#include <string>
+
+#include "lbm_device.h"
+#include "lbm_internal.h"
+
+namespace lbm {
+
+#define HIP_OK(expr)                                                                                    \
+    do {                                                                                                \
+        hipError_t e_ = (expr);                                                                         \
+        if (e_ != hipSuccess) throw std::runtime_error(std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+static constexpr int NT = 256;  // threads per block of the one-cell-per-thread kernels
+
+template <class T>
+static Consts<T> make_consts(const lbm_config_t& c, int omega_band_top) {
+    Consts<T> k{};
+    k.nx = c.nx;
+    k.ny = c.ny;
+    k.nz = c.nz;
+    k.physics_water = c.physics_mode == LBM_PHYS_WATER;
+    k.collision_reg = c.collision == LBM_COLL_REGULARIZED;
+    k.use_les = c.use_les;
+    k.x_boundary = c.x_boundary;
+    k.y_boundary = c.y_boundary;
+    k.top_boundary = c.top_boundary;
+    k.bed_profile = c.bed_profile;
+    k.bed_moving = c.bed_profile == LBM_BED_MOVING_SINE || c.bed_profile == LBM_BED_MOVING_SINE1;
+    k.include_bed = c.include_bed;
+    k.wall_link = c.wall_velocity_link;
+    k.top_drive = c.enable_top_drive;
+    k.x_fringe = c.enable_x_fringe;
+    k.top_sponge = c.enable_top_sponge;
+    k.fs_vof = c.fs_enabled && c.fs_tracking == LBM_FST_VOF;
+    k.pf = c.pressure_formulation;
+    k.fs_boundary_hydro = c.fs_boundary_mode == LBM_FSB_HYDROSTATIC;
+    k.refill_mode = c.refill_mode;
+    k.link_aperture = c.vof_link_aperture;
+    k.thin_gap_bridge = c.vof_thin_gap_bridge;
+    k.omega_band_top = omega_band_top;
+    k.seed = c.seed;
+    k.dx = (T)c.dx;
+    k.rho0 = (T)c.rho0;
+    k.tau0 = (T)c.tau0;
+    k.nu0 = (T)c.nu0;
+    T csd = (T)c.cs_les * (T)c.delta_les;
+    k.cs2_les = csd * csd;
+    k.nu_t_max = (T)c.nu_t_max;
+    k.kappa = (T)c.kappa;
+    k.z0_phys = (T)c.z0_phys;
+    k.u_top_lb = (T)c.u_top_lb;
+    k.u_current_lb = (T)c.u_current_lb;
+    k.g_lb = (T)c.g_lb;
+    k.g_body_lb = (T)c.g_body_lb;
+    k.c_backscatter = (T)c.c_backscatter;
+    k.free_surface_base_m = (T)c.free_surface_base_m;
+    k.bridge_strength = (T)c.vof_thin_gap_bridge_strength;
+    k.offset_m = (T)c.offset_m;
+    k.bed_amp_m = (T)c.bed_amp_m;
+    k.bed_wavelength_m = (T)c.bed_wavelength_m;
+    k.wave_speed = (T)c.wave_speed;
+    k.wave_period = (T)c.wave_period;
+    k.depth_swl_m = (T)c.depth_swl_m;
+    k.vel_scale = (T)c.vel_scale;
+    k.phase_speed = (T)c.bed_phase_speed_mps;
+    k.dt_phys = (T)(float)c.dt_phys;
+    float sc = 1.0f;
+    for (int i = 0; i < c.storage_shift; i++) sc *= 2.0f;
+    k.store_scale = sc;
+    k.store_inv_scale = 1.0f / sc;
+    return k;
+}
+
+template <class T>
+static Slot<T> make_slot(DevBufs& b, int s) {
+    Slot<T> sl;
+    sl.h = (const T*)b.h[s];
+    sl.uw = (const T*)b.uw[s];
+    sl.vw = (const T*)b.vw[s];
+    sl.ww = (const T*)b.ww[s];
+    sl.eta9 = (const T*)b.eta9[s];
+    sl.denom = (const T*)b.denom[s];
+    sl.kband = b.kband[s];
+    return sl;
+}
+
+static inline unsigned nblocks(size_t n, int nt = NT) { return (unsigned)((n + nt - 1) / nt); }
+
+__device__ __forceinline__ void cell_ijk(size_t id, int nx, int ny, int& i, int& j, int& k) {
+    i = (int)(id % (size_t)nx);
+    size_t r = id / (size_t)nx;
+    j = (int)(r % (size_t)ny);
+    k = (int)(r / (size_t)ny);
+}
+
+// ===========================================================================
+// 2-D bed kernels
+// ===========================================================================
+
+// update_wave_bed_and_velocities (CODEX:1082-1162), one thread per column;
+// moving_sine1 is the single-harmonic bed of the new configuration
+template <class T>
+__global__ void k_bed2d_a(Consts<T> c, T t, T* h, T* uw, T* vw, T* ww, const T* h_last, const T* piecewise) {
+    size_t id = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= (size_t)c.nx * c.ny) return;
+    int i = (int)(id % (size_t)c.nx);
+    T dx = c.dx;
+    T kw = T(2.0) * T(3.141592653589793) / c.bed_wavelength_m;
+    T cs = c.wave_speed;
+    T ramp = T(2.0) * c.wave_period;
+    T amp_now = T(0);
+    if (c.bed_profile == LBM_BED_MOVING_SINE)
+        amp_now = t < ramp ? c.bed_amp_m * smoothstep01(t / ramp) : c.bed_amp_m;
+    T x0 = (T)i * dx;
+    T hl = h_last[id];
+    T eta1 = T(0), eta2 = T(0), eta = T(0), phase = T(0), phase2 = T(0);
+    if (c.bed_profile == LBM_BED_MOVING_SINE) {
+        T x_phase = x0 - cs * t;
+        T x_phase2 = x0 - T(0.5) * cs * t;
+        phase = T(0.5) * kw * x_phase;
+        phase2 = kw * x_phase2;
+        eta1 = amp_now * (T)det_sincos((double)phase, 0);
+        eta2 = amp_now * (T)det_sincos((double)phase2, 0);
+        eta = eta1 + eta2;
+    } else if (c.bed_profile == LBM_BED_MOVING_SINE1) {
+        phase = kw * (x0 - c.phase_speed * t);
+        eta1 = c.bed_amp_m * (T)det_sincos((double)phase, 0);
+        eta = eta1;
+    }
+    T bed_h = c.offset_m + eta;
+    if (c.bed_profile == LBM_BED_PIECEWISE) bed_h = piecewise[i];
+    h[id] = bed_h;
+    T u_s = T(0), v_s = T(0), w_s = T(0);
+    T h0 = c.depth_swl_m;
+    if (c.bed_profile == LBM_BED_MOVING_SINE) {
+        T eta_x1 = amp_now * T(0.5) * kw * (T)det_sincos((double)phase, 1);
+        T eta_x2 = amp_now * kw * (T)det_sincos((double)phase2, 1);
+        T eta_x = eta_x1 + eta_x2;
+        T eta_y = T(0);
+        T eta_t = (bed_h - hl) / c.dt_phys;
+        u_s = cs * eta1 / (h0 + eta1) + T(0.5) * cs * eta2 / (h0 + eta2);
+        w_s = eta_t + u_s * eta_x + v_s * eta_y;
+    } else if (c.bed_profile == LBM_BED_MOVING_SINE1) {
+        T eta_x = c.bed_amp_m * kw * (T)det_sincos((double)phase, 1);
+        T eta_t = (bed_h - hl) / c.dt_phys;
+        u_s = c.phase_speed * eta1 / (h0 + eta1);
+        w_s = eta_t + u_s * eta_x;
+    }
+    uw[id] = u_s / c.vel_scale;
+    vw[id] = v_s / c.vel_scale;
+    ww[id] = w_s / c.vel_scale;
+}
+
+// per-column link-endpoint bed heights (CODEX:2536-2569 at the 9 (cx,cy)
+// endpoints), phi slope normaliser (CODEX:2451-2469) and the highest k with
+// a cut or closed link (cells above it are fully open: frac = 1 everywhere)
+template <class T>
+__global__ void k_bed2d_b(Consts<T> c, const T* h, T* eta9, T* denom, int* kband) {
+    size_t id = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    size_t plane = (size_t)c.nx * c.ny;
+    if (id >= plane) return;
+    int i = (int)(id % (size_t)c.nx), j = (int)(id / (size_t)c.nx);
+    T dx = c.dx;
+    T xc = ((T)i + T(0.5)) * dx, yc = ((T)j + T(0.5)) * dx;
+    T hmax = T(-1e30);
+    for (int a = -1; a <= 1; a++)
+        for (int bb = -1; bb <= 1; bb++) {
+            T xq = xc + T(0.5) * dx * (T)a;
+            T yq = yc + T(0.5) * dx * (T)bb;
+            T e = sample2d(h, xq, yq, c.nx, c.ny, dx);
+            eta9[(size_t)((a + 1) * 3 + (bb + 1)) * plane + id] = e;
+            hmax = e > hmax ? e : hmax;
+        }
+    int ip = (i + 1) % c.nx, im = (i - 1 + c.nx) % c.nx;
+    int jp = (j + 1) % c.ny, jm = (j - 1 + c.ny) % c.ny;
+    T dh_dx = (h[(size_t)j * c.nx + ip] - h[(size_t)j * c.nx + im]) * (T(0.5) / dx);
+    if (c.x_boundary != LBM_XB_PERIODIC) {
+        if (i == 0)
+            dh_dx = (h[(size_t)j * c.nx + i + 1] - h[(size_t)j * c.nx + i]) / dx;
+        else if (i == c.nx - 1)
+            dh_dx = (h[(size_t)j * c.nx + i] - h[(size_t)j * c.nx + i - 1]) / dx;
+    }
+    T dh_dy = (h[(size_t)jp * c.nx + i] - h[(size_t)jm * c.nx + i]) * (T(0.5) / dx);
+    T d = sqrt(T(1.0) + dh_dx * dh_dx + dh_dy * dh_dy);
+    if (d < T(1e-6)) d = T(1.0);
+    denom[id] = d;
+    // a link of cell k has frac < 1 only when zq - eta < dx/2 with
+    // zq >= k dx, i.e. k < eta/dx + 1/2; one extra cell of margin
+    int kb = -1;
+    if (c.include_bed) {
+        double kk = ceil((double)hmax / (double)dx + 0.5) + 1.0;
+        kb = kk < -1.0 ? -1 : (kk > (double)c.nz ? c.nz : (int)kk);
+    }
+    bool wall_col = (c.x_boundary == LBM_XB_SOLID && (i == 0 || i == c.nx - 1)) ||
+                    (c.y_boundary == LBM_YB_SOLID && (j == 0 || j == c.ny - 1));
+    if (wall_col) kb = c.nz;
+    kband[id] = kb;
+}
+
+// ===========================================================================
+// geometry materialisation (downloads) and per-cell flags
+// ===========================================================================
+template <class T>
+__global__ void k_geometry_fields(Consts<T> c, Slot<T> s, int32_t* lo, T* lf, int32_t* wt, int32_t* near, T* phi) {
+    size_t id = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    size_t n = (size_t)c.nx * c.ny * c.nz;
+    if (id >= n) return;
+    int i, j, k;
+    cell_ijk(id, c.nx, c.ny, i, j, k);
+    CellGeo<T> g;
+    cell_geometry(c, s, i, j, k, g);
+    for (int q = 0; q < 19; q++) {
+        lo[(size_t)q * n + id] = (g.open_mask >> q) & 1u;
+        lf[(size_t)q * n + id] = g.lf[q];
+        wt[(size_t)q * n + id] = q == 0 ? (c.include_bed ? LBM_WALL_BED : LBM_WALL_NONE) : link_wall_type(c, g, i, j, q);
+    }
+    near[id] = g.near;
+    phi[id] = cell_phi(c, s, i, j, k);
+}
+
+template <class T>
+__global__ void k_cellflags(Consts<T> c, Slot<T> s, uint8_t* flags) {
+    size_t id = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    size_t n = (size_t)c.nx * c.ny * c.nz;
+    if (id >= n) return;
+    int i, j, k;
+    cell_ijk(id, c.nx, c.ny, i, j, k);
+    CellGeo<T> g;
+    cell_geometry(c, s, i, j, k, g);
+    flags[id] = (uint8_t)((g.open_mask & 1u) | (g.near ? 2u : 0u));
+}
+
+// gravity of one cell for the Guo force (CODEX:646-651 with
+// update_pressure_reference_fields CODEX:710-740 evaluated in place)
+template <class T>
+__device__ __forceinline__ T gravity_fraction(const Consts<T>& c, int k, bool open0, int type) {
+    if (!(c.physics_water && open0 && type != LBM_FS_GAS)) return T(0);
+    if (c.pf == LBM_PF_HYDROSTATIC_BALANCED) return T(0);
+    if (c.pf == LBM_PF_VOF_COMPONENT_BALANCED) {
+        T zc = ((T)k + T(0.5)) * c.dx;
+        return zc < c.free_surface_base_m ? T(0) : T(1);
+    }
+    return T(1);
+}
+template <class T>
+__device__ __forceinline__ T local_g(const Consts<T>& c, int k, bool open0, int type) {
+    if (c.physics_water && c.pf == LBM_PF_VOF_COMPONENT_BALANCED) return c.g_lb * gravity_fraction(c, k, open0, type);
+    return c.g_body_lb;
+}
+
+// ===========================================================================
+// initialisation: init_fields populations (CODEX:2711-2818)
+// ===========================================================================
+template <class S, class T>
+__global__ void k_init_populations(Consts<T> c, Slot<T> s, S* f, T* rho, T* ux, T* uy, T* uz, int* obst, const int* type) {
+    size_t id = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    size_t n = (size_t)c.nx * c.ny * c.nz;
+    if (id >= n) return;
+    int i, j, k;
+    cell_ijk(id, c.nx, c.ny, i, j, k);
+    CellGeo<T> g;
+    cell_geometry(c, s, i, j, k, g);
+    bool open0 = g.open_mask & 1u;
+    obst[id] = open0 ? 0 : 1;
+    if (!open0) {
+        T a = T(0), bq = T(0), e = T(0);
+        if (c.include_bed) {
+            size_t col = (size_t)j * c.nx + i;
+            a = s.uw[col];
+            bq = s.vw[col];
+            e = s.ww[col];
+        }
+        ux[id] = a;
+        uy[id] = bq;
+        uz[id] = e;
+        rho[id] = c.rho0;
+    }
+    T u = ux[id], v = uy[id], w = uz[id];
+    int ty = c.physics_water ? type[id] : LBM_FS_LIQUID;
+    if (c.physics_water && open0 && ty != LBM_FS_GAS) w += T(0.5) * local_g(c, k, open0, ty);
+    T r = rho[id];
+    T u2 = u * u + v * v + w * w;
+#pragma unroll
+    for (int q = 0; q < 19; q++) {
+        T cu = T(3.0) * ((T)cxq(q) * u + (T)cyq(q) * v + (T)czq(q) * w);
+        T fv = r * wq<T>(q) * (T(1.0) + cu + T(0.5) * cu * cu - T(1.5) * u2);
+        Store<S, T>::store(f, (size_t)q * n + id, q, fv, c.store_scale);
+    }
+}
+
+// ===========================================================================
+// staged path: one kernel per reference stage
+// ===========================================================================
+
+// update_bed_populations_and_reinit (CODEX:2878-2940)
+template <class T>
+__global__ void k_cond(Consts<T> c, Slot<T> s, T* f, int* obst, int* obst_prev) {
+    size_t id = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    size_t n = (size_t)c.nx * c.ny * c.nz;
+    if (id >= n) return;
+    int i, j, k;
+    cell_ijk(id, c.nx, c.ny, i, j, k);
+    CellGeo<T> g;
+    cell_geometry(c, s, i, j, k, g);
+    obst_prev[id] = obst[id];
+    obst[id] = (g.open_mask & 1u) ? 0 : 1;
+    if ((g.open_mask & 0x7ffffu) == 0x7ffffu) return;  // no closed link
+    T fl[19];
+#pragma unroll
+    for (int q = 0; q < 19; q++) fl[q] = f[(size_t)q * n + id];
+    condition_cell(c, s, g, i, j, fl);
+#pragma unroll
+    for (int q = 0; q < 19; q++) f[(size_t)q * n + id] = fl[q];
+}
+
+// macro_step (CODEX:2943-3008)
+template <class T>
+__global__ void k_macro(Consts<T> c, const T* f, T* rho, T* ux, T* uy, T* uz, const int* type, const uint8_t* flags) {
+    size_t id = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    size_t n = (size_t)c.nx * c.ny * c.nz;
+    if (id >= n) return;
+    int i, j, k;
+    cell_ijk(id, c.nx, c.ny, i, j, k);
+    T fl[19];
+#pragma unroll
+    for (int q = 0; q < 19; q++) fl[q] = f[(size_t)q * n + id];
+    T r = T(0);
+#pragma unroll
+    for (int q = 0; q < 19; q++) r += fl[q];
+    T jx = T(0), jy = T(0), jz = T(0);
+#pragma unroll
+    for (int q = 0; q < 19; q++) {
+        if (cxq(q)) jx += fl[q] * (T)cxq(q);
+        if (cyq(q)) jy += fl[q] * (T)cyq(q);
+        if (czq(q)) jz += fl[q] * (T)czq(q);
+    }
+    T u, v, w;
+    if (c.physics_water) {
+        int ty = type[id];
+        if (ty == LBM_FS_GAS) {
+            r = c.rho0;
+            u = v = w = T(0);
+        } else {
+            T force_z = -r * local_g(c, k, flags[id] & 1u, ty);
+            T inv = T(1.0) / r;
+            u = jx * inv;
+            v = jy * inv;
+            w = (jz + T(0.5) * force_z) * inv;
+        }
+    } else {
+        T inv = T(1.0) / r;
+        u = jx * inv;
+        v = jy * inv;
+        w = jz * inv;
+    }
+    if (c.top_drive && k == c.nz - 1) {
+        u = c.u_top_lb;
+        v = T(0);
+        w = T(0);
+    }
+    rho[id] = r;
+    ux[id] = u;
+    uy[id] = v;
+    uz[id] = w;
+}
+
+// compute_LES (CODEX:3011-3141)
+template <class T>
+__global__ void k_les(Consts<T> c, Slot<T> s, const T* ux, const T* uy, const T* uz, T* omega, T* nu_t, T* smag,
+                      const int* type, const uint8_t* flags) {
+    size_t id = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    size_t n = (size_t)c.nx * c.ny * c.nz;
+    if (id >= n) return;
+    int i, j, k;
+    cell_ijk(id, c.nx, c.ny, i, j, k);
+    bool active = flags[id] & 1u;
+    if (c.physics_water && type[id] == LBM_FS_GAS) active = false;
+    if (!active) return;
+    int nx = c.nx, ny = c.ny, nz = c.nz;
+    int ip = (i + 1) % nx, im = (i + nx - 1) % nx, jp = (j + 1) % ny, jm = (j + ny - 1) % ny;
+    int kp = k + 1 >= nz ? nz - 1 : k + 1, km = k - 1 < 0 ? 0 : k - 1;
+    size_t pl = (size_t)nx * ny;
+    size_t base = (size_t)k * pl;
+    size_t xp = base + (size_t)j * nx + ip, xm = base + (size_t)j * nx + im;
+    size_t yp = base + (size_t)jp * nx + i, ym = base + (size_t)jm * nx + i;
+    size_t zp = (size_t)kp * pl + (size_t)j * nx + i, zm = (size_t)km * pl + (size_t)j * nx + i;
+    bool near_wall = k > 0 && !(flags[id - pl] & 1u);
+    T phi = T(0), uwx = T(0), uwy = T(0);
+    if (near_wall) {
+        phi = cell_phi(c, s, i, j, k);
+        if (c.include_bed) {
+            uwx = s.uw[(size_t)j * nx + i];
+            uwy = s.vw[(size_t)j * nx + i];
+        }
+    }
+    T nt, sm;
+    T om = les_omega(c, ux[xp], ux[xm], ux[yp], ux[ym], ux[zp], ux[zm], uy[xp], uy[xm], uy[yp], uy[ym], uy[zp], uy[zm], uz[xp],
+                     uz[xm], uz[yp], uz[ym], uz[zp], uz[zm], near_wall, phi, ux[id], uy[id], uwx, uwy, &nt, &sm);
+    omega[id] = om;
+    nu_t[id] = nt;
+    smag[id] = sm;
+}
+
+// collide_regularized / collide_KBC (CODEX:3143-3416)
+template <class T>
+__global__ void k_collide(Consts<T> c, T* f, const T* rho, const T* ux, const T* uy, const T* uz, const T* omega,
+                          const int* type, const uint8_t* flags, uint32_t step) {
+    size_t id = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    size_t n = (size_t)c.nx * c.ny * c.nz;
+    if (id >= n) return;
+    int i, j, k;
+    cell_ijk(id, c.nx, c.ny, i, j, k);
+    T r = rho[id], u = ux[id], v = uy[id], w = uz[id];
+    bool open0 = flags[id] & 1u;
+    bool gas = false;
+    int ty = LBM_FS_LIQUID;
+    if (c.physics_water) {
+        ty = type[id];
+        if (ty == LBM_FS_GAS) {
+            gas = true;
+            r = c.rho0;
+            u = v = w = T(0);
+        }
+    }
+    T fl[19];
+#pragma unroll
+    for (int q = 0; q < 19; q++) fl[q] = f[(size_t)q * n + id];
+    T om = omega[id];
+    if (c.collision_reg) {
+        if (gas) {
+#pragma unroll
+            for (int q = 0; q < 19; q++) fl[q] = feq(r, u, v, w, q);
+        } else if (c.physics_water) {
+            T force_z = -r * local_g(c, k, open0, ty);
+            collide_regularized<T, true>(fl, r, u, v, w, om, force_z);
+        } else {
+            collide_regularized<T, false>(fl, r, u, v, w, om, T(0));
+        }
+    } else {
+        if (c.top_sponge && k > c.nz - 3) {
+            sponge_relax(fl, r, u, v, w);
+        } else {
+            bool bs = c.c_backscatter > T(0) && open0;
+            float xi = bs ? bs_uniform(c.seed, step, (uint32_t)id) : 0.0f;
+            if (c.physics_water && !gas) {
+                T force_z = -r * local_g(c, k, open0, ty);
+                collide_kbc<T, true>(fl, r, u, v, w, om, force_z, bs, c.c_backscatter, xi);
+            } else {
+                collide_kbc<T, false>(fl, r, u, v, w, om, T(0), bs, c.c_backscatter, xi);
+            }
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < 19; q++) f[(size_t)q * n + id] = fl[q];
+}
+
+// free-surface gas density at a link (CODEX:821-867)
+template <class T>
+__device__ __forceinline__ T fs_density_link(const Consts<T>& c, int k, int q, bool open0, int type, const T* fs_h, size_t col) {
+    T dx = c.dx;
+    if (c.physics_water && c.pf == LBM_PF_VOF_COMPONENT_BALANCED) {
+        T z_if = ((T)k + T(0.5)) * dx - T(0.5) * (T)czq(q) * dx;
+        T head = T(0);
+        if (gravity_fraction(c, k, open0, type) < T(0.5)) head += z_if - c.free_surface_base_m;
+        return c.rho0 + c.g_lb * (head / dx) / (T(1.0) / T(3.0));
+    }
+    if (c.physics_water && c.pf == LBM_PF_HYDROSTATIC_BALANCED && c.fs_vof) {
+        T z_if = ((T)k + T(0.5)) * dx - T(0.5) * (T)czq(q) * dx;
+        T head = z_if - c.free_surface_base_m;
+        return c.rho0 + c.g_lb * (head / dx) / (T(1.0) / T(3.0));
+    }
+    // free_surface_hydrostatic_density_at_cell, total-pressure form
+    T zc = ((T)k + T(0.5)) * dx;
+    T head = fs_h[col] - zc;
+    if (head < T(0)) head = T(0);
+    return c.rho0 + c.g_lb * (head / dx) / (T(1.0) / T(3.0));
+}
+
+// post-collision value of population q at (si,sj,sk), with the ghost layers
+// of the inflow/outflow x boundary (new): inflow = feq(rho0, u_in),
+// outflow = copy of the last interior layer
+template <class S, class T>
+__device__ __forceinline__ T src_value(const Consts<T>& c, const S* f, size_t n, const T* inflow, int si, int sj, int sk, int q) {
+    if (si < 0) return feq(c.rho0, inflow[(size_t)sk * c.ny + sj], T(0), T(0), q);
+    if (si >= c.nx) si = c.nx - 1;
+    size_t id = ((size_t)sk * c.ny + sj) * c.nx + si;
+    return Store<S, T>::load(f, (size_t)q * n + id, q, (T)c.store_inv_scale);
+}
+
+// stream + apply_x_solid_wall_bounceback + apply_open_boundary_conditions
+// (CODEX:3418-3712) for one destination cell; fin = post-collision state.
+template <class S, class T>
+__global__ void k_stream_bc(Consts<T> c, Slot<T> s, const S* fin, T* fout, const T* ux, const T* uy, const T* uz,
+                            const int* type, const T* fill, const T* u_inlet, const T* inflow, const T* fs_h,
+                            const uint8_t* flags, int* icnt, int count) {
+    size_t id = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    size_t n = (size_t)c.nx * c.ny * c.nz;
+    if (id >= n) return;
+    int i, j, k;
+    cell_ijk(id, c.nx, c.ny, i, j, k);
+    const int nx = c.nx, ny = c.ny, nz = c.nz;
+    const bool water = c.physics_water;
+    const bool xper = c.x_boundary == LBM_XB_PERIODIC, xio = c.x_boundary == LBM_XB_INFLOW_OUTFLOW;
+    const bool yper = c.y_boundary == LBM_YB_PERIODIC;
+    const T inv = (T)c.store_inv_scale;
+    CellGeo<T> g;
+    cell_geometry(c, s, i, j, k, g);
+    bool open0 = g.open_mask & 1u;
+    int ty = water ? type[id] : LBM_FS_LIQUID;
+    bool dest_gas = water && ty == LBM_FS_GAS && open0;
+    T fpre[19], fo[19];
+#pragma unroll
+    for (int q = 0; q < 19; q++) fpre[q] = Store<S, T>::load(fin, (size_t)q * n + id, q, inv);
+    int n_cut = 0, n_fs = 0, n_bridge = 0;
+    fo[0] = dest_gas ? feq(c.rho0, T(0), T(0), T(0), 0) : fpre[0];
+    size_t col = (size_t)j * nx + i;
+#pragma unroll
+    for (int q = 1; q < 19; q++) {
+        const int qb = oppq(q);
+        int sir = i - cxq(q), si = sir;
+        bool fluid = true;
+        if (xper)
+            si = pmod(sir, nx);
+        else if (!xio && (sir < 0 || sir >= nx))
+            fluid = false;
+        int sjr = j - cyq(q), sj = yper ? pmod(sjr, ny) : sjr;
+        if (!yper && (sjr < 0 || sjr >= ny)) fluid = false;
+        int sk = k - czq(q);
+        T out;
+        if (water) {
+            bool src_gas = false;
+            if (fluid && sk >= 0 && sk < nz) {
+                if (si >= 0 && si < nx) {
+                    size_t sid = ((size_t)sk * ny + sj) * nx + si;
+                    src_gas = type[sid] == LBM_FS_GAS && (flags[sid] & 1u);
+                }
+            } else if (sk >= nz) {
+                src_gas = true;
+            }
+            if (dest_gas) {
+                out = feq(c.rho0, T(0), T(0), T(0), q);
+            } else if (!fluid) {
+                out = fpre[qb];
+                n_cut++;
+            } else if (sk >= 0 && sk < nz && !src_gas) {
+                T frac = g.lf[q];
+                T fstr = frac * src_value<S, T>(c, fin, n, inflow, si, sj, sk, q) + (T(1.0) - frac) * fpre[qb];
+                out = fstr;
+                if (c.fs_vof && c.link_aperture) {
+                    T la = T(1);
+                    if (si >= 0 && si < nx) {
+                        size_t sid = ((size_t)sk * ny + sj) * nx + si;
+                        if (flags[sid] & 1u) {
+                            int st = type[sid];
+                            if (st == LBM_FS_GAS)
+                                la = T(0);
+                            else if (st == LBM_FS_INTERFACE) {
+                                la = fill[sid];
+                                if (la < T(0)) la = T(0);
+                                if (la > T(1)) la = T(1);
+                            }
+                        }
+                    }
+                    if (la < T(1.0)) {
+                        T rfs = c.rho0;
+                        if (c.pf != LBM_PF_TOTAL || c.fs_boundary_hydro) rfs = fs_density_link(c, k, q, open0, ty, fs_h, col);
+                        T fa = feq(rfs, ux[id], uy[id], uz[id], q);
+                        T fb = feq(rfs, ux[id], uy[id], uz[id], qb);
+                        T ffs = fa + fb - fpre[qb];
+                        out = la * fstr + (T(1.0) - la) * ffs;
+                        n_fs++;
+                    }
+                }
+                if (frac < T(1.0)) n_cut++;
+            } else if (sk < 0) {
+                out = fpre[qb];
+                n_cut++;
+            } else {
+                T rfs = c.rho0;
+                if (c.pf != LBM_PF_TOTAL || c.fs_boundary_hydro) rfs = fs_density_link(c, k, q, open0, ty, fs_h, col);
+                T fa = feq(rfs, ux[id], uy[id], uz[id], q);
+                T fb = feq(rfs, ux[id], uy[id], uz[id], qb);
+                T ffs = fa + fb - fpre[qb];
+                out = ffs;
+                if (c.fs_vof && c.thin_gap_bridge && sk >= 0 && sk < nz && src_gas) {
+                    int bir = si - cxq(q), bi = bir;
+                    int bj = yper ? pmod(sj - cyq(q), ny) : sj - cyq(q);
+                    int bk = sk - czq(q);
+                    bool bv = true;
+                    if (xper)
+                        bi = pmod(bir, nx);
+                    else if (bir < 0 || bir >= nx)
+                        bv = false;
+                    if (!yper && (bj < 0 || bj >= ny)) bv = false;
+                    if (bk < 0 || bk >= nz) bv = false;
+                    if (bv) {
+                        size_t bid = ((size_t)bk * ny + bj) * nx + bi;
+                        if ((flags[bid] & 1u) && type[bid] != LBM_FS_GAS) {
+                            T bw = fill[id];
+                            if (fill[bid] < bw) bw = fill[bid];
+                            bw *= c.bridge_strength;
+                            if (bw > T(0)) {
+                                if (bw > T(1)) bw = T(1);
+                                out = (T(1.0) - bw) * ffs + bw * Store<S, T>::load(fin, (size_t)q * n + bid, q, inv);
+                                n_bridge++;
+                            }
+                        }
+                    }
+                }
+                n_fs++;
+            }
+        } else if (!fluid) {
+            out = fpre[qb];
+            n_cut++;
+        } else if (sk >= 0 && sk < nz) {
+            T frac = g.lf[q];
+            out = frac * src_value<S, T>(c, fin, n, inflow, si, sj, sk, q) + (T(1.0) - frac) * fpre[qb];
+            if (frac < T(1.0)) n_cut++;
+        } else if (sk >= nz && c.top_boundary == LBM_TOP_FREE_SLIP) {
+            out = src_value<S, T>(c, fin, n, inflow, si, sj, nz - 1, mirzq(q));
+        } else {
+            out = fpre[q];
+        }
+        fo[q] = out;
+    }
+    // (A) x fringe (CODEX:3620-3644)
+    if (c.x_fringe && i < 5 && open0 && k > nz / 2) {
+        T sfr = T(1.0) - (T)i / T(4.0);
+        T kr = (T)k / (T)(nz - 1);
+        T strength = T(0.05) * (kr * kr - T(0.25));
+        T sigma = strength * smoothstep01(sfr);
+        T uxt = u_inlet[k] + T(0.0);
+#pragma unroll
+        for (int q = 0; q < 19; q++) fo[q] = (T(1.0) - sigma) * fo[q] + sigma * feq(c.rho0, uxt, T(0), T(0), q);
+    }
+    // (B) top drive (CODEX:3650-3662)
+    if (c.top_drive && k == nz - 1 && open0) {
+#pragma unroll
+        for (int q = 0; q < 19; q++) fo[q] = feq(c.rho0, c.u_top_lb, T(0), T(0), q);
+    }
+    // (C) moving-wall IBB (CODEX:3674-3712)
+    if (g.near) {
+#pragma unroll
+        for (int q = 1; q < 19; q++) fo[q] = ibb_link(c, s, g, i, j, q, fo[q], fpre[oppq(q)]);
+    }
+#pragma unroll
+    for (int q = 0; q < 19; q++) fout[(size_t)q * n + id] = fo[q];
+    if (count) {
+        if (n_cut) atomicAdd(&icnt[IC_BOUNDARY_CUT], n_cut);
+        if (n_fs) atomicAdd(&icnt[IC_FS_CUT], n_fs);
+        if (n_bridge) atomicAdd(&icnt[IC_FS_BRIDGE], n_bridge);
+    }
+}
+
+// number of cut links of the air stream (the boundary_cut_link_count of
+// CODEX:3429-3549) from geometry alone, for runs on the fused path
+template <class T>
+__global__ void k_cut_links(Consts<T> c, Slot<T> s, int* icnt) {
+    size_t id = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    size_t n = (size_t)c.nx * c.ny * c.nz;
+    int cnt = 0;
+    if (id < n) {
+        int i, j, k;
+        cell_ijk(id, c.nx, c.ny, i, j, k);
+        CellGeo<T> g;
+        cell_geometry(c, s, i, j, k, g);
+        for (int q = 1; q < 19; q++) {
+            int sir = i - cxq(q), sjr = j - cyq(q), sk = k - czq(q);
+            bool fluid = true;
+            if (c.x_boundary == LBM_XB_SOLID && (sir < 0 || sir >= c.nx)) fluid = false;
+            if (c.y_boundary == LBM_YB_SOLID && (sjr < 0 || sjr >= c.ny)) fluid = false;
+            if (!fluid)
+                cnt++;
+            else if (sk >= 0 && sk < c.nz && g.lf[q] < T(1.0))
+                cnt++;
+        }
+    }
+    if (cnt) atomicAdd(&icnt[IC_BOUNDARY_CUT], cnt);
+}
+
+// ===========================================================================
+// VOF pipeline (CODEX:1305-2374), deterministic gather formulation
+// ===========================================================================
+template <class T>
+struct VofArgs {
+    Consts<T> c;
+    const T* f;
+    T *rho, *ux, *uy, *uz;
+    T *fill, *mass, *mass2, *mdelta, *excess;
+    T *sa, *sb, *sc, *sd, *se, *smv;
+    int *type, *type_prev, *excess_dir, *ext_gas, *si;
+    const uint8_t* flags;
+    int* icnt;
+    double* dcnt;
+};
+
+__device__ __forceinline__ bool nbr(int nx, int ny, int nz, int xb, int yb, int i, int j, int k, int di, int dj, int dk, int& ni, int& nj,
+                                    int& nk) {
+    int a = i + di, b = j + dj, e = k + dk;
+    bool valid = true;
+    if (xb == LBM_XB_PERIODIC)
+        a = pmod(a, nx);
+    else if (a < 0 || a >= nx)
+        valid = false;
+    if (yb == LBM_YB_PERIODIC)
+        b = pmod(b, ny);
+    else if (b < 0 || b >= ny)
+        valid = false;
+    if (e < 0 || e >= nz) valid = false;
+    ni = a;
+    nj = b;
+    nk = e;
+    return valid;
+}
+#define NBR(di, dj, dk, ni, nj, nk) nbr(a.c.nx, a.c.ny, a.c.nz, a.c.x_boundary, a.c.y_boundary, i, j, k, di, dj, dk, ni, nj, nk)
+#define CID(kk, jj, ii) (((size_t)(kk) * a.c.ny + (jj)) * a.c.nx + (ii))
+#define VOF_PROLOGUE                                                      \
+    size_t id = (size_t)blockIdx.x * blockDim.x + threadIdx.x;            \
+    size_t n = (size_t)a.c.nx * a.c.ny * a.c.nz;                          \
+    if (id >= n) return;                                                  \
+    int i, j, k;                                                          \
+    cell_ijk(id, a.c.nx, a.c.ny, i, j, k);
+
+// compute_vof_mass_exchange (CODEX:1305-1350)
+template <class T>
+__global__ void k_vof_exchange(VofArgs<T> a) {
+    VOF_PROLOGUE
+    a.mdelta[id] = T(0);
+    a.excess[id] = T(0);
+    a.excess_dir[id] = 0;
+    if (!((a.flags[id] & 1u) && a.type[id] == LBM_FS_INTERFACE)) return;
+    T dm = T(0);
+#pragma unroll
+    for (int q = 1; q < 19; q++) {
+        int ni, nj, nk;
+        if (!NBR(cxq(q), cyq(q), czq(q), ni, nj, nk)) continue;
+        size_t nid = CID(nk, nj, ni);
+        if ((a.flags[nid] & 1u) && a.type[nid] != LBM_FS_GAS) {
+            T lw = T(1.0);
+            if (a.type[nid] == LBM_FS_INTERFACE) lw = T(0.5) * (a.fill[id] + a.fill[nid]);
+            if (lw > T(0)) dm += lw * (a.f[(size_t)oppq(q) * n + id] - a.f[(size_t)q * n + nid]);
+        }
+    }
+    a.mdelta[id] = dm;
+}
+
+// apply_vof_mass_update_and_preclassify (CODEX:1353-1414)
+template <class T>
+__global__ void k_vof_preclassify(VofArgs<T> a, T empty_thr) {
+    VOF_PROLOGUE
+    int old = a.type[id];
+    a.type_prev[id] = old;
+    a.excess[id] = T(0);
+    a.excess_dir[id] = 0;
+    if (!(a.flags[id] & 1u)) {
+        a.mass[id] = T(0);
+        a.fill[id] = T(0);
+        a.type[id] = LBM_FS_GAS;
+        return;
+    }
+    T rr = a.rho[id];
+    if (rr < T(1.0e-8)) rr = a.c.rho0;
+    T m = a.mass[id];
+    if (old == LBM_FS_INTERFACE)
+        m += a.mdelta[id];
+    else if (old == LBM_FS_LIQUID)
+        m = rr;
+    else
+        m = T(0);
+    T ex = T(0);
+    int ed = 0;
+    if (m < T(0)) {
+        ex = m;
+        ed = -1;
+        m = T(0);
+    } else if (m > rr) {
+        ex = m - rr;
+        ed = 1;
+        m = rr;
+    }
+    T fl = m / rr;
+    int ct = LBM_FS_INTERFACE;
+    if (fl > T(1.0e-4) && fl < empty_thr) {
+        ex = m;
+        ed = -1;
+        m = T(0);
+        fl = T(0);
+        ct = LBM_FS_GAS;
+    } else if (fl <= T(1.0e-4)) {
+        fl = T(0);
+        ct = LBM_FS_GAS;
+    } else if (fl >= T(1.0) - T(1.0e-4)) {
+        fl = T(1.0);
+        ct = LBM_FS_LIQUID;
+    }
+    a.mass[id] = m;
+    a.excess[id] = ex;
+    a.excess_dir[id] = ed;
+    if (fabs(ex) > T(1.0e-7)) atomicAdd(&a.icnt[IC_VOF_EXCESS], 1);
+    a.fill[id] = fl;
+    a.type[id] = ct;
+}
+
+template <class T>
+__device__ __forceinline__ T vof_room(const VofArgs<T>& a, size_t nid, T ex) {
+    if (ex > T(0)) {
+        T rr = a.rho[nid];
+        if (rr < T(1.0e-8)) rr = a.c.rho0;
+        return rr - a.mass[nid];
+    }
+    return a.mass[nid];
+}
+
+// redistribute_vof_mass_excess (CODEX:1417-1574) phase A: donor sums
+template <class T>
+__global__ void k_vof_redist_a(VofArgs<T> a) {
+    VOF_PROLOGUE
+    a.si[id] = 0;
+    T ex = a.excess[id];
+    if (!(fabs(ex) > T(1.0e-12))) return;
+    int ed = a.excess_dir[id];
+    if (ed == 0) ed = ex < T(0) ? -1 : 1;
+    int nx = a.c.nx, ny = a.c.ny, nz = a.c.nz;
+    int ip = i + 1, im = i - 1, jp = j + 1, jm = j - 1;
+    if (a.c.x_boundary == LBM_XB_PERIODIC) {
+        ip = (i + 1) % nx;
+        im = (i - 1 + nx) % nx;
+    } else {
+        if (ip >= nx) ip = nx - 1;
+        if (im < 0) im = 0;
+    }
+    if (a.c.y_boundary == LBM_YB_PERIODIC) {
+        jp = (j + 1) % ny;
+        jm = (j - 1 + ny) % ny;
+    } else {
+        if (jp >= ny) jp = ny - 1;
+        if (jm < 0) jm = 0;
+    }
+    int kp = k + 1 >= nz ? nz - 1 : k + 1, km = k - 1 < 0 ? 0 : k - 1;
+    T nxv = -(a.fill[CID(k, j, ip)] - a.fill[CID(k, j, im)]);
+    T nyv = -(a.fill[CID(k, jp, i)] - a.fill[CID(k, jm, i)]);
+    T nzv = -(a.fill[CID(kp, j, i)] - a.fill[CID(km, j, i)]);
+    T nm = sqrt(nxv * nxv + nyv * nyv + nzv * nzv);
+    if (nm > T(1.0e-8)) {
+        T invn = T(1.0) / nm;
+        nxv *= invn;
+        nyv *= invn;
+        nzv *= invn;
+    }
+    T ws = T(0), cs = T(0);
+#pragma unroll
+    for (int q = 1; q < 19; q++) {
+        int ni, nj, nk;
+        if (!NBR(cxq(q), cyq(q), czq(q), ni, nj, nk)) continue;
+        size_t nid = CID(nk, nj, ni);
+        if (!(a.flags[nid] & 1u)) continue;
+        T nd = (T)cxq(q) * nxv + (T)cyq(q) * nyv + (T)czq(q) * nzv;
+        bool ok = ed > 0 ? a.type[nid] != LBM_FS_LIQUID : a.type[nid] != LBM_FS_GAS;
+        if (!ok) continue;
+        T room = vof_room(a, nid, ex);
+        if (room > T(1.0e-12)) {
+            cs += room;
+            T wgt = ed > 0 ? nd : -nd;
+            if (wgt > T(0)) ws += wgt * room;
+        }
+    }
+    a.si[id] = ed;
+    a.sa[id] = nxv;
+    a.sb[id] = nyv;
+    a.sc[id] = nzv;
+    a.sd[id] = ws;
+    a.se[id] = cs;
+}
+
+// phase B: every receiver adds its shares (own retained excess, then q order)
+template <class T>
+__global__ void k_vof_redist_b(VofArgs<T> a) {
+    VOF_PROLOGUE
+    T acc = a.mass[id];
+    if (a.si[id] != 0 && !(a.se[id] > T(1.0e-12))) acc += a.excess[id];
+    if (a.flags[id] & 1u) {
+        int ty = a.type[id];
+#pragma unroll
+        for (int q = 1; q < 19; q++) {
+            int si_, sj, sk;
+            if (!NBR(-cxq(q), -cyq(q), -czq(q), si_, sj, sk)) continue;
+            size_t sid = CID(sk, sj, si_);
+            int ed = a.si[sid];
+            if (ed == 0 || !(a.se[sid] > T(1.0e-12))) continue;
+            T ex = a.excess[sid];
+            bool ok = ed > 0 ? ty != LBM_FS_LIQUID : ty != LBM_FS_GAS;
+            if (!ok) continue;
+            T room = vof_room(a, id, ex);
+            if (!(room > T(1.0e-12))) continue;
+            T nd = (T)cxq(q) * a.sa[sid] + (T)cyq(q) * a.sb[sid] + (T)czq(q) * a.sc[sid];
+            T wgt = ed > 0 ? nd : -nd;
+            T share = ex * room / a.se[sid];
+            if (a.sd[sid] > T(1.0e-12) && wgt > T(0))
+                share = ex * wgt * room / a.sd[sid];
+            else if (a.sd[sid] > T(1.0e-12))
+                share = T(0);
+            acc += share;
+        }
+    }
+    a.mass2[id] = acc;
+}
+
+// finalize_vof_classification_from_mass (CODEX:1577-1624)
+template <class T>
+__global__ void k_vof_finalize(VofArgs<T> a, T empty_thr) {
+    VOF_PROLOGUE
+    a.excess[id] = T(0);
+    a.excess_dir[id] = 0;
+    if (!(a.flags[id] & 1u)) {
+        a.mass[id] = T(0);
+        a.fill[id] = T(0);
+        a.type[id] = LBM_FS_GAS;
+        return;
+    }
+    T rr = a.rho[id];
+    if (rr < T(1.0e-8)) rr = a.c.rho0;
+    T m = a.mass[id];
+    T ex = T(0);
+    int ed = 0, cnt = 0;
+    if (m < T(0)) {
+        ex = m;
+        ed = -1;
+        m = T(0);
+        cnt++;
+    }
+    if (m > rr) {
+        ex = m - rr;
+        ed = 1;
+        m = rr;
+        cnt++;
+    }
+    T fl = m / rr;
+    int ct = LBM_FS_INTERFACE;
+    if (fl > T(1.0e-4) && fl < empty_thr) {
+        ex = m;
+        ed = -1;
+        m = T(0);
+        fl = T(0);
+        ct = LBM_FS_GAS;
+        cnt++;
+    } else if (fl <= T(1.0e-4)) {
+        fl = T(0);
+        ct = LBM_FS_GAS;
+    } else if (fl >= T(1.0) - T(1.0e-4)) {
+        fl = T(1.0);
+        ct = LBM_FS_LIQUID;
+    }
+    a.mass[id] = m;
+    a.excess[id] = ex;
+    a.excess_dir[id] = ed;
+    a.fill[id] = fl;
+    a.type[id] = ct;
+    if (cnt) atomicAdd(&a.icnt[IC_VOF_EXCESS], cnt);
+}
+
+// enforce_vof_interface_layer (CODEX:1627-1656): only LIQUID cells change
+// and only GAS neighbours are read, so the in-place update has no race
+template <class T>
+__global__ void k_vof_enforce(VofArgs<T> a) {
+    VOF_PROLOGUE
+    if (!((a.flags[id] & 1u) && a.type[id] == LBM_FS_LIQUID)) return;
+    bool touches = false;
+#pragma unroll
+    for (int q = 1; q < 19; q++) {
+        int ni, nj, nk;
+        if (!NBR(cxq(q), cyq(q), czq(q), ni, nj, nk)) continue;
+        size_t nid = CID(nk, nj, ni);
+        if ((a.flags[nid] & 1u) && a.type[nid] == LBM_FS_GAS) touches = true;
+    }
+    if (touches) {
+        a.type[id] = LBM_FS_INTERFACE;
+        a.fill[id] = T(1.0);
+        a.mass[id] = a.rho[id];
+    }
+}
+
+// remove_orphan_vof_interface_cells (CODEX:1659-1776) phase A: decide
+template <class T>
+__global__ void k_vof_orphan_a(VofArgs<T> a, T empty_thr, T thr, int rmax, int max_weak) {
+    VOF_PROLOGUE
+    a.si[id] = 0;
+    if (!((a.flags[id] & 1u) && a.type[id] != LBM_FS_GAS && a.fill[id] > empty_thr)) return;
+    int wet = 0, res = 0;
+#pragma unroll
+    for (int q = 1; q < 19; q++) {
+        int ni, nj, nk;
+        if (!NBR(cxq(q), cyq(q), czq(q), ni, nj, nk)) continue;
+        size_t nid = CID(nk, nj, ni);
+        if ((a.flags[nid] & 1u) && a.type[nid] != LBM_FS_GAS) {
+            wet++;
+            if (a.fill[nid] >= thr) res++;
+        }
+    }
+    if (!(res == 0 && (a.fill[id] < thr || wet <= max_weak))) return;
+    T m = a.mass[id];
+    T cw = T(0), tw = T(0);
+    for (int q = 1; q < 19; q++)
+        for (int d = 2; d <= rmax; d++) {
+            int ni, nj, nk;
+            int di = d * cxq(q), dj = d * cyq(q), dk = d * czq(q);
+            if (!NBR(di, dj, dk, ni, nj, nk)) continue;
+            size_t nid = CID(nk, nj, ni);
+            if ((a.flags[nid] & 1u) && a.type[nid] != LBM_FS_GAS && a.fill[nid] >= thr) {
+                T rr = a.rho[nid];
+                if (rr < T(1.0e-8)) rr = a.c.rho0;
+                T room = rr - a.mass[nid];
+                T d2 = (T)(di * di + dj * dj + dk * dk);
+                tw += a.fill[nid] / d2;
+                if (room > T(1.0e-12)) cw += room / d2;
+            }
+        }
+    if (m > T(1.0e-12) && (cw > T(1.0e-12) || tw > T(1.0e-12))) {
+        a.si[id] = 1;
+        a.sb[id] = cw;
+        a.sc[id] = tw;
+        atomicAdd(&a.icnt[IC_VOF_ORPHAN], 1);
+    }
+}
+
+// phase B: receivers gather the scattered orphan mass into mass2
+template <class T>
+__global__ void k_vof_orphan_b(VofArgs<T> a, T thr, int rmax) {
+    VOF_PROLOGUE
+    bool orphan = a.si[id] != 0;
+    T acc = orphan ? T(0) : a.mass[id];
+    if ((a.flags[id] & 1u) && a.type[id] != LBM_FS_GAS && a.fill[id] >= thr) {
+        T rr = a.rho[id];
+        if (rr < T(1.0e-8)) rr = a.c.rho0;
+        T room = rr - a.mass[id];
+        for (int q = 1; q < 19; q++)
+            for (int d = 2; d <= rmax; d++) {
+                int si_, sj, sk;
+                int di = d * cxq(q), dj = d * cyq(q), dk = d * czq(q);
+                if (!NBR(-di, -dj, -dk, si_, sj, sk)) continue;
+                size_t sid = CID(sk, sj, si_);
+                if (!a.si[sid]) continue;
+                T d2 = (T)(di * di + dj * dj + dk * dk);
+                T wgt = T(0), den = a.sb[sid];
+                if (a.sb[sid] > T(1.0e-12)) {
+                    if (room > T(1.0e-12)) wgt = room / d2;
+                } else {
+                    wgt = a.fill[id] / d2;
+                    den = a.sc[sid];
+                }
+                if (den > T(1.0e-12) && wgt > T(0)) acc += a.mass[sid] * wgt / den;
+            }
+    }
+    a.mass2[id] = acc;
+}
+
+// phase C: the orphans become gas
+template <class T>
+__global__ void k_vof_orphan_c(VofArgs<T> a) {
+    VOF_PROLOGUE
+    if (a.si[id]) {
+        a.fill[id] = T(0);
+        a.type[id] = LBM_FS_GAS;
+    }
+}
+
+// vof_unresolved_detached_cell (CODEX:1779-1810)
+template <class T>
+__device__ bool vof_detached(const VofArgs<T>& a, int i, int j, int k, T orphan_thr, int max_wet, int max_res) {
+    size_t id = CID(k, j, i);
+    if (!((a.flags[id] & 1u) && a.type[id] != LBM_FS_GAS && a.fill[id] > T(1.0e-4))) return false;
+    int wet = 0, res = 0;
+#pragma unroll
+    for (int q = 1; q < 19; q++) {
+        int ni, nj, nk;
+        if (!NBR(cxq(q), cyq(q), czq(q), ni, nj, nk)) continue;
+        size_t nid = CID(nk, nj, ni);
+        if ((a.flags[nid] & 1u) && a.type[nid] != LBM_FS_GAS) {
+            wet++;
+            if (a.fill[nid] >= orphan_thr) res++;
+        }
+    }
+    return wet <= max_wet && res <= max_res;
+}
+
+__device__ __constant__ int FD6[6][3] = {{1, 0, 0}, {-1, 0, 0}, {0, 1, 0}, {0, -1, 0}, {0, 0, 1}, {0, 0, -1}};
+
+// advect_detached_vof_cells / compress_detached_vof_residuals phase A
+// (CODEX:1813-1993): mass each detached cell moves through each face
+template <class T>
+__global__ void k_vof_detached_a(VofArgs<T> a, int mode, T orphan_thr, int max_wet, int max_res, T resid_thr) {
+    VOF_PROLOGUE
+    T mv[6] = {T(0), T(0), T(0), T(0), T(0), T(0)};
+    if (vof_detached(a, i, j, k, orphan_thr, max_wet, max_res)) {
+        T m = a.mass[id];
+        T u = a.ux[id], v = a.uy[id], w = a.uz[id];
+        T sg[6] = {u, -u, v, -v, w, -w};
+        T r[6];
+        for (int d = 0; d < 6; d++) {
+            r[d] = T(0);
+            if (sg[d] > T(0)) {
+                int ni, nj, nk;
+                if (NBR(FD6[d][0], FD6[d][1], FD6[d][2], ni, nj, nk)) {
+                    size_t nid = CID(nk, nj, ni);
+                    if ((a.flags[nid] & 1u) && a.type[nid] == LBM_FS_GAS) r[d] = sg[d];
+                }
+            }
+        }
+        if (mode == 0) {
+            if (m > T(1.0e-12)) {
+                T rs = r[0] + r[1] + r[2] + r[3] + r[4] + r[5];
+                if (rs > T(1.0e-12)) {
+                    T scale = T(1.0);
+                    if (rs > T(0.95)) scale = T(0.95) / rs;
+                    for (int d = 0; d < 6; d++)
+                        if (r[d] > T(0)) {
+                            T x = m * r[d] * scale;
+                            mv[d] = x > T(1.0e-12) ? x : T(0);
+                        }
+                }
+            }
+        } else {
+            T rr = a.rho[id];
+            if (rr < T(1.0e-8)) rr = a.c.rho0;
+            T fl = m / rr;
+            if (fl > T(1.0e-4) && fl < resid_thr) {
+                int best = -1;
+                T br = T(0);
+                for (int d = 0; d < 6; d++)
+                    if (r[d] > br) {
+                        br = r[d];
+                        best = d;
+                    }
+                if (br > T(1.0e-12)) {
+                    T x = m * T(1.0) * T(1.0);
+                    mv[best] = x > T(1.0e-12) ? x : T(0);
+                }
+            }
+        }
+    }
+    T moved = T(0);
+    for (int d = 0; d < 6; d++) {
+        a.smv[6 * id + d] = mv[d];
+        moved += mv[d];
+    }
+    if (moved > T(1.0e-12)) {
+        atomicAdd(&a.icnt[mode == 0 ? IC_VOF_ADVECT : IC_VOF_COMPRESS], 1);
+        atomicAdd(&a.dcnt[mode == 0 ? DC_ADVECT_MASS : DC_COMPRESS_MASS], (double)moved);
+    }
+}
+
+// phase B: apply outgoing then incoming moves (fixed direction order)
+template <class T>
+__global__ void k_vof_detached_b(VofArgs<T> a) {
+    VOF_PROLOGUE
+    T out = T(0);
+    for (int d = 0; d < 6; d++) out -= a.smv[6 * id + d];
+    T in = T(0);
+    for (int d = 0; d < 6; d++) {
+        int si_, sj, sk;
+        if (!NBR(-FD6[d][0], -FD6[d][1], -FD6[d][2], si_, sj, sk)) continue;
+        in += a.smv[6 * CID(sk, sj, si_) + d];
+    }
+    T dm = out + in;
+    if (fabs(dm) > T(1.0e-12)) a.mass[id] = a.mass[id] + dm;
+}
+
+// vof_gas_fraction_at (CODEX:1996-2009)
+template <class T>
+__device__ __forceinline__ T gas_fraction(const VofArgs<T>& a, size_t id) {
+    T g = T(0);
+    if (a.flags[id] & 1u) {
+        if (a.type[id] == LBM_FS_GAS)
+            g = T(1.0);
+        else if (a.fill[id] < T(1.0)) {
+            g = T(1.0) - a.fill[id];
+            if (g < T(0)) g = T(0);
+            if (g > T(1)) g = T(1);
+        }
+    }
+    return g;
+}
+
+// initialize_vof_exterior_gas_marks (CODEX:2021-2030)
+template <class T>
+__global__ void k_flood_seed(VofArgs<T> a) {
+    VOF_PROLOGUE
+    a.ext_gas[id] = (gas_fraction(a, id) > T(1.0e-6) && k == a.c.nz - 1) ? 1 : 0;
+}
+
+// propagate_vof_exterior_gas_marks (CODEX:2033-2055), run to the fixed
+// point: every sweep after a sweep that marked nothing returns at once
+template <class T>
+__global__ void k_flood_sweep(VofArgs<T> a, int* changed_prev, int* changed_this) {
+    if (*changed_prev == 0) return;
+    VOF_PROLOGUE
+    if (a.ext_gas[id] != 0 || !(gas_fraction(a, id) > T(1.0e-6))) return;
+    bool conn = false;
+#pragma unroll
+    for (int q = 1; q < 7; q++) {
+        int ni, nj, nk;
+        if (!NBR(cxq(q), cyq(q), czq(q), ni, nj, nk)) continue;
+        if (a.ext_gas[CID(nk, nj, ni)] == 1) conn = true;
+    }
+    if (conn) {
+        a.ext_gas[id] = 1;
+        *changed_this = 1;
+    }
+}
+
+// summarize_vof_trapped_gas_collapse sums (CODEX:2058-2075) in FP64
+template <class T>
+__global__ void k_collapse_sum(VofArgs<T> a) {
+    size_t id = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    size_t n = (size_t)a.c.nx * a.c.ny * a.c.nz;
+    double cm = 0.0, cv = 0.0, cap = 0.0;
+    if (id < n) {
+        T g = gas_fraction(a, id);
+        if (g > T(1.0e-6)) {
+            if (a.ext_gas[id] == 0) {
+                T rr = a.rho[id];
+                if (rr < T(1.0e-8)) rr = a.c.rho0;
+                cm = (double)(g * rr);
+                cv = (double)(g * a.c.dx * a.c.dx * a.c.dx);
+            } else if (a.type[id] != LBM_FS_GAS) {
+                T m = a.mass[id];
+                if (m > T(0)) cap = (double)m;
+            }
+        }
+    }
+    // wave reduction, one atomic per wave
+    for (int off = 32; off > 0; off >>= 1) {
+        cm += __shfl_down(cm, off);
+        cv += __shfl_down(cv, off);
+        cap += __shfl_down(cap, off);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        if (cm != 0.0) atomicAdd(&a.dcnt[DC_CAND_MASS], cm);
+        if (cv != 0.0) atomicAdd(&a.dcnt[DC_CAND_VOLUME], cv);
+        if (cap != 0.0) atomicAdd(&a.dcnt[DC_EXT_CAPACITY], cap);
+    }
+}
+
+// accept test of summarize_vof_trapped_gas_collapse (CODEX:2077-2091)
+__global__ void k_collapse_decide(double* dcnt, double max_volume) {
+    double cm = dcnt[DC_CAND_MASS], cv = dcnt[DC_CAND_VOLUME], cap = dcnt[DC_EXT_CAPACITY];
+    dcnt[DC_APPLIED_VOLUME] = 0.0;
+    dcnt[DC_REMOVAL_SCALE] = 0.0;
+    if (cv > 0.0 && cv <= max_volume && cap >= cm && cap > 1.0e-12) {
+        dcnt[DC_APPLIED_VOLUME] = cv;
+        dcnt[DC_REMOVAL_SCALE] = cm / cap;
+    }
+}
+
+// apply_vof_trapped_gas_collapse (CODEX:2094-2126)
+template <class T>
+__global__ void k_collapse_apply(VofArgs<T> a) {
+    VOF_PROLOGUE
+    T scale = (T)a.dcnt[DC_REMOVAL_SCALE];
+    if (!(scale > T(0))) return;
+    T g = gas_fraction(a, id);
+    if (!(g > T(1.0e-6))) return;
+    T rr = a.rho[id];
+    if (rr < T(1.0e-8)) rr = a.c.rho0;
+    if (a.ext_gas[id] == 0) {
+        a.mass[id] = rr;
+        a.fill[id] = T(1.0);
+        a.type[id] = LBM_FS_LIQUID;
+        atomicAdd(&a.icnt[IC_VOF_COLLAPSE_CELLS], 1);
+    } else if (a.type[id] != LBM_FS_GAS) {
+        T m = a.mass[id] * (T(1.0) - scale);
+        if (m < T(0)) m = T(0);
+        T fl = m / rr;
+        int ct = LBM_FS_INTERFACE;
+        if (fl <= T(1.0e-4)) {
+            fl = T(0);
+            m = T(0);
+            ct = LBM_FS_GAS;
+        } else if (fl >= T(1.0) - T(1.0e-4)) {
+            fl = T(1.0);
+            m = rr;
+            ct = LBM_FS_LIQUID;
+        }
+        a.mass[id] = m;
+        a.fill[id] = fl;
+        a.type[id] = ct;
+    }
+}
+
+// update_free_surface_height_from_vof (CODEX:2146-2161), one thread per column
+template <class T>
+__global__ void k_vof_height(Consts<T> c, Slot<T> s, const T* fill, const uint8_t* flags, T* fs_h) {
+    size_t col = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    size_t plane = (size_t)c.nx * c.ny;
+    if (col >= plane) return;
+    int i = (int)(col % (size_t)c.nx), j = (int)(col / (size_t)c.nx);
+    T depth = T(0);
+    for (int k = 0; k < c.nz; k++) {
+        size_t id = (size_t)k * plane + col;
+        if (flags[id] & 1u) {
+            CellGeo<T> g;
+            cell_geometry(c, s, i, j, k, g);
+            depth += fill[id] * g.lf[0] * c.dx;
+        }
+    }
+    fs_h[col] = s.h[col] + depth;
+}
+
+// refill_new_free_surface_cells (CODEX:2163-2316), VOF tracking
+template <class T>
+__global__ void k_vof_refill(VofArgs<T> a, T* f) {
+    VOF_PROLOGUE
+    if (!(a.type_prev[id] == LBM_FS_GAS && a.type[id] != LBM_FS_GAS && (a.flags[id] & 1u))) return;
+    const Consts<T>& c = a.c;
+    T ws = 0, fws = 0, ra = 0, xa = 0, ya = 0, za = 0, rf = 0, xf = 0, yf = 0, zf = 0;
+    T fna[19], fnf[19];
+#pragma unroll
+    for (int q = 0; q < 19; q++) fna[q] = fnf[q] = T(0);
+    for (int q = 1; q < 19; q++) {
+        int ni, nj, nk;
+        if (!NBR(cxq(q), cyq(q), czq(q), ni, nj, nk)) continue;
+        size_t nid = CID(nk, nj, ni);
+        if (!((a.flags[nid] & 1u) && a.type_prev[nid] != LBM_FS_GAS && a.type[nid] != LBM_FS_GAS)) continue;
+        T wn = wq<T>(q), fl = a.fill[nid];
+        T rn = a.rho[nid], uxn = a.ux[nid], uyn = a.uy[nid], uzn = a.uz[nid];
+        fws += wn;
+        rf += wn * rn;
+        xf += wn * uxn;
+        yf += wn * uyn;
+        zf += wn * uzn;
+        T lw = wn * fl;
+        ws += lw;
+        ra += lw * rn;
+        xa += lw * uxn;
+        ya += lw * uyn;
+        za += lw * uzn;
+        if (c.refill_mode == LBM_REFILL_NONEQ)
+            for (int qq = 0; qq < 19; qq++) {
+                T fneq = f[(size_t)qq * n + nid] - feq(rn, uxn, uyn, uzn, qq);
+                fna[qq] += lw * fneq;
+                fnf[qq] += wn * fneq;
+            }
+    }
+    T rr = c.rho0, urx = c.u_current_lb, ury = T(0), urz = T(0);
+    if (ws > T(0)) {
+        T inv = T(1.0) / ws;
+        rr = ra * inv;
+        urx = xa * inv;
+        ury = ya * inv;
+        urz = za * inv;
+    } else if (fws > T(0)) {
+        T inv = T(1.0) / fws;
+        rr = rf * inv;
+        urx = xf * inv;
+        ury = yf * inv;
+        urz = zf * inv;
+    } else {
+        rr = c.rho0;
+    }
+    T fv[19];
+    T rp = T(0);
+    for (int q = 0; q < 19; q++) {
+        T fr = feq(rr, urx, ury, urz, q);
+        if (c.refill_mode == LBM_REFILL_NONEQ) {
+            if (ws > T(0))
+                fr += fna[q] / ws;
+            else if (fws > T(0))
+                fr += fnf[q] / fws;
+        } else if (c.refill_mode == LBM_REFILL_DIRECTIONAL) {
+            int si_, sj, sk;
+            if (NBR(-cxq(q), -cyq(q), -czq(q), si_, sj, sk)) {
+                size_t sid = CID(sk, sj, si_);
+                if ((a.flags[sid] & 1u) && a.type_prev[sid] != LBM_FS_GAS && a.type[sid] != LBM_FS_GAS) fr = f[(size_t)q * n + sid];
+            }
+        }
+        fv[q] = fr;
+        rp += fr;
+    }
+    if (c.refill_mode == LBM_REFILL_DIRECTIONAL) {
+        T scale = T(1.0);
+        if (rp > T(1.0e-8)) scale = rr / rp;
+        T r2 = 0, mx = 0, my = 0, mz = 0;
+        for (int q = 0; q < 19; q++) {
+            T fvv = fv[q] * scale;
+            f[(size_t)q * n + id] = fvv;
+            r2 += fvv;
+            mx += fvv * (T)cxq(q);
+            my += fvv * (T)cyq(q);
+            mz += fvv * (T)czq(q);
+        }
+        a.rho[id] = r2;
+        if (r2 > T(1.0e-8)) {
+            a.ux[id] = mx / r2;
+            a.uy[id] = my / r2;
+            a.uz[id] = mz / r2;
+        } else {
+            a.ux[id] = urx;
+            a.uy[id] = ury;
+            a.uz[id] = urz;
+        }
+    } else {
+        a.rho[id] = rr;
+        a.ux[id] = urx;
+        a.uy[id] = ury;
+        a.uz[id] = urz;
+        for (int q = 0; q < 19; q++) f[(size_t)q * n + id] = fv[q];
+    }
+    atomicAdd(&a.icnt[IC_FS_REFILL], 1);
+}
+
+// repair_vof_coalescence_links (CODEX:2319-2373): donors (refilled cells,
+// type_prev gas) and repaired cells (type_prev wet) are disjoint
+template <class T>
+__global__ void k_vof_repair(VofArgs<T> a, T* f) {
+    VOF_PROLOGUE
+    if (!((a.flags[id] & 1u) && a.type[id] != LBM_FS_GAS && a.type_prev[id] != LBM_FS_GAS)) return;
+    bool rep = false;
+    int cnt = 0;
+    for (int q = 1; q < 19; q++) {
+        int si_, sj, sk;
+        if (!NBR(-cxq(q), -cyq(q), -czq(q), si_, sj, sk)) continue;
+        size_t sid = CID(sk, sj, si_);
+        if ((a.flags[sid] & 1u) && a.type_prev[sid] == LBM_FS_GAS && a.type[sid] != LBM_FS_GAS) {
+            f[(size_t)q * n + id] = f[(size_t)q * n + sid];
+            rep = true;
+            cnt++;
+        }
+    }
+    if (rep) {
+        T r = 0, mx = 0, my = 0, mz = 0;
+        for (int q = 0; q < 19; q++) {
+            T fq = f[(size_t)q * n + id];
+            r += fq;
+            mx += fq * (T)cxq(q);
+            my += fq * (T)cyq(q);
+            mz += fq * (T)czq(q);
+        }
+        a.rho[id] = r;
+        if (r > T(1.0e-8)) {
+            a.ux[id] = mx / r;
+            a.uy[id] = my / r;
+            a.uz[id] = mz / r;
+        }
+        atomicAdd(&a.icnt[IC_FS_REPAIR], cnt);
+    }
+}
+
+// initialize_vof_mass_from_fill (CODEX:1289-1302)
+template <class T>
+__global__ void k_vof_init_mass(VofArgs<T> a) {
+    VOF_PROLOGUE
+    if ((a.flags[id] & 1u) && a.type[id] != LBM_FS_GAS) {
+        a.mass[id] = a.fill[id] * a.rho[id];
+    } else {
+        a.mass[id] = T(0);
+        a.fill[id] = T(0);
+        a.type[id] = LBM_FS_GAS;
+    }
+    a.type_prev[id] = a.type[id];
+    a.mdelta[id] = T(0);
+    a.excess[id] = T(0);
+    a.excess_dir[id] = 0;
+}
+
+// ===========================================================================
+// fused z-marching step (air mode)
+// ===========================================================================
+//
+// Block = 32 x 16 columns (8 waves of interior cells) + 2 waves for the
+// 96-cell halo ring whose velocities the LES central differences need.  The
+// block marches up a chunk of kc planes; at every plane it
+//   1. streams + applies the boundary rules of the previous step to plane
+//      k+1 (interior and halo), conditions its closed links for the new
+//      step and forms rho and u; u goes to a 3-plane LDS ring, the interior
+//      keeps its populations in registers;
+//   2. evaluates the LES viscosity of plane k from the ring, collides and
+//      writes f* of plane k.
+// The halo ring and the planes k0-1 / k0+kc are recomputed by the
+// neighbouring blocks: extra arithmetic, no extra HBM traffic.
+constexpr int FT_X = 32, FT_Y = 16, FT_THREADS = 640;
+constexpr int FL_X = FT_X + 2, FL_Y = FT_Y + 2;
+
+template <class S, class T>
+struct FusedArgs {
+    Consts<T> c;
+    Slot<T> s0;  // time level of the pending stream (step n)
+    Slot<T> s1;  // time level of the new step (n+1)
+    const S* fin;
+    S* fout;
+    T *rho, *ux, *uy, *uz, *omega;
+    const T *u_inlet, *inflow;
+    int mode;  // 0: fin holds post-stream populations (no stream)
+    int write_macro;
+    int kc;
+    uint32_t step;
+};
+
+// stream (+ boundary rules) of step n and conditioning of step n+1 for one
+// cell; returns the populations entering the moments of step n+1
+template <class S, class T>
+__device__ __forceinline__ void fused_gather(const FusedArgs<S, T>& a, int i, int j, int k, T f[19], bool& open0_out) {
+    const Consts<T>& c = a.c;
+    const int nx = c.nx, ny = c.ny, nz = c.nz;
+    const size_t n = (size_t)nx * ny * nz;
+    const size_t id = ((size_t)k * ny + j) * nx + i;
+    const T inv = (T)c.store_inv_scale;
+    const size_t col = (size_t)j * nx + i;
+    if (a.mode == 0) {
+#pragma unroll
+        for (int q = 0; q < 19; q++) f[q] = Store<S, T>::load(a.fin, (size_t)q * n + id, q, inv);
+    } else {
+        const bool xper = c.x_boundary == LBM_XB_PERIODIC, xio = c.x_boundary == LBM_XB_INFLOW_OUTFLOW;
+        const bool yper = c.y_boundary == LBM_YB_PERIODIC;
+        const bool fast0 = k > a.s0.kband[col];
+        f[0] = Store<S, T>::load(a.fin, id, 0, inv);
+        if (fast0) {
+            // no cut or closed link at this cell: plain pull + the x/y/top rules
+#pragma unroll
+            for (int q = 1; q < 19; q++) {
+                int sir = i - cxq(q), sjr = j - cyq(q), sk = k - czq(q);
+                int si = xper ? pmod(sir, nx) : sir;
+                int sj = yper ? pmod(sjr, ny) : sjr;
+                bool fluid = (xper || xio || (sir >= 0 && sir < nx)) && (yper || (sjr >= 0 && sjr < ny));
+                T out;
+                if (!fluid)
+                    out = Store<S, T>::load(a.fin, (size_t)oppq(q) * n + id, oppq(q), inv);
+                else if (sk >= 0 && sk < nz)
+                    out = src_value<S, T>(c, a.fin, n, a.inflow, si, sj, sk, q);
+                else if (sk >= nz && c.top_boundary == LBM_TOP_FREE_SLIP)
+                    out = src_value<S, T>(c, a.fin, n, a.inflow, si, sj, nz - 1, mirzq(q));
+                else
+                    out = Store<S, T>::load(a.fin, (size_t)q * n + id, q, inv);
+                f[q] = out;
+            }
+        } else {
+            CellGeo<T> g;
+            cell_geometry(c, a.s0, i, j, k, g);
+            T fpre[19];
+#pragma unroll
+            for (int q = 0; q < 19; q++) fpre[q] = Store<S, T>::load(a.fin, (size_t)q * n + id, q, inv);
+#pragma unroll
+            for (int q = 1; q < 19; q++) {
+                const int qb = oppq(q);
+                int sir = i - cxq(q), sjr = j - cyq(q), sk = k - czq(q);
+                int si = xper ? pmod(sir, nx) : sir;
+                int sj = yper ? pmod(sjr, ny) : sjr;
+                bool fluid = (xper || xio || (sir >= 0 && sir < nx)) && (yper || (sjr >= 0 && sjr < ny));
+                T out;
+                if (!fluid) {
+                    out = fpre[qb];
+                } else if (sk >= 0 && sk < nz) {
+                    T frac = g.lf[q];
+                    out = frac * src_value<S, T>(c, a.fin, n, a.inflow, si, sj, sk, q) + (T(1.0) - frac) * fpre[qb];
+                } else if (sk >= nz && c.top_boundary == LBM_TOP_FREE_SLIP) {
+                    out = src_value<S, T>(c, a.fin, n, a.inflow, si, sj, nz - 1, mirzq(q));
+                } else {
+                    out = fpre[q];
+                }
+                f[q] = out;
+            }
+            bool open0 = g.open_mask & 1u;
+            if (c.x_fringe && i < 5 && open0 && k > nz / 2) {
+                T sfr = T(1.0) - (T)i / T(4.0);
+                T kr = (T)k / (T)(nz - 1);
+                T strength = T(0.05) * (kr * kr - T(0.25));
+                T sigma = strength * smoothstep01(sfr);
+                T uxt = a.u_inlet[k] + T(0.0);
+#pragma unroll
+                for (int q = 0; q < 19; q++) f[q] = (T(1.0) - sigma) * f[q] + sigma * feq(c.rho0, uxt, T(0), T(0), q);
+            }
+            if (c.top_drive && k == nz - 1 && open0) {
+#pragma unroll
+                for (int q = 0; q < 19; q++) f[q] = feq(c.rho0, c.u_top_lb, T(0), T(0), q);
+            }
+            if (g.near) {
+#pragma unroll
+                for (int q = 1; q < 19; q++) f[q] = ibb_link(c, a.s0, g, i, j, q, f[q], fpre[oppq(q)]);
+            }
+        }
+        if (fast0) {
+            // fringe and top drive act on fully open cells as well
+            if (c.x_fringe && i < 5 && k > nz / 2) {
+                T sfr = T(1.0) - (T)i / T(4.0);
+                T kr = (T)k / (T)(nz - 1);
+                T strength = T(0.05) * (kr * kr - T(0.25));
+                T sigma = strength * smoothstep01(sfr);
+                T uxt = a.u_inlet[k] + T(0.0);
+#pragma unroll
+                for (int q = 0; q < 19; q++) f[q] = (T(1.0) - sigma) * f[q] + sigma * feq(c.rho0, uxt, T(0), T(0), q);
+            }
+            if (c.top_drive && k == nz - 1) {
+#pragma unroll
+                for (int q = 0; q < 19; q++) f[q] = feq(c.rho0, c.u_top_lb, T(0), T(0), q);
+            }
+        }
+    }
+    // update_bed_populations_and_reinit of step n+1
+    bool open0 = true;
+    if (k <= a.s1.kband[col]) {
+        CellGeo<T> g;
+        cell_geometry(c, a.s1, i, j, k, g);
+        open0 = g.open_mask & 1u;
+        if ((g.open_mask & 0x7ffffu) != 0x7ffffu) condition_cell(c, a.s1, g, i, j, f);
+    }
+    open0_out = open0;
+}
+
+template <class S, class T>
+__global__ __launch_bounds__(FT_THREADS) void k_fused(FusedArgs<S, T> a) {
+    __shared__ T s_u[3][3][FL_Y * FL_X];  // [plane ring][component][cell]
+    const Consts<T>& c = a.c;
+    const int nx = c.nx, ny = c.ny, nz = c.nz;
+    const size_t n = (size_t)nx * ny * nz;
+    const int tid = threadIdx.x;
+    const int i0 = blockIdx.x * FT_X, j0 = blockIdx.y * FT_Y;
+    const int k0 = blockIdx.z * a.kc;
+    const int k1 = min(k0 + a.kc, nz);
+    int lx, ly;
+    bool interior = tid < FT_X * FT_Y, alive = true;
+    if (interior) {
+        lx = tid % FT_X;
+        ly = tid / FT_X;
+    } else {
+        int h = tid - FT_X * FT_Y;
+        if (h < FT_X) {
+            lx = h;
+            ly = -1;
+        } else if (h < 2 * FT_X) {
+            lx = h - FT_X;
+            ly = FT_Y;
+        } else if (h < 2 * FT_X + FT_Y) {
+            lx = -1;
+            ly = h - 2 * FT_X;
+        } else if (h < 2 * FT_X + 2 * FT_Y) {
+            lx = FT_X;
+            ly = h - 2 * FT_X - FT_Y;
+        } else {
+            lx = 0;
+            ly = 0;
+            alive = false;
+        }
+    }
+    // LES neighbours wrap periodically in x and y (CODEX:3047-3053)
+    const int gi = pmod(i0 + lx, nx), gj = pmod(j0 + ly, ny);
+    const int lcell = (ly + 1) * FL_X + (lx + 1);
+    const size_t col = (size_t)gj * nx + gi;
+
+    T fcur[19], fnext[19];
+    T rcur = T(0), ucur = T(0), vcur = T(0), wcur = T(0);
+    bool open_cur = true, open_below = true;
+
+    auto compute_plane = [&](int kk, T* fdst, T& r, T& u, T& v, T& w, bool& open0) {
+        fused_gather<S, T>(a, gi, gj, kk, fdst, open0);
+        T rr = T(0);
+#pragma unroll
+        for (int q = 0; q < 19; q++) rr += fdst[q];
+        T jx = T(0), jy = T(0), jz = T(0);
+#pragma unroll
+        for (int q = 0; q < 19; q++) {
+            if (cxq(q)) jx += fdst[q] * (T)cxq(q);
+            if (cyq(q)) jy += fdst[q] * (T)cyq(q);
+            if (czq(q)) jz += fdst[q] * (T)czq(q);
+        }
+        T inv = T(1.0) / rr;
+        u = jx * inv;
+        v = jy * inv;
+        w = jz * inv;
+        if (c.top_drive && kk == nz - 1) {
+            u = c.u_top_lb;
+            v = T(0);
+            w = T(0);
+        }
+        r = rr;
+        int slot = kk % 3;
+        s_u[slot][0][lcell] = u;
+        s_u[slot][1][lcell] = v;
+        s_u[slot][2][lcell] = w;
+    };
+
+    // prologue: plane below the chunk (for the LES z difference) and plane k0
+    if (alive && k0 > 0) {
+        T r, u, v, w;
+        bool o0;
+        compute_plane(k0 - 1, fnext, r, u, v, w, o0);
+        open_below = o0;
+    }
+    if (alive) compute_plane(k0, fcur, rcur, ucur, vcur, wcur, open_cur);
+
+    for (int k = k0; k < k1; k++) {
+        T rn = T(0), un = T(0), vn = T(0), wn = T(0);
+        bool open_next = true;
+        // plane k+1 (for the top plane of the chunk only its velocity is used)
+        if (alive && k + 1 < nz) compute_plane(k + 1, fnext, rn, un, vn, wn, open_next);
+        __syncthreads();
+        if (interior) {
+            const int i = i0 + lx, j = j0 + ly;
+            const size_t id = ((size_t)k * ny + j) * nx + i;
+            // LES (CODEX:3011-3141)
+            T om = T(1.0) / c.tau0;
+            if (c.use_les) {
+                if (open_cur) {
+                    const int sk = k % 3;
+                    const int sp = (k + 1 < nz ? k + 1 : k) % 3;
+                    const int sm = (k > 0 ? k - 1 : k) % 3;
+                    const int cx_p = lcell + 1, cx_m = lcell - 1, cy_p = lcell + FL_X, cy_m = lcell - FL_X;
+                    bool near_wall = k > 0 && !open_below;
+                    T phi = T(0), uwx = T(0), uwy = T(0);
+                    if (near_wall) {
+                        phi = cell_phi(c, a.s1, i, j, k);
+                        if (c.include_bed) {
+                            uwx = a.s1.uw[col];
+                            uwy = a.s1.vw[col];
+                        }
+                    }
+                    om = les_omega(c, s_u[sk][0][cx_p], s_u[sk][0][cx_m], s_u[sk][0][cy_p], s_u[sk][0][cy_m], s_u[sp][0][lcell],
+                                   s_u[sm][0][lcell], s_u[sk][1][cx_p], s_u[sk][1][cx_m], s_u[sk][1][cy_p], s_u[sk][1][cy_m],
+                                   s_u[sp][1][lcell], s_u[sm][1][lcell], s_u[sk][2][cx_p], s_u[sk][2][cx_m], s_u[sk][2][cy_p],
+                                   s_u[sk][2][cy_m], s_u[sp][2][lcell], s_u[sm][2][lcell], near_wall, phi, ucur, vcur, uwx, uwy,
+                                   (T*)nullptr, (T*)nullptr);
+                    if (k <= c.omega_band_top) a.omega[id] = om;
+                } else if (k <= c.omega_band_top) {
+                    om = a.omega[id];  // inactive cells keep their last omega
+                }
+            }
+            // collision (CODEX:3143-3416), air mode
+            if (c.collision_reg) {
+                collide_regularized<T, false>(fcur, rcur, ucur, vcur, wcur, om, T(0));
+            } else if (c.top_sponge && k > nz - 3) {
+                sponge_relax(fcur, rcur, ucur, vcur, wcur);
+            } else {
+                bool bs = c.c_backscatter > T(0) && open_cur;
+                float xi = bs ? bs_uniform(c.seed, a.step, (uint32_t)id) : 0.0f;
+                collide_kbc<T, false>(fcur, rcur, ucur, vcur, wcur, om, T(0), bs, c.c_backscatter, xi);
+            }
+#pragma unroll
+            for (int q = 0; q < 19; q++) Store<S, T>::store(a.fout, (size_t)q * n + id, q, fcur[q], c.store_scale);
+            if (a.write_macro) {
+                a.rho[id] = rcur;
+                a.ux[id] = ucur;
+                a.uy[id] = vcur;
+                a.uz[id] = wcur;
+                a.omega[id] = om;
+            }
+        }
+        __syncthreads();
+        open_below = open_cur;
+#pragma unroll
+        for (int q = 0; q < 19; q++) fcur[q] = fnext[q];
+        rcur = rn;
+        ucur = un;
+        vcur = vn;
+        wcur = wn;
+        open_cur = open_next;
+    }
+}
+
+// ===========================================================================
+// reductions (FP64)
+// ===========================================================================
+template <class S, class T>
+__global__ void k_reduce(Consts<T> c, const S* f, const T* ux, const T* uy, const T* uz, const T* mass, const T* fill,
+                         const uint8_t* flags, const int* type, int what, double* out) {
+    size_t id = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    size_t n = (size_t)c.nx * c.ny * c.nz;
+    double v = 0.0;
+    if (id < n) {
+        if (what == LBM_REDUCE_MASS) {
+            if (flags[id] & 1u)
+                for (int q = 0; q < 19; q++) v += (double)Store<S, T>::load(f, (size_t)q * n + id, q, (T)c.store_inv_scale);
+        } else if (what == LBM_REDUCE_VOF_MASS) {
+            v = (double)mass[id];
+        } else if (what == LBM_REDUCE_VOF_VOLUME) {
+            v = (double)fill[id];
+        } else if (what == LBM_REDUCE_MAX_SPEED) {
+            bool act = flags[id] & 1u;
+            if (c.physics_water && type[id] == LBM_FS_GAS) act = false;
+            if (act) {
+                double a = ux[id], b = uy[id], e = uz[id];
+                v = sqrt(a * a + b * b + e * e);
+                if (v != v) v = 0.0;  // np.nanmax ignores NaN
+            }
+        } else if (what == LBM_REDUCE_NONFINITE) {
+            for (int q = 0; q < 19; q++) {
+                T x = Store<S, T>::load(f, (size_t)q * n + id, q, (T)c.store_inv_scale);
+                if (!__builtin_isfinite(x)) v += 1.0;
+            }
+        }
+    }
+    if (what == LBM_REDUCE_MAX_SPEED) {
+        for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_down(v, off));
+        if ((threadIdx.x & 63) == 0 && v > 0.0) {
+            unsigned long long* p = (unsigned long long*)out;
+            unsigned long long bits = (unsigned long long)__double_as_longlong(v);
+            atomicMax(p, bits);  // non-negative doubles order like their bits
+        }
+    } else {
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off);
+        if ((threadIdx.x & 63) == 0 && v != 0.0) atomicAdd(out, v);
+    }
+}
+
+template <class S, class T>
+__global__ void k_store_populations(Consts<T> c, const T* src, S* dst) {
+    size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    size_t n = (size_t)c.nx * c.ny * c.nz;
+    if (idx >= 19 * n) return;
+    int q = (int)(idx / n);
+    Store<S, T>::store(dst, idx, q, src[idx], c.store_scale);
+}
+
+template <class S, class T>
+__global__ void k_load_populations(Consts<T> c, const S* src, T* dst) {
+    size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    size_t n = (size_t)c.nx * c.ny * c.nz;
+    if (idx >= 19 * n) return;
+    int q = (int)(idx / n);
+    dst[idx] = Store<S, T>::load(src, idx, q, (T)c.store_inv_scale);
+}
+
+// ===========================================================================
+// launchers
+// ===========================================================================
+bool fused_supported(const lbm_config_t& c) {
+    return c.physics_mode == LBM_PHYS_AIR && c.nx % FT_X == 0 && c.ny % FT_Y == 0 && c.nz >= 3;
+}
+
+template <class T>
+static void bed2d_t(const lbm_config_t& cfg, DevBufs& b, float t, int slot, hipStream_t s) {
+    Consts<T> c = make_consts<T>(cfg, -1);
+    size_t n2 = b.n2;
+    int prev = slot ^ 1;
+    k_bed2d_a<T><<<nblocks(n2), NT, 0, s>>>(c, (T)t, (T*)b.h[slot], (T*)b.uw[slot], (T*)b.vw[slot], (T*)b.ww[slot],
+                                             (const T*)b.h[prev], (const T*)b.piecewise);
+    k_bed2d_b<T><<<nblocks(n2), NT, 0, s>>>(c, (const T*)b.h[slot], (T*)b.eta9[slot], (T*)b.denom[slot], b.kband[slot]);
+    HIP_OK(hipGetLastError());
+}
+void launch_bed2d(const lbm_config_t& c, DevBufs& b, float t, int slot, hipStream_t s) {
+    if (b.storage == LBM_STORE_FP64)
+        bed2d_t<double>(c, b, t, slot, s);
+    else
+        bed2d_t<float>(c, b, t, slot, s);
+}
+
+template <class T>
+static void geometry_fields_t(const lbm_config_t& cfg, DevBufs& b, int slot, int32_t* lo, void* lf, int32_t* wt, int32_t* near,
+                              void* phi, hipStream_t s) {
+    Consts<T> c = make_consts<T>(cfg, -1);
+    k_geometry_fields<T><<<nblocks(b.n), NT, 0, s>>>(c, make_slot<T>(b, slot), lo, (T*)lf, wt, near, (T*)phi);
+    HIP_OK(hipGetLastError());
+}
+void launch_geometry_fields(const lbm_config_t& c, DevBufs& b, int slot, int32_t* lo, void* lf, int32_t* wt, int32_t* near,
+                            void* phi, hipStream_t s) {
+    if (b.storage == LBM_STORE_FP64)
+        geometry_fields_t<double>(c, b, slot, lo, lf, wt, near, phi, s);
+    else
+        geometry_fields_t<float>(c, b, slot, lo, lf, wt, near, phi, s);
+}
+
+template <class T>
+static void cellflags_t(const lbm_config_t& cfg, DevBufs& b, int slot, hipStream_t s) {
+    Consts<T> c = make_consts<T>(cfg, -1);
+    k_cellflags<T><<<nblocks(b.n), NT, 0, s>>>(c, make_slot<T>(b, slot), b.cellflags);
+    HIP_OK(hipGetLastError());
+}
+void launch_cellflags(const lbm_config_t& c, DevBufs& b, int slot, hipStream_t s) {
+    if (b.storage == LBM_STORE_FP64)
+        cellflags_t<double>(c, b, slot, s);
+    else
+        cellflags_t<float>(c, b, slot, s);
+}
+
+template <class T>
+static VofArgs<T> make_vof(const lbm_config_t& cfg, DevBufs& b, const T* f) {
+    VofArgs<T> a;
+    a.c = make_consts<T>(cfg, -1);
+    a.f = f;
+    a.rho = (T*)b.rho;
+    a.ux = (T*)b.ux;
+    a.uy = (T*)b.uy;
+    a.uz = (T*)b.uz;
+    a.fill = (T*)b.fill;
+    a.mass = (T*)b.mass;
+    a.mass2 = (T*)b.mass2;
+    a.mdelta = (T*)b.mdelta;
+    a.excess = (T*)b.excess;
+    a.sa = (T*)b.sa;
+    a.sb = (T*)b.sb;
+    a.sc = (T*)b.sc;
+    a.sd = (T*)b.sd;
+    a.se = (T*)b.se;
+    a.smv = (T*)b.smv;
+    a.type = b.type;
+    a.type_prev = b.type_prev;
+    a.excess_dir = b.excess_dir;
+    a.ext_gas = b.ext_gas;
+    a.si = b.si;
+    a.flags = b.cellflags;
+    a.icnt = b.icnt;
+    a.dcnt = b.dcnt;
+    return a;
+}
+
+template <class S, class T>
+static void init_t(const lbm_config_t& cfg, DevBufs& b, int slot, hipStream_t s) {
+    Consts<T> c = make_consts<T>(cfg, -1);
+    Slot<T> sl = make_slot<T>(b, slot);
+    k_cellflags<T><<<nblocks(b.n), NT, 0, s>>>(c, sl, b.cellflags);
+    k_init_populations<S, T><<<nblocks(b.n), NT, 0, s>>>(c, sl, (S*)b.f[0], (T*)b.rho, (T*)b.ux, (T*)b.uy, (T*)b.uz, b.obst,
+                                                         b.type);
+    if (cfg.fs_enabled && cfg.fs_tracking == LBM_FST_VOF) {
+        VofArgs<T> a = make_vof<T>(cfg, b, (const T*)b.f[0]);
+        k_vof_init_mass<T><<<nblocks(b.n), NT, 0, s>>>(a);
+        k_vof_enforce<T><<<nblocks(b.n), NT, 0, s>>>(a);
+        k_vof_height<T><<<nblocks(b.n2), NT, 0, s>>>(c, sl, (const T*)b.fill, b.cellflags, (T*)b.fs_h);
+    }
+    HIP_OK(hipGetLastError());
+}
+void launch_init_populations(const lbm_config_t& c, DevBufs& b, int slot, hipStream_t s) {
+    switch (b.storage) {
+        case LBM_STORE_FP64: init_t<double, double>(c, b, slot, s); break;
+        case LBM_STORE_FP16: init_t<__half, float>(c, b, slot, s); break;
+        case LBM_STORE_BF16: init_t<bf16_t, float>(c, b, slot, s); break;
+        default: init_t<float, float>(c, b, slot, s); break;
+    }
+}
+
+template <class T>
+static void vof_pipeline(const lbm_config_t& cfg, DevBufs& b, T* f, int slot, int64_t step, hipStream_t s) {
+    const size_t n = b.n;
+    const unsigned nb = nblocks(n);
+    auto A = [&]() { return make_vof<T>(cfg, b, f); };
+    auto swap_mass = [&]() { std::swap(b.mass, b.mass2); };
+    T empty_thr = (T)cfg.vof_empty_fill_threshold;
+    HIP_OK(hipMemsetAsync(b.icnt + IC_VOF_EXCESS, 0, sizeof(int), s));
+    k_vof_exchange<T><<<nb, NT, 0, s>>>(A());
+    k_vof_preclassify<T><<<nb, NT, 0, s>>>(A(), empty_thr);
+    auto redistribute = [&]() {
+        k_vof_redist_a<T><<<nb, NT, 0, s>>>(A());
+        k_vof_redist_b<T><<<nb, NT, 0, s>>>(A());
+        swap_mass();
+    };
+    auto finalize = [&]() { k_vof_finalize<T><<<nb, NT, 0, s>>>(A(), empty_thr); };
+    for (int r = 0; r < 3; r++) {
+        redistribute();
+        finalize();
+    }
+    k_vof_enforce<T><<<nb, NT, 0, s>>>(A());
+    HIP_OK(hipMemsetAsync(b.icnt + IC_VOF_ORPHAN, 0, sizeof(int), s));
+    T othr = (T)cfg.vof_orphan_empty_threshold;
+    k_vof_orphan_a<T><<<nb, NT, 0, s>>>(A(), empty_thr, othr, cfg.vof_orphan_search_radius, cfg.vof_orphan_max_weak_neighbors);
+    k_vof_orphan_b<T><<<nb, NT, 0, s>>>(A(), othr, cfg.vof_orphan_search_radius);
+    swap_mass();
+    k_vof_orphan_c<T><<<nb, NT, 0, s>>>(A());
+    HIP_OK(hipMemsetAsync(b.icnt + IC_VOF_ADVECT, 0, 2 * sizeof(int), s));
+    HIP_OK(hipMemsetAsync(b.dcnt + DC_ADVECT_MASS, 0, 2 * sizeof(double), s));
+    for (int mode = 0; mode < 2; mode++) {
+        bool on = cfg.vof_detached_advection && (mode == 0 || cfg.vof_detached_residual_fill_threshold > 0.0f);
+        if (on) {
+            k_vof_detached_a<T><<<nb, NT, 0, s>>>(A(), mode, othr, cfg.vof_detached_max_wet_neighbors,
+                                                  cfg.vof_detached_max_resolved_neighbors,
+                                                  (T)cfg.vof_detached_residual_fill_threshold);
+            k_vof_detached_b<T><<<nb, NT, 0, s>>>(A());
+        }
+        finalize();
+    }
+    redistribute();
+    finalize();
+    k_vof_enforce<T><<<nb, NT, 0, s>>>(A());
+    // collapse_trapped_gas_if_due (CODEX:2129-2143)
+    HIP_OK(hipMemsetAsync(b.icnt + IC_VOF_COLLAPSE_CELLS, 0, sizeof(int), s));
+    HIP_OK(hipMemsetAsync(b.dcnt + DC_CAND_MASS, 0, 5 * sizeof(double), s));
+    if (cfg.vof_collapse_trapped_gas && step % cfg.vof_collapse_interval == 0) {
+        k_flood_seed<T><<<nb, NT, 0, s>>>(A());
+        // sweep flags: flagbuf[0] = 1 primes the first sweep; each sweep
+        // clears its own flag before running
+        int* flagbuf = b.si;  // reuse the int scratch: first (sweeps+1) ints
+        int sweeps = std::max(1, cfg.nx + cfg.ny + cfg.nz) * 4;
+        if ((size_t)(sweeps + 1) > n) sweeps = (int)n - 1;
+        int one = 1;
+        HIP_OK(hipMemcpyAsync(flagbuf, &one, sizeof(int), hipMemcpyHostToDevice, s));
+        HIP_OK(hipMemsetAsync(flagbuf + 1, 0, sweeps * sizeof(int), s));
+        for (int w = 0; w < sweeps; w++) k_flood_sweep<T><<<nb, NT, 0, s>>>(A(), flagbuf + w, flagbuf + w + 1);
+        HIP_OK(hipMemcpyAsync(b.icnt + IC_FLOOD_UNCONVERGED, flagbuf + sweeps, sizeof(int), hipMemcpyDeviceToDevice, s));
+        k_collapse_sum<T><<<nb, NT, 0, s>>>(A());
+        k_collapse_decide<<<1, 1, 0, s>>>(b.dcnt, (double)cfg.vof_collapse_max_volume_m3);
+        k_collapse_apply<T><<<nb, NT, 0, s>>>(A());
+    }
+    k_vof_enforce<T><<<nb, NT, 0, s>>>(A());
+    {
+        Consts<T> c = make_consts<T>(cfg, -1);
+        k_vof_height<T><<<nblocks(b.n2), NT, 0, s>>>(c, make_slot<T>(b, slot), (const T*)b.fill, b.cellflags, (T*)b.fs_h);
+    }
+    HIP_OK(hipMemsetAsync(b.icnt + IC_FS_REFILL, 0, 2 * sizeof(int), s));
+    k_vof_refill<T><<<nb, NT, 0, s>>>(A(), f);
+    k_vof_repair<T><<<nb, NT, 0, s>>>(A(), f);
+    HIP_OK(hipGetLastError());
+}
+
+template <class T>
+static void staged_t(const lbm_config_t& cfg, DevBufs& b, int& cur, int slot, int64_t step, int omega_band_top, hipStream_t s) {
+    Consts<T> c = make_consts<T>(cfg, omega_band_top);
+    Slot<T> sl = make_slot<T>(b, slot);
+    const unsigned nb = nblocks(b.n);
+    T* f = (T*)b.f[cur];
+    T* fn = (T*)b.f[cur ^ 1];
+    k_cellflags<T><<<nb, NT, 0, s>>>(c, sl, b.cellflags);
+    k_cond<T><<<nb, NT, 0, s>>>(c, sl, f, b.obst, b.obst_prev);
+    k_macro<T><<<nb, NT, 0, s>>>(c, f, (T*)b.rho, (T*)b.ux, (T*)b.uy, (T*)b.uz, b.type, b.cellflags);
+    if (cfg.use_les)
+        k_les<T><<<nb, NT, 0, s>>>(c, sl, (const T*)b.ux, (const T*)b.uy, (const T*)b.uz, (T*)b.omega, (T*)b.nu_t, (T*)b.smag,
+                                   b.type, b.cellflags);
+    k_collide<T><<<nb, NT, 0, s>>>(c, f, (const T*)b.rho, (const T*)b.ux, (const T*)b.uy, (const T*)b.uz, (const T*)b.omega,
+                                   b.type, b.cellflags, (uint32_t)step);
+    HIP_OK(hipMemsetAsync(b.icnt + IC_BOUNDARY_CUT, 0, 3 * sizeof(int), s));
+    k_stream_bc<T, T><<<nb, NT, 0, s>>>(c, sl, f, fn, (const T*)b.ux, (const T*)b.uy, (const T*)b.uz, b.type, (const T*)b.fill,
+                                        (const T*)b.u_inlet, (const T*)b.inflow, (const T*)b.fs_h, b.cellflags, b.icnt, 1);
+    cur ^= 1;
+    HIP_OK(hipGetLastError());
+    if (cfg.fs_enabled) {
+        T* f2 = (T*)b.f[cur];
+        k_macro<T><<<nb, NT, 0, s>>>(c, f2, (T*)b.rho, (T*)b.ux, (T*)b.uy, (T*)b.uz, b.type, b.cellflags);
+        if (cfg.fs_tracking == LBM_FST_VOF) vof_pipeline<T>(cfg, b, f2, slot, step, s);
+    }
+}
+void launch_staged_step(const lbm_config_t& c, DevBufs& b, int& cur, int slot, int64_t step, int omega_band_top,
+                        hipStream_t s) {
+    if (b.storage == LBM_STORE_FP64)
+        staged_t<double>(c, b, cur, slot, step, omega_band_top, s);
+    else if (b.storage == LBM_STORE_FP32)
+        staged_t<float>(c, b, cur, slot, step, omega_band_top, s);
+    else
+        throw std::runtime_error("the staged path needs fp32 or fp64 storage");
+}
+
+template <class S, class T>
+static void fused_t(const lbm_config_t& cfg, DevBufs& b, int& cur, int mode, int slot_prev, int slot, int64_t step, int write_macro,
+                    int omega_band_top, int kchunk, hipStream_t s) {
+    FusedArgs<S, T> a;
+    a.c = make_consts<T>(cfg, omega_band_top);
+    a.s0 = make_slot<T>(b, slot_prev);
+    a.s1 = make_slot<T>(b, slot);
+    a.fin = (const S*)b.f[cur];
+    a.fout = (S*)b.f[cur ^ 1];
+    a.rho = (T*)b.rho;
+    a.ux = (T*)b.ux;
+    a.uy = (T*)b.uy;
+    a.uz = (T*)b.uz;
+    a.omega = (T*)b.omega;
+    a.u_inlet = (const T*)b.u_inlet;
+    a.inflow = (const T*)b.inflow;
+    a.mode = mode;
+    a.write_macro = write_macro;
+    a.kc = kchunk;
+    a.step = (uint32_t)step;
+    dim3 grid(cfg.nx / FT_X, cfg.ny / FT_Y, (cfg.nz + kchunk - 1) / kchunk);
+    k_fused<S, T><<<grid, FT_THREADS, 0, s>>>(a);
+    HIP_OK(hipGetLastError());
+    cur ^= 1;
+}
+void launch_fused_step(const lbm_config_t& c, DevBufs& b, int& cur, int mode, int slot_prev, int slot, int64_t step,
+                       int write_macro, int omega_band_top, int kchunk, hipStream_t s) {
+    if (!fused_supported(c)) throw std::runtime_error("fused path: needs air mode, nx % 32 == 0 and ny % 16 == 0");
+    switch (b.storage) {
+        case LBM_STORE_FP64: fused_t<double, double>(c, b, cur, mode, slot_prev, slot, step, write_macro, omega_band_top, kchunk, s); break;
+        case LBM_STORE_FP16: fused_t<__half, float>(c, b, cur, mode, slot_prev, slot, step, write_macro, omega_band_top, kchunk, s); break;
+        case LBM_STORE_BF16: fused_t<bf16_t, float>(c, b, cur, mode, slot_prev, slot, step, write_macro, omega_band_top, kchunk, s); break;
+        default: fused_t<float, float>(c, b, cur, mode, slot_prev, slot, step, write_macro, omega_band_top, kchunk, s); break;
+    }
+}
+
+template <class S, class T>
+static void materialize_t(const lbm_config_t& cfg, DevBufs& b, int cur, int slot, void* dst, hipStream_t s) {
+    Consts<T> c = make_consts<T>(cfg, -1);
+    Slot<T> sl = make_slot<T>(b, slot);
+    k_cellflags<T><<<nblocks(b.n), NT, 0, s>>>(c, sl, b.cellflags);
+    k_stream_bc<S, T><<<nblocks(b.n), NT, 0, s>>>(c, sl, (const S*)b.f[cur], (T*)dst, (const T*)b.ux, (const T*)b.uy,
+                                                  (const T*)b.uz, b.type, (const T*)b.fill, (const T*)b.u_inlet,
+                                                  (const T*)b.inflow, (const T*)b.fs_h, b.cellflags, b.icnt, 0);
+    HIP_OK(hipGetLastError());
+}
+void launch_materialize(const lbm_config_t& c, DevBufs& b, int cur, int slot, void* dst, hipStream_t s) {
+    switch (b.storage) {
+        case LBM_STORE_FP64: materialize_t<double, double>(c, b, cur, slot, dst, s); break;
+        case LBM_STORE_FP16: materialize_t<__half, float>(c, b, cur, slot, dst, s); break;
+        case LBM_STORE_BF16: materialize_t<bf16_t, float>(c, b, cur, slot, dst, s); break;
+        default: materialize_t<float, float>(c, b, cur, slot, dst, s); break;
+    }
+}
+
+template <class S, class T>
+static void store_pop_t(const lbm_config_t& cfg, DevBufs& b, int cur, const void* src, hipStream_t s) {
+    Consts<T> c = make_consts<T>(cfg, -1);
+    k_store_populations<S, T><<<nblocks(19 * b.n), NT, 0, s>>>(c, (const T*)src, (S*)b.f[cur]);
+    HIP_OK(hipGetLastError());
+}
+void launch_store_populations(const lbm_config_t& c, DevBufs& b, int cur, const void* src, hipStream_t s) {
+    switch (b.storage) {
+        case LBM_STORE_FP64: store_pop_t<double, double>(c, b, cur, src, s); break;
+        case LBM_STORE_FP16: store_pop_t<__half, float>(c, b, cur, src, s); break;
+        case LBM_STORE_BF16: store_pop_t<bf16_t, float>(c, b, cur, src, s); break;
+        default: store_pop_t<float, float>(c, b, cur, src, s); break;
+    }
+}
+
+template <class S, class T>
+static void reduce_t(const lbm_config_t& cfg, DevBufs& b, const void* f, int what, hipStream_t s) {
+    Consts<T> c = make_consts<T>(cfg, -1);
+    HIP_OK(hipMemsetAsync(b.dcnt + DC_REDUCE, 0, sizeof(double), s));
+    k_reduce<S, T><<<nblocks(b.n), NT, 0, s>>>(c, (const S*)f, (const T*)b.ux, (const T*)b.uy, (const T*)b.uz, (const T*)b.mass,
+                                               (const T*)b.fill, b.cellflags, b.type, what, b.dcnt + DC_REDUCE);
+    HIP_OK(hipGetLastError());
+}
+void launch_reduce(const lbm_config_t& c, DevBufs& b, const void* f, int f_is_storage, int what, hipStream_t s) {
+    if (b.storage == LBM_STORE_FP64)
+        reduce_t<double, double>(c, b, f, what, s);
+    else if (!f_is_storage || b.storage == LBM_STORE_FP32)
+        reduce_t<float, float>(c, b, f, what, s);
+    else if (b.storage == LBM_STORE_FP16)
+        reduce_t<__half, float>(c, b, f, what, s);
+    else
+        reduce_t<bf16_t, float>(c, b, f, what, s);
+}
+
+template <class T>
+static void cut_links_t(const lbm_config_t& cfg, DevBufs& b, int slot, hipStream_t s) {
+    Consts<T> c = make_consts<T>(cfg, -1);
+    HIP_OK(hipMemsetAsync(b.icnt + IC_BOUNDARY_CUT, 0, sizeof(int), s));
+    k_cut_links<T><<<nblocks(b.n), NT, 0, s>>>(c, make_slot<T>(b, slot), b.icnt);
+    HIP_OK(hipGetLastError());
+}
+void launch_cut_link_count(const lbm_config_t& c, DevBufs& b, int slot, hipStream_t s) {
+    if (b.storage == LBM_STORE_FP64)
+        cut_links_t<double>(c, b, slot, s);
+    else
+        cut_links_t<float>(c, b, slot, s);
+}
+
+template <class T>
+__global__ void k_obstacle_from_flags(size_t n, const uint8_t* flags, int* obst) {
+    size_t id = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (id < n) obst[id] = (flags[id] & 1u) ? 0 : 1;
+}
+void launch_obstacle(const lbm_config_t& c, DevBufs& b, int slot, int* dst, hipStream_t s) {
+    launch_cellflags(c, b, slot, s);
+    k_obstacle_from_flags<float><<<nblocks(b.n), NT, 0, s>>>(b.n, b.cellflags, dst);
+    HIP_OK(hipGetLastError());
+}
+
+template <class S, class T>
+static void load_pop_t(const lbm_config_t& cfg, DevBufs& b, int cur, void* dst, hipStream_t s) {
+    Consts<T> c = make_consts<T>(cfg, -1);
+    k_load_populations<S, T><<<nblocks(19 * b.n), NT, 0, s>>>(c, (const S*)b.f[cur], (T*)dst);
+    HIP_OK(hipGetLastError());
+}
+void launch_load_populations(const lbm_config_t& c, DevBufs& b, int cur, void* dst, hipStream_t s) {
+    switch (b.storage) {
+        case LBM_STORE_FP64: load_pop_t<double, double>(c, b, cur, dst, s); break;
+        case LBM_STORE_FP16: load_pop_t<__half, float>(c, b, cur, dst, s); break;
+        case LBM_STORE_BF16: load_pop_t<bf16_t, float>(c, b, cur, dst, s); break;
+        default: load_pop_t<float, float>(c, b, cur, dst, s); break;
+    }
+}
+
+}  // namespace lbm

@@ -1,0 +1,165 @@
+"""Host-side initial state of a case (the "initialise" half of the drop-in).
+
+Restates the host/NumPy parts of the reference's init sequence
+(LBM_3D_LES_air_over_waves_CODEX.py:3777-3792, "CODEX"):
+
+  * the log-law inlet profile and the randomised initial velocity of
+    init_fields (CODEX:2714-2749), with NumPy's PCG64 in place of Taichi's
+    RNG (the reference seeds ti.random through run_ab_case.py:261-273);
+  * initialize_free_surface_height + classify_free_surface_cells +
+    apply_vof_initial_disconnected_shape (CODEX:1197-1287) for VOF pools;
+  * prepare_initial_density_field (CODEX:743-818) for flat pools;
+  * the water branch of init_fields (CODEX:2751-2760).
+
+New initial states (LBM_INITIAL_STATE): uniform_noise (u0 + U(-a, a) per
+component) and loglaw_noise (log-law mean profile with uniform relative
+perturbations, also used as the inflow plane).  All draws use
+numpy.random.default_rng(seed), arrays in C order (nz, ny, nx), components
+ux -> uy -> uz.
+
+The device finishes the sequence (solid-cell wall velocities, f = feq,
+VOF mass, interface layer) in lbm_init_populations.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+import lbm_abi as abi
+from lbm_config import SolverConfig
+
+
+def inlet_profile(cfg: SolverConfig) -> np.ndarray:
+    """u_inlet_profile of init_fields (CODEX:2719-2738), float32 arithmetic."""
+    f32 = np.float32
+    B = f32(5.0)
+    u_tau = f32(0.1) * f32(cfg.u_top_lb)
+    nu_lbm = (f32(cfg.tau0) - f32(0.5)) / f32(3.0)
+    k = np.arange(cfg.Nz, dtype=np.float32)
+    y_plus = k * u_tau / nu_lbm
+    y_plus = np.where(y_plus <= f32(1.0), f32(1.0), y_plus).astype(np.float32)
+    tmp = (f32(1.0) / f32(cfg.kappa)) * np.log(y_plus).astype(np.float32) + B
+    top = tmp[-1] if tmp[-1] != 0.0 else f32(1.0)
+    return (tmp / top * f32(cfg.u_top_lb)).astype(np.float32)
+
+
+def loglaw_profile(cfg: SolverConfig) -> np.ndarray:
+    """Log-law mean profile (new): u(z) = A ln(max(z - z_b, z0) / z0) with
+    z_b the bed level, z0 in cells, A set so the mean over the cells above the
+    bed equals LBM_INFLOW_U_LU."""
+    dx = cfg.dx
+    z = (np.arange(cfg.Nz, dtype=np.float64) + 0.5) * dx
+    z_rel = z - cfg.offset_m
+    z0 = cfg.inflow_z0_cells * dx
+    prof = np.log(np.maximum(z_rel, z0) / z0)
+    above = z_rel > 0.0
+    mean = prof[above].mean() if np.any(above) else 1.0
+    return (cfg.inflow_u_lu * prof / mean).astype(np.float64)
+
+
+def initial_fields(cfg: SolverConfig, obstacle0: np.ndarray) -> dict:
+    """Macroscopic initial fields and boundary arrays to upload.
+
+    obstacle0 is the t=0 solid mask (1 = no open link) from the device
+    geometry; it classifies closed cells as gas in water runs."""
+    nz, ny, nx = cfg.Nz, cfg.Ny, cfg.Nx
+    real = np.float64 if cfg.storage == "fp64" else np.float32
+    rng = np.random.default_rng(cfg.seed)
+    out: dict = {}
+    u_inlet = inlet_profile(cfg)
+    out[abi.FIELD_U_INLET_PROFILE] = u_inlet.astype(real)
+    rho = np.full((nz, ny, nx), cfg.rho0, dtype=real)
+    uy = np.zeros((nz, ny, nx), dtype=real)
+    uz = np.zeros((nz, ny, nx), dtype=real)
+    inflow = np.zeros((nz, ny), dtype=real)
+    if cfg.initial_state == "uniform_noise":
+        a = cfg.init_noise_lu
+        noise = rng.uniform(-a, a, size=(3, nz, ny, nx))
+        ux = (cfg.init_u_lu + noise[0]).astype(real)
+        uy = noise[1].astype(real)
+        uz = noise[2].astype(real)
+    elif cfg.initial_state == "loglaw_noise":
+        prof = loglaw_profile(cfg)
+        pert = rng.uniform(-1.0, 1.0, size=(nz, ny, nx))
+        ux = (prof[:, None, None] * (1.0 + cfg.inflow_noise * pert)).astype(real)
+        pin = rng.uniform(-1.0, 1.0, size=(nz, ny))
+        inflow = (prof[:, None] * (1.0 + cfg.inflow_noise * pin)).astype(real)
+    else:
+        # init_fields: ux = u_inlet_profile[k] * (1 + (2U - 1) * TI)
+        rand_kick = (rng.random((nz, ny, nx), dtype=np.float32) - np.float32(0.5)) * np.float32(2.0)
+        ux = (u_inlet[:, None, None] * (np.float32(1.0) + rand_kick * np.float32(cfg.TI))).astype(real)
+    out[abi.FIELD_INFLOW_UX] = inflow
+    if cfg.water_free_surface_enabled:
+        fill, ctype, fs_h = free_surface_state(cfg, obstacle0)
+        rho_init = initial_density(cfg, fs_h)
+        gas = ctype == abi.FS_GAS
+        rho = np.where(gas, cfg.rho0, rho_init).astype(real)
+        ux = np.where(gas, 0.0, cfg.u_current_lb).astype(real)
+        uy = np.zeros_like(ux)
+        uz = np.zeros_like(ux)
+        out[abi.FIELD_FS_FILL] = fill.astype(real)
+        out[abi.FIELD_FS_TYPE] = ctype.astype(np.int32)
+        out[abi.FIELD_FS_TYPE_PREV] = ctype.astype(np.int32)
+        out[abi.FIELD_FS_H] = fs_h.astype(real)
+    out[abi.FIELD_RHO] = rho
+    out[abi.FIELD_UX] = ux
+    out[abi.FIELD_UY] = uy
+    out[abi.FIELD_UZ] = uz
+    return out
+
+
+def free_surface_state(cfg: SolverConfig, obstacle0: np.ndarray):
+    """initialize_free_surface_height (flat), classify_free_surface_cells and
+    apply_vof_initial_disconnected_shape (CODEX:1197-1287) in float32."""
+    f32 = np.float32
+    nz, ny, nx = cfg.Nz, cfg.Ny, cfg.Nx
+    dx = f32(cfg.dx)
+    fs_h = np.full((ny, nx), f32(cfg.free_surface_base_m), dtype=np.float32)
+    z_bottom = np.arange(nz, dtype=np.float32)[:, None, None] * dx
+    fill = ((fs_h[None, :, :] - z_bottom) / dx).astype(np.float32)
+    fill = np.clip(fill, f32(0.0), f32(1.0))
+    ctype = np.where(fill <= f32(1e-4), abi.FS_GAS, np.where(fill >= f32(1.0) - f32(1e-4), abi.FS_LIQUID, abi.FS_INTERFACE))
+    closed = obstacle0 != 0
+    fill = np.where(closed, f32(0.0), fill).astype(np.float32)
+    ctype = np.where(closed, abi.FS_GAS, ctype).astype(np.int32)
+    if cfg.vof_initial_shape == "block":
+        def overlap(lo, hi, bmin, bmax):
+            a = np.maximum(lo, f32(bmin))
+            b = np.minimum(hi, f32(bmax))
+            ov = np.maximum(b - a, f32(0.0))
+            return (ov / (hi - lo)).astype(np.float32)
+
+        x0 = np.arange(nx, dtype=np.float32) * dx
+        y0 = np.arange(ny, dtype=np.float32) * dx
+        z0 = np.arange(nz, dtype=np.float32) * dx
+        fx = overlap(x0, x0 + dx, cfg.vof_block_center_x_m - 0.5 * cfg.vof_block_size_x_m,
+                     cfg.vof_block_center_x_m + 0.5 * cfg.vof_block_size_x_m)
+        fy = overlap(y0, y0 + dx, cfg.vof_block_center_y_m - 0.5 * cfg.vof_block_size_y_m,
+                     cfg.vof_block_center_y_m + 0.5 * cfg.vof_block_size_y_m)
+        fz = overlap(z0, z0 + dx, cfg.vof_block_center_z_m - 0.5 * cfg.vof_block_size_z_m,
+                     cfg.vof_block_center_z_m + 0.5 * cfg.vof_block_size_z_m)
+        block = (fx[None, None, :] * fy[None, :, None] * fz[:, None, None]).astype(np.float32)
+        sel = (~closed) & (block > f32(1.0e-4))
+        nf = np.maximum(fill, block)
+        liquid = nf >= f32(1.0) - f32(1.0e-4)
+        nf = np.where(liquid, f32(1.0), nf)
+        nt = np.where(liquid, abi.FS_LIQUID, abi.FS_INTERFACE)
+        fill = np.where(sel, nf, fill).astype(np.float32)
+        ctype = np.where(sel, nt, ctype).astype(np.int32)
+    return fill, ctype, fs_h
+
+
+def initial_density(cfg: SolverConfig, fs_h: np.ndarray) -> np.ndarray:
+    """prepare_initial_density_field (CODEX:743-818), hydrostatic branch."""
+    nz, ny, nx = cfg.Nz, cfg.Ny, cfg.Nx
+    if cfg.initial_pressure_mode == "linear_wave":
+        raise NotImplementedError("LBM_INITIAL_PRESSURE_MODE=linear_wave is not supported")
+    rho = np.empty((nz, ny, nx), dtype=np.float32)
+    h = fs_h.astype(np.float64)
+    z = (np.arange(nz, dtype=np.float64) + 0.5) * cfg.dx
+    for kk, zc in enumerate(z):
+        if cfg.use_component_balanced_pressure == 1 or cfg.use_hydrostatic_balanced_pressure == 1:
+            head = h - cfg.free_surface_base_m
+        else:
+            head = np.maximum(h - zc, 0.0)
+        rho[kk] = np.asarray(cfg.rho0 + cfg.g_lb * head / cfg.dx / (1.0 / 3.0), dtype=np.float32)
+    return rho

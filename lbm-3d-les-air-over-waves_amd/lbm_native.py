@@ -1,0 +1,185 @@
+"""ctypes binding of liblbm.so (the C ABI declared in include/lbm.h).
+
+``Case`` wraps one lbm_handle_t.  There is no fallback: when the library is
+missing or fails to load, constructing a Case raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from pathlib import Path
+from typing import Optional
+
+import numpy as np
+
+import lbm_abi as abi
+
+PKG_DIR = Path(__file__).resolve().parent
+LIB_PATH = PKG_DIR / "liblbm.so"
+
+_lib = None
+
+EXPORTS = [
+    "lbm_create",
+    "lbm_destroy",
+    "lbm_geometry",
+    "lbm_init_populations",
+    "lbm_step",
+    "lbm_upload_field",
+    "lbm_download_field",
+    "lbm_get_counters",
+    "lbm_reduce",
+    "lbm_get_step",
+    "lbm_sync",
+    "lbm_last_error",
+    "lbm_last_step_timing",
+    "lbm_device_bytes",
+    "lbm_set_option",
+    "lbm_get_path",
+]
+
+
+def load_library(path: Optional[os.PathLike] = None) -> ctypes.CDLL:
+    """Load liblbm.so once and declare the prototypes."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = Path(path) if path else LIB_PATH
+    if not p.exists():
+        raise RuntimeError(f"{p} is missing: build it with __graft_entry__.build() (hipcc, gfx950)")
+    lib = ctypes.CDLL(str(p))
+    H = ctypes.c_void_p
+    lib.lbm_create.argtypes = [ctypes.POINTER(abi.LbmConfig), ctypes.POINTER(H)]
+    lib.lbm_create.restype = ctypes.c_int
+    lib.lbm_destroy.argtypes = [H]
+    lib.lbm_destroy.restype = None
+    lib.lbm_geometry.argtypes = [H, ctypes.c_float]
+    lib.lbm_init_populations.argtypes = [H]
+    lib.lbm_step.argtypes = [H, ctypes.c_int32]
+    lib.lbm_upload_field.argtypes = [H, ctypes.c_int32, ctypes.c_void_p, ctypes.c_size_t]
+    lib.lbm_download_field.argtypes = [H, ctypes.c_int32, ctypes.c_void_p, ctypes.c_size_t]
+    lib.lbm_get_counters.argtypes = [H, ctypes.POINTER(abi.LbmCounters)]
+    lib.lbm_reduce.argtypes = [H, ctypes.c_int32, ctypes.POINTER(ctypes.c_double)]
+    lib.lbm_get_step.argtypes = [H, ctypes.POINTER(ctypes.c_int64)]
+    lib.lbm_sync.argtypes = [H]
+    lib.lbm_last_error.argtypes = [H]
+    lib.lbm_last_error.restype = ctypes.c_char_p
+    lib.lbm_last_step_timing.argtypes = [H, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]
+    lib.lbm_device_bytes.argtypes = [ctypes.POINTER(abi.LbmConfig), ctypes.POINTER(ctypes.c_size_t)]
+    lib.lbm_set_option.argtypes = [H, ctypes.c_char_p, ctypes.c_int64]
+    lib.lbm_get_path.argtypes = [H, ctypes.POINTER(ctypes.c_int32)]
+    for name in EXPORTS:
+        if name not in ("lbm_destroy", "lbm_last_error"):
+            getattr(lib, name).restype = ctypes.c_int
+    if path is None:
+        _lib = lib
+    return lib
+
+
+class LbmError(RuntimeError):
+    pass
+
+
+def device_bytes(cfg: abi.LbmConfig) -> int:
+    lib = load_library()
+    out = ctypes.c_size_t(0)
+    rc = lib.lbm_device_bytes(ctypes.byref(cfg), ctypes.byref(out))
+    if rc != 0:
+        raise LbmError(f"lbm_device_bytes: config rejected ({rc})")
+    return int(out.value)
+
+
+class Case:
+    """One simulation case on the GPU (one lbm_handle_t)."""
+
+    def __init__(self, cfg: abi.LbmConfig):
+        self.lib = load_library()
+        self.cfg = cfg.copy()
+        self.h = ctypes.c_void_p()
+        rc = self.lib.lbm_create(ctypes.byref(self.cfg), ctypes.byref(self.h))
+        if rc != 0:
+            msg = self.lib.lbm_last_error(None)
+            raise LbmError(f"lbm_create failed ({rc}): {msg.decode() if msg else ''}")
+        self.real = np.float64 if cfg.storage == abi.STORE_FP64 else np.float32
+
+    # -- plumbing ------------------------------------------------------------
+    def _check(self, rc: int, what: str):
+        if rc != 0:
+            msg = self.lib.lbm_last_error(self.h)
+            raise LbmError(f"{what} failed ({rc}): {msg.decode() if msg else ''}")
+
+    def close(self):
+        if self.h:
+            self.lib.lbm_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # -- API -----------------------------------------------------------------
+    def geometry(self, t: float = 0.0):
+        self._check(self.lib.lbm_geometry(self.h, ctypes.c_float(t)), "lbm_geometry")
+
+    def init_populations(self):
+        self._check(self.lib.lbm_init_populations(self.h), "lbm_init_populations")
+
+    def step(self, n: int = 1):
+        self._check(self.lib.lbm_step(self.h, int(n)), "lbm_step")
+
+    def sync(self):
+        self._check(self.lib.lbm_sync(self.h), "lbm_sync")
+
+    def dtype_of(self, field: int):
+        return np.int32 if field in abi.INT_FIELDS else self.real
+
+    def upload(self, field: int, array: np.ndarray):
+        c = self.cfg
+        shape = abi.field_shape(field, c.nx, c.ny, c.nz)
+        a = np.ascontiguousarray(array, dtype=self.dtype_of(field)).reshape(shape)
+        self._check(self.lib.lbm_upload_field(self.h, field, a.ctypes.data, a.nbytes), f"upload field {field}")
+
+    def download(self, field: int) -> np.ndarray:
+        c = self.cfg
+        shape = abi.field_shape(field, c.nx, c.ny, c.nz)
+        out = np.empty(shape, dtype=self.dtype_of(field))
+        self._check(self.lib.lbm_download_field(self.h, field, out.ctypes.data, out.nbytes), f"download field {field}")
+        return out
+
+    def counters(self) -> dict:
+        cnt = abi.LbmCounters()
+        self._check(self.lib.lbm_get_counters(self.h, ctypes.byref(cnt)), "lbm_get_counters")
+        d = cnt.as_dict()
+        d["flood_fill_unconverged"] = cnt.pad0
+        return d
+
+    def reduce(self, what: int) -> float:
+        out = ctypes.c_double(0.0)
+        self._check(self.lib.lbm_reduce(self.h, what, ctypes.byref(out)), "lbm_reduce")
+        return float(out.value)
+
+    def step_index(self) -> int:
+        out = ctypes.c_int64(0)
+        self._check(self.lib.lbm_get_step(self.h, ctypes.byref(out)), "lbm_get_step")
+        return int(out.value)
+
+    def set_option(self, key: str, value: int):
+        self._check(self.lib.lbm_set_option(self.h, key.encode(), int(value)), f"lbm_set_option({key})")
+
+    def path(self) -> int:
+        out = ctypes.c_int32(0)
+        self._check(self.lib.lbm_get_path(self.h, ctypes.byref(out)), "lbm_get_path")
+        return int(out.value)
+
+    def last_timing(self):
+        a, b = ctypes.c_double(0.0), ctypes.c_double(0.0)
+        self._check(self.lib.lbm_last_step_timing(self.h, ctypes.byref(a), ctypes.byref(b)), "lbm_last_step_timing")
+        return float(a.value), float(b.value)

@@ -9,6 +9,8 @@
 // always see the reference's f(n).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <stdexcept>
@@ -39,6 +41,8 @@ struct lbm_handle {
     int64_t step = 0;
     int path = LBM_PATH_STAGED;
     int omega_band_top = -1;
+    int wall_top = 0;
+    double piecewise_max = 0.0;
     int kchunk = 16;
     int initialized = 0;
     int geometry_done = 0;
@@ -94,13 +98,32 @@ static int resolve_path(const lbm_config_t& c) {
     return fused_supported(c) ? LBM_PATH_FUSED : LBM_PATH_STAGED;
 }
 
+// upper bound of the bed height over the whole run
+static double bed_height_bound(const lbm_config_t& c, double piecewise_max) {
+    double amp = 0.0;
+    if (c.bed_profile == LBM_BED_MOVING_SINE) amp = 2.0 * fabs((double)c.bed_amp_m);
+    if (c.bed_profile == LBM_BED_MOVING_SINE1) amp = fabs((double)c.bed_amp_m);
+    if (c.bed_profile == LBM_BED_PIECEWISE) return piecewise_max;
+    return (double)c.offset_m + amp;
+}
+
 // highest k at which a cell can switch between solid and fluid (moving beds)
 static int band_top(const lbm_config_t& c) {
     if (!(c.bed_profile == LBM_BED_MOVING_SINE || c.bed_profile == LBM_BED_MOVING_SINE1) || !c.include_bed) return -1;
-    double amp = c.bed_profile == LBM_BED_MOVING_SINE ? 2.0 * fabs((double)c.bed_amp_m) : fabs((double)c.bed_amp_m);
-    double hmax = (double)c.offset_m + amp;
-    int k = (int)ceil(hmax / (double)c.dx) + 2;
+    int k = (int)ceil(bed_height_bound(c, 0.0) / (double)c.dx) + 2;
     return k >= c.nz ? c.nz - 1 : k;
+}
+
+// highest k at which any cell can have a cut or closed link: the per-column
+// band of k_bed2d_b (ceil(h/dx + 1/2) + 1) plus one cell; every k for
+// solid x or y walls
+static int wall_band_top(const lbm_config_t& c, double piecewise_max) {
+    if (c.x_boundary == LBM_XB_SOLID || c.y_boundary == LBM_YB_SOLID) return c.nz - 1;
+    if (!c.include_bed) return -1;
+    double h = bed_height_bound(c, piecewise_max);
+    double k = ceil(h / (double)c.dx + 0.5) + 2.0;
+    if (k < -1.0) return -1;
+    return k >= (double)(c.nz - 1) ? c.nz - 1 : (int)k;
 }
 
 static size_t device_bytes(const lbm_config_t& c, int path) {
@@ -109,7 +132,7 @@ static size_t device_bytes(const lbm_config_t& c, int path) {
     size_t bytes = 2 * 19 * n * store_size(c);  // ping-pong populations
     bytes += 5 * n * R + n;                      // rho, u, omega, cell flags
     if (path == LBM_PATH_STAGED) bytes += 2 * n * R + 2 * n * 4;  // nu_t, smag, obstacle masks
-    bytes += 2 * n2 * (13 * R + 4) + (c.nx + c.nz + (size_t)c.nz * c.ny) * R;
+    bytes += 2 * n2 * (40 * R + 4) + (c.nx + c.nz + (size_t)c.nz * c.ny) * R;
     if (c.fs_enabled) bytes += n * (11 * R + 7 * 4) + 6 * n * R + n2 * R;
     return bytes + (1 << 20);
 }
@@ -177,6 +200,7 @@ static void allocate(lbm_handle* h) {
         b.vw[s] = dalloc(h, n2 * R);
         b.ww[s] = dalloc(h, n2 * R);
         b.eta9[s] = dalloc(h, 9 * n2 * R);
+        b.uw9[s] = dalloc(h, 27 * n2 * R);
         b.denom[s] = dalloc(h, n2 * R);
         b.kband[s] = (int*)dalloc(h, n2 * 4);
         CHECK(hipMemsetAsync(b.uw[s], 0, n2 * R, h->stream));
@@ -288,6 +312,8 @@ int lbm_create(const lbm_config_t* cfg, lbm_handle_t** out) {
     h->cfg = *cfg;
     h->path = resolve_path(*cfg);
     h->omega_band_top = band_top(*cfg);
+    h->piecewise_max = (double)cfg->offset_m;
+    h->wall_top = wall_band_top(*cfg, h->piecewise_max);
     h->kchunk = cfg->nz >= 64 ? 16 : 8;
     try {
         int ndev = 0;
@@ -363,7 +389,7 @@ int lbm_step(lbm_handle_t* h, int32_t nsteps) {
         } else {
             int last = s == nsteps - 1;
             launch_fused_step(c, h->b, h->cur, h->post_stream ? 0 : 1, slot_of(step - 1), slot, step, last, h->omega_band_top,
-                              h->kchunk, h->stream);
+                              h->wall_top, h->kchunk, h->stream);
             h->post_stream = 0;
         }
         if (h->timing) CHECK(hipEventRecord(h->ev[3 + 2 * s], h->stream));
@@ -429,6 +455,11 @@ int lbm_upload_field(lbm_handle_t* h, int32_t field, const void* src, size_t byt
     } else {
         CHECK(hipMemcpyAsync(dst, src, need, hipMemcpyHostToDevice, h->stream));
         if (field == LBM_FIELD_PIECEWISE_BED_Z && c.bed_profile == LBM_BED_PIECEWISE) {
+            double mx = -1e300;
+            for (int i = 0; i < c.nx; i++)
+                mx = std::max(mx, R == 8 ? ((const double*)src)[i] : (double)((const float*)src)[i]);
+            h->piecewise_max = mx;
+            h->wall_top = wall_band_top(c, mx);
             // init_constants seeds the bed height with the profile (CODEX:1050-1054)
             std::vector<char> plane(n2 * R);
             for (size_t j = 0; j < (size_t)c.ny; j++) memcpy(plane.data() + j * c.nx * R, src, c.nx * R);

@@ -30,7 +30,7 @@ __host__ __device__ constexpr int oppq(int q) {
 }
 // z-mirror (free-slip top, new)
 __host__ __device__ constexpr int mirzq(int q) {
-    return q == 5 ? 6 : q == 6 ? 5 : q == 11 ? 13 : q == 13 ? 11 : q == 12 ? 14 : q == 14 ? 12 : q == 15 ? 17 : q == 17 ? 15 : q == 16 ? 18 : q == 18 ? 16 : q;
+    return q == 5 ? 6 : q == 6 ? 5 : q == 11 ? 14 : q == 14 ? 11 : q == 12 ? 13 : q == 13 ? 12 : q == 15 ? 18 : q == 18 ? 15 : q == 16 ? 17 : q == 17 ? 16 : q;
 }
 // index of the (cx,cy) column sample used by link q: (cx+1)*3 + (cy+1)
 __host__ __device__ constexpr int c9q(int q) { return (cxq(q) + 1) * 3 + (cyq(q) + 1); }
@@ -49,15 +49,23 @@ template <class S, class T>
 struct Store;
 template <class T>
 struct Store<T, T> {  // full-precision storage: the population itself
-    static __device__ __forceinline__ T load(const T* p, size_t i, int q, T) { return p[i]; }
-    static __device__ __forceinline__ void store(T* p, size_t i, int q, T v, T) { p[i] = v; }
+    template <class I>
+    static __device__ __forceinline__ T load(const T* p, I i, int q, T) {
+        return p[i];
+    }
+    template <class I>
+    static __device__ __forceinline__ void store(T* p, I i, int q, T v, T) {
+        p[i] = v;
+    }
 };
 template <>
 struct Store<__half, float> {  // (f_q - w_q) * 2^k as IEEE half
-    static __device__ __forceinline__ float load(const __half* p, size_t i, int q, float inv_scale) {
+    template <class I>
+    static __device__ __forceinline__ float load(const __half* p, I i, int q, float inv_scale) {
         return wq<float>(q) + __half2float(p[i]) * inv_scale;
     }
-    static __device__ __forceinline__ void store(__half* p, size_t i, int q, float v, float scale) {
+    template <class I>
+    static __device__ __forceinline__ void store(__half* p, I i, int q, float v, float scale) {
         p[i] = __float2half_rn((v - wq<float>(q)) * scale);
     }
 };
@@ -75,10 +83,12 @@ __device__ __forceinline__ bf16_t float_to_bf16(float f) {
 }
 template <>
 struct Store<bf16_t, float> {  // (f_q - w_q) * 2^k as bfloat16
-    static __device__ __forceinline__ float load(const bf16_t* p, size_t i, int q, float inv_scale) {
+    template <class I>
+    static __device__ __forceinline__ float load(const bf16_t* p, I i, int q, float inv_scale) {
         return wq<float>(q) + bf16_to_float(p[i]) * inv_scale;
     }
-    static __device__ __forceinline__ void store(bf16_t* p, size_t i, int q, float v, float scale) {
+    template <class I>
+    static __device__ __forceinline__ void store(bf16_t* p, I i, int q, float v, float scale) {
         p[i] = float_to_bf16((v - wq<float>(q)) * scale);
     }
 };
@@ -214,8 +224,16 @@ struct Consts {
     T free_surface_base_m, bridge_strength;
     T offset_m, bed_amp_m, bed_wavelength_m, wave_speed, wave_period, depth_swl_m, vel_scale, phase_speed;
     T dt_phys;
+    T inv_dx;     // 1/dx, used in place of a division by dx when dx is a power of two (exact)
+    int dx_pow2;
     float store_scale, store_inv_scale;  // 2^k and 2^-k of reduced-precision storage
 };
+
+// x / dx, exactly: a product with 1/dx when dx is a power of two
+template <class T>
+__device__ __forceinline__ T div_dx(const Consts<T>& c, T x) {
+    return c.dx_pow2 ? x * c.inv_dx : x / c.dx;
+}
 
 // 2-D time-slot arrays produced by the bed kernels for one time level
 template <class T>
@@ -227,6 +245,7 @@ struct Slot {
     const T* eta9;   // [9][ny][nx] bed height at the 9 link endpoints of each column
     const T* denom;  // [ny][nx] slope normaliser of phi
     const int* kband;  // [ny][nx] highest k with a cut or closed link
+    const T* uw9;      // [3][9][ny][nx] wall velocity at the 9 link endpoints
 };
 
 // link geometry of one cell (CODEX:2492-2686): open flag, open fraction,
@@ -250,7 +269,7 @@ __device__ __forceinline__ void cell_geometry(const Consts<T>& c, const Slot<T>&
         T eta_c = s.eta9[4 * plane + col];
         T gap_c = zc - eta_c;
         T ggap = c.include_bed ? gap_c : T(1.0e6);
-        T frac = T(0.5) + ggap / dx;
+        T frac = T(0.5) + div_dx(c, ggap);
         if (frac < T(0)) frac = T(0);
         if (frac > T(1)) frac = T(1);
         g.lf[0] = frac;
@@ -261,7 +280,7 @@ __device__ __forceinline__ void cell_geometry(const Consts<T>& c, const Slot<T>&
         T eta = s.eta9[(size_t)c9q(q) * plane + col];
         T gap = zq - eta;
         T ggap = c.include_bed ? gap : T(1.0e6);
-        T frac = T(0.5) + ggap / dx;
+        T frac = T(0.5) + div_dx(c, ggap);
         if (frac < T(0)) frac = T(0);
         if (frac > T(1)) frac = T(1);
         int lo = ggap < T(0) ? 0 : 1;
@@ -309,13 +328,12 @@ __device__ __forceinline__ int link_wall_type(const Consts<T>& c, const CellGeo<
 template <class T>
 __device__ __forceinline__ void wall_velocity(const Consts<T>& c, const Slot<T>& s, int i, int j, int q, T& ux, T& uy, T& uz) {
     if (c.wall_link) {
-        const T dx = c.dx;
-        T xc = ((T)i + T(0.5)) * dx, yc = ((T)j + T(0.5)) * dx;
-        T xq = xc + T(0.5) * dx * (T)cxq(q);
-        T yq = yc + T(0.5) * dx * (T)cyq(q);
-        ux = sample2d(s.uw, xq, yq, c.nx, c.ny, dx);
-        uy = sample2d(s.vw, xq, yq, c.nx, c.ny, dx);
-        uz = sample2d(s.ww, xq, yq, c.nx, c.ny, dx);
+        const size_t plane2 = (size_t)c.nx * c.ny;
+        const size_t col = (size_t)j * c.nx + i;
+        const int e = c9q(q);
+        ux = s.uw9[(size_t)e * plane2 + col];
+        uy = s.uw9[(size_t)(9 + e) * plane2 + col];
+        uz = s.uw9[(size_t)(18 + e) * plane2 + col];
     } else {
         size_t col = (size_t)j * c.nx + i;
         ux = s.uw[col];

@@ -83,7 +83,13 @@ static Consts<T> make_consts(const lbm_config_t& c, int omega_band_top) {
     k.depth_swl_m = (T)c.depth_swl_m;
     k.vel_scale = (T)c.vel_scale;
     k.phase_speed = (T)c.bed_phase_speed_mps;
-    k.dt_phys = (T)(float)c.dt_phys;
+    k.dt_phys = (T)c.dt_phys;
+    {
+        int e = 0;
+        double m = frexp((double)c.dx, &e);
+        k.dx_pow2 = m == 0.5 ? 1 : 0;
+        k.inv_dx = k.dx_pow2 ? (T)ldexp(1.0, 1 - e) : T(1) / k.dx;
+    }
     float sc = 1.0f;
     for (int i = 0; i < c.storage_shift; i++) sc *= 2.0f;
     k.store_scale = sc;
@@ -101,6 +107,7 @@ static Slot<T> make_slot(DevBufs& b, int s) {
     sl.eta9 = (const T*)b.eta9[s];
     sl.denom = (const T*)b.denom[s];
     sl.kband = b.kband[s];
+    sl.uw9 = (const T*)b.uw9[s];
     return sl;
 }
 
@@ -120,7 +127,7 @@ __device__ __forceinline__ void cell_ijk(size_t id, int nx, int ny, int& i, int&
 // update_wave_bed_and_velocities (CODEX:1082-1162), one thread per column;
 // moving_sine1 is the single-harmonic bed of the new configuration
 template <class T>
-__global__ void k_bed2d_a(Consts<T> c, T t, T* h, T* uw, T* vw, T* ww, const T* h_last, const T* piecewise) {
+__global__ __launch_bounds__(NT) void k_bed2d_a(Consts<T> c, T t, T* h, T* uw, T* vw, T* ww, const T* h_last, const T* piecewise) {
     size_t id = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (id >= (size_t)c.nx * c.ny) return;
     int i = (int)(id % (size_t)c.nx);
@@ -175,7 +182,7 @@ __global__ void k_bed2d_a(Consts<T> c, T t, T* h, T* uw, T* vw, T* ww, const T* 
 // endpoints), phi slope normaliser (CODEX:2451-2469) and the highest k with
 // a cut or closed link (cells above it are fully open: frac = 1 everywhere)
 template <class T>
-__global__ void k_bed2d_b(Consts<T> c, const T* h, T* eta9, T* denom, int* kband) {
+__global__ __launch_bounds__(NT) void k_bed2d_b(Consts<T> c, const T* h, T* eta9, T* denom, int* kband, const T* uw, const T* vw, const T* ww, T* uw9) {
     size_t id = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     size_t plane = (size_t)c.nx * c.ny;
     if (id >= plane) return;
@@ -188,8 +195,13 @@ __global__ void k_bed2d_b(Consts<T> c, const T* h, T* eta9, T* denom, int* kband
             T xq = xc + T(0.5) * dx * (T)a;
             T yq = yc + T(0.5) * dx * (T)bb;
             T e = sample2d(h, xq, yq, c.nx, c.ny, dx);
-            eta9[(size_t)((a + 1) * 3 + (bb + 1)) * plane + id] = e;
+            const int c9 = (a + 1) * 3 + (bb + 1);
+            eta9[(size_t)c9 * plane + id] = e;
             hmax = e > hmax ? e : hmax;
+            // wall velocity at the link endpoint (CODEX:937-988)
+            uw9[(size_t)c9 * plane + id] = sample2d(uw, xq, yq, c.nx, c.ny, dx);
+            uw9[(size_t)(9 + c9) * plane + id] = sample2d(vw, xq, yq, c.nx, c.ny, dx);
+            uw9[(size_t)(18 + c9) * plane + id] = sample2d(ww, xq, yq, c.nx, c.ny, dx);
         }
     int ip = (i + 1) % c.nx, im = (i - 1 + c.nx) % c.nx;
     int jp = (j + 1) % c.ny, jm = (j - 1 + c.ny) % c.ny;
@@ -221,7 +233,7 @@ __global__ void k_bed2d_b(Consts<T> c, const T* h, T* eta9, T* denom, int* kband
 // geometry materialisation (downloads) and per-cell flags
 // ===========================================================================
 template <class T>
-__global__ void k_geometry_fields(Consts<T> c, Slot<T> s, int32_t* lo, T* lf, int32_t* wt, int32_t* near, T* phi) {
+__global__ __launch_bounds__(NT) void k_geometry_fields(Consts<T> c, Slot<T> s, int32_t* lo, T* lf, int32_t* wt, int32_t* near, T* phi) {
     size_t id = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     size_t n = (size_t)c.nx * c.ny * c.nz;
     if (id >= n) return;
@@ -239,7 +251,7 @@ __global__ void k_geometry_fields(Consts<T> c, Slot<T> s, int32_t* lo, T* lf, in
 }
 
 template <class T>
-__global__ void k_cellflags(Consts<T> c, Slot<T> s, uint8_t* flags) {
+__global__ __launch_bounds__(NT) void k_cellflags(Consts<T> c, Slot<T> s, uint8_t* flags) {
     size_t id = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     size_t n = (size_t)c.nx * c.ny * c.nz;
     if (id >= n) return;
@@ -272,7 +284,7 @@ __device__ __forceinline__ T local_g(const Consts<T>& c, int k, bool open0, int 
 // initialisation: init_fields populations (CODEX:2711-2818)
 // ===========================================================================
 template <class S, class T>
-__global__ void k_init_populations(Consts<T> c, Slot<T> s, S* f, T* rho, T* ux, T* uy, T* uz, int* obst, const int* type) {
+__global__ __launch_bounds__(NT) void k_init_populations(Consts<T> c, Slot<T> s, S* f, T* rho, T* ux, T* uy, T* uz, int* obst, const int* type) {
     size_t id = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     size_t n = (size_t)c.nx * c.ny * c.nz;
     if (id >= n) return;
@@ -314,7 +326,7 @@ __global__ void k_init_populations(Consts<T> c, Slot<T> s, S* f, T* rho, T* ux, 
 
 // update_bed_populations_and_reinit (CODEX:2878-2940)
 template <class T>
-__global__ void k_cond(Consts<T> c, Slot<T> s, T* f, int* obst, int* obst_prev) {
+__global__ __launch_bounds__(NT) void k_cond(Consts<T> c, Slot<T> s, T* f, int* obst, int* obst_prev) {
     size_t id = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     size_t n = (size_t)c.nx * c.ny * c.nz;
     if (id >= n) return;
@@ -335,7 +347,7 @@ __global__ void k_cond(Consts<T> c, Slot<T> s, T* f, int* obst, int* obst_prev) 
 
 // macro_step (CODEX:2943-3008)
 template <class T>
-__global__ void k_macro(Consts<T> c, const T* f, T* rho, T* ux, T* uy, T* uz, const int* type, const uint8_t* flags) {
+__global__ __launch_bounds__(NT) void k_macro(Consts<T> c, const T* f, T* rho, T* ux, T* uy, T* uz, const int* type, const uint8_t* flags) {
     size_t id = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     size_t n = (size_t)c.nx * c.ny * c.nz;
     if (id >= n) return;
@@ -386,7 +398,7 @@ __global__ void k_macro(Consts<T> c, const T* f, T* rho, T* ux, T* uy, T* uz, co
 
 // compute_LES (CODEX:3011-3141)
 template <class T>
-__global__ void k_les(Consts<T> c, Slot<T> s, const T* ux, const T* uy, const T* uz, T* omega, T* nu_t, T* smag,
+__global__ __launch_bounds__(NT) void k_les(Consts<T> c, Slot<T> s, const T* ux, const T* uy, const T* uz, T* omega, T* nu_t, T* smag,
                       const int* type, const uint8_t* flags) {
     size_t id = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     size_t n = (size_t)c.nx * c.ny * c.nz;
@@ -423,7 +435,7 @@ __global__ void k_les(Consts<T> c, Slot<T> s, const T* ux, const T* uy, const T*
 
 // collide_regularized / collide_KBC (CODEX:3143-3416)
 template <class T>
-__global__ void k_collide(Consts<T> c, T* f, const T* rho, const T* ux, const T* uy, const T* uz, const T* omega,
+__global__ __launch_bounds__(NT) void k_collide(Consts<T> c, T* f, const T* rho, const T* ux, const T* uy, const T* uz, const T* omega,
                           const int* type, const uint8_t* flags, uint32_t step) {
     size_t id = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     size_t n = (size_t)c.nx * c.ny * c.nz;
@@ -510,7 +522,7 @@ __device__ __forceinline__ T src_value(const Consts<T>& c, const S* f, size_t n,
 // stream + apply_x_solid_wall_bounceback + apply_open_boundary_conditions
 // (CODEX:3418-3712) for one destination cell; fin = post-collision state.
 template <class S, class T>
-__global__ void k_stream_bc(Consts<T> c, Slot<T> s, const S* fin, T* fout, const T* ux, const T* uy, const T* uz,
+__global__ __launch_bounds__(NT) void k_stream_bc(Consts<T> c, Slot<T> s, const S* fin, T* fout, const T* ux, const T* uy, const T* uz,
                             const int* type, const T* fill, const T* u_inlet, const T* inflow, const T* fs_h,
                             const uint8_t* flags, int* icnt, int count) {
     size_t id = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -675,7 +687,7 @@ __global__ void k_stream_bc(Consts<T> c, Slot<T> s, const S* fin, T* fout, const
 // number of cut links of the air stream (the boundary_cut_link_count of
 // CODEX:3429-3549) from geometry alone, for runs on the fused path
 template <class T>
-__global__ void k_cut_links(Consts<T> c, Slot<T> s, int* icnt) {
+__global__ __launch_bounds__(NT) void k_cut_links(Consts<T> c, Slot<T> s, int* icnt) {
     size_t id = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     size_t n = (size_t)c.nx * c.ny * c.nz;
     int cnt = 0;
@@ -743,7 +755,7 @@ __device__ __forceinline__ bool nbr(int nx, int ny, int nz, int xb, int yb, int 
 
 // compute_vof_mass_exchange (CODEX:1305-1350)
 template <class T>
-__global__ void k_vof_exchange(VofArgs<T> a) {
+__global__ __launch_bounds__(NT) void k_vof_exchange(VofArgs<T> a) {
     VOF_PROLOGUE
     a.mdelta[id] = T(0);
     a.excess[id] = T(0);
@@ -766,7 +778,7 @@ __global__ void k_vof_exchange(VofArgs<T> a) {
 
 // apply_vof_mass_update_and_preclassify (CODEX:1353-1414)
 template <class T>
-__global__ void k_vof_preclassify(VofArgs<T> a, T empty_thr) {
+__global__ __launch_bounds__(NT) void k_vof_preclassify(VofArgs<T> a, T empty_thr) {
     VOF_PROLOGUE
     int old = a.type[id];
     a.type_prev[id] = old;
@@ -833,7 +845,7 @@ __device__ __forceinline__ T vof_room(const VofArgs<T>& a, size_t nid, T ex) {
 
 // redistribute_vof_mass_excess (CODEX:1417-1574) phase A: donor sums
 template <class T>
-__global__ void k_vof_redist_a(VofArgs<T> a) {
+__global__ __launch_bounds__(NT) void k_vof_redist_a(VofArgs<T> a) {
     VOF_PROLOGUE
     a.si[id] = 0;
     T ex = a.excess[id];
@@ -894,7 +906,7 @@ __global__ void k_vof_redist_a(VofArgs<T> a) {
 
 // phase B: every receiver adds its shares (own retained excess, then q order)
 template <class T>
-__global__ void k_vof_redist_b(VofArgs<T> a) {
+__global__ __launch_bounds__(NT) void k_vof_redist_b(VofArgs<T> a) {
     VOF_PROLOGUE
     T acc = a.mass[id];
     if (a.si[id] != 0 && !(a.se[id] > T(1.0e-12))) acc += a.excess[id];
@@ -927,7 +939,7 @@ __global__ void k_vof_redist_b(VofArgs<T> a) {
 
 // finalize_vof_classification_from_mass (CODEX:1577-1624)
 template <class T>
-__global__ void k_vof_finalize(VofArgs<T> a, T empty_thr) {
+__global__ __launch_bounds__(NT) void k_vof_finalize(VofArgs<T> a, T empty_thr) {
     VOF_PROLOGUE
     a.excess[id] = T(0);
     a.excess_dir[id] = 0;
@@ -981,7 +993,7 @@ __global__ void k_vof_finalize(VofArgs<T> a, T empty_thr) {
 // enforce_vof_interface_layer (CODEX:1627-1656): only LIQUID cells change
 // and only GAS neighbours are read, so the in-place update has no race
 template <class T>
-__global__ void k_vof_enforce(VofArgs<T> a) {
+__global__ __launch_bounds__(NT) void k_vof_enforce(VofArgs<T> a) {
     VOF_PROLOGUE
     if (!((a.flags[id] & 1u) && a.type[id] == LBM_FS_LIQUID)) return;
     bool touches = false;
@@ -1001,7 +1013,7 @@ __global__ void k_vof_enforce(VofArgs<T> a) {
 
 // remove_orphan_vof_interface_cells (CODEX:1659-1776) phase A: decide
 template <class T>
-__global__ void k_vof_orphan_a(VofArgs<T> a, T empty_thr, T thr, int rmax, int max_weak) {
+__global__ __launch_bounds__(NT) void k_vof_orphan_a(VofArgs<T> a, T empty_thr, T thr, int rmax, int max_weak) {
     VOF_PROLOGUE
     a.si[id] = 0;
     if (!((a.flags[id] & 1u) && a.type[id] != LBM_FS_GAS && a.fill[id] > empty_thr)) return;
@@ -1044,7 +1056,7 @@ __global__ void k_vof_orphan_a(VofArgs<T> a, T empty_thr, T thr, int rmax, int m
 
 // phase B: receivers gather the scattered orphan mass into mass2
 template <class T>
-__global__ void k_vof_orphan_b(VofArgs<T> a, T thr, int rmax) {
+__global__ __launch_bounds__(NT) void k_vof_orphan_b(VofArgs<T> a, T thr, int rmax) {
     VOF_PROLOGUE
     bool orphan = a.si[id] != 0;
     T acc = orphan ? T(0) : a.mass[id];
@@ -1075,7 +1087,7 @@ __global__ void k_vof_orphan_b(VofArgs<T> a, T thr, int rmax) {
 
 // phase C: the orphans become gas
 template <class T>
-__global__ void k_vof_orphan_c(VofArgs<T> a) {
+__global__ __launch_bounds__(NT) void k_vof_orphan_c(VofArgs<T> a) {
     VOF_PROLOGUE
     if (a.si[id]) {
         a.fill[id] = T(0);
@@ -1107,7 +1119,7 @@ __device__ __constant__ int FD6[6][3] = {{1, 0, 0}, {-1, 0, 0}, {0, 1, 0}, {0, -
 // advect_detached_vof_cells / compress_detached_vof_residuals phase A
 // (CODEX:1813-1993): mass each detached cell moves through each face
 template <class T>
-__global__ void k_vof_detached_a(VofArgs<T> a, int mode, T orphan_thr, int max_wet, int max_res, T resid_thr) {
+__global__ __launch_bounds__(NT) void k_vof_detached_a(VofArgs<T> a, int mode, T orphan_thr, int max_wet, int max_res, T resid_thr) {
     VOF_PROLOGUE
     T mv[6] = {T(0), T(0), T(0), T(0), T(0), T(0)};
     if (vof_detached(a, i, j, k, orphan_thr, max_wet, max_res)) {
@@ -1170,7 +1182,7 @@ __global__ void k_vof_detached_a(VofArgs<T> a, int mode, T orphan_thr, int max_w
 
 // phase B: apply outgoing then incoming moves (fixed direction order)
 template <class T>
-__global__ void k_vof_detached_b(VofArgs<T> a) {
+__global__ __launch_bounds__(NT) void k_vof_detached_b(VofArgs<T> a) {
     VOF_PROLOGUE
     T out = T(0);
     for (int d = 0; d < 6; d++) out -= a.smv[6 * id + d];
@@ -1202,7 +1214,7 @@ __device__ __forceinline__ T gas_fraction(const VofArgs<T>& a, size_t id) {
 
 // initialize_vof_exterior_gas_marks (CODEX:2021-2030)
 template <class T>
-__global__ void k_flood_seed(VofArgs<T> a) {
+__global__ __launch_bounds__(NT) void k_flood_seed(VofArgs<T> a) {
     VOF_PROLOGUE
     a.ext_gas[id] = (gas_fraction(a, id) > T(1.0e-6) && k == a.c.nz - 1) ? 1 : 0;
 }
@@ -1210,7 +1222,7 @@ __global__ void k_flood_seed(VofArgs<T> a) {
 // propagate_vof_exterior_gas_marks (CODEX:2033-2055), run to the fixed
 // point: every sweep after a sweep that marked nothing returns at once
 template <class T>
-__global__ void k_flood_sweep(VofArgs<T> a, int* changed_prev, int* changed_this) {
+__global__ __launch_bounds__(NT) void k_flood_sweep(VofArgs<T> a, int* changed_prev, int* changed_this) {
     if (*changed_prev == 0) return;
     VOF_PROLOGUE
     if (a.ext_gas[id] != 0 || !(gas_fraction(a, id) > T(1.0e-6))) return;
@@ -1229,7 +1241,7 @@ __global__ void k_flood_sweep(VofArgs<T> a, int* changed_prev, int* changed_this
 
 // summarize_vof_trapped_gas_collapse sums (CODEX:2058-2075) in FP64
 template <class T>
-__global__ void k_collapse_sum(VofArgs<T> a) {
+__global__ __launch_bounds__(NT) void k_collapse_sum(VofArgs<T> a) {
     size_t id = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     size_t n = (size_t)a.c.nx * a.c.ny * a.c.nz;
     double cm = 0.0, cv = 0.0, cap = 0.0;
@@ -1261,7 +1273,7 @@ __global__ void k_collapse_sum(VofArgs<T> a) {
 }
 
 // accept test of summarize_vof_trapped_gas_collapse (CODEX:2077-2091)
-__global__ void k_collapse_decide(double* dcnt, double max_volume) {
+__global__ __launch_bounds__(NT) void k_collapse_decide(double* dcnt, double max_volume) {
     double cm = dcnt[DC_CAND_MASS], cv = dcnt[DC_CAND_VOLUME], cap = dcnt[DC_EXT_CAPACITY];
     dcnt[DC_APPLIED_VOLUME] = 0.0;
     dcnt[DC_REMOVAL_SCALE] = 0.0;
@@ -1273,7 +1285,7 @@ __global__ void k_collapse_decide(double* dcnt, double max_volume) {
 
 // apply_vof_trapped_gas_collapse (CODEX:2094-2126)
 template <class T>
-__global__ void k_collapse_apply(VofArgs<T> a) {
+__global__ __launch_bounds__(NT) void k_collapse_apply(VofArgs<T> a) {
     VOF_PROLOGUE
     T scale = (T)a.dcnt[DC_REMOVAL_SCALE];
     if (!(scale > T(0))) return;
@@ -1308,7 +1320,7 @@ __global__ void k_collapse_apply(VofArgs<T> a) {
 
 // update_free_surface_height_from_vof (CODEX:2146-2161), one thread per column
 template <class T>
-__global__ void k_vof_height(Consts<T> c, Slot<T> s, const T* fill, const uint8_t* flags, T* fs_h) {
+__global__ __launch_bounds__(NT) void k_vof_height(Consts<T> c, Slot<T> s, const T* fill, const uint8_t* flags, T* fs_h) {
     size_t col = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     size_t plane = (size_t)c.nx * c.ny;
     if (col >= plane) return;
@@ -1327,7 +1339,7 @@ __global__ void k_vof_height(Consts<T> c, Slot<T> s, const T* fill, const uint8_
 
 // refill_new_free_surface_cells (CODEX:2163-2316), VOF tracking
 template <class T>
-__global__ void k_vof_refill(VofArgs<T> a, T* f) {
+__global__ __launch_bounds__(NT) void k_vof_refill(VofArgs<T> a, T* f) {
     VOF_PROLOGUE
     if (!(a.type_prev[id] == LBM_FS_GAS && a.type[id] != LBM_FS_GAS && (a.flags[id] & 1u))) return;
     const Consts<T>& c = a.c;
@@ -1430,7 +1442,7 @@ __global__ void k_vof_refill(VofArgs<T> a, T* f) {
 // repair_vof_coalescence_links (CODEX:2319-2373): donors (refilled cells,
 // type_prev gas) and repaired cells (type_prev wet) are disjoint
 template <class T>
-__global__ void k_vof_repair(VofArgs<T> a, T* f) {
+__global__ __launch_bounds__(NT) void k_vof_repair(VofArgs<T> a, T* f) {
     VOF_PROLOGUE
     if (!((a.flags[id] & 1u) && a.type[id] != LBM_FS_GAS && a.type_prev[id] != LBM_FS_GAS)) return;
     bool rep = false;
@@ -1466,7 +1478,7 @@ __global__ void k_vof_repair(VofArgs<T> a, T* f) {
 
 // initialize_vof_mass_from_fill (CODEX:1289-1302)
 template <class T>
-__global__ void k_vof_init_mass(VofArgs<T> a) {
+__global__ __launch_bounds__(NT) void k_vof_init_mass(VofArgs<T> a) {
     VOF_PROLOGUE
     if ((a.flags[id] & 1u) && a.type[id] != LBM_FS_GAS) {
         a.mass[id] = a.fill[id] * a.rho[id];
@@ -1496,6 +1508,12 @@ __global__ void k_vof_init_mass(VofArgs<T> a) {
 //      writes f* of plane k.
 // The halo ring and the planes k0-1 / k0+kc are recomputed by the
 // neighbouring blocks: extra arithmetic, no extra HBM traffic.
+//
+// Cells above the per-column wall band (every link open with fraction 1)
+// take a fast path: plain pull with the domain-boundary rules only.  Inside
+// the band the link geometry is evaluated link by link from the 9 column
+// bed samples, and the cell's own post-collision populations (needed by
+// the bounce-back terms) are re-read from cache instead of being held.
 constexpr int FT_X = 32, FT_Y = 16, FT_THREADS = 640;
 constexpr int FL_X = FT_X + 2, FL_Y = FT_Y + 2;
 
@@ -1511,131 +1529,326 @@ struct FusedArgs {
     int mode;  // 0: fin holds post-stream populations (no stream)
     int write_macro;
     int kc;
+    int chunk0;  // index of the first z chunk of this launch
     uint32_t step;
 };
 
-// stream (+ boundary rules) of step n and conditioning of step n+1 for one
-// cell; returns the populations entering the moments of step n+1
-template <class S, class T>
-__device__ __forceinline__ void fused_gather(const FusedArgs<S, T>& a, int i, int j, int k, T f[19], bool& open0_out) {
-    const Consts<T>& c = a.c;
-    const int nx = c.nx, ny = c.ny, nz = c.nz;
-    const size_t n = (size_t)nx * ny * nz;
-    const size_t id = ((size_t)k * ny + j) * nx + i;
-    const T inv = (T)c.store_inv_scale;
-    const size_t col = (size_t)j * nx + i;
-    if (a.mode == 0) {
+// c_q . u over the non-zero lattice components, in x, y, z order; a zero
+// component contributes an exact zero to the reference's sum, so the value
+// equals (cx ux + cy uy) + cz uz
+template <class T>
+__device__ __forceinline__ T cdotu(int q, T ux, T uy, T uz) {
+    const int a = cxq(q), b = cyq(q), e = czq(q);
+    T s = T(0);
+    bool first = true;
+    if (a) {
+        s = a > 0 ? ux : -ux;
+        first = false;
+    }
+    if (b) {
+        T v = b > 0 ? uy : -uy;
+        s = first ? v : s + v;
+        first = false;
+    }
+    if (e) {
+        T v = e > 0 ? uz : -uz;
+        s = first ? v : s + v;
+    }
+    return s;
+}
+
+// feq with the velocity terms hoisted: u15 = 1.5 |u|^2 (CODEX:641-643)
+template <class T>
+__device__ __forceinline__ T feq_h(T rho, T ux, T uy, T uz, T u15, int q) {
+    T cu = T(3.0) * cdotu<T>(q, ux, uy, uz);
+    return rho * wq<T>(q) * (T(1.0) + cu + T(0.5) * cu * cu - u15);
+}
+
+// Q_q : Pi over the non-zero entries of Q (zero entries add exact zeros)
+template <class T>
+__device__ __forceinline__ T q_dot_pi(int q, T Pxx, T Pyy, T Pzz, T Pxy, T Pxz, T Pyz) {
+    const T cs2 = T(1.0) / T(3.0);
+    const int a = cxq(q), b = cyq(q), e = czq(q);
+    T d = ((T)(a * a) - cs2) * Pxx + ((T)(b * b) - cs2) * Pyy + ((T)(e * e) - cs2) * Pzz;
+    T o;
+    if (a * b)
+        o = (T)(a * b) * Pxy;
+    else if (a * e)
+        o = (T)(a * e) * Pxz;
+    else if (b * e)
+        o = (T)(b * e) * Pyz;
+    else
+        return d;
+    return d + T(2.0) * o;
+}
+
+// collide_regularized, air mode (CODEX:3324-3416), in place
+template <class T>
+__device__ __forceinline__ void fused_regularized(T f[19], T rho, T ux, T uy, T uz, T om) {
+    const T cs2 = T(1.0) / T(3.0);
+    const T inv2 = T(1.0) / (T(2.0) * cs2 * cs2);
+    const T eps = T(1e-6);
+    T u2 = ux * ux + uy * uy + uz * uz;
+    T u15 = T(1.5) * u2;
+    T fe[19];
+    T Pxx = T(0), Pyy = T(0), Pzz = T(0), Pxy = T(0), Pxz = T(0), Pyz = T(0);
 #pragma unroll
-        for (int q = 0; q < 19; q++) f[q] = Store<S, T>::load(a.fin, (size_t)q * n + id, q, inv);
-    } else {
-        const bool xper = c.x_boundary == LBM_XB_PERIODIC, xio = c.x_boundary == LBM_XB_INFLOW_OUTFLOW;
-        const bool yper = c.y_boundary == LBM_YB_PERIODIC;
-        const bool fast0 = k > a.s0.kband[col];
-        f[0] = Store<S, T>::load(a.fin, id, 0, inv);
-        if (fast0) {
-            // no cut or closed link at this cell: plain pull + the x/y/top rules
+    for (int q = 0; q < 19; q++) {
+        fe[q] = feq_h<T>(rho, ux, uy, uz, u15, q);
+        T d = f[q] - fe[q];
+        const int a = cxq(q), b = cyq(q), e = czq(q);
+        if (a) Pxx += d;
+        if (b) Pyy += d;
+        if (e) Pzz += d;
+        if (a * b) Pxy += a * b > 0 ? d : -d;
+        if (a * e) Pxz += a * e > 0 ? d : -d;
+        if (b * e) Pyz += b * e > 0 ? d : -d;
+    }
+    T alpha = T(1.0);
+    const T one_m = T(1.0) - om;
 #pragma unroll
-            for (int q = 1; q < 19; q++) {
-                int sir = i - cxq(q), sjr = j - cyq(q), sk = k - czq(q);
-                int si = xper ? pmod(sir, nx) : sir;
-                int sj = yper ? pmod(sjr, ny) : sjr;
-                bool fluid = (xper || xio || (sir >= 0 && sir < nx)) && (yper || (sjr >= 0 && sjr < ny));
-                T out;
-                if (!fluid)
-                    out = Store<S, T>::load(a.fin, (size_t)oppq(q) * n + id, oppq(q), inv);
-                else if (sk >= 0 && sk < nz)
-                    out = src_value<S, T>(c, a.fin, n, a.inflow, si, sj, sk, q);
-                else if (sk >= nz && c.top_boundary == LBM_TOP_FREE_SLIP)
-                    out = src_value<S, T>(c, a.fin, n, a.inflow, si, sj, nz - 1, mirzq(q));
-                else
-                    out = Store<S, T>::load(a.fin, (size_t)q * n + id, q, inv);
-                f[q] = out;
-            }
-        } else {
-            CellGeo<T> g;
-            cell_geometry(c, a.s0, i, j, k, g);
-            T fpre[19];
-#pragma unroll
-            for (int q = 0; q < 19; q++) fpre[q] = Store<S, T>::load(a.fin, (size_t)q * n + id, q, inv);
-#pragma unroll
-            for (int q = 1; q < 19; q++) {
-                const int qb = oppq(q);
-                int sir = i - cxq(q), sjr = j - cyq(q), sk = k - czq(q);
-                int si = xper ? pmod(sir, nx) : sir;
-                int sj = yper ? pmod(sjr, ny) : sjr;
-                bool fluid = (xper || xio || (sir >= 0 && sir < nx)) && (yper || (sjr >= 0 && sjr < ny));
-                T out;
-                if (!fluid) {
-                    out = fpre[qb];
-                } else if (sk >= 0 && sk < nz) {
-                    T frac = g.lf[q];
-                    out = frac * src_value<S, T>(c, a.fin, n, a.inflow, si, sj, sk, q) + (T(1.0) - frac) * fpre[qb];
-                } else if (sk >= nz && c.top_boundary == LBM_TOP_FREE_SLIP) {
-                    out = src_value<S, T>(c, a.fin, n, a.inflow, si, sj, nz - 1, mirzq(q));
-                } else {
-                    out = fpre[q];
-                }
-                f[q] = out;
-            }
-            bool open0 = g.open_mask & 1u;
-            if (c.x_fringe && i < 5 && open0 && k > nz / 2) {
-                T sfr = T(1.0) - (T)i / T(4.0);
-                T kr = (T)k / (T)(nz - 1);
-                T strength = T(0.05) * (kr * kr - T(0.25));
-                T sigma = strength * smoothstep01(sfr);
-                T uxt = a.u_inlet[k] + T(0.0);
-#pragma unroll
-                for (int q = 0; q < 19; q++) f[q] = (T(1.0) - sigma) * f[q] + sigma * feq(c.rho0, uxt, T(0), T(0), q);
-            }
-            if (c.top_drive && k == nz - 1 && open0) {
-#pragma unroll
-                for (int q = 0; q < 19; q++) f[q] = feq(c.rho0, c.u_top_lb, T(0), T(0), q);
-            }
-            if (g.near) {
-#pragma unroll
-                for (int q = 1; q < 19; q++) f[q] = ibb_link(c, a.s0, g, i, j, q, f[q], fpre[oppq(q)]);
-            }
-        }
-        if (fast0) {
-            // fringe and top drive act on fully open cells as well
-            if (c.x_fringe && i < 5 && k > nz / 2) {
-                T sfr = T(1.0) - (T)i / T(4.0);
-                T kr = (T)k / (T)(nz - 1);
-                T strength = T(0.05) * (kr * kr - T(0.25));
-                T sigma = strength * smoothstep01(sfr);
-                T uxt = a.u_inlet[k] + T(0.0);
-#pragma unroll
-                for (int q = 0; q < 19; q++) f[q] = (T(1.0) - sigma) * f[q] + sigma * feq(c.rho0, uxt, T(0), T(0), q);
-            }
-            if (c.top_drive && k == nz - 1) {
-#pragma unroll
-                for (int q = 0; q < 19; q++) f[q] = feq(c.rho0, c.u_top_lb, T(0), T(0), q);
+    for (int q = 0; q < 19; q++) {
+        T freg = wq<T>(q) * inv2 * q_dot_pi<T>(q, Pxx, Pyy, Pzz, Pxy, Pxz, Pyz);
+        T fs = fe[q] + one_m * freg;
+        f[q] = fs;
+        if (fs < eps) {
+            T den = fs - fe[q];
+            if (den < T(0)) {
+                T cand = (eps - fe[q]) / den;
+                if (cand < alpha) alpha = cand;
             }
         }
     }
-    // update_bed_populations_and_reinit of step n+1
+    if (alpha < T(0)) alpha = T(0);
+    if (alpha > T(1)) alpha = T(1);
+    if (alpha != T(1.0)) {
+#pragma unroll
+        for (int q = 0; q < 19; q++) f[q] = fe[q] + alpha * (f[q] - fe[q]);
+    } else {
+        // alpha == 1: fe + 1 * (fs - fe) rounds back to fs unless the
+        // difference itself rounds; evaluate it for bit-identity
+#pragma unroll
+        for (int q = 0; q < 19; q++) f[q] = fe[q] + (f[q] - fe[q]);
+    }
+}
+
+// collide_KBC without backscatter, air mode (CODEX:3210-3243), in place
+template <class T>
+__device__ __forceinline__ void fused_kbc(T f[19], T rho, T ux, T uy, T uz, T om) {
+    const T eps = T(1e-6);
+    T u2 = ux * ux + uy * uy + uz * uz;
+    T u15 = T(1.5) * u2;
+    T fe[19];
+    T alpha = T(1.0);
+#pragma unroll
+    for (int q = 0; q < 19; q++) {
+        fe[q] = feq_h<T>(rho, ux, uy, uz, u15, q);
+        T fs = f[q] - om * (f[q] - fe[q]);
+        f[q] = fs;
+        if (fs < eps) {
+            T den = fs - fe[q];
+            if (den < T(0)) {
+                T cand = (eps - fe[q]) / den;
+                if (cand < alpha) alpha = cand;
+            }
+        }
+    }
+    if (alpha < T(0)) alpha = T(0);
+    if (alpha > T(1)) alpha = T(1);
+#pragma unroll
+    for (int q = 0; q < 19; q++) f[q] = fe[q] + alpha * (f[q] - fe[q]);
+}
+
+// open fraction and open flag of link q of cell (i,j,k) from the column bed
+// samples (CODEX:2573-2680), bed obstacle + x/y wall overrides
+template <class T>
+__device__ __forceinline__ T link_geo(const Consts<T>& c, const T* eta9, size_t col, size_t plane2, int i, int j, int k, int q,
+                                      int& lo, bool& wall) {
+    const T dx = c.dx;
+    T zc = ((T)k + T(0.5)) * dx;
+    T zq = zc + T(0.5) * dx * (T)czq(q);
+    T gap = zq - eta9[(size_t)c9q(q) * plane2 + col];
+    T ggap = c.include_bed ? gap : T(1.0e6);
+    T frac = T(0.5) + div_dx(c, ggap);
+    if (frac < T(0)) frac = T(0);
+    if (frac > T(1)) frac = T(1);
+    lo = ggap < T(0) ? 0 : 1;
+    wall = false;
+    if (q > 0) {
+        if ((c.x_boundary == LBM_XB_SOLID && ((i == 0 && cxq(q) > 0) || (i == c.nx - 1 && cxq(q) < 0))) ||
+            (c.y_boundary == LBM_YB_SOLID && ((j == 0 && cyq(q) > 0) || (j == c.ny - 1 && cyq(q) < 0)))) {
+            lo = 0;
+            frac = T(0);
+            wall = true;
+        }
+    }
+    return frac;
+}
+
+// open mask (bit q, q = 1..18) of a cell
+template <class T>
+__device__ __forceinline__ uint32_t link_mask(const Consts<T>& c, const T* eta9, size_t col, size_t plane2, int i, int j, int k) {
+    uint32_t m = 0;
+#pragma unroll
+    for (int q = 1; q < 19; q++) {
+        int lo;
+        bool w;
+        link_geo(c, eta9, col, plane2, i, j, k, q, lo, w);
+        m |= (uint32_t)lo << q;
+    }
+    return m;
+}
+
+// wall type of link q (bed unless an x/y wall override)
+template <class T>
+__device__ __forceinline__ int link_wt(const Consts<T>& c, bool wall, int i, int j, int q) {
+    if (!wall) return c.include_bed ? LBM_WALL_BED : LBM_WALL_NONE;
+    if (c.y_boundary == LBM_YB_SOLID && j == 0 && cyq(q) > 0) return LBM_WALL_YMIN;
+    if (c.y_boundary == LBM_YB_SOLID && j == c.ny - 1 && cyq(q) < 0) return LBM_WALL_YMAX;
+    return (i == 0 && cxq(q) > 0) ? LBM_WALL_XMIN : LBM_WALL_XMAX;
+}
+
+// stream (+ boundary rules) of step n and conditioning of step n+1 for one
+// cell; returns the populations entering the moments of step n+1
+template <class S, class T, bool BAND>
+__device__ __forceinline__ void fused_gather(const FusedArgs<S, T>& a, int i, int j, int k, T f[19], bool& open0_out) {
+    const Consts<T>& c = a.c;
+    const int nx = c.nx, ny = c.ny, nz = c.nz;
+    const uint32_t n = (uint32_t)nx * ny * nz;
+    const uint32_t plane = (uint32_t)nx * ny;
+    const uint32_t id = (uint32_t)k * plane + (uint32_t)j * nx + i;
+    const T inv = (T)c.store_inv_scale;
+    const size_t col = (size_t)j * nx + i;
+    const S* fin = a.fin;
+#define LD(q, idx) Store<S, T>::load(fin + (size_t)(q) * n, (uint32_t)(idx), (q), inv)
+    if (a.mode == 0) {
+#pragma unroll
+        for (int q = 0; q < 19; q++) f[q] = LD(q, id);
+    } else {
+        const bool xper = c.x_boundary == LBM_XB_PERIODIC, xio = c.x_boundary == LBM_XB_INFLOW_OUTFLOW;
+        const bool xsolid = c.x_boundary == LBM_XB_SOLID;
+        const bool yper = c.y_boundary == LBM_YB_PERIODIC;
+        // source columns / rows for c = -1, 0, +1 (index 1 - c)
+        const int im = xper ? (i == 0 ? nx - 1 : i - 1) : i - 1;
+        const int ip = xper ? (i == nx - 1 ? 0 : i + 1) : (xio ? (i == nx - 1 ? nx - 1 : i + 1) : i + 1);
+        const int jm = yper ? (j == 0 ? ny - 1 : j - 1) : j - 1;
+        const int jp = yper ? (j == ny - 1 ? 0 : j + 1) : j + 1;
+        const int cols[3] = {ip, i, im};  // source column for cx = -1, 0, +1
+        const int rows[3] = {jp, j, jm};
+        const bool xout[3] = {!xper && !xio && i == nx - 1, false, !xper && !xio && i == 0};
+        const bool yout[3] = {!yper && j == ny - 1, false, !yper && j == 0};
+        const bool inflow_col = xio && i == 0;
+        const bool slow = BAND && k <= a.s0.kband[col];
+        const size_t plane2 = (size_t)nx * ny;
+        uint32_t mask = 0x7fffeu;
+        if (slow) mask = link_mask(c, a.s0.eta9, col, plane2, i, j, k);
+        const bool any_open = (mask & 0x7fffeu) != 0;
+        f[0] = LD(0, id);
+#pragma unroll
+        for (int q = 1; q < 19; q++) {
+            const int cxi = cxq(q), cyi = cyq(q), czi = czq(q);
+            const int sk = k - czi;
+            const int si = cols[cxi + 1], sj = rows[cyi + 1];
+            T out;
+            if (xout[cxi + 1] || yout[cyi + 1]) {
+                out = LD(oppq(q), id);
+            } else if (sk >= 0 && sk < nz) {
+                T src;
+                if (inflow_col && cxi > 0)
+                    src = feq(c.rho0, a.inflow[(size_t)sk * ny + sj], T(0), T(0), q);
+                else
+                    src = LD(q, (uint32_t)sk * plane + (uint32_t)sj * nx + si);
+                if (slow) {
+                    int lo;
+                    bool w;
+                    T frac = any_open ? link_geo(c, a.s0.eta9, col, plane2, i, j, k, q, lo, w) : T(0);
+                    out = frac * src + (T(1.0) - frac) * LD(oppq(q), id);
+                } else {
+                    out = src;
+                }
+            } else if (sk >= nz && c.top_boundary == LBM_TOP_FREE_SLIP) {
+                if (inflow_col && cxi > 0)
+                    out = feq(c.rho0, a.inflow[(size_t)(nz - 1) * ny + sj], T(0), T(0), mirzq(q));
+                else
+                    out = LD(mirzq(q), (uint32_t)(nz - 1) * plane + (uint32_t)sj * nx + si);
+            } else {
+                out = LD(q, id);
+            }
+            f[q] = out;
+        }
+        const bool open0 = any_open;
+        // (A) x fringe (CODEX:3620-3644)
+        if (c.x_fringe && i < 5 && open0 && k > nz / 2) {
+            T sfr = T(1.0) - (T)i / T(4.0);
+            T kr = (T)k / (T)(nz - 1);
+            T strength = T(0.05) * (kr * kr - T(0.25));
+            T sigma = strength * smoothstep01(sfr);
+            T uxt = a.u_inlet[k] + T(0.0);
+#pragma unroll
+            for (int q = 0; q < 19; q++) f[q] = (T(1.0) - sigma) * f[q] + sigma * feq(c.rho0, uxt, T(0), T(0), q);
+        }
+        // (B) top drive (CODEX:3650-3662)
+        if (c.top_drive && k == nz - 1 && open0) {
+#pragma unroll
+            for (int q = 0; q < 19; q++) f[q] = feq(c.rho0, c.u_top_lb, T(0), T(0), q);
+        }
+        // (C) moving-wall IBB on near-obstacle cells (CODEX:3674-3712)
+        if (slow && (mask & 0x7fffeu) != 0x7fffeu) {
+#pragma unroll
+            for (int q = 1; q < 19; q++) {
+                int lo;
+                bool w;
+                T fo = link_geo(c, a.s0.eta9, col, plane2, i, j, k, q, lo, w);
+                if (!any_open) fo = T(0);
+                const int qb = oppq(q);
+                T ux = T(0), uy = T(0), uz = T(0);
+                if (link_wt(c, w, i, j, q) == LBM_WALL_BED) wall_velocity(c, a.s0, i, j, q, ux, uy, uz);
+                T cuw = (T)cxq(qb) * ux + (T)cyq(qb) * uy + (T)czq(qb) * uz;
+                T fw = LD(qb, id) - T(6.0) * wq<T>(qb) * c.rho0 * cuw;
+                f[q] = f[q] * fo + fw * (T(1.0) - fo);
+            }
+        }
+    }
+#undef LD
+    // update_bed_populations_and_reinit of step n+1 (CODEX:2894-2940)
     bool open0 = true;
-    if (k <= a.s1.kband[col]) {
-        CellGeo<T> g;
-        cell_geometry(c, a.s1, i, j, k, g);
-        open0 = g.open_mask & 1u;
-        if ((g.open_mask & 0x7ffffu) != 0x7ffffu) condition_cell(c, a.s1, g, i, j, f);
+    if (BAND && k <= a.s1.kband[col]) {
+        const size_t plane2 = (size_t)nx * ny;
+        uint32_t mask = link_mask(c, a.s1.eta9, col, plane2, i, j, k);
+        open0 = (mask & 0x7fffeu) != 0;
+        if ((mask & 0x7fffeu) != 0x7fffeu || !open0) {
+#pragma unroll
+            for (int q = 0; q < 19; q++) {
+                int lo;
+                bool w;
+                T frac = link_geo(c, a.s1.eta9, col, plane2, i, j, k, q, lo, w);
+                if (q == 0) lo = open0 ? 1 : 0;
+                else if (!open0) frac = T(0);
+                if (lo) continue;
+                int wt = q == 0 ? (c.include_bed ? LBM_WALL_BED : LBM_WALL_NONE) : link_wt(c, w, i, j, q);
+                T ux = T(0), uy = T(0), uz = T(0);
+                if (wt == LBM_WALL_BED) wall_velocity(c, a.s1, i, j, q, ux, uy, uz);
+                T fe = feq(c.rho0, ux, uy, uz, q);
+                f[q] = frac * f[q] + (T(1.0) - frac) * fe;
+            }
+        }
     }
     open0_out = open0;
 }
 
-template <class S, class T>
+template <class S, class T, bool BAND>
 __global__ __launch_bounds__(FT_THREADS) void k_fused(FusedArgs<S, T> a) {
     __shared__ T s_u[3][3][FL_Y * FL_X];  // [plane ring][component][cell]
     const Consts<T>& c = a.c;
     const int nx = c.nx, ny = c.ny, nz = c.nz;
-    const size_t n = (size_t)nx * ny * nz;
+    const uint32_t n = (uint32_t)nx * ny * nz;
     const int tid = threadIdx.x;
     const int i0 = blockIdx.x * FT_X, j0 = blockIdx.y * FT_Y;
-    const int k0 = blockIdx.z * a.kc;
+    const int k0 = (blockIdx.z + a.chunk0) * a.kc;
     const int k1 = min(k0 + a.kc, nz);
     int lx, ly;
-    bool interior = tid < FT_X * FT_Y, alive = true;
+    const bool interior = tid < FT_X * FT_Y;
+    bool alive = true;
     if (interior) {
         lx = tid % FT_X;
         ly = tid / FT_X;
@@ -1666,55 +1879,48 @@ __global__ __launch_bounds__(FT_THREADS) void k_fused(FusedArgs<S, T> a) {
 
     T fcur[19], fnext[19];
     T rcur = T(0), ucur = T(0), vcur = T(0), wcur = T(0);
-    bool open_cur = true, open_below = true;
+    bool open_cur = true, open_below = true, open_m = true;
+    S* const fout = a.fout;
 
-    auto compute_plane = [&](int kk, T* fdst, T& r, T& u, T& v, T& w, bool& open0) {
-        fused_gather<S, T>(a, gi, gj, kk, fdst, open0);
-        T rr = T(0);
-#pragma unroll
-        for (int q = 0; q < 19; q++) rr += fdst[q];
-        T jx = T(0), jy = T(0), jz = T(0);
-#pragma unroll
-        for (int q = 0; q < 19; q++) {
-            if (cxq(q)) jx += fdst[q] * (T)cxq(q);
-            if (cyq(q)) jy += fdst[q] * (T)cyq(q);
-            if (czq(q)) jz += fdst[q] * (T)czq(q);
-        }
-        T inv = T(1.0) / rr;
-        u = jx * inv;
-        v = jy * inv;
-        w = jz * inv;
-        if (c.top_drive && kk == nz - 1) {
-            u = c.u_top_lb;
-            v = T(0);
-            w = T(0);
-        }
-        r = rr;
-        int slot = kk % 3;
-        s_u[slot][0][lcell] = u;
-        s_u[slot][1][lcell] = v;
-        s_u[slot][2][lcell] = w;
-    };
-
-    // prologue: plane below the chunk (for the LES z difference) and plane k0
-    if (alive && k0 > 0) {
-        T r, u, v, w;
-        bool o0;
-        compute_plane(k0 - 1, fnext, r, u, v, w, o0);
-        open_below = o0;
-    }
-    if (alive) compute_plane(k0, fcur, rcur, ucur, vcur, wcur, open_cur);
-
-    for (int k = k0; k < k1; k++) {
+    // planes m = k0-1 .. k1: gather + moments of plane m, then LES +
+    // collision of plane m-1 (which needs the velocity of plane m)
+    const int m0 = k0 > 0 ? k0 - 1 : 0;
+    for (int m = m0; m <= k1; m++) {
         T rn = T(0), un = T(0), vn = T(0), wn = T(0);
-        bool open_next = true;
-        // plane k+1 (for the top plane of the chunk only its velocity is used)
-        if (alive && k + 1 < nz) compute_plane(k + 1, fnext, rn, un, vn, wn, open_next);
+        open_m = true;
+        if (alive && m < nz) {
+            fused_gather<S, T, BAND>(a, gi, gj, m, fnext, open_m);
+            T rr = T(0);
+#pragma unroll
+            for (int q = 0; q < 19; q++) rr += fnext[q];
+            T jx = T(0), jy = T(0), jz = T(0);
+#pragma unroll
+            for (int q = 0; q < 19; q++) {
+                if (cxq(q)) jx = (q == 1) ? fnext[q] : (cxq(q) > 0 ? jx + fnext[q] : jx - fnext[q]);
+                if (cyq(q)) jy = (q == 3) ? fnext[q] : (cyq(q) > 0 ? jy + fnext[q] : jy - fnext[q]);
+                if (czq(q)) jz = (q == 5) ? fnext[q] : (czq(q) > 0 ? jz + fnext[q] : jz - fnext[q]);
+            }
+            T inv = T(1.0) / rr;
+            un = jx * inv;
+            vn = jy * inv;
+            wn = jz * inv;
+            if (c.top_drive && m == nz - 1) {
+                un = c.u_top_lb;
+                vn = T(0);
+                wn = T(0);
+            }
+            rn = rr;
+            const int slot = m % 3;
+            s_u[slot][0][lcell] = un;
+            s_u[slot][1][lcell] = vn;
+            s_u[slot][2][lcell] = wn;
+        }
         __syncthreads();
-        if (interior) {
+        const int k = m - 1;
+        if (interior && k >= k0) {
             const int i = i0 + lx, j = j0 + ly;
-            const size_t id = ((size_t)k * ny + j) * nx + i;
-            // LES (CODEX:3011-3141)
+            const uint32_t id = ((uint32_t)k * ny + j) * nx + i;
+            // compute_LES (CODEX:3011-3141)
             T om = T(1.0) / c.tau0;
             if (c.use_les) {
                 if (open_cur) {
@@ -1722,7 +1928,7 @@ __global__ __launch_bounds__(FT_THREADS) void k_fused(FusedArgs<S, T> a) {
                     const int sp = (k + 1 < nz ? k + 1 : k) % 3;
                     const int sm = (k > 0 ? k - 1 : k) % 3;
                     const int cx_p = lcell + 1, cx_m = lcell - 1, cy_p = lcell + FL_X, cy_m = lcell - FL_X;
-                    bool near_wall = k > 0 && !open_below;
+                    const bool near_wall = k > 0 && !open_below;
                     T phi = T(0), uwx = T(0), uwy = T(0);
                     if (near_wall) {
                         phi = cell_phi(c, a.s1, i, j, k);
@@ -1743,16 +1949,18 @@ __global__ __launch_bounds__(FT_THREADS) void k_fused(FusedArgs<S, T> a) {
             }
             // collision (CODEX:3143-3416), air mode
             if (c.collision_reg) {
-                collide_regularized<T, false>(fcur, rcur, ucur, vcur, wcur, om, T(0));
+                fused_regularized(fcur, rcur, ucur, vcur, wcur, om);
             } else if (c.top_sponge && k > nz - 3) {
                 sponge_relax(fcur, rcur, ucur, vcur, wcur);
-            } else {
-                bool bs = c.c_backscatter > T(0) && open_cur;
-                float xi = bs ? bs_uniform(c.seed, a.step, (uint32_t)id) : 0.0f;
+            } else if (c.c_backscatter > T(0)) {
+                bool bs = open_cur;
+                float xi = bs ? bs_uniform(c.seed, a.step, id) : 0.0f;
                 collide_kbc<T, false>(fcur, rcur, ucur, vcur, wcur, om, T(0), bs, c.c_backscatter, xi);
+            } else {
+                fused_kbc(fcur, rcur, ucur, vcur, wcur, om);
             }
 #pragma unroll
-            for (int q = 0; q < 19; q++) Store<S, T>::store(a.fout, (size_t)q * n + id, q, fcur[q], c.store_scale);
+            for (int q = 0; q < 19; q++) Store<S, T>::store(fout + (size_t)q * n, id, q, fcur[q], c.store_scale);
             if (a.write_macro) {
                 a.rho[id] = rcur;
                 a.ux[id] = ucur;
@@ -1763,13 +1971,219 @@ __global__ __launch_bounds__(FT_THREADS) void k_fused(FusedArgs<S, T> a) {
         }
         __syncthreads();
         open_below = open_cur;
+        open_cur = open_m;
 #pragma unroll
         for (int q = 0; q < 19; q++) fcur[q] = fnext[q];
         rcur = rn;
         ucur = un;
         vcur = vn;
         wcur = wn;
-        open_cur = open_next;
+    }
+}
+
+// Lean fused kernel for z chunks above the wall band: every cell it touches
+// has all links open with fraction 1, so the stream is a plain pull with the
+// domain-boundary rules only.  The raw loads of plane m+1 are issued before
+// plane m is reduced and plane m-1 collided, so they are in flight while the
+// block computes (two-stage software pipeline).
+template <class S, class T>
+struct LeanSrc {
+    uint32_t idx[19];  // source cell of each population load
+};
+
+template <class S, class T>
+__device__ __forceinline__ void lean_issue(const FusedArgs<S, T>& a, int i, int j, int m, const int cols[3], const int rows[3],
+                                           T raw[19]) {
+    const Consts<T>& c = a.c;
+    const int nx = c.nx, ny = c.ny, nz = c.nz;
+    const uint32_t n = (uint32_t)nx * ny * nz;
+    const uint32_t plane = (uint32_t)nx * ny;
+    const uint32_t id = (uint32_t)m * plane + (uint32_t)j * nx + i;
+    const T inv = (T)c.store_inv_scale;
+    const S* fin = a.fin;
+    if (a.mode == 0) {
+#pragma unroll
+        for (int q = 0; q < 19; q++) raw[q] = Store<S, T>::load(fin + (size_t)q * n, id, q, inv);
+        return;
+    }
+    const bool top = m == nz - 1;
+    const bool free_slip = c.top_boundary == LBM_TOP_FREE_SLIP;
+#pragma unroll
+    for (int q = 0; q < 19; q++) {
+        const int cxi = cxq(q), cyi = cyq(q), czi = czq(q);
+        uint32_t src;
+        int pq = q;
+        if (top && czi < 0) {
+            if (free_slip) {
+                src = (uint32_t)(nz - 1) * plane + (uint32_t)rows[cyi + 1] * nx + cols[cxi + 1];
+                pq = mirzq(q);
+            } else {
+                src = id;
+            }
+        } else {
+            src = (uint32_t)(m - czi) * plane + (uint32_t)rows[cyi + 1] * nx + cols[cxi + 1];
+        }
+        raw[q] = Store<S, T>::load(fin + (size_t)pq * n, src, pq, inv);
+    }
+}
+
+template <class S, class T>
+__global__ __launch_bounds__(FT_THREADS) void k_fused_lean(FusedArgs<S, T> a) {
+    __shared__ T s_u[3][3][FL_Y * FL_X];
+    const Consts<T>& c = a.c;
+    const int nx = c.nx, ny = c.ny, nz = c.nz;
+    const uint32_t n = (uint32_t)nx * ny * nz;
+    const int tid = threadIdx.x;
+    const int i0 = blockIdx.x * FT_X, j0 = blockIdx.y * FT_Y;
+    const int k0 = (blockIdx.z + a.chunk0) * a.kc;
+    const int k1 = min(k0 + a.kc, nz);
+    int lx, ly;
+    const bool interior = tid < FT_X * FT_Y;
+    bool alive = true;
+    if (interior) {
+        lx = tid % FT_X;
+        ly = tid / FT_X;
+    } else {
+        int h = tid - FT_X * FT_Y;
+        if (h < FT_X) {
+            lx = h;
+            ly = -1;
+        } else if (h < 2 * FT_X) {
+            lx = h - FT_X;
+            ly = FT_Y;
+        } else if (h < 2 * FT_X + FT_Y) {
+            lx = -1;
+            ly = h - 2 * FT_X;
+        } else if (h < 2 * FT_X + 2 * FT_Y) {
+            lx = FT_X;
+            ly = h - 2 * FT_X - FT_Y;
+        } else {
+            lx = 0;
+            ly = 0;
+            alive = false;
+        }
+    }
+    const int gi = pmod(i0 + lx, nx), gj = pmod(j0 + ly, ny);
+    const int lcell = (ly + 1) * FL_X + (lx + 1);
+    const bool xper = c.x_boundary == LBM_XB_PERIODIC, xio = c.x_boundary == LBM_XB_INFLOW_OUTFLOW;
+    const bool yper = c.y_boundary == LBM_YB_PERIODIC;
+    // source column / row for c = -1, 0, +1 (index 1 + c); the outflow ghost
+    // column is a copy of the last interior column
+    const int cols[3] = {xper ? (gi == nx - 1 ? 0 : gi + 1) : (gi == nx - 1 ? nx - 1 : gi + 1), gi,
+                         xper ? (gi == 0 ? nx - 1 : gi - 1) : (gi == 0 ? 0 : gi - 1)};
+    const int rows[3] = {yper ? (gj == ny - 1 ? 0 : gj + 1) : gj + 1, gj, yper ? (gj == 0 ? ny - 1 : gj - 1) : gj - 1};
+    const bool inflow_col = xio && gi == 0;
+
+    T raw[19], fcur[19], fnext[19];
+    T rcur = T(0), ucur = T(0), vcur = T(0), wcur = T(0);
+    const int m0 = k0 > 0 ? k0 - 1 : 0;
+    if (alive) lean_issue<S, T>(a, gi, gj, m0, cols, rows, raw);
+    for (int m = m0; m <= k1; m++) {
+        T rn = T(0), un = T(0), vn = T(0), wn = T(0);
+        if (alive && m < nz) {
+#pragma unroll
+            for (int q = 0; q < 19; q++) fnext[q] = raw[q];
+            if (m + 1 <= k1 && m + 1 < nz) lean_issue<S, T>(a, gi, gj, m + 1, cols, rows, raw);
+            if (a.mode != 0 && inflow_col) {
+                // inflow ghost: feq(rho0, u_in) of the source row/plane
+#pragma unroll
+                for (int q = 1; q < 19; q++) {
+                    if (cxq(q) <= 0) continue;
+                    int sk = m - czq(q);
+                    int pq = q;
+                    if (m == nz - 1 && czq(q) < 0) {
+                        if (c.top_boundary != LBM_TOP_FREE_SLIP) continue;
+                        sk = nz - 1;
+                        pq = mirzq(q);
+                    }
+                    fnext[q] = feq(c.rho0, a.inflow[(size_t)sk * ny + rows[cyq(q) + 1]], T(0), T(0), pq);
+                }
+            }
+            if (a.mode != 0) {
+                if (c.x_fringe && gi < 5 && m > nz / 2) {
+                    T sfr = T(1.0) - (T)gi / T(4.0);
+                    T kr = (T)m / (T)(nz - 1);
+                    T strength = T(0.05) * (kr * kr - T(0.25));
+                    T sigma = strength * smoothstep01(sfr);
+                    T uxt = a.u_inlet[m] + T(0.0);
+#pragma unroll
+                    for (int q = 0; q < 19; q++) fnext[q] = (T(1.0) - sigma) * fnext[q] + sigma * feq(c.rho0, uxt, T(0), T(0), q);
+                }
+                if (c.top_drive && m == nz - 1) {
+#pragma unroll
+                    for (int q = 0; q < 19; q++) fnext[q] = feq(c.rho0, c.u_top_lb, T(0), T(0), q);
+                }
+            }
+            T rr = T(0);
+#pragma unroll
+            for (int q = 0; q < 19; q++) rr += fnext[q];
+            T jx = T(0), jy = T(0), jz = T(0);
+#pragma unroll
+            for (int q = 0; q < 19; q++) {
+                if (cxq(q)) jx = (q == 1) ? fnext[q] : (cxq(q) > 0 ? jx + fnext[q] : jx - fnext[q]);
+                if (cyq(q)) jy = (q == 3) ? fnext[q] : (cyq(q) > 0 ? jy + fnext[q] : jy - fnext[q]);
+                if (czq(q)) jz = (q == 5) ? fnext[q] : (czq(q) > 0 ? jz + fnext[q] : jz - fnext[q]);
+            }
+            T inv = T(1.0) / rr;
+            un = jx * inv;
+            vn = jy * inv;
+            wn = jz * inv;
+            if (c.top_drive && m == nz - 1) {
+                un = c.u_top_lb;
+                vn = T(0);
+                wn = T(0);
+            }
+            rn = rr;
+            const int slot = m % 3;
+            s_u[slot][0][lcell] = un;
+            s_u[slot][1][lcell] = vn;
+            s_u[slot][2][lcell] = wn;
+        }
+        __syncthreads();
+        const int k = m - 1;
+        if (interior && k >= k0) {
+            const int i = i0 + lx, j = j0 + ly;
+            const uint32_t id = ((uint32_t)k * ny + j) * nx + i;
+            T om = T(1.0) / c.tau0;
+            if (c.use_les) {
+                const int sk = k % 3;
+                const int sp = (k + 1 < nz ? k + 1 : k) % 3;
+                const int sm = (k > 0 ? k - 1 : k) % 3;
+                const int cx_p = lcell + 1, cx_m = lcell - 1, cy_p = lcell + FL_X, cy_m = lcell - FL_X;
+                om = les_omega(c, s_u[sk][0][cx_p], s_u[sk][0][cx_m], s_u[sk][0][cy_p], s_u[sk][0][cy_m], s_u[sp][0][lcell],
+                               s_u[sm][0][lcell], s_u[sk][1][cx_p], s_u[sk][1][cx_m], s_u[sk][1][cy_p], s_u[sk][1][cy_m],
+                               s_u[sp][1][lcell], s_u[sm][1][lcell], s_u[sk][2][cx_p], s_u[sk][2][cx_m], s_u[sk][2][cy_p],
+                               s_u[sk][2][cy_m], s_u[sp][2][lcell], s_u[sm][2][lcell], false, T(0), ucur, vcur, T(0), T(0),
+                               (T*)nullptr, (T*)nullptr);
+            }
+            if (c.collision_reg) {
+                fused_regularized(fcur, rcur, ucur, vcur, wcur, om);
+            } else if (c.top_sponge && k > nz - 3) {
+                sponge_relax(fcur, rcur, ucur, vcur, wcur);
+            } else if (c.c_backscatter > T(0)) {
+                float xi = bs_uniform(c.seed, a.step, id);
+                collide_kbc<T, false>(fcur, rcur, ucur, vcur, wcur, om, T(0), true, c.c_backscatter, xi);
+            } else {
+                fused_kbc(fcur, rcur, ucur, vcur, wcur, om);
+            }
+            S* const fout = a.fout;
+#pragma unroll
+            for (int q = 0; q < 19; q++) Store<S, T>::store(fout + (size_t)q * n, id, q, fcur[q], c.store_scale);
+            if (a.write_macro) {
+                a.rho[id] = rcur;
+                a.ux[id] = ucur;
+                a.uy[id] = vcur;
+                a.uz[id] = wcur;
+                a.omega[id] = om;
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < 19; q++) fcur[q] = fnext[q];
+        rcur = rn;
+        ucur = un;
+        vcur = vn;
+        wcur = wn;
     }
 }
 
@@ -1777,7 +2191,7 @@ __global__ __launch_bounds__(FT_THREADS) void k_fused(FusedArgs<S, T> a) {
 // reductions (FP64)
 // ===========================================================================
 template <class S, class T>
-__global__ void k_reduce(Consts<T> c, const S* f, const T* ux, const T* uy, const T* uz, const T* mass, const T* fill,
+__global__ __launch_bounds__(NT) void k_reduce(Consts<T> c, const S* f, const T* ux, const T* uy, const T* uz, const T* mass, const T* fill,
                          const uint8_t* flags, const int* type, int what, double* out) {
     size_t id = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     size_t n = (size_t)c.nx * c.ny * c.nz;
@@ -1819,7 +2233,7 @@ __global__ void k_reduce(Consts<T> c, const S* f, const T* ux, const T* uy, cons
 }
 
 template <class S, class T>
-__global__ void k_store_populations(Consts<T> c, const T* src, S* dst) {
+__global__ __launch_bounds__(NT) void k_store_populations(Consts<T> c, const T* src, S* dst) {
     size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     size_t n = (size_t)c.nx * c.ny * c.nz;
     if (idx >= 19 * n) return;
@@ -1828,7 +2242,7 @@ __global__ void k_store_populations(Consts<T> c, const T* src, S* dst) {
 }
 
 template <class S, class T>
-__global__ void k_load_populations(Consts<T> c, const S* src, T* dst) {
+__global__ __launch_bounds__(NT) void k_load_populations(Consts<T> c, const S* src, T* dst) {
     size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     size_t n = (size_t)c.nx * c.ny * c.nz;
     if (idx >= 19 * n) return;
@@ -1850,7 +2264,8 @@ static void bed2d_t(const lbm_config_t& cfg, DevBufs& b, float t, int slot, hipS
     int prev = slot ^ 1;
     k_bed2d_a<T><<<nblocks(n2), NT, 0, s>>>(c, (T)t, (T*)b.h[slot], (T*)b.uw[slot], (T*)b.vw[slot], (T*)b.ww[slot],
                                              (const T*)b.h[prev], (const T*)b.piecewise);
-    k_bed2d_b<T><<<nblocks(n2), NT, 0, s>>>(c, (const T*)b.h[slot], (T*)b.eta9[slot], (T*)b.denom[slot], b.kband[slot]);
+    k_bed2d_b<T><<<nblocks(n2), NT, 0, s>>>(c, (const T*)b.h[slot], (T*)b.eta9[slot], (T*)b.denom[slot], b.kband[slot],
+                                             (const T*)b.uw[slot], (const T*)b.vw[slot], (const T*)b.ww[slot], (T*)b.uw9[slot]);
     HIP_OK(hipGetLastError());
 }
 void launch_bed2d(const lbm_config_t& c, DevBufs& b, float t, int slot, hipStream_t s) {
@@ -2053,7 +2468,7 @@ void launch_staged_step(const lbm_config_t& c, DevBufs& b, int& cur, int slot, i
 
 template <class S, class T>
 static void fused_t(const lbm_config_t& cfg, DevBufs& b, int& cur, int mode, int slot_prev, int slot, int64_t step, int write_macro,
-                    int omega_band_top, int kchunk, hipStream_t s) {
+                    int omega_band_top, int band_top, int kchunk, hipStream_t s) {
     FusedArgs<S, T> a;
     a.c = make_consts<T>(cfg, omega_band_top);
     a.s0 = make_slot<T>(b, slot_prev);
@@ -2071,19 +2486,30 @@ static void fused_t(const lbm_config_t& cfg, DevBufs& b, int& cur, int mode, int
     a.write_macro = write_macro;
     a.kc = kchunk;
     a.step = (uint32_t)step;
-    dim3 grid(cfg.nx / FT_X, cfg.ny / FT_Y, (cfg.nz + kchunk - 1) / kchunk);
-    k_fused<S, T><<<grid, FT_THREADS, 0, s>>>(a);
+    // chunks whose planes k0-1 .. k1 reach the wall band run the general
+    // kernel; the others cannot meet a cut or closed link and run the lean one
+    const int nchunks = (cfg.nz + kchunk - 1) / kchunk;
+    int nband = nchunks;
+    if (band_top < cfg.nz - 1) nband = std::min(nchunks, (band_top + 1) / kchunk + 1);
+    if (nband > 0) {
+        a.chunk0 = 0;
+        k_fused<S, T, true><<<dim3(cfg.nx / FT_X, cfg.ny / FT_Y, nband), FT_THREADS, 0, s>>>(a);
+    }
+    if (nchunks > nband) {
+        a.chunk0 = nband;
+        k_fused_lean<S, T><<<dim3(cfg.nx / FT_X, cfg.ny / FT_Y, nchunks - nband), FT_THREADS, 0, s>>>(a);
+    }
     HIP_OK(hipGetLastError());
     cur ^= 1;
 }
 void launch_fused_step(const lbm_config_t& c, DevBufs& b, int& cur, int mode, int slot_prev, int slot, int64_t step,
-                       int write_macro, int omega_band_top, int kchunk, hipStream_t s) {
+                       int write_macro, int omega_band_top, int band_top, int kchunk, hipStream_t s) {
     if (!fused_supported(c)) throw std::runtime_error("fused path: needs air mode, nx % 32 == 0 and ny % 16 == 0");
     switch (b.storage) {
-        case LBM_STORE_FP64: fused_t<double, double>(c, b, cur, mode, slot_prev, slot, step, write_macro, omega_band_top, kchunk, s); break;
-        case LBM_STORE_FP16: fused_t<__half, float>(c, b, cur, mode, slot_prev, slot, step, write_macro, omega_band_top, kchunk, s); break;
-        case LBM_STORE_BF16: fused_t<bf16_t, float>(c, b, cur, mode, slot_prev, slot, step, write_macro, omega_band_top, kchunk, s); break;
-        default: fused_t<float, float>(c, b, cur, mode, slot_prev, slot, step, write_macro, omega_band_top, kchunk, s); break;
+        case LBM_STORE_FP64: fused_t<double, double>(c, b, cur, mode, slot_prev, slot, step, write_macro, omega_band_top, band_top, kchunk, s); break;
+        case LBM_STORE_FP16: fused_t<__half, float>(c, b, cur, mode, slot_prev, slot, step, write_macro, omega_band_top, band_top, kchunk, s); break;
+        case LBM_STORE_BF16: fused_t<bf16_t, float>(c, b, cur, mode, slot_prev, slot, step, write_macro, omega_band_top, band_top, kchunk, s); break;
+        default: fused_t<float, float>(c, b, cur, mode, slot_prev, slot, step, write_macro, omega_band_top, band_top, kchunk, s); break;
     }
 }
 
@@ -2155,7 +2581,7 @@ void launch_cut_link_count(const lbm_config_t& c, DevBufs& b, int slot, hipStrea
 }
 
 template <class T>
-__global__ void k_obstacle_from_flags(size_t n, const uint8_t* flags, int* obst) {
+__global__ __launch_bounds__(NT) void k_obstacle_from_flags(size_t n, const uint8_t* flags, int* obst) {
     size_t id = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (id < n) obst[id] = (flags[id] & 1u) ? 0 : 1;
 }

@@ -22,7 +22,7 @@
 extern "C" {
 #endif
 
-#define LBM_CONFIG_VERSION 3
+#define LBM_CONFIG_VERSION 4
 
 /* ---- error codes ---------------------------------------------------- */
 #define LBM_OK 0
@@ -58,9 +58,14 @@ extern "C" {
 #define LBM_BED_MOVING_SINE1 3
 /* storage of the populations in HBM (new) */
 #define LBM_STORE_FP32 0
-#define LBM_STORE_FP16 1 /* (f_i - w_i) * 2^shift as IEEE half */
-#define LBM_STORE_BF16 2 /* (f_i - w_i) * 2^shift as bfloat16   */
+#define LBM_STORE_FP16 1 /* (f_i - s_i) * 2^shift as IEEE half */
+#define LBM_STORE_BF16 2 /* (f_i - s_i) * 2^shift as bfloat16   */
 #define LBM_STORE_FP64 3 /* FP64 storage and FP64 arithmetic    */
+/* Reduced storage: s_i = feq_i(1, storage_u_ref) (= w_i when the reference
+ * velocity is zero), rounded into the 16-bit format either to nearest even
+ * or stochastically with a counter-based dither of (cell, step, i). */
+#define LBM_ROUND_NEAREST 0
+#define LBM_ROUND_STOCHASTIC 1
 /* kernel path */
 #define LBM_PATH_AUTO 0
 #define LBM_PATH_STAGED 1 /* one kernel per reference stage */
@@ -105,7 +110,7 @@ typedef struct lbm_config {
     int32_t x_boundary, y_boundary, top_boundary;
     int32_t bed_profile, include_bed, wall_velocity_link;
     int32_t enable_top_drive, enable_x_fringe, enable_top_sponge;
-    int32_t storage, storage_shift, kernel_path;
+    int32_t storage, storage_shift, storage_rounding, kernel_path;
 
     /* water / free surface */
     int32_t fs_enabled, fs_tracking, pressure_formulation, fs_boundary_mode;
@@ -126,6 +131,7 @@ typedef struct lbm_config {
     float vof_empty_fill_threshold, vof_orphan_empty_threshold;
     float vof_detached_residual_fill_threshold, vof_thin_gap_bridge_strength;
     float vof_collapse_max_volume_m3;
+    float storage_u_ref[3]; /* reference velocity of the FP16/BF16 shift */
     float pad0;
     double dt_phys; /* per-step time, the host forms t=(step+1)*dt_phys */
 } lbm_config_t;

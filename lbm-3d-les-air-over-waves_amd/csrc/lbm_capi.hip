@@ -76,6 +76,11 @@ static const char* validate(const lbm_config_t* c) {
     if ((size_t)c->nx * c->ny * c->nz > (size_t)1 << 31) return "grid larger than 2^31 cells";
     if (c->storage < 0 || c->storage > 3) return "unknown storage";
     if (c->storage_shift < 0 || c->storage_shift > 24) return "storage_shift must be in [0, 24]";
+    if (c->storage_rounding != LBM_ROUND_NEAREST && c->storage_rounding != LBM_ROUND_STOCHASTIC) return "unknown storage_rounding";
+    for (int a = 0; a < 3; a++)
+        if (!(fabsf(c->storage_u_ref[a]) <= 0.3f)) return "storage_u_ref components must lie in [-0.3, 0.3]";
+    if ((c->storage == LBM_STORE_FP16 || c->storage == LBM_STORE_BF16) && c->kernel_path == LBM_PATH_STAGED)
+        return "fp16/bf16 storage runs on the fused path only";
     if (c->physics_mode == LBM_PHYS_WATER) {
         if (!c->fs_enabled) return "water mode without a free surface is not supported";
         if (c->fs_tracking != LBM_FST_VOF) return "height-function tracking is not supported";
@@ -448,7 +453,7 @@ int lbm_upload_field(lbm_handle_t* h, int32_t field, const void* src, size_t byt
         } else {
             ensure_ftmp(h);
             CHECK(hipMemcpyAsync(b.ftmp, src, need, hipMemcpyHostToDevice, h->stream));
-            launch_store_populations(c, b, h->cur, b.ftmp, h->stream);
+            launch_store_populations(c, b, h->cur, b.ftmp, h->step, h->stream);
         }
         h->post_stream = 1;
         h->initialized = 1;

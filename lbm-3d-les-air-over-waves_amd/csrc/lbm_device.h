@@ -45,28 +45,68 @@ struct bf16_t {
     uint16_t v;
 };
 
+// Reduced-precision population codec (new).  A population is stored as
+//   h_q = round((f_q - s_q) * 2^k),   f_q = s_q + h_q * 2^-k,
+// with s_q = feq_q(1, u_ref) (s_q = w_q for u_ref = 0).  round() is either
+// IEEE round-to-nearest-even or stochastic rounding driven by a counter-based
+// dither of (cell, step, q): the 32-bit word r is added to the magnitude's
+// bits below the kept precision before truncation, so a value is rounded up
+// with probability equal to its distance from the lower neighbour.  The
+// stochastic form removes the per-step rounding bias that otherwise
+// accumulates into a momentum drift.
+struct Codec {
+    float scale, inv_scale;  // 2^k and 2^-k
+    float shift[19];         // s_q
+    int stochastic;          // LBM_ROUND_STOCHASTIC
+};
+
+__host__ __device__ inline uint32_t fmix32_hd(uint32_t h) {
+    h ^= h >> 16;
+    h *= 0x85EBCA6Bu;
+    h ^= h >> 13;
+    h *= 0xC2B2AE35u;
+    h ^= h >> 16;
+    return h;
+}
+// per-cell dither base; population q uses base + q * 0x9E3779B9 (a Weyl
+// sequence over q, so the 19 populations of a cell draw well-spread values)
+__host__ __device__ inline uint32_t dither_base(uint32_t step, uint32_t cell) {
+    return fmix32_hd(cell * 0x9E3779B1u ^ step * 0x85EBCA77u);
+}
+__host__ __device__ constexpr uint32_t dither_q(uint32_t base, int q) { return base + (uint32_t)q * 0x9E3779B9u; }
+
 template <class S, class T>
 struct Store;
 template <class T>
 struct Store<T, T> {  // full-precision storage: the population itself
     template <class I>
-    static __device__ __forceinline__ T load(const T* p, I i, int q, T) {
+    static __device__ __forceinline__ T load(const T* p, I i, int, const Codec&) {
         return p[i];
     }
     template <class I>
-    static __device__ __forceinline__ void store(T* p, I i, int q, T v, T) {
+    static __device__ __forceinline__ void store(T* p, I i, int, T v, const Codec&, uint32_t) {
         p[i] = v;
     }
 };
 template <>
-struct Store<__half, float> {  // (f_q - w_q) * 2^k as IEEE half
+struct Store<__half, float> {  // (f_q - s_q) * 2^k as IEEE half
     template <class I>
-    static __device__ __forceinline__ float load(const __half* p, I i, int q, float inv_scale) {
-        return wq<float>(q) + __half2float(p[i]) * inv_scale;
+    static __device__ __forceinline__ float load(const __half* p, I i, int q, const Codec& k) {
+        return k.shift[q] + __half2float(p[i]) * k.inv_scale;
     }
     template <class I>
-    static __device__ __forceinline__ void store(__half* p, I i, int q, float v, float scale) {
-        p[i] = __float2half_rn((v - wq<float>(q)) * scale);
+    static __device__ __forceinline__ void store(__half* p, I i, int q, float v, const Codec& k, uint32_t base) {
+        float d = (v - k.shift[q]) * k.scale;
+        if (k.stochastic) {
+            // keep 10 fraction bits: add 13 dither bits below them, truncate;
+            // the result is a half value (or rounds once more, to nearest, in
+            // half's subnormal range)
+            uint32_t u = __float_as_uint(d);
+            uint32_t a = u & 0x7fffffffu;
+            if (a < 0x7f800000u) a = (a + (dither_q(base, q) >> 19)) & ~0x1fffu;
+            d = __uint_as_float(a | (u & 0x80000000u));
+        }
+        p[i] = __float2half_rn(d);
     }
 };
 __device__ __forceinline__ float bf16_to_float(bf16_t b) { return __uint_as_float(((uint32_t)b.v) << 16); }
@@ -81,15 +121,29 @@ __device__ __forceinline__ bf16_t float_to_bf16(float f) {
     r.v = (uint16_t)(u >> 16);
     return r;
 }
+__device__ __forceinline__ bf16_t float_to_bf16_sr(float f, uint32_t r) {
+    uint32_t u = __float_as_uint(f);
+    bf16_t b;
+    if ((u & 0x7fffffffu) > 0x7f800000u) {
+        b.v = (uint16_t)((u >> 16) | 0x40u);
+        return b;
+    }
+    uint32_t a = u & 0x7fffffffu;
+    if (a < 0x7f800000u) a = a + (r >> 16);  // may carry into the exponent: rounds up a binade
+    if (a > 0x7f800000u) a = 0x7f800000u;
+    b.v = (uint16_t)(((a | (u & 0x80000000u)) >> 16));
+    return b;
+}
 template <>
-struct Store<bf16_t, float> {  // (f_q - w_q) * 2^k as bfloat16
+struct Store<bf16_t, float> {  // (f_q - s_q) * 2^k as bfloat16
     template <class I>
-    static __device__ __forceinline__ float load(const bf16_t* p, I i, int q, float inv_scale) {
-        return wq<float>(q) + bf16_to_float(p[i]) * inv_scale;
+    static __device__ __forceinline__ float load(const bf16_t* p, I i, int q, const Codec& k) {
+        return k.shift[q] + bf16_to_float(p[i]) * k.inv_scale;
     }
     template <class I>
-    static __device__ __forceinline__ void store(bf16_t* p, I i, int q, float v, float scale) {
-        p[i] = float_to_bf16((v - wq<float>(q)) * scale);
+    static __device__ __forceinline__ void store(bf16_t* p, I i, int q, float v, const Codec& k, uint32_t base) {
+        float d = (v - k.shift[q]) * k.scale;
+        p[i] = k.stochastic ? float_to_bf16_sr(d, dither_q(base, q)) : float_to_bf16(d);
     }
 };
 
@@ -226,7 +280,8 @@ struct Consts {
     T dt_phys;
     T inv_dx;     // 1/dx, used in place of a division by dx when dx is a power of two (exact)
     int dx_pow2;
-    float store_scale, store_inv_scale;  // 2^k and 2^-k of reduced-precision storage
+    Codec store;          // reduced-precision population codec
+    uint32_t store_step;  // step index of the dither for stores outside the step kernels
 };
 
 // x / dx, exactly: a product with 1/dx when dx is a power of two
@@ -396,16 +451,8 @@ __device__ __forceinline__ T ibb_link(const Consts<T>& c, const Slot<T>& s, cons
 }
 
 // counter-based uniform [0,1) for backscatter (same hash as the oracle)
-__device__ __forceinline__ uint32_t fmix32(uint32_t h) {
-    h ^= h >> 16;
-    h *= 0x85EBCA6Bu;
-    h ^= h >> 13;
-    h *= 0xC2B2AE35u;
-    h ^= h >> 16;
-    return h;
-}
 __device__ __forceinline__ float bs_uniform(uint32_t seed, uint32_t step, uint32_t cell) {
-    uint32_t h = fmix32(seed ^ fmix32(cell + fmix32(step * 0x9E3779B9u)));
+    uint32_t h = fmix32_hd(seed ^ fmix32_hd(cell + fmix32_hd(step * 0x9E3779B9u)));
     return (float)(h >> 8) * (1.0f / 16777216.0f);
 }
 

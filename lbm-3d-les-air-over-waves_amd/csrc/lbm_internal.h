@@ -76,7 +76,7 @@ void launch_fused_step(const lbm_config_t& c, DevBufs& b, int& cur, int mode, in
 // post-collision state -> post-stream populations (arithmetic type) in dst
 void launch_materialize(const lbm_config_t& c, DevBufs& b, int cur, int slot, void* dst, hipStream_t s);
 // convert arithmetic-type populations in src into the storage buffer f[cur]
-void launch_store_populations(const lbm_config_t& c, DevBufs& b, int cur, const void* src, hipStream_t s);
+void launch_store_populations(const lbm_config_t& c, DevBufs& b, int cur, const void* src, int64_t step, hipStream_t s);
 void launch_reduce(const lbm_config_t& c, DevBufs& b, const void* f, int f_is_storage, int what, hipStream_t s);
 void launch_cut_link_count(const lbm_config_t& c, DevBufs& b, int slot, hipStream_t s);
 bool fused_supported(const lbm_config_t& c);

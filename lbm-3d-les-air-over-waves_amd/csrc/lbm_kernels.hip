@@ -92,8 +92,19 @@ static Consts<T> make_consts(const lbm_config_t& c, int omega_band_top) {
     }
     float sc = 1.0f;
     for (int i = 0; i < c.storage_shift; i++) sc *= 2.0f;
-    k.store_scale = sc;
-    k.store_inv_scale = 1.0f / sc;
+    k.store.scale = sc;
+    k.store.inv_scale = 1.0f / sc;
+    k.store.stochastic = c.storage_rounding == LBM_ROUND_STOCHASTIC ? 1 : 0;
+    {
+        // s_q = feq_q(1, u_ref) in float, the order of the equilibrium (CODEX:3326-3333)
+        const float ux = c.storage_u_ref[0], uy = c.storage_u_ref[1], uz = c.storage_u_ref[2];
+        const float u2 = ux * ux + uy * uy + uz * uz;
+        for (int q = 0; q < 19; q++) {
+            float cu = 3.0f * ((float)cxq(q) * ux + (float)cyq(q) * uy + (float)czq(q) * uz);
+            k.store.shift[q] = wq<float>(q) * (1.0f + cu + 0.5f * cu * cu - 1.5f * u2);
+        }
+    }
+    k.store_step = 0;
     return k;
 }
 
@@ -312,11 +323,12 @@ __global__ __launch_bounds__(NT) void k_init_populations(Consts<T> c, Slot<T> s,
     if (c.physics_water && open0 && ty != LBM_FS_GAS) w += T(0.5) * local_g(c, k, open0, ty);
     T r = rho[id];
     T u2 = u * u + v * v + w * w;
+    const uint32_t dbase = dither_base(c.store_step, (uint32_t)id);
 #pragma unroll
     for (int q = 0; q < 19; q++) {
         T cu = T(3.0) * ((T)cxq(q) * u + (T)cyq(q) * v + (T)czq(q) * w);
         T fv = r * wq<T>(q) * (T(1.0) + cu + T(0.5) * cu * cu - T(1.5) * u2);
-        Store<S, T>::store(f, (size_t)q * n + id, q, fv, c.store_scale);
+        Store<S, T>::store(f, (size_t)q * n + id, q, fv, c.store, dbase);
     }
 }
 
@@ -516,7 +528,7 @@ __device__ __forceinline__ T src_value(const Consts<T>& c, const S* f, size_t n,
     if (si < 0) return feq(c.rho0, inflow[(size_t)sk * c.ny + sj], T(0), T(0), q);
     if (si >= c.nx) si = c.nx - 1;
     size_t id = ((size_t)sk * c.ny + sj) * c.nx + si;
-    return Store<S, T>::load(f, (size_t)q * n + id, q, (T)c.store_inv_scale);
+    return Store<S, T>::load(f, (size_t)q * n + id, q, c.store);
 }
 
 // stream + apply_x_solid_wall_bounceback + apply_open_boundary_conditions
@@ -534,7 +546,7 @@ __global__ __launch_bounds__(NT) void k_stream_bc(Consts<T> c, Slot<T> s, const 
     const bool water = c.physics_water;
     const bool xper = c.x_boundary == LBM_XB_PERIODIC, xio = c.x_boundary == LBM_XB_INFLOW_OUTFLOW;
     const bool yper = c.y_boundary == LBM_YB_PERIODIC;
-    const T inv = (T)c.store_inv_scale;
+    const Codec& inv = c.store;
     CellGeo<T> g;
     cell_geometry(c, s, i, j, k, g);
     bool open0 = g.open_mask & 1u;
@@ -1717,7 +1729,7 @@ __device__ __forceinline__ void fused_gather(const FusedArgs<S, T>& a, int i, in
     const uint32_t n = (uint32_t)nx * ny * nz;
     const uint32_t plane = (uint32_t)nx * ny;
     const uint32_t id = (uint32_t)k * plane + (uint32_t)j * nx + i;
-    const T inv = (T)c.store_inv_scale;
+    const Codec& inv = c.store;
     const size_t col = (size_t)j * nx + i;
     const S* fin = a.fin;
 #define LD(q, idx) Store<S, T>::load(fin + (size_t)(q) * n, (uint32_t)(idx), (q), inv)
@@ -1959,8 +1971,11 @@ __global__ __launch_bounds__(FT_THREADS) void k_fused(FusedArgs<S, T> a) {
             } else {
                 fused_kbc(fcur, rcur, ucur, vcur, wcur, om);
             }
+            {
+                const uint32_t dbase = c.store.stochastic ? dither_base(a.step, (uint32_t)id) : 0u;
 #pragma unroll
-            for (int q = 0; q < 19; q++) Store<S, T>::store(fout + (size_t)q * n, id, q, fcur[q], c.store_scale);
+                for (int q = 0; q < 19; q++) Store<S, T>::store(fout + (size_t)q * n, id, q, fcur[q], c.store, dbase);
+            }
             if (a.write_macro) {
                 a.rho[id] = rcur;
                 a.ux[id] = ucur;
@@ -1999,7 +2014,7 @@ __device__ __forceinline__ void lean_issue(const FusedArgs<S, T>& a, int i, int 
     const uint32_t n = (uint32_t)nx * ny * nz;
     const uint32_t plane = (uint32_t)nx * ny;
     const uint32_t id = (uint32_t)m * plane + (uint32_t)j * nx + i;
-    const T inv = (T)c.store_inv_scale;
+    const Codec& inv = c.store;
     const S* fin = a.fin;
     if (a.mode == 0) {
 #pragma unroll
@@ -2167,8 +2182,11 @@ __global__ __launch_bounds__(FT_THREADS) void k_fused_lean(FusedArgs<S, T> a) {
                 fused_kbc(fcur, rcur, ucur, vcur, wcur, om);
             }
             S* const fout = a.fout;
+            {
+                const uint32_t dbase = c.store.stochastic ? dither_base(a.step, (uint32_t)id) : 0u;
 #pragma unroll
-            for (int q = 0; q < 19; q++) Store<S, T>::store(fout + (size_t)q * n, id, q, fcur[q], c.store_scale);
+                for (int q = 0; q < 19; q++) Store<S, T>::store(fout + (size_t)q * n, id, q, fcur[q], c.store, dbase);
+            }
             if (a.write_macro) {
                 a.rho[id] = rcur;
                 a.ux[id] = ucur;
@@ -2199,7 +2217,7 @@ __global__ __launch_bounds__(NT) void k_reduce(Consts<T> c, const S* f, const T*
     if (id < n) {
         if (what == LBM_REDUCE_MASS) {
             if (flags[id] & 1u)
-                for (int q = 0; q < 19; q++) v += (double)Store<S, T>::load(f, (size_t)q * n + id, q, (T)c.store_inv_scale);
+                for (int q = 0; q < 19; q++) v += (double)Store<S, T>::load(f, (size_t)q * n + id, q, c.store);
         } else if (what == LBM_REDUCE_VOF_MASS) {
             v = (double)mass[id];
         } else if (what == LBM_REDUCE_VOF_VOLUME) {
@@ -2214,7 +2232,7 @@ __global__ __launch_bounds__(NT) void k_reduce(Consts<T> c, const S* f, const T*
             }
         } else if (what == LBM_REDUCE_NONFINITE) {
             for (int q = 0; q < 19; q++) {
-                T x = Store<S, T>::load(f, (size_t)q * n + id, q, (T)c.store_inv_scale);
+                T x = Store<S, T>::load(f, (size_t)q * n + id, q, c.store);
                 if (!__builtin_isfinite(x)) v += 1.0;
             }
         }
@@ -2238,7 +2256,7 @@ __global__ __launch_bounds__(NT) void k_store_populations(Consts<T> c, const T* 
     size_t n = (size_t)c.nx * c.ny * c.nz;
     if (idx >= 19 * n) return;
     int q = (int)(idx / n);
-    Store<S, T>::store(dst, idx, q, src[idx], c.store_scale);
+    Store<S, T>::store(dst, idx, q, src[idx], c.store, dither_base(c.store_step, (uint32_t)(idx - (size_t)q * n)));
 }
 
 template <class S, class T>
@@ -2247,7 +2265,7 @@ __global__ __launch_bounds__(NT) void k_load_populations(Consts<T> c, const S* s
     size_t n = (size_t)c.nx * c.ny * c.nz;
     if (idx >= 19 * n) return;
     int q = (int)(idx / n);
-    dst[idx] = Store<S, T>::load(src, idx, q, (T)c.store_inv_scale);
+    dst[idx] = Store<S, T>::load(src, idx, q, c.store);
 }
 
 // ===========================================================================
@@ -2533,17 +2551,18 @@ void launch_materialize(const lbm_config_t& c, DevBufs& b, int cur, int slot, vo
 }
 
 template <class S, class T>
-static void store_pop_t(const lbm_config_t& cfg, DevBufs& b, int cur, const void* src, hipStream_t s) {
+static void store_pop_t(const lbm_config_t& cfg, DevBufs& b, int cur, const void* src, int64_t step, hipStream_t s) {
     Consts<T> c = make_consts<T>(cfg, -1);
+    c.store_step = (uint32_t)step;
     k_store_populations<S, T><<<nblocks(19 * b.n), NT, 0, s>>>(c, (const T*)src, (S*)b.f[cur]);
     HIP_OK(hipGetLastError());
 }
-void launch_store_populations(const lbm_config_t& c, DevBufs& b, int cur, const void* src, hipStream_t s) {
+void launch_store_populations(const lbm_config_t& c, DevBufs& b, int cur, const void* src, int64_t step, hipStream_t s) {
     switch (b.storage) {
-        case LBM_STORE_FP64: store_pop_t<double, double>(c, b, cur, src, s); break;
-        case LBM_STORE_FP16: store_pop_t<__half, float>(c, b, cur, src, s); break;
-        case LBM_STORE_BF16: store_pop_t<bf16_t, float>(c, b, cur, src, s); break;
-        default: store_pop_t<float, float>(c, b, cur, src, s); break;
+        case LBM_STORE_FP64: store_pop_t<double, double>(c, b, cur, src, step, s); break;
+        case LBM_STORE_FP16: store_pop_t<__half, float>(c, b, cur, src, step, s); break;
+        case LBM_STORE_BF16: store_pop_t<bf16_t, float>(c, b, cur, src, step, s); break;
+        default: store_pop_t<float, float>(c, b, cur, src, step, s); break;
     }
 }
 

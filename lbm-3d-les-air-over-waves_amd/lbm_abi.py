@@ -2,13 +2,13 @@
 
 Shared by the solver facade (lbm_solver.py), the ensemble server and the
 tests.  The struct layouts must match include/lbm.h field for field;
-tests/test_abi.py checks sizes and offsets against the compiled library.
+tests/test_abi_config.py checks sizes and offsets against the C header.
 """
 from __future__ import annotations
 
 import ctypes
 
-LBM_CONFIG_VERSION = 3
+LBM_CONFIG_VERSION = 4
 
 # error codes
 LBM_OK = 0
@@ -26,6 +26,7 @@ YB_PERIODIC, YB_SOLID = 0, 1
 TOP_KEEP, TOP_FREE_SLIP = 0, 1
 BED_MOVING_SINE, BED_FLAT, BED_PIECEWISE, BED_MOVING_SINE1 = 0, 1, 2, 3
 STORE_FP32, STORE_FP16, STORE_BF16, STORE_FP64 = 0, 1, 2, 3
+ROUND_NEAREST, ROUND_STOCHASTIC = 0, 1
 PATH_AUTO, PATH_STAGED, PATH_FUSED = 0, 1, 2
 FST_HEIGHT, FST_VOF = 0, 1
 PF_TOTAL, PF_HYDROSTATIC_BALANCED, PF_VOF_COMPONENT_BALANCED = 0, 1, 2
@@ -128,6 +129,7 @@ class LbmConfig(ctypes.Structure):
         ("enable_top_sponge", _I),
         ("storage", _I),
         ("storage_shift", _I),
+        ("storage_rounding", _I),
         ("kernel_path", _I),
         ("fs_enabled", _I),
         ("fs_tracking", _I),
@@ -173,6 +175,7 @@ class LbmConfig(ctypes.Structure):
         ("vof_detached_residual_fill_threshold", _F),
         ("vof_thin_gap_bridge_strength", _F),
         ("vof_collapse_max_volume_m3", _F),
+        ("storage_u_ref", _F * 3),
         ("pad0", _F),
         ("dt_phys", ctypes.c_double),
     ]

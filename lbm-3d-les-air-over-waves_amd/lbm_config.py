@@ -20,7 +20,11 @@ behaviour):
   LBM_BED_PROFILE         adds moving_sine1: eta = A sin(k(x - c t))
   LBM_BED_PHASE_SPEED_LU  phase speed c of moving_sine1, lattice units
   LBM_STORAGE             fp32 | fp16 | bf16 | fp64 population storage
-  LBM_STORAGE_SHIFT       k of the (f_i - w_i) * 2^k scaling (fp16/bf16)
+  LBM_STORAGE_SHIFT       k of the (f_i - s_i) * 2^k scaling (fp16/bf16);
+                          default chosen from a bound of max |f_i - s_i|
+  LBM_STORAGE_U_REF       auto | 0 | "ux,uy,uz": velocity of the shift
+                          s_i = feq_i(1, u_ref) (0 gives s_i = w_i)
+  LBM_STORAGE_ROUNDING    stochastic (default) | nearest
   LBM_KERNEL_PATH         auto | staged | fused
   LBM_INITIAL_STATE       reference | uniform_noise | loglaw_noise
   LBM_INIT_U_LU, LBM_INIT_NOISE_LU     uniform_noise mean and half-width
@@ -98,6 +102,9 @@ class SolverConfig:
             v["storage"]
         ]
         s.storage_shift = v["storage_shift"]
+        s.storage_rounding = abi.ROUND_STOCHASTIC if v["storage_rounding"] == "stochastic" else abi.ROUND_NEAREST
+        for a in range(3):
+            s.storage_u_ref[a] = v["storage_u_ref"][a]
         s.kernel_path = {"auto": abi.PATH_AUTO, "staged": abi.PATH_STAGED, "fused": abi.PATH_FUSED}[v["kernel_path"]]
         s.fs_enabled = v["water_free_surface_enabled"]
         s.fs_tracking = abi.FST_VOF if v["free_surface_tracking_mode"] == "vof" else abi.FST_HEIGHT
@@ -469,11 +476,7 @@ def parse_env(env: Optional[Mapping[str, str]] = None) -> SolverConfig:
     bed_phase_speed_lu = float(env.get("LBM_BED_PHASE_SPEED_LU", u_bed_lb))
     bed_phase_speed_mps = bed_phase_speed_lu * vel_scale
     storage = env_choice(env, "LBM_STORAGE", "fp32", ("fp32", "fp16", "bf16", "fp64"))
-    storage_shift = env_int(env, "LBM_STORAGE_SHIFT", -1)
-    if storage_shift < 0:
-        storage_shift = default_storage_shift(storage)
-    if storage_shift > 24:
-        raise ValueError("LBM_STORAGE_SHIFT must be in [0, 24]")
+    storage_rounding = env_choice(env, "LBM_STORAGE_ROUNDING", "stochastic", ("stochastic", "nearest"))
     kernel_path = env_choice(env, "LBM_KERNEL_PATH", "auto", ("auto", "staged", "fused"))
     initial_state = env_choice(
         env, "LBM_INITIAL_STATE", "reference", ("reference", "uniform_noise", "loglaw_noise")
@@ -484,6 +487,25 @@ def parse_env(env: Optional[Mapping[str, str]] = None) -> SolverConfig:
     inflow_z0_cells = float(env.get("LBM_INFLOW_Z0_CELLS", 0.5))
     inflow_noise = float(env.get("LBM_INFLOW_NOISE", 0.02))
     seed = env_int(env, "LBM_SEED", 12345)
+
+    u_ref_text = str(env.get("LBM_STORAGE_U_REF", "auto")).lower()
+    if u_ref_text == "auto":
+        storage_u_ref = auto_storage_u_ref(physics_mode, initial_state, init_u_lu, inflow_u_lu, u_top_lb)
+    else:
+        parts = [float(x) for x in u_ref_text.split(",")]
+        if len(parts) == 1:
+            parts = [parts[0], 0.0, 0.0]
+        if len(parts) != 3:
+            raise ValueError("LBM_STORAGE_U_REF must be 'auto', one value or 'ux,uy,uz'")
+        storage_u_ref = tuple(parts)
+    if max(abs(x) for x in storage_u_ref) > 0.3:
+        raise ValueError("LBM_STORAGE_U_REF components must lie in [-0.3, 0.3]")
+    storage_shift = env_int(env, "LBM_STORAGE_SHIFT", -1)
+    if storage_shift < 0:
+        speeds = [abs(init_u_lu) + init_noise_lu, abs(inflow_u_lu) * (1.0 + inflow_noise), abs(u_top_lb)]
+        storage_shift = default_storage_shift(storage, max(speeds), storage_u_ref)
+    if storage_shift > 24:
+        raise ValueError("LBM_STORAGE_SHIFT must be in [0, 24]")
 
     # features this build does not implement are refused, not ignored
     unsupported = []
@@ -612,6 +634,8 @@ def parse_env(env: Optional[Mapping[str, str]] = None) -> SolverConfig:
             bed_phase_speed_mps=bed_phase_speed_mps,
             storage=storage,
             storage_shift=storage_shift,
+            storage_rounding=storage_rounding,
+            storage_u_ref=storage_u_ref,
             kernel_path=kernel_path,
             initial_state=initial_state,
             init_u_lu=init_u_lu,
@@ -626,16 +650,31 @@ def parse_env(env: Optional[Mapping[str, str]] = None) -> SolverConfig:
     return SolverConfig(values=v)
 
 
-def default_storage_shift(storage: str) -> int:
-    """Scale 2^k of the stored deviations f_i - w_i.  Deviations are
-    O(w_i * (3|u| + |rho-1|)) <= ~0.1, so k is chosen to put them near the
-    top of each format's normal range without overflow: half keeps 11
-    significant bits at any exponent in range, so k only has to keep the
-    smallest deviations out of the subnormals; bfloat16 shares float32's
-    exponent range and needs no scaling."""
-    if storage == "fp16":
-        return 10
-    return 0
+def auto_storage_u_ref(physics_mode, initial_state, init_u_lu, inflow_u_lu, u_top_lb):
+    """Reference velocity of the reduced-storage shift: the run's mean flow
+    (x direction), so that the stored deviations f_i - feq_i(1, u_ref) of
+    the bulk are small and keep their relative precision."""
+    if physics_mode == "water":
+        return (0.0, 0.0, 0.0)
+    if initial_state == "uniform_noise":
+        return (float(init_u_lu), 0.0, 0.0)
+    if initial_state == "loglaw_noise":
+        return (float(inflow_u_lu), 0.0, 0.0)
+    return (float(np.float32(0.8) * np.float32(u_top_lb)), 0.0, 0.0)
+
+
+def default_storage_shift(storage: str, speed: float = 0.1, u_ref=(0.0, 0.0, 0.0)) -> int:
+    """k of the (f_i - s_i) * 2^k scaling, chosen from a bound of the largest
+    deviation.  With |u - u_ref| <= U = 2 * (speed + |u_ref|) and density
+    excursions of 5 %, |f_i - s_i| <= D = (3U + 6U^2) / 18 + 0.05 / 3; k puts
+    D at 2^12 so the stored values keep a 16x margin below the largest half
+    (65504) while deviations down to D * 2^-26 stay in half's normal range.
+    bfloat16 has float32's exponent range and is not scaled."""
+    if storage != "fp16":
+        return 0
+    U = 2.0 * (abs(speed) + math.sqrt(sum(x * x for x in u_ref)))
+    D = (3.0 * U + 6.0 * U * U) / 18.0 + 0.05 / 3.0
+    return int(min(24, max(0, math.floor(math.log2(4096.0 / D)))))
 
 
 def bed_max_height(cfg: SolverConfig) -> float:
@@ -650,7 +689,7 @@ def bed_max_height(cfg: SolverConfig) -> float:
     return cfg.offset_m
 
 
-# the BASELINE workloads as environment presets ----------------------------
+# the benchmark workloads as environment presets ----------------------------
 def preset_env(name: str) -> dict:
     """LBM_* settings of the benchmark workloads (all lattice-unit choices are
     made with dx = 1 m so that cells and metres coincide)."""

@@ -30,6 +30,10 @@
  *    counter-based hash of (seed, step, cell).
  *  - New options: inflow/outflow x boundary, solid y walls, free-slip top,
  *    single-harmonic moving bed, settable C_s and tau (via the config).
+ *  - FP16/BF16 population storage (new): the FP32 build applies the
+ *    device's storage round trip wherever the device stores populations
+ *    (store_round below); the FP64 build ignores the storage setting and is
+ *    the full-precision run the reduced forms are measured against.
  */
 #include <math.h>
 #include <stdint.h>
@@ -687,6 +691,93 @@ static uint32_t fmix32(uint32_t h) {
 static float bs_uniform(uint32_t seed, uint32_t step, uint32_t cell) {
     uint32_t h = fmix32(seed ^ fmix32(cell + fmix32(step * 0x9E3779B9u)));
     return (float)(h >> 8) * (1.0f / 16777216.0f);
+}
+
+/* ---- reduced-precision population storage (new) ----------------------
+ * The device stores h_q = round((f_q - s_q) 2^k) as binary16 or bfloat16
+ * and reads f_q = s_q + h_q 2^-k, s_q = feq_q(1, u_ref).  The FP32 oracle
+ * applies the same round trip wherever the device stores populations
+ * (initialisation, upload of F, after every collision), so the FP16/BF16
+ * device run is checked against it like the FP32 one.  The roundings are
+ * restated from the IEEE 754 binary16 and bfloat16 formats:
+ *  - nearest: round to nearest, ties to even (half: 10 fraction bits above
+ *    2^-14, quantum 2^-24 below; overflow to infinity);
+ *  - stochastic: a 32-bit dither r of (step, cell, q) is added to the
+ *    magnitude's bits below the kept fraction bits, which are then
+ *    truncated (half: r >> 19 onto 13 dropped bits, then the nearest-even
+ *    conversion, exact except in half's subnormal range; bfloat16: r >> 16
+ *    onto 16 dropped bits). */
+static float half_rne(float x) {
+    union { float f; uint32_t u; } v;
+    v.f = x;
+    uint32_t a = v.u & 0x7fffffffu, sg = v.u & 0x80000000u;
+    if (a >= 0x7f800000u) return x;
+    if (a < 0x38800000u) return rintf(x * 16777216.0f) * (1.0f / 16777216.0f);
+    a += 0xfffu + ((a >> 13) & 1u);
+    a &= ~0x1fffu;
+    if (a >= 0x47800000u) a = 0x7f800000u;
+    v.u = a | sg;
+    return v.f;
+}
+static float half_sr(float x, uint32_t r) {
+    union { float f; uint32_t u; } v;
+    v.f = x;
+    uint32_t a = v.u & 0x7fffffffu, sg = v.u & 0x80000000u;
+    if (a < 0x7f800000u) a = (a + (r >> 19)) & ~0x1fffu;
+    v.u = a | sg;
+    return half_rne(v.f);
+}
+static float bf16_rne(float x) {
+    union { float f; uint32_t u; } v;
+    v.f = x;
+    if ((v.u & 0x7fffffffu) > 0x7f800000u) {
+        v.u = ((v.u >> 16) | 0x40u) << 16;
+        return v.f;
+    }
+    v.u += 0x7fffu + ((v.u >> 16) & 1u);
+    v.u &= 0xffff0000u;
+    return v.f;
+}
+static float bf16_sr(float x, uint32_t r) {
+    union { float f; uint32_t u; } v;
+    v.f = x;
+    if ((v.u & 0x7fffffffu) > 0x7f800000u) {
+        v.u = ((v.u >> 16) | 0x40u) << 16;
+        return v.f;
+    }
+    uint32_t a = v.u & 0x7fffffffu, sg = v.u & 0x80000000u;
+    if (a < 0x7f800000u) a = a + (r >> 16);
+    if (a > 0x7f800000u) a = 0x7f800000u;
+    v.u = (a | sg) & 0xffff0000u;
+    return v.f;
+}
+static uint32_t dither_base(uint32_t step, uint32_t cell) { return fmix32(cell * 0x9E3779B1u ^ step * 0x85EBCA77u); }
+
+static void store_round(O *o, uint32_t step) {
+    const lbm_config_t *c = &o->c;
+    if (sizeof(REAL) != sizeof(float)) return;
+    if (c->storage != LBM_STORE_FP16 && c->storage != LBM_STORE_BF16) return;
+    float sh[19];
+    const float ux = c->storage_u_ref[0], uy = c->storage_u_ref[1], uz = c->storage_u_ref[2];
+    const float u2 = ux * ux + uy * uy + uz * uz;
+    for (int q = 0; q < 19; q++) {
+        float cu = 3.0f * ((float)CX[q] * ux + (float)CY[q] * uy + (float)CZ[q] * uz);
+        sh[q] = (float)WD[q] * (1.0f + cu + 0.5f * cu * cu - 1.5f * u2);
+    }
+    float scale = 1.0f;
+    for (int i = 0; i < c->storage_shift; i++) scale *= 2.0f;
+    const float inv = 1.0f / scale;
+    const int sr = c->storage_rounding == LBM_ROUND_STOCHASTIC;
+    const int half = c->storage == LBM_STORE_FP16;
+    for (size_t id = 0; id < o->n; id++) {
+        uint32_t base = dither_base(step, (uint32_t)id);
+        for (int q = 0; q < 19; q++) {
+            float d = ((float)F(q, id) - sh[q]) * scale;
+            uint32_t r = base + (uint32_t)q * 0x9E3779B9u;
+            float h = half ? (sr ? half_sr(d, r) : half_rne(d)) : (sr ? bf16_sr(d, r) : bf16_rne(d));
+            F(q, id) = (REAL)(sh[q] + h * inv);
+        }
+    }
 }
 
 /* collide_KBC (CODEX:3143-3321) */
@@ -1997,6 +2088,7 @@ int API(orc_field)(void *p, int field, void *data, size_t bytes, int dir) {
     if (bytes != need) return LBM_E_ARG;
     if (dir == 0) {
         memcpy(ptr, data, need);
+        if (field == LBM_FIELD_F) store_round(o, (uint32_t)o->step);
         if (field == LBM_FIELD_PIECEWISE_BED_Z && o->c.bed_profile == LBM_BED_PIECEWISE)
             for (size_t id = 0; id < o->n2; id++) {
                 o->water_h[id] = o->piecewise_bed_z[id % (size_t)o->nx];
@@ -2046,6 +2138,7 @@ void API(orc_init_populations)(void *p) {
             F(q, id) = o->rho[id] * o->w[q] * (R(1.0) + cu + R(0.5) * cu * cu - R(1.5) * u2);
         }
     }
+    store_round(o, 0);
     memcpy(o->fn, o->f, 19 * o->n * sizeof(REAL));
     if (is_vof(o)) {
         vof_init_mass(o);
@@ -2072,6 +2165,7 @@ static void one_step(O *o) {
         collide_reg(o);
     else
         collide_kbc(o);
+    store_round(o, (uint32_t)o->step);
     stream(o);
     x_wall_bounceback(o);
     open_bc(o);

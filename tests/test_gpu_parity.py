@@ -4,8 +4,9 @@ Bar: flags, link fractions, wall types, bed kinematics and VOF cell types
 bit-exact; FP32 populations / velocity / density within max-relative 1e-5
 after 100 steps (the device and the oracle evaluate the same operations in
 the same order without contraction, so they are expected to agree to the
-bit; the tolerance is the stated bound); FP16-storage velocity within 1e-3
-of the FP64 run.
+bit; the tolerance is the stated bound); FP16/BF16 storage held to the
+same bar against the oracle's FP32 build with the same storage round trip,
+and FP16-storage velocity within 1e-3 of the FP64 run after 1000 steps.
 """
 import numpy as np
 import pytest
@@ -140,25 +141,43 @@ def test_fp64_parity_with_fp64_oracle():
     o.close()
 
 
-@pytest.mark.parametrize("storage,tol", [("fp16", 1e-3), ("bf16", 3e-2)])
+@pytest.mark.parametrize("name", ["plain", "waves"])
+@pytest.mark.parametrize("storage,rounding", [("fp16", "stochastic"), ("fp16", "nearest"), ("bf16", "stochastic")])
+def test_reduced_storage_parity_with_oracle(name, storage, rounding):
+    """FP16/BF16 storage: the FP32 oracle applies the same storage round trip
+    (same shift, scale and dither), so the device run is held to the FP32
+    bar."""
+    env = dict(CASES[name], LBM_STORAGE=storage, LBM_STORAGE_ROUNDING=rounding)
+    _, o = oracle_case(env)
+    _, c = device_case(env)
+    o.step(100)
+    c.step(100)
+    for fid in (abi.FIELD_F, abi.FIELD_RHO, abi.FIELD_UX, abi.FIELD_UZ):
+        a, b = c.download(fid), o.download(fid)
+        assert relerr(a, b) <= F32_TOL, (name, storage, fid, int(np.sum(a != b)), relerr(a, b))
+    c.close()
+    o.close()
+
+
+@pytest.mark.parametrize("storage,tol", [("fp16", 1e-3), ("bf16", 1e-2)])
 def test_reduced_storage_velocity_error(storage, tol):
-    """FP16 / BF16 shifted-deviation storage against the FP64 run of the
-    same case: max |u - u64| / max |u64| after 200 steps."""
+    """FP16 / BF16 storage (stochastic rounding, mean-flow shift) against the
+    FP64 run of the plain channel 64x32x32 after 1000 steps:
+    max |u - u64| / max |u64| <= tol, mass drift <= 1e-5."""
     base = env_of("plain_64x32x32")
     _, ref = device_case(dict(base, LBM_STORAGE="fp64"))
     _, c = device_case(dict(base, LBM_STORAGE=storage))
-    ref.step(200)
-    c.step(200)
-    errs = []
-    for fid in (abi.FIELD_UX, abi.FIELD_UY, abi.FIELD_UZ):
-        errs.append(relerr(c.download(fid), ref.download(fid)))
+    ref.step(1000)
+    c.step(1000)
     u = np.stack([ref.download(x) for x in (abi.FIELD_UX, abi.FIELD_UY, abi.FIELD_UZ)])
     du = np.stack([c.download(x) for x in (abi.FIELD_UX, abi.FIELD_UY, abi.FIELD_UZ)]) - u
     vel_err = float(np.max(np.linalg.norm(du, axis=0)) / np.max(np.linalg.norm(u, axis=0)))
-    print(storage, "velocity rel error", vel_err, "per component", errs)
-    assert vel_err <= tol
+    dp = float(np.max(np.abs(c.download(abi.FIELD_RHO) - ref.download(abi.FIELD_RHO))) / 3.0)
     m64 = ref.reduce(abi.REDUCE_MASS)
-    assert abs(c.reduce(abi.REDUCE_MASS) - m64) / m64 <= tol
+    dm = abs(c.reduce(abi.REDUCE_MASS) - m64) / m64
+    print(storage, "velocity rel error", vel_err, "max |dp|", dp, "mass drift", dm)
+    assert vel_err <= tol
+    assert dm <= 1e-5
     ref.close()
     c.close()
 

@@ -106,6 +106,9 @@ def main():
     ap.add_argument("--workload", default="waves_256x128x128")
     ap.add_argument("--no-variants", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="run only a plain device copy of 1 GiB (3 launches): the reference kernel for "
+                         "scaling the FETCH_SIZE / WRITE_SIZE counters")
     args = ap.parse_args()
     if args.gpus != 1:
         print(
@@ -120,6 +123,14 @@ def main():
 
     assert torch.cuda.is_available(), "bench.py needs a HIP device"
     torch.cuda.set_device(0)
+    if args.copy_calibration:
+        x = torch.ones(256 << 20, dtype=torch.float32, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(3):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes": x.numel() * 4, "launches": 3}))
+        return
     env = preset_env(args.workload)
     env["LBM_KERNEL_PATH"] = "auto"
     cfg = parse_env(env)

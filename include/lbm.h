@@ -258,7 +258,8 @@ int lbm_device_bytes(const lbm_config_t *cfg, size_t *out);
 
 /* Tuning and instrumentation knobs (no counterpart in the reference):
    "timing" (0/1: record HIP events around every step kernel),
-   "kchunk" (z planes per block of the fused kernel, >= 1). */
+   "block_x" (64, 128 or 256: x extent of a block of the fused passes;
+   default from nx). */
 int lbm_set_option(lbm_handle_t *h, const char *key, int64_t value);
 
 /* Resolved execution path of the handle: LBM_PATH_STAGED or LBM_PATH_FUSED. */

@@ -43,7 +43,7 @@ struct lbm_handle {
     int omega_band_top = -1;
     int wall_top = 0;
     double piecewise_max = 0.0;
-    int kchunk = 16;
+    int block_x = 64;  // x extent of a fused-kernel block (64, 128 or 256 cells; set from nx at creation)
     int initialized = 0;
     int geometry_done = 0;
     int timing = 0;
@@ -91,8 +91,8 @@ static const char* validate(const lbm_config_t* c) {
     if (!(c->dx > 0.0f)) return "dx must be positive";
     if (c->vof_collapse_interval < 1) return "vof_collapse_interval must be positive";
     if ((c->storage == LBM_STORE_FP16 || c->storage == LBM_STORE_BF16) && !fused_supported(*c))
-        return "fp16/bf16 storage runs on the fused path: air mode with nx % 32 == 0 and ny % 16 == 0";
-    if (c->kernel_path == LBM_PATH_FUSED && !fused_supported(*c)) return "the fused path needs air mode, nx % 32 == 0, ny % 16 == 0";
+        return "fp16/bf16 storage runs on the fused path: air mode only";
+    if (c->kernel_path == LBM_PATH_FUSED && !fused_supported(*c)) return "the fused path covers air mode only";
     return nullptr;
 }
 
@@ -208,6 +208,8 @@ static void allocate(lbm_handle* h) {
         b.uw9[s] = dalloc(h, 27 * n2 * R);
         b.denom[s] = dalloc(h, n2 * R);
         b.kband[s] = (int*)dalloc(h, n2 * 4);
+        b.ksolid[s] = (int*)dalloc(h, n2 * 4);
+        b.fsol[s] = dalloc(h, 19 * n2 * R);
         CHECK(hipMemsetAsync(b.uw[s], 0, n2 * R, h->stream));
         CHECK(hipMemsetAsync(b.vw[s], 0, n2 * R, h->stream));
         CHECK(hipMemsetAsync(b.ww[s], 0, n2 * R, h->stream));
@@ -316,10 +318,10 @@ int lbm_create(const lbm_config_t* cfg, lbm_handle_t** out) {
     lbm_handle* h = new lbm_handle();
     h->cfg = *cfg;
     h->path = resolve_path(*cfg);
+    h->block_x = cfg->nx >= 256 ? 256 : (cfg->nx >= 128 ? 128 : 64);
     h->omega_band_top = band_top(*cfg);
     h->piecewise_max = (double)cfg->offset_m;
     h->wall_top = wall_band_top(*cfg, h->piecewise_max);
-    h->kchunk = cfg->nz >= 64 ? 16 : 8;
     try {
         int ndev = 0;
         if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) throw std::runtime_error("no HIP device visible");
@@ -394,7 +396,7 @@ int lbm_step(lbm_handle_t* h, int32_t nsteps) {
         } else {
             int last = s == nsteps - 1;
             launch_fused_step(c, h->b, h->cur, h->post_stream ? 0 : 1, slot_of(step - 1), slot, step, last, h->omega_band_top,
-                              h->wall_top, h->kchunk, h->stream);
+                              h->wall_top, h->block_x, h->stream);
             h->post_stream = 0;
         }
         if (h->timing) CHECK(hipEventRecord(h->ev[3 + 2 * s], h->stream));
@@ -687,9 +689,9 @@ int lbm_set_option(lbm_handle_t* h, const char* key, int64_t value) {
         h->timing = value ? 1 : 0;
         return LBM_OK;
     }
-    if (!strcmp(key, "kchunk")) {
-        if (value < 1 || value > h->cfg.nz) return fail(h, LBM_E_ARG, "kchunk must be in [1, nz]");
-        h->kchunk = (int)value;
+    if (!strcmp(key, "block_x")) {
+        if (value != 64 && value != 128 && value != 256) return fail(h, LBM_E_ARG, "block_x must be 64, 128 or 256");
+        h->block_x = (int)value;
         return LBM_OK;
     }
     return fail(h, LBM_E_ARG, std::string("unknown option ") + key);

@@ -299,8 +299,10 @@ struct Slot {
     const T* ww;     // w_wave
     const T* eta9;   // [9][ny][nx] bed height at the 9 link endpoints of each column
     const T* denom;  // [ny][nx] slope normaliser of phi
-    const int* kband;  // [ny][nx] highest k with a cut or closed link
-    const T* uw9;      // [3][9][ny][nx] wall velocity at the 9 link endpoints
+    const int* kband;   // [ny][nx] highest k with a cut or closed link
+    const T* uw9;       // [3][9][ny][nx] wall velocity at the 9 link endpoints
+    const int* ksolid;  // [ny][nx] highest k whose 18 links the bed closes (-1: none)
+    const T* fsol;      // [19][ny][nx] conditioned populations of those closed cells
 };
 
 // link geometry of one cell (CODEX:2492-2686): open flag, open fraction,

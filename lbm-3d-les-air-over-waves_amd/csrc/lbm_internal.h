@@ -50,6 +50,8 @@ struct DevBufs {
     void *rho, *ux, *uy, *uz, *omega, *nu_t, *smag;
     void *h[2], *uw[2], *vw[2], *ww[2], *eta9[2], *denom[2], *uw9[2];
     int* kband[2];
+    int* ksolid[2];
+    void* fsol[2];
     void *piecewise, *u_inlet, *inflow;
     uint8_t* cellflags;  // bit0 lattice_open[0], bit1 near_obstacle (staged path, current slot)
     // water / VOF
@@ -72,7 +74,7 @@ void launch_staged_step(const lbm_config_t& c, DevBufs& b, int& cur, int slot, i
 // ones; slot_prev is the time level of the pending stream, slot the new one
 // band_top: highest k at which any cell can have a cut or closed link
 void launch_fused_step(const lbm_config_t& c, DevBufs& b, int& cur, int mode, int slot_prev, int slot, int64_t step,
-                       int write_macro, int omega_band_top, int band_top, int kchunk, hipStream_t s);
+                       int write_macro, int omega_band_top, int band_top, int block_x, hipStream_t s);
 // post-collision state -> post-stream populations (arithmetic type) in dst
 void launch_materialize(const lbm_config_t& c, DevBufs& b, int cur, int slot, void* dst, hipStream_t s);
 // convert arithmetic-type populations in src into the storage buffer f[cur]

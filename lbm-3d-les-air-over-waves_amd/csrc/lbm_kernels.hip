@@ -119,6 +119,8 @@ static Slot<T> make_slot(DevBufs& b, int s) {
     sl.denom = (const T*)b.denom[s];
     sl.kband = b.kband[s];
     sl.uw9 = (const T*)b.uw9[s];
+    sl.ksolid = b.ksolid[s];
+    sl.fsol = (const T*)b.fsol[s];
     return sl;
 }
 
@@ -193,7 +195,8 @@ __global__ __launch_bounds__(NT) void k_bed2d_a(Consts<T> c, T t, T* h, T* uw, T
 // endpoints), phi slope normaliser (CODEX:2451-2469) and the highest k with
 // a cut or closed link (cells above it are fully open: frac = 1 everywhere)
 template <class T>
-__global__ __launch_bounds__(NT) void k_bed2d_b(Consts<T> c, const T* h, T* eta9, T* denom, int* kband, const T* uw, const T* vw, const T* ww, T* uw9) {
+__global__ __launch_bounds__(NT) void k_bed2d_b(Consts<T> c, const T* h, T* eta9, T* denom, int* kband, const T* uw, const T* vw, const T* ww, T* uw9,
+                                                int* ksolid) {
     size_t id = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     size_t plane = (size_t)c.nx * c.ny;
     if (id >= plane) return;
@@ -238,6 +241,49 @@ __global__ __launch_bounds__(NT) void k_bed2d_b(Consts<T> c, const T* h, T* eta9
                     (c.y_boundary == LBM_YB_SOLID && (j == 0 || j == c.ny - 1));
     if (wall_col) kb = c.nz;
     kband[id] = kb;
+    // highest k whose 18 links all end below the bed (gap < 0 with the
+    // arithmetic of link_geo); closedness is monotone in k
+    int ks = -1;
+    if (c.include_bed) {
+        for (int k = 0; k < c.nz; k++) {
+            T zc = ((T)k + T(0.5)) * dx;
+            bool closed = true;
+#pragma unroll
+            for (int q = 1; q < 19; q++) {
+                T zq = zc + T(0.5) * dx * (T)czq(q);
+                T gap = zq - eta9[(size_t)c9q(q) * plane + id];
+                if (!(gap < T(0))) closed = false;
+            }
+            if (!closed) break;
+            ks = k;
+        }
+    }
+    ksolid[id] = ks;
+}
+
+// Populations of a cell whose links the bed closes all: the conditioning
+// of update_bed_populations_and_reinit (CODEX:2894-2940) sets every one to
+// feq(rho0, wall velocity of its link) and they depend on the column only.
+template <class T>
+__global__ __launch_bounds__(NT) void k_bed2d_c(Consts<T> c, Slot<T> s, T* fsol) {
+    size_t id = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    size_t plane = (size_t)c.nx * c.ny;
+    if (id >= plane) return;
+    int i = (int)(id % (size_t)c.nx), j = (int)(id / (size_t)c.nx);
+    const int k = s.ksolid[id];
+    if (k < 0) return;
+    for (int q = 0; q < 19; q++) {
+        int wt = c.include_bed ? LBM_WALL_BED : LBM_WALL_NONE;
+        if (q > 0) {
+            int lo;
+            bool w;
+            link_geo(c, s.eta9, id, plane, i, j, k, q, lo, w);
+            wt = link_wt(c, w, i, j, q);
+        }
+        T ux = T(0), uy = T(0), uz = T(0);
+        if (wt == LBM_WALL_BED) wall_velocity(c, s, i, j, q, ux, uy, uz);
+        fsol[(size_t)q * plane + id] = feq(c.rho0, ux, uy, uz, q);
+    }
 }
 
 // ===========================================================================
@@ -1506,28 +1552,30 @@ __global__ __launch_bounds__(NT) void k_vof_init_mass(VofArgs<T> a) {
 }
 
 // ===========================================================================
-// fused z-marching step (air mode)
+// fused step (air mode): moments pre-pass + fused stream-LES-collide pass
 // ===========================================================================
 //
-// Block = 32 x 16 columns (8 waves of interior cells) + 2 waves for the
-// 96-cell halo ring whose velocities the LES central differences need.  The
-// block marches up a chunk of kc planes; at every plane it
-//   1. streams + applies the boundary rules of the previous step to plane
-//      k+1 (interior and halo), conditions its closed links for the new
-//      step and forms rho and u; u goes to a 3-plane LDS ring, the interior
-//      keeps its populations in registers;
-//   2. evaluates the LES viscosity of plane k from the ring, collides and
-//      writes f* of plane k.
-// The halo ring and the planes k0-1 / k0+kc are recomputed by the
-// neighbouring blocks: extra arithmetic, no extra HBM traffic.
+// The state between steps is the post-collision population set f* of step
+// n (one ping-pong buffer pair).  A step is two launches over the grid, one
+// thread per cell:
+//   K1 (k_moments): pull-streams f* of step n with the boundary rules of
+//      step n, conditions the closed links for step n+1 and writes the
+//      velocity of step n+1 (ux, uy, uz) - 76 B read + 12 B written per
+//      cell (FP32);
+//   K2 (k_collide): repeats the identical pull + conditioning, evaluates the
+//      LES viscosity from the 6 neighbouring velocities written by K1,
+//      collides and writes f* of step n+1 - 76 B + 12 B read, 76 B written.
+// The LES central differences (CODEX:3040-3141) need the velocity of the
+// neighbours of step n+1, which exist only after their own pull; K1 makes
+// them available to K2 for 12 B/cell of extra traffic instead of a halo
+// recomputation inside a tile.
 //
-// Cells above the per-column wall band (every link open with fraction 1)
-// take a fast path: plain pull with the domain-boundary rules only.  Inside
-// the band the link geometry is evaluated link by link from the 9 column
-// bed samples, and the cell's own post-collision populations (needed by
-// the bounce-back terms) are re-read from cache instead of being held.
-constexpr int FT_X = 32, FT_Y = 16, FT_THREADS = 640;
-constexpr int FL_X = FT_X + 2, FL_Y = FT_Y + 2;
+// Planes up to the wall band (the highest plane a moving bed can cut) run
+// the BAND instantiation, which evaluates the link geometry link by link
+// from the 9 column bed samples; the planes above it run the lean one
+// (plain pull with the domain-boundary rules only).
+constexpr int FB_THREADS = 256;  // block: bx x (256 / bx) cells of one plane, bx = 64 .. 256
+
 
 template <class S, class T>
 struct FusedArgs {
@@ -1538,12 +1586,13 @@ struct FusedArgs {
     S* fout;
     T *rho, *ux, *uy, *uz, *omega;
     const T *u_inlet, *inflow;
-    int mode;  // 0: fin holds post-stream populations (no stream)
+    uint8_t* open;  // band planes: 1 if the cell has an open link at step n+1 (K1 writes, K2 reads)
+    int mode;       // 0: fin holds post-stream populations (no stream)
     int write_macro;
-    int kc;
-    int chunk0;  // index of the first z chunk of this launch
+    int plane0;  // first plane of this launch
     uint32_t step;
 };
+
 
 // c_q . u over the non-zero lattice components, in x, y, z order; a zero
 // component contributes an exact zero to the reference's sum, so the value
@@ -1720,6 +1769,27 @@ __device__ __forceinline__ int link_wt(const Consts<T>& c, bool wall, int i, int
     return (i == 0 && cxq(q) > 0) ? LBM_WALL_XMIN : LBM_WALL_XMAX;
 }
 
+// x fringe (CODEX:3620-3644) and top drive (CODEX:3650-3662) applied to the
+// pulled populations of an open cell
+template <class S, class T>
+__device__ __forceinline__ void stream_edges(const FusedArgs<S, T>& a, int i, int k, bool open0, T f[19]) {
+    const Consts<T>& c = a.c;
+    const int nz = c.nz;
+    if (c.x_fringe && i < 5 && open0 && k > nz / 2) {
+        T sfr = T(1.0) - (T)i / T(4.0);
+        T kr = (T)k / (T)(nz - 1);
+        T strength = T(0.05) * (kr * kr - T(0.25));
+        T sigma = strength * smoothstep01(sfr);
+        T uxt = a.u_inlet[k] + T(0.0);
+#pragma unroll
+        for (int q = 0; q < 19; q++) f[q] = (T(1.0) - sigma) * f[q] + sigma * feq(c.rho0, uxt, T(0), T(0), q);
+    }
+    if (c.top_drive && k == nz - 1 && open0) {
+#pragma unroll
+        for (int q = 0; q < 19; q++) f[q] = feq(c.rho0, c.u_top_lb, T(0), T(0), q);
+    }
+}
+
 // stream (+ boundary rules) of step n and conditioning of step n+1 for one
 // cell; returns the populations entering the moments of step n+1
 template <class S, class T, bool BAND>
@@ -1732,6 +1802,18 @@ __device__ __forceinline__ void fused_gather(const FusedArgs<S, T>& a, int i, in
     const Codec& inv = c.store;
     const size_t col = (size_t)j * nx + i;
     const S* fin = a.fin;
+    if (BAND && k <= a.s1.ksolid[col]) {
+        // closed at step n+1: the conditioning replaces every population by
+        // feq(rho0, wall velocity) with weight 1 and the streamed value with
+        // weight 0 (equal whenever the streamed values are finite); the
+        // column's values come from k_bed2d_c
+        const T* fs = a.s1.fsol + col;
+        const size_t plane2 = (size_t)nx * ny;
+#pragma unroll
+        for (int q = 0; q < 19; q++) f[q] = fs[(size_t)q * plane2];
+        open0_out = false;
+        return;
+    }
 #define LD(q, idx) Store<S, T>::load(fin + (size_t)(q) * n, (uint32_t)(idx), (q), inv)
     if (a.mode == 0) {
 #pragma unroll
@@ -1788,22 +1870,7 @@ __device__ __forceinline__ void fused_gather(const FusedArgs<S, T>& a, int i, in
             }
             f[q] = out;
         }
-        const bool open0 = any_open;
-        // (A) x fringe (CODEX:3620-3644)
-        if (c.x_fringe && i < 5 && open0 && k > nz / 2) {
-            T sfr = T(1.0) - (T)i / T(4.0);
-            T kr = (T)k / (T)(nz - 1);
-            T strength = T(0.05) * (kr * kr - T(0.25));
-            T sigma = strength * smoothstep01(sfr);
-            T uxt = a.u_inlet[k] + T(0.0);
-#pragma unroll
-            for (int q = 0; q < 19; q++) f[q] = (T(1.0) - sigma) * f[q] + sigma * feq(c.rho0, uxt, T(0), T(0), q);
-        }
-        // (B) top drive (CODEX:3650-3662)
-        if (c.top_drive && k == nz - 1 && open0) {
-#pragma unroll
-            for (int q = 0; q < 19; q++) f[q] = feq(c.rho0, c.u_top_lb, T(0), T(0), q);
-        }
+        stream_edges(a, i, k, any_open, f);
         // (C) moving-wall IBB on near-obstacle cells (CODEX:3674-3712)
         if (slow && (mask & 0x7fffeu) != 0x7fffeu) {
 #pragma unroll
@@ -1848,362 +1915,143 @@ __device__ __forceinline__ void fused_gather(const FusedArgs<S, T>& a, int i, in
     open0_out = open0;
 }
 
-template <class S, class T, bool BAND>
-__global__ __launch_bounds__(FT_THREADS) void k_fused(FusedArgs<S, T> a) {
-    __shared__ T s_u[3][3][FL_Y * FL_X];  // [plane ring][component][cell]
-    const Consts<T>& c = a.c;
-    const int nx = c.nx, ny = c.ny, nz = c.nz;
-    const uint32_t n = (uint32_t)nx * ny * nz;
-    const int tid = threadIdx.x;
-    const int i0 = blockIdx.x * FT_X, j0 = blockIdx.y * FT_Y;
-    const int k0 = (blockIdx.z + a.chunk0) * a.kc;
-    const int k1 = min(k0 + a.kc, nz);
-    int lx, ly;
-    const bool interior = tid < FT_X * FT_Y;
-    bool alive = true;
-    if (interior) {
-        lx = tid % FT_X;
-        ly = tid / FT_X;
-    } else {
-        int h = tid - FT_X * FT_Y;
-        if (h < FT_X) {
-            lx = h;
-            ly = -1;
-        } else if (h < 2 * FT_X) {
-            lx = h - FT_X;
-            ly = FT_Y;
-        } else if (h < 2 * FT_X + FT_Y) {
-            lx = -1;
-            ly = h - 2 * FT_X;
-        } else if (h < 2 * FT_X + 2 * FT_Y) {
-            lx = FT_X;
-            ly = h - 2 * FT_X - FT_Y;
-        } else {
-            lx = 0;
-            ly = 0;
-            alive = false;
-        }
-    }
-    // LES neighbours wrap periodically in x and y (CODEX:3047-3053)
-    const int gi = pmod(i0 + lx, nx), gj = pmod(j0 + ly, ny);
-    const int lcell = (ly + 1) * FL_X + (lx + 1);
-    const size_t col = (size_t)gj * nx + gi;
-
-    T fcur[19], fnext[19];
-    T rcur = T(0), ucur = T(0), vcur = T(0), wcur = T(0);
-    bool open_cur = true, open_below = true, open_m = true;
-    S* const fout = a.fout;
-
-    // planes m = k0-1 .. k1: gather + moments of plane m, then LES +
-    // collision of plane m-1 (which needs the velocity of plane m)
-    const int m0 = k0 > 0 ? k0 - 1 : 0;
-    for (int m = m0; m <= k1; m++) {
-        T rn = T(0), un = T(0), vn = T(0), wn = T(0);
-        open_m = true;
-        if (alive && m < nz) {
-            fused_gather<S, T, BAND>(a, gi, gj, m, fnext, open_m);
-            T rr = T(0);
+// rho and u of step n+1 from the conditioned populations, in the order of
+// the staged macro kernel (CODEX:2962-2987); the top drive fixes u on the
+// top plane (CODEX:3650-3662)
+template <class T>
+__device__ __forceinline__ void fused_moments(const Consts<T>& c, int k, const T f[19], T& rr, T& un, T& vn, T& wn) {
+    rr = T(0);
 #pragma unroll
-            for (int q = 0; q < 19; q++) rr += fnext[q];
-            T jx = T(0), jy = T(0), jz = T(0);
-#pragma unroll
-            for (int q = 0; q < 19; q++) {
-                if (cxq(q)) jx = (q == 1) ? fnext[q] : (cxq(q) > 0 ? jx + fnext[q] : jx - fnext[q]);
-                if (cyq(q)) jy = (q == 3) ? fnext[q] : (cyq(q) > 0 ? jy + fnext[q] : jy - fnext[q]);
-                if (czq(q)) jz = (q == 5) ? fnext[q] : (czq(q) > 0 ? jz + fnext[q] : jz - fnext[q]);
-            }
-            T inv = T(1.0) / rr;
-            un = jx * inv;
-            vn = jy * inv;
-            wn = jz * inv;
-            if (c.top_drive && m == nz - 1) {
-                un = c.u_top_lb;
-                vn = T(0);
-                wn = T(0);
-            }
-            rn = rr;
-            const int slot = m % 3;
-            s_u[slot][0][lcell] = un;
-            s_u[slot][1][lcell] = vn;
-            s_u[slot][2][lcell] = wn;
-        }
-        __syncthreads();
-        const int k = m - 1;
-        if (interior && k >= k0) {
-            const int i = i0 + lx, j = j0 + ly;
-            const uint32_t id = ((uint32_t)k * ny + j) * nx + i;
-            // compute_LES (CODEX:3011-3141)
-            T om = T(1.0) / c.tau0;
-            if (c.use_les) {
-                if (open_cur) {
-                    const int sk = k % 3;
-                    const int sp = (k + 1 < nz ? k + 1 : k) % 3;
-                    const int sm = (k > 0 ? k - 1 : k) % 3;
-                    const int cx_p = lcell + 1, cx_m = lcell - 1, cy_p = lcell + FL_X, cy_m = lcell - FL_X;
-                    const bool near_wall = k > 0 && !open_below;
-                    T phi = T(0), uwx = T(0), uwy = T(0);
-                    if (near_wall) {
-                        phi = cell_phi(c, a.s1, i, j, k);
-                        if (c.include_bed) {
-                            uwx = a.s1.uw[col];
-                            uwy = a.s1.vw[col];
-                        }
-                    }
-                    om = les_omega(c, s_u[sk][0][cx_p], s_u[sk][0][cx_m], s_u[sk][0][cy_p], s_u[sk][0][cy_m], s_u[sp][0][lcell],
-                                   s_u[sm][0][lcell], s_u[sk][1][cx_p], s_u[sk][1][cx_m], s_u[sk][1][cy_p], s_u[sk][1][cy_m],
-                                   s_u[sp][1][lcell], s_u[sm][1][lcell], s_u[sk][2][cx_p], s_u[sk][2][cx_m], s_u[sk][2][cy_p],
-                                   s_u[sk][2][cy_m], s_u[sp][2][lcell], s_u[sm][2][lcell], near_wall, phi, ucur, vcur, uwx, uwy,
-                                   (T*)nullptr, (T*)nullptr);
-                    if (k <= c.omega_band_top) a.omega[id] = om;
-                } else if (k <= c.omega_band_top) {
-                    om = a.omega[id];  // inactive cells keep their last omega
-                }
-            }
-            // collision (CODEX:3143-3416), air mode
-            if (c.collision_reg) {
-                fused_regularized(fcur, rcur, ucur, vcur, wcur, om);
-            } else if (c.top_sponge && k > nz - 3) {
-                sponge_relax(fcur, rcur, ucur, vcur, wcur);
-            } else if (c.c_backscatter > T(0)) {
-                bool bs = open_cur;
-                float xi = bs ? bs_uniform(c.seed, a.step, id) : 0.0f;
-                collide_kbc<T, false>(fcur, rcur, ucur, vcur, wcur, om, T(0), bs, c.c_backscatter, xi);
-            } else {
-                fused_kbc(fcur, rcur, ucur, vcur, wcur, om);
-            }
-            {
-                const uint32_t dbase = c.store.stochastic ? dither_base(a.step, (uint32_t)id) : 0u;
-#pragma unroll
-                for (int q = 0; q < 19; q++) Store<S, T>::store(fout + (size_t)q * n, id, q, fcur[q], c.store, dbase);
-            }
-            if (a.write_macro) {
-                a.rho[id] = rcur;
-                a.ux[id] = ucur;
-                a.uy[id] = vcur;
-                a.uz[id] = wcur;
-                a.omega[id] = om;
-            }
-        }
-        __syncthreads();
-        open_below = open_cur;
-        open_cur = open_m;
-#pragma unroll
-        for (int q = 0; q < 19; q++) fcur[q] = fnext[q];
-        rcur = rn;
-        ucur = un;
-        vcur = vn;
-        wcur = wn;
-    }
-}
-
-// Lean fused kernel for z chunks above the wall band: every cell it touches
-// has all links open with fraction 1, so the stream is a plain pull with the
-// domain-boundary rules only.  The raw loads of plane m+1 are issued before
-// plane m is reduced and plane m-1 collided, so they are in flight while the
-// block computes (two-stage software pipeline).
-template <class S, class T>
-struct LeanSrc {
-    uint32_t idx[19];  // source cell of each population load
-};
-
-template <class S, class T>
-__device__ __forceinline__ void lean_issue(const FusedArgs<S, T>& a, int i, int j, int m, const int cols[3], const int rows[3],
-                                           T raw[19]) {
-    const Consts<T>& c = a.c;
-    const int nx = c.nx, ny = c.ny, nz = c.nz;
-    const uint32_t n = (uint32_t)nx * ny * nz;
-    const uint32_t plane = (uint32_t)nx * ny;
-    const uint32_t id = (uint32_t)m * plane + (uint32_t)j * nx + i;
-    const Codec& inv = c.store;
-    const S* fin = a.fin;
-    if (a.mode == 0) {
-#pragma unroll
-        for (int q = 0; q < 19; q++) raw[q] = Store<S, T>::load(fin + (size_t)q * n, id, q, inv);
-        return;
-    }
-    const bool top = m == nz - 1;
-    const bool free_slip = c.top_boundary == LBM_TOP_FREE_SLIP;
+    for (int q = 0; q < 19; q++) rr += f[q];
+    T jx = T(0), jy = T(0), jz = T(0);
 #pragma unroll
     for (int q = 0; q < 19; q++) {
-        const int cxi = cxq(q), cyi = cyq(q), czi = czq(q);
-        uint32_t src;
-        int pq = q;
-        if (top && czi < 0) {
-            if (free_slip) {
-                src = (uint32_t)(nz - 1) * plane + (uint32_t)rows[cyi + 1] * nx + cols[cxi + 1];
-                pq = mirzq(q);
-            } else {
-                src = id;
-            }
-        } else {
-            src = (uint32_t)(m - czi) * plane + (uint32_t)rows[cyi + 1] * nx + cols[cxi + 1];
-        }
-        raw[q] = Store<S, T>::load(fin + (size_t)pq * n, src, pq, inv);
+        if (cxq(q)) jx = (q == 1) ? f[q] : (cxq(q) > 0 ? jx + f[q] : jx - f[q]);
+        if (cyq(q)) jy = (q == 3) ? f[q] : (cyq(q) > 0 ? jy + f[q] : jy - f[q]);
+        if (czq(q)) jz = (q == 5) ? f[q] : (czq(q) > 0 ? jz + f[q] : jz - f[q]);
+    }
+    T inv = T(1.0) / rr;
+    un = jx * inv;
+    vn = jy * inv;
+    wn = jz * inv;
+    if (c.top_drive && k == c.nz - 1) {
+        un = c.u_top_lb;
+        vn = T(0);
+        wn = T(0);
     }
 }
 
+// cell of this thread; false outside the grid
 template <class S, class T>
-__global__ __launch_bounds__(FT_THREADS) void k_fused_lean(FusedArgs<S, T> a) {
-    __shared__ T s_u[3][3][FL_Y * FL_X];
+__device__ __forceinline__ bool fused_cell(const FusedArgs<S, T>& a, int& i, int& j, int& k) {
+    i = blockIdx.x * (int)blockDim.x + (int)threadIdx.x;
+    j = blockIdx.y * (int)blockDim.y + (int)threadIdx.y;
+    k = blockIdx.z + a.plane0;
+    return i < a.c.nx && j < a.c.ny;
+}
+
+// waves per SIMD the kernels are compiled for (register budget 512 / waves):
+// the lean moments pass fits 8 waves in FP32, the band passes and the
+// collision 4; FP64 arithmetic halves each
+template <class T, bool BAND>
+struct FusedOcc {
+    static constexpr int k1 = (sizeof(T) == 8 ? 4 : 8) / (BAND ? 2 : 1);
+    static constexpr int k2 = sizeof(T) == 8 ? 2 : 4;
+};
+
+template <class S, class T, bool BAND>
+__global__ __launch_bounds__(FB_THREADS, (FusedOcc<T, BAND>::k1)) void k_moments(FusedArgs<S, T> a) {
+    int i, j, k;
+    if (!fused_cell(a, i, j, k)) return;
+    const Consts<T>& c = a.c;
+    T f[19];
+    bool open0;
+    fused_gather<S, T, BAND>(a, i, j, k, f, open0);
+    T rr, un, vn, wn;
+    fused_moments(c, k, f, rr, un, vn, wn);
+    const uint32_t id = ((uint32_t)k * c.ny + j) * c.nx + i;
+    a.ux[id] = un;
+    a.uy[id] = vn;
+    a.uz[id] = wn;
+    if (BAND) a.open[id] = open0 ? 1 : 0;
+}
+
+// LES viscosity (CODEX:3011-3141), collision (CODEX:3143-3416, air mode)
+// and store of one cell whose conditioned populations, moments and
+// neighbour velocities are known.  nbu(comp, dir) returns the neighbour
+// velocity component comp (0..2) in direction dir = x+, x-, y+, y-, z+, z-
+// (x and y wrap periodically, CODEX:3047-3053; z clamps at the ends).
+template <class S, class T, class NB>
+__device__ __forceinline__ void les_collide_store(const FusedArgs<S, T>& a, int i, int j, int k, uint32_t id, T f[19], bool open_cur,
+                                                  bool open_below, T rcur, T ucur, T vcur, T wcur, const NB& nbu) {
+    const Consts<T>& c = a.c;
+    const uint32_t n = (uint32_t)c.nx * c.ny * c.nz;
+    T om = T(1.0) / c.tau0;
+    if (c.use_les) {
+        if (open_cur) {
+            const bool near_wall = k > 0 && !open_below;
+            T phi = T(0), uwx = T(0), uwy = T(0);
+            if (near_wall) {
+                phi = cell_phi(c, a.s1, i, j, k);
+                if (c.include_bed) {
+                    const size_t col = (size_t)j * c.nx + i;
+                    uwx = a.s1.uw[col];
+                    uwy = a.s1.vw[col];
+                }
+            }
+            om = les_omega(c, nbu(0, 0), nbu(0, 1), nbu(0, 2), nbu(0, 3), nbu(0, 4), nbu(0, 5), nbu(1, 0), nbu(1, 1), nbu(1, 2),
+                           nbu(1, 3), nbu(1, 4), nbu(1, 5), nbu(2, 0), nbu(2, 1), nbu(2, 2), nbu(2, 3), nbu(2, 4), nbu(2, 5), near_wall,
+                           phi, ucur, vcur, uwx, uwy, (T*)nullptr, (T*)nullptr);
+            if (k <= c.omega_band_top) a.omega[id] = om;
+        } else if (k <= c.omega_band_top) {
+            om = a.omega[id];  // inactive cells keep their last omega
+        }
+    }
+    if (c.collision_reg) {
+        fused_regularized(f, rcur, ucur, vcur, wcur, om);
+    } else if (c.top_sponge && k > c.nz - 3) {
+        sponge_relax(f, rcur, ucur, vcur, wcur);
+    } else if (c.c_backscatter > T(0)) {
+        const bool bs = open_cur;
+        const float xi = bs ? bs_uniform(c.seed, a.step, id) : 0.0f;
+        collide_kbc<T, false>(f, rcur, ucur, vcur, wcur, om, T(0), bs, c.c_backscatter, xi);
+    } else {
+        fused_kbc(f, rcur, ucur, vcur, wcur, om);
+    }
+    S* const fout = a.fout;
+    const uint32_t dbase = c.store.stochastic ? dither_base(a.step, id) : 0u;
+#pragma unroll
+    for (int q = 0; q < 19; q++) Store<S, T>::store(fout + (size_t)q * n, id, q, f[q], c.store, dbase);
+    if (a.write_macro) {
+        a.rho[id] = rcur;
+        a.omega[id] = om;
+    }
+}
+
+template <class S, class T, bool BAND>
+__global__ __launch_bounds__(FB_THREADS, (FusedOcc<T, BAND>::k2)) void k_collide(FusedArgs<S, T> a) {
+    int i, j, k;
+    if (!fused_cell(a, i, j, k)) return;
     const Consts<T>& c = a.c;
     const int nx = c.nx, ny = c.ny, nz = c.nz;
-    const uint32_t n = (uint32_t)nx * ny * nz;
-    const int tid = threadIdx.x;
-    const int i0 = blockIdx.x * FT_X, j0 = blockIdx.y * FT_Y;
-    const int k0 = (blockIdx.z + a.chunk0) * a.kc;
-    const int k1 = min(k0 + a.kc, nz);
-    int lx, ly;
-    const bool interior = tid < FT_X * FT_Y;
-    bool alive = true;
-    if (interior) {
-        lx = tid % FT_X;
-        ly = tid / FT_X;
-    } else {
-        int h = tid - FT_X * FT_Y;
-        if (h < FT_X) {
-            lx = h;
-            ly = -1;
-        } else if (h < 2 * FT_X) {
-            lx = h - FT_X;
-            ly = FT_Y;
-        } else if (h < 2 * FT_X + FT_Y) {
-            lx = -1;
-            ly = h - 2 * FT_X;
-        } else if (h < 2 * FT_X + 2 * FT_Y) {
-            lx = FT_X;
-            ly = h - 2 * FT_X - FT_Y;
-        } else {
-            lx = 0;
-            ly = 0;
-            alive = false;
-        }
-    }
-    const int gi = pmod(i0 + lx, nx), gj = pmod(j0 + ly, ny);
-    const int lcell = (ly + 1) * FL_X + (lx + 1);
-    const bool xper = c.x_boundary == LBM_XB_PERIODIC, xio = c.x_boundary == LBM_XB_INFLOW_OUTFLOW;
-    const bool yper = c.y_boundary == LBM_YB_PERIODIC;
-    // source column / row for c = -1, 0, +1 (index 1 + c); the outflow ghost
-    // column is a copy of the last interior column
-    const int cols[3] = {xper ? (gi == nx - 1 ? 0 : gi + 1) : (gi == nx - 1 ? nx - 1 : gi + 1), gi,
-                         xper ? (gi == 0 ? nx - 1 : gi - 1) : (gi == 0 ? 0 : gi - 1)};
-    const int rows[3] = {yper ? (gj == ny - 1 ? 0 : gj + 1) : gj + 1, gj, yper ? (gj == 0 ? ny - 1 : gj - 1) : gj - 1};
-    const bool inflow_col = xio && gi == 0;
-
-    T raw[19], fcur[19], fnext[19];
-    T rcur = T(0), ucur = T(0), vcur = T(0), wcur = T(0);
-    const int m0 = k0 > 0 ? k0 - 1 : 0;
-    if (alive) lean_issue<S, T>(a, gi, gj, m0, cols, rows, raw);
-    for (int m = m0; m <= k1; m++) {
-        T rn = T(0), un = T(0), vn = T(0), wn = T(0);
-        if (alive && m < nz) {
-#pragma unroll
-            for (int q = 0; q < 19; q++) fnext[q] = raw[q];
-            if (m + 1 <= k1 && m + 1 < nz) lean_issue<S, T>(a, gi, gj, m + 1, cols, rows, raw);
-            if (a.mode != 0 && inflow_col) {
-                // inflow ghost: feq(rho0, u_in) of the source row/plane
-#pragma unroll
-                for (int q = 1; q < 19; q++) {
-                    if (cxq(q) <= 0) continue;
-                    int sk = m - czq(q);
-                    int pq = q;
-                    if (m == nz - 1 && czq(q) < 0) {
-                        if (c.top_boundary != LBM_TOP_FREE_SLIP) continue;
-                        sk = nz - 1;
-                        pq = mirzq(q);
-                    }
-                    fnext[q] = feq(c.rho0, a.inflow[(size_t)sk * ny + rows[cyq(q) + 1]], T(0), T(0), pq);
-                }
-            }
-            if (a.mode != 0) {
-                if (c.x_fringe && gi < 5 && m > nz / 2) {
-                    T sfr = T(1.0) - (T)gi / T(4.0);
-                    T kr = (T)m / (T)(nz - 1);
-                    T strength = T(0.05) * (kr * kr - T(0.25));
-                    T sigma = strength * smoothstep01(sfr);
-                    T uxt = a.u_inlet[m] + T(0.0);
-#pragma unroll
-                    for (int q = 0; q < 19; q++) fnext[q] = (T(1.0) - sigma) * fnext[q] + sigma * feq(c.rho0, uxt, T(0), T(0), q);
-                }
-                if (c.top_drive && m == nz - 1) {
-#pragma unroll
-                    for (int q = 0; q < 19; q++) fnext[q] = feq(c.rho0, c.u_top_lb, T(0), T(0), q);
-                }
-            }
-            T rr = T(0);
-#pragma unroll
-            for (int q = 0; q < 19; q++) rr += fnext[q];
-            T jx = T(0), jy = T(0), jz = T(0);
-#pragma unroll
-            for (int q = 0; q < 19; q++) {
-                if (cxq(q)) jx = (q == 1) ? fnext[q] : (cxq(q) > 0 ? jx + fnext[q] : jx - fnext[q]);
-                if (cyq(q)) jy = (q == 3) ? fnext[q] : (cyq(q) > 0 ? jy + fnext[q] : jy - fnext[q]);
-                if (czq(q)) jz = (q == 5) ? fnext[q] : (czq(q) > 0 ? jz + fnext[q] : jz - fnext[q]);
-            }
-            T inv = T(1.0) / rr;
-            un = jx * inv;
-            vn = jy * inv;
-            wn = jz * inv;
-            if (c.top_drive && m == nz - 1) {
-                un = c.u_top_lb;
-                vn = T(0);
-                wn = T(0);
-            }
-            rn = rr;
-            const int slot = m % 3;
-            s_u[slot][0][lcell] = un;
-            s_u[slot][1][lcell] = vn;
-            s_u[slot][2][lcell] = wn;
-        }
-        __syncthreads();
-        const int k = m - 1;
-        if (interior && k >= k0) {
-            const int i = i0 + lx, j = j0 + ly;
-            const uint32_t id = ((uint32_t)k * ny + j) * nx + i;
-            T om = T(1.0) / c.tau0;
-            if (c.use_les) {
-                const int sk = k % 3;
-                const int sp = (k + 1 < nz ? k + 1 : k) % 3;
-                const int sm = (k > 0 ? k - 1 : k) % 3;
-                const int cx_p = lcell + 1, cx_m = lcell - 1, cy_p = lcell + FL_X, cy_m = lcell - FL_X;
-                om = les_omega(c, s_u[sk][0][cx_p], s_u[sk][0][cx_m], s_u[sk][0][cy_p], s_u[sk][0][cy_m], s_u[sp][0][lcell],
-                               s_u[sm][0][lcell], s_u[sk][1][cx_p], s_u[sk][1][cx_m], s_u[sk][1][cy_p], s_u[sk][1][cy_m],
-                               s_u[sp][1][lcell], s_u[sm][1][lcell], s_u[sk][2][cx_p], s_u[sk][2][cx_m], s_u[sk][2][cy_p],
-                               s_u[sk][2][cy_m], s_u[sp][2][lcell], s_u[sm][2][lcell], false, T(0), ucur, vcur, T(0), T(0),
-                               (T*)nullptr, (T*)nullptr);
-            }
-            if (c.collision_reg) {
-                fused_regularized(fcur, rcur, ucur, vcur, wcur, om);
-            } else if (c.top_sponge && k > nz - 3) {
-                sponge_relax(fcur, rcur, ucur, vcur, wcur);
-            } else if (c.c_backscatter > T(0)) {
-                float xi = bs_uniform(c.seed, a.step, id);
-                collide_kbc<T, false>(fcur, rcur, ucur, vcur, wcur, om, T(0), true, c.c_backscatter, xi);
-            } else {
-                fused_kbc(fcur, rcur, ucur, vcur, wcur, om);
-            }
-            S* const fout = a.fout;
-            {
-                const uint32_t dbase = c.store.stochastic ? dither_base(a.step, (uint32_t)id) : 0u;
-#pragma unroll
-                for (int q = 0; q < 19; q++) Store<S, T>::store(fout + (size_t)q * n, id, q, fcur[q], c.store, dbase);
-            }
-            if (a.write_macro) {
-                a.rho[id] = rcur;
-                a.ux[id] = ucur;
-                a.uy[id] = vcur;
-                a.uz[id] = wcur;
-                a.omega[id] = om;
-            }
-        }
-        __syncthreads();
-#pragma unroll
-        for (int q = 0; q < 19; q++) fcur[q] = fnext[q];
-        rcur = rn;
-        ucur = un;
-        vcur = vn;
-        wcur = wn;
-    }
+    const uint32_t plane = (uint32_t)nx * ny;
+    const uint32_t id = (uint32_t)k * plane + (uint32_t)j * nx + i;
+    T f[19];
+    bool open_cur;
+    fused_gather<S, T, BAND>(a, i, j, k, f, open_cur);
+    T rcur, ucur, vcur, wcur;
+    fused_moments(c, k, f, rcur, ucur, vcur, wcur);
+    const bool open_below = BAND && k > 0 ? a.open[id - plane] != 0 : true;
+    const int ip = i == nx - 1 ? 0 : i + 1, im = i == 0 ? nx - 1 : i - 1;
+    const int jp = j == ny - 1 ? 0 : j + 1, jm = j == 0 ? ny - 1 : j - 1;
+    const uint32_t row = (uint32_t)k * plane + (uint32_t)j * nx;
+    const uint32_t nbi[6] = {row + ip, row + im, (uint32_t)k * plane + (uint32_t)jp * nx + i,
+                             (uint32_t)k * plane + (uint32_t)jm * nx + i, (k + 1 < nz ? id + plane : id), (k > 0 ? id - plane : id)};
+    const T* u3[3] = {a.ux, a.uy, a.uz};
+    auto nbu = [&](int comp, int dir) { return u3[comp][nbi[dir]]; };
+    les_collide_store(a, i, j, k, id, f, open_cur, open_below, rcur, ucur, vcur, wcur, nbu);
 }
+
 
 // ===========================================================================
 // reductions (FP64)
@@ -2271,9 +2119,7 @@ __global__ __launch_bounds__(NT) void k_load_populations(Consts<T> c, const S* s
 // ===========================================================================
 // launchers
 // ===========================================================================
-bool fused_supported(const lbm_config_t& c) {
-    return c.physics_mode == LBM_PHYS_AIR && c.nx % FT_X == 0 && c.ny % FT_Y == 0 && c.nz >= 3;
-}
+bool fused_supported(const lbm_config_t& c) { return c.physics_mode == LBM_PHYS_AIR && c.nz >= 3; }
 
 template <class T>
 static void bed2d_t(const lbm_config_t& cfg, DevBufs& b, float t, int slot, hipStream_t s) {
@@ -2283,7 +2129,9 @@ static void bed2d_t(const lbm_config_t& cfg, DevBufs& b, float t, int slot, hipS
     k_bed2d_a<T><<<nblocks(n2), NT, 0, s>>>(c, (T)t, (T*)b.h[slot], (T*)b.uw[slot], (T*)b.vw[slot], (T*)b.ww[slot],
                                              (const T*)b.h[prev], (const T*)b.piecewise);
     k_bed2d_b<T><<<nblocks(n2), NT, 0, s>>>(c, (const T*)b.h[slot], (T*)b.eta9[slot], (T*)b.denom[slot], b.kband[slot],
-                                             (const T*)b.uw[slot], (const T*)b.vw[slot], (const T*)b.ww[slot], (T*)b.uw9[slot]);
+                                             (const T*)b.uw[slot], (const T*)b.vw[slot], (const T*)b.ww[slot], (T*)b.uw9[slot],
+                                             b.ksolid[slot]);
+    k_bed2d_c<T><<<nblocks(n2), NT, 0, s>>>(c, make_slot<T>(b, slot), (T*)b.fsol[slot]);
     HIP_OK(hipGetLastError());
 }
 void launch_bed2d(const lbm_config_t& c, DevBufs& b, float t, int slot, hipStream_t s) {
@@ -2486,7 +2334,7 @@ void launch_staged_step(const lbm_config_t& c, DevBufs& b, int& cur, int slot, i
 
 template <class S, class T>
 static void fused_t(const lbm_config_t& cfg, DevBufs& b, int& cur, int mode, int slot_prev, int slot, int64_t step, int write_macro,
-                    int omega_band_top, int band_top, int kchunk, hipStream_t s) {
+                    int omega_band_top, int band_top, int block_x, hipStream_t s) {
     FusedArgs<S, T> a;
     a.c = make_consts<T>(cfg, omega_band_top);
     a.s0 = make_slot<T>(b, slot_prev);
@@ -2500,34 +2348,43 @@ static void fused_t(const lbm_config_t& cfg, DevBufs& b, int& cur, int mode, int
     a.omega = (T*)b.omega;
     a.u_inlet = (const T*)b.u_inlet;
     a.inflow = (const T*)b.inflow;
+    a.open = b.cellflags;
     a.mode = mode;
     a.write_macro = write_macro;
-    a.kc = kchunk;
     a.step = (uint32_t)step;
-    // chunks whose planes k0-1 .. k1 reach the wall band run the general
-    // kernel; the others cannot meet a cut or closed link and run the lean one
-    const int nchunks = (cfg.nz + kchunk - 1) / kchunk;
-    int nband = nchunks;
-    if (band_top < cfg.nz - 1) nband = std::min(nchunks, (band_top + 1) / kchunk + 1);
-    if (nband > 0) {
-        a.chunk0 = 0;
-        k_fused<S, T, true><<<dim3(cfg.nx / FT_X, cfg.ny / FT_Y, nband), FT_THREADS, 0, s>>>(a);
+    // planes 0 .. band_top can meet a cut or closed link (general kernels);
+    // the planes above cannot (lean kernels)
+    const int nb = band_top < 0 ? 0 : std::min(cfg.nz, band_top + 1);
+    const int bxd = block_x > 0 ? block_x : 64;
+    const dim3 blk(bxd, FB_THREADS / bxd);
+    const unsigned gx = (cfg.nx + blk.x - 1) / blk.x, gy = (cfg.ny + blk.y - 1) / blk.y;
+    if (nb > 0) {
+        a.plane0 = 0;
+        k_moments<S, T, true><<<dim3(gx, gy, nb), blk, 0, s>>>(a);
     }
-    if (nchunks > nband) {
-        a.chunk0 = nband;
-        k_fused_lean<S, T><<<dim3(cfg.nx / FT_X, cfg.ny / FT_Y, nchunks - nband), FT_THREADS, 0, s>>>(a);
+    if (nb < cfg.nz) {
+        a.plane0 = nb;
+        k_moments<S, T, false><<<dim3(gx, gy, cfg.nz - nb), blk, 0, s>>>(a);
+    }
+    if (nb > 0) {
+        a.plane0 = 0;
+        k_collide<S, T, true><<<dim3(gx, gy, nb), blk, 0, s>>>(a);
+    }
+    if (nb < cfg.nz) {
+        a.plane0 = nb;
+        k_collide<S, T, false><<<dim3(gx, gy, cfg.nz - nb), blk, 0, s>>>(a);
     }
     HIP_OK(hipGetLastError());
     cur ^= 1;
 }
 void launch_fused_step(const lbm_config_t& c, DevBufs& b, int& cur, int mode, int slot_prev, int slot, int64_t step,
-                       int write_macro, int omega_band_top, int band_top, int kchunk, hipStream_t s) {
-    if (!fused_supported(c)) throw std::runtime_error("fused path: needs air mode, nx % 32 == 0 and ny % 16 == 0");
+                       int write_macro, int omega_band_top, int band_top, int block_x, hipStream_t s) {
+    if (!fused_supported(c)) throw std::runtime_error("fused path: needs air mode");
     switch (b.storage) {
-        case LBM_STORE_FP64: fused_t<double, double>(c, b, cur, mode, slot_prev, slot, step, write_macro, omega_band_top, band_top, kchunk, s); break;
-        case LBM_STORE_FP16: fused_t<__half, float>(c, b, cur, mode, slot_prev, slot, step, write_macro, omega_band_top, band_top, kchunk, s); break;
-        case LBM_STORE_BF16: fused_t<bf16_t, float>(c, b, cur, mode, slot_prev, slot, step, write_macro, omega_band_top, band_top, kchunk, s); break;
-        default: fused_t<float, float>(c, b, cur, mode, slot_prev, slot, step, write_macro, omega_band_top, band_top, kchunk, s); break;
+        case LBM_STORE_FP64: fused_t<double, double>(c, b, cur, mode, slot_prev, slot, step, write_macro, omega_band_top, band_top, block_x, s); break;
+        case LBM_STORE_FP16: fused_t<__half, float>(c, b, cur, mode, slot_prev, slot, step, write_macro, omega_band_top, band_top, block_x, s); break;
+        case LBM_STORE_BF16: fused_t<bf16_t, float>(c, b, cur, mode, slot_prev, slot, step, write_macro, omega_band_top, band_top, block_x, s); break;
+        default: fused_t<float, float>(c, b, cur, mode, slot_prev, slot, step, write_macro, omega_band_top, band_top, block_x, s); break;
     }
 }
 

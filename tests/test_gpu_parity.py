@@ -91,8 +91,6 @@ def test_geometry_bit_exact(name):
 def test_step_parity_with_oracle(name, path):
     env = dict(CASES[name], LBM_KERNEL_PATH=path)
     cfg = parse_env(env)
-    if path == "fused" and (cfg.Nx % 32 or cfg.Ny % 16):
-        pytest.skip("fused tile needs nx % 32 == 0 and ny % 16 == 0")
     _, o = oracle_case(env)
     _, c = device_case(env)
     steps = 100
@@ -125,6 +123,22 @@ def test_fused_equals_staged_bitwise(name):
             assert np.array_equal(a, b), (name, fid, int(np.sum(a != b)), relerr(a, b))
     s.close()
     f.close()
+
+
+@pytest.mark.parametrize("opts", [{"block_x": 64}, {"block_x": 128}, {"block_x": 256}])
+def test_fused_launch_shapes_bitwise(opts):
+    """The block shape of the fused passes changes the schedule only."""
+    env = dict(CASES["waves"], LBM_KERNEL_PATH="fused")
+    _, a = device_case(env)
+    _, b = device_case(env)
+    for key, value in opts.items():
+        b.set_option(key, value)
+    a.step(12)
+    b.step(12)
+    for fid in (abi.FIELD_F, abi.FIELD_UX, abi.FIELD_OMEGA):
+        assert np.array_equal(a.download(fid), b.download(fid)), (opts, fid)
+    a.close()
+    b.close()
 
 
 def test_fp64_parity_with_fp64_oracle():

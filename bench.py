@@ -98,6 +98,68 @@ def cpu_baseline(env: dict, budget_s: float = 15.0):
     }
 
 
+def serving(n_cases: int, storage: str, seed: int = 7):
+    """Serving stream (half air over a moving sinusoidal bed, half water
+    Gaussian mounds; sizes 32^3 .. 128x64x64; U[500, 2000] steps):
+    the same requests run one at a time (sequential plain runs) and through
+    the ensemble server, first as a burst (its capacity), then as Poisson
+    arrivals at 0.8 x that capacity (latency)."""
+    import numpy as np
+
+    from lbm_server import EnsembleServer, ensemble_stream
+    from lbm_solver import Solver
+    from lbm_config import parse_env
+
+    stream = ensemble_stream(n_cases, seed=seed, storage=storage)
+    lups = sum(parse_env(e).Nx * parse_env(e).Ny * parse_env(e).Nz * st for e, st in stream)
+    t0 = time.perf_counter()
+    for env, steps in stream:
+        s = Solver(config=parse_env(env))
+        s.initialize()
+        s.step(steps)
+        s.case.download(1)
+        s.close()
+    seq_wall = time.perf_counter() - t0
+    budget = 200 << 30
+    srv = EnsembleServer(budget, max_active=64, chunk_steps=16, n_workers=4)
+    t0 = time.perf_counter()
+    for env, steps in stream:
+        srv.submit(env, steps)
+    srv.run_until_idle()
+    burst = srv.stats(t0)
+    srv.close()
+    capacity = n_cases / burst["wall_s"]
+    rate = 0.8 * capacity
+    rng = np.random.default_rng(seed + 1)
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, size=n_cases))
+    srv = EnsembleServer(budget, max_active=64, chunk_steps=16, n_workers=4)
+    t0 = time.perf_counter()
+    nxt = 0
+    while nxt < n_cases or not srv.idle():
+        now = time.perf_counter() - t0
+        while nxt < n_cases and arrivals[nxt] <= now:
+            srv.submit(*stream[nxt])
+            nxt += 1
+        srv.pump()
+        time.sleep(2e-4)
+    poisson = srv.stats(t0)
+    srv.close()
+    return {
+        "cases": n_cases,
+        "storage_air": storage,
+        "lattice_updates": lups,
+        "sequential_mlups": round(lups / seq_wall / 1e6, 1),
+        "ensemble_mlups": round(burst["aggregate_mlups"], 1),
+        "speedup_vs_sequential": round(burst["aggregate_mlups"] / (lups / seq_wall / 1e6), 3),
+        "poisson_rate_per_s": round(rate, 3),
+        "poisson_mlups": round(poisson["aggregate_mlups"], 1),
+        "latency_p50_s": round(poisson["latency_p50_s"], 4),
+        "latency_p99_s": round(poisson["latency_p99_s"], 4),
+        "peak_active": max(burst["peak_active"], poisson["peak_active"]),
+        "failed": burst["failed"] + poisson["failed"],
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -106,6 +168,7 @@ def main():
     ap.add_argument("--workload", default="waves_256x128x128")
     ap.add_argument("--no-variants", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--serving-cases", type=int, default=64, help="requests per precision of the serving stream (0: skip)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a plain device copy of 1 GiB (3 launches): the reference kernel for "
                          "scaling the FETCH_SIZE / WRITE_SIZE counters")
@@ -184,6 +247,8 @@ def main():
                 "hbm_frac": round(r["cells"] * b / (r["ms_kernel"] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
             }
         result["variants"] = variants
+    if args.serving_cases > 0 and not args.no_variants:
+        result["serving"] = {st: serving(args.serving_cases, st) for st in ("fp32", "fp16")}
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(env)
     print(json.dumps(result))

@@ -265,6 +265,40 @@ int lbm_set_option(lbm_handle_t *h, const char *key, int64_t value);
 /* Resolved execution path of the handle: LBM_PATH_STAGED or LBM_PATH_FUSED. */
 int lbm_get_path(lbm_handle_t *h, int32_t *out);
 
+/* *idle = 1 when all work queued on the handle's stream has finished, 0
+   otherwise; never blocks (no counterpart: used by the ensemble server). */
+int lbm_query(lbm_handle_t *h, int32_t *idle);
+
+/* ---- ensemble server (no counterpart in the reference) ------------------
+ * A native stepping runtime for many independent cases on one GPU.  Worker
+ * threads own the cases handed to them and advance each on its own stream in
+ * chunks of chunk_steps, so that the device work of all active cases
+ * overlaps; a case is reported finished once its last step has completed on
+ * the device.  Between lbm_server_add and the poll that reports it finished,
+ * the handle belongs to the server: the caller must not use it.  Admission
+ * (by device footprint, lbm_device_bytes), initialisation and result
+ * download stay with the caller (the Python EnsembleServer). */
+typedef struct lbm_server lbm_server_t;
+
+/* Start n_workers (1..64) stepping threads. */
+int lbm_server_create(int32_t chunk_steps, int32_t n_workers, lbm_server_t **out);
+
+/* Hand over an initialised case to advance by nsteps; ticket is the caller's
+   identifier, reported back by lbm_server_poll. */
+int lbm_server_add(lbm_server_t *s, lbm_handle_t *h, int32_t nsteps, int64_t ticket);
+
+/* Move up to max finished tickets into tickets[] (*n_done of them) together
+   with their status (0, or the error code of a failed lbm_step). */
+int lbm_server_poll(lbm_server_t *s, int64_t *tickets, int32_t *status, int32_t max, int32_t *n_done);
+
+/* Cases still advancing. */
+int lbm_server_active(lbm_server_t *s, int32_t *n_active);
+
+/* Stop the workers after the queued chunks of every case have completed
+   (cases not yet finished are abandoned, their handles stay valid) and free
+   the server. */
+int lbm_server_destroy(lbm_server_t *s);
+
 #ifdef __cplusplus
 }
 #endif

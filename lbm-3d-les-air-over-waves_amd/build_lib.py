@@ -21,7 +21,7 @@ PKG_DIR = Path(__file__).resolve().parent
 CSRC = PKG_DIR / "csrc"
 BUILD = PKG_DIR / "_build"
 LIB = PKG_DIR / "liblbm.so"
-SOURCES = ["lbm_kernels.hip", "lbm_capi.hip"]
+SOURCES = ["lbm_kernels.hip", "lbm_capi.hip", "lbm_server.cpp"]
 ARCH = os.environ.get("LBM_OFFLOAD_ARCH", "gfx950")
 
 FLAGS = [
@@ -48,7 +48,7 @@ def hipcc() -> str:
 
 def _compile(src: str) -> Path:
     obj = BUILD / (Path(src).stem + ".o")
-    cmd = [hipcc(), *FLAGS, "-c", str(CSRC / src), "-o", str(obj)]
+    cmd = [hipcc(), *FLAGS, "-pthread", "-c", str(CSRC / src), "-o", str(obj)]
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         raise RuntimeError(f"hipcc failed for {src}:\n{res.stderr}")
@@ -61,7 +61,7 @@ def build(verbose: bool = False) -> Path:
     with ThreadPoolExecutor(max_workers=len(SOURCES)) as pool:
         objs = list(pool.map(_compile, SOURCES))
     tmp = LIB.with_suffix(".so.tmp")
-    cmd = [hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", *map(str, objs), "-o", str(tmp)]
+    cmd = [hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-pthread", *map(str, objs), "-o", str(tmp)]
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         raise RuntimeError(f"link failed:\n{res.stderr}")

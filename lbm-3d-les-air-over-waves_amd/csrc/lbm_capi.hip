@@ -668,6 +668,20 @@ int lbm_sync(lbm_handle_t* h) {
     API_END(h)
 }
 
+int lbm_query(lbm_handle_t* h, int32_t* idle) {
+    if (!h || !idle) return LBM_E_ARG;
+    hipError_t e = hipStreamQuery(h->stream);
+    if (e == hipSuccess) {
+        *idle = 1;
+        return LBM_OK;
+    }
+    if (e == hipErrorNotReady) {
+        *idle = 0;
+        return LBM_OK;
+    }
+    return fail(h, LBM_E_HIP, std::string("hipStreamQuery: ") + hipGetErrorString(e));
+}
+
 int lbm_last_step_timing(lbm_handle_t* h, double* ms_per_step, double* ms_main_kernel) {
     if (!h) return LBM_E_ARG;
     if (!h->timing) return fail(h, LBM_E_STATE, "timing is off (lbm_set_option(h, \"timing\", 1))");

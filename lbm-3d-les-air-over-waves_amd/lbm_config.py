@@ -515,8 +515,10 @@ def parse_env(env: Optional[Mapping[str, str]] = None) -> SolverConfig:
         unsupported.append("LBM_BOUNDARY_GEOMETRY=phi_fraction|phi")
     if water_free_surface_enabled and tracking != "vof":
         unsupported.append("height-function free-surface tracking")
-    if water_free_surface_enabled and fs_initial != "flat":
-        unsupported.append("LBM_FREE_SURFACE_INITIAL=solitary|gaussian")
+    if water_free_surface_enabled and fs_initial == "solitary":
+        unsupported.append("LBM_FREE_SURFACE_INITIAL=solitary")
+    if water_free_surface_enabled and initial_pressure_mode == "linear_wave":
+        unsupported.append("LBM_INITIAL_PRESSURE_MODE=linear_wave")
     if init_fs_pops:
         unsupported.append("LBM_INIT_FREE_SURFACE_POPULATIONS=1")
     if physics_mode == "water" and not water_free_surface_enabled:

@@ -107,13 +107,32 @@ def initial_fields(cfg: SolverConfig, obstacle0: np.ndarray) -> dict:
     return out
 
 
+def surface_height(cfg: SolverConfig) -> np.ndarray:
+    """initialize_free_surface_height (CODEX:1197-1207): flat, or the
+    Gaussian hump gaussian_surface_height_at_x (CODEX:1185-1194), float32."""
+    f32 = np.float32
+    ny, nx = cfg.Ny, cfg.Nx
+    dx = f32(cfg.dx)
+    if cfg.free_surface_initial_condition != "gaussian":
+        return np.full((ny, nx), f32(cfg.free_surface_base_m), dtype=np.float32)
+    lx = f32(nx) * dx
+    xp = ((np.arange(nx, dtype=np.float32) + f32(0.5)) * dx).astype(np.float32)
+    xrel = (xp - f32(cfg.gaussian_center_x_m)).astype(np.float32)
+    if cfg.x_boundary_mode == "periodic":
+        xrel = np.where(xrel > f32(0.5) * lx, xrel - lx, xrel).astype(np.float32)
+        xrel = np.where(xrel < f32(-0.5) * lx, xrel + lx, xrel).astype(np.float32)
+    arg = (xrel / f32(cfg.gaussian_sigma_m)).astype(np.float32)
+    h = (f32(cfg.free_surface_base_m) + f32(cfg.gaussian_amp_m) * np.exp(f32(-0.5) * arg * arg)).astype(np.float32)
+    return np.broadcast_to(h[None, :], (ny, nx)).astype(np.float32)
+
+
 def free_surface_state(cfg: SolverConfig, obstacle0: np.ndarray):
-    """initialize_free_surface_height (flat), classify_free_surface_cells and
+    """initialize_free_surface_height, classify_free_surface_cells and
     apply_vof_initial_disconnected_shape (CODEX:1197-1287) in float32."""
     f32 = np.float32
     nz, ny, nx = cfg.Nz, cfg.Ny, cfg.Nx
     dx = f32(cfg.dx)
-    fs_h = np.full((ny, nx), f32(cfg.free_surface_base_m), dtype=np.float32)
+    fs_h = surface_height(cfg)
     z_bottom = np.arange(nz, dtype=np.float32)[:, None, None] * dx
     fill = ((fs_h[None, :, :] - z_bottom) / dx).astype(np.float32)
     fill = np.clip(fill, f32(0.0), f32(1.0))

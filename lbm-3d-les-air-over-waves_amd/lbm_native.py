@@ -36,6 +36,12 @@ EXPORTS = [
     "lbm_device_bytes",
     "lbm_set_option",
     "lbm_get_path",
+    "lbm_query",
+    "lbm_server_create",
+    "lbm_server_add",
+    "lbm_server_poll",
+    "lbm_server_active",
+    "lbm_server_destroy",
 ]
 
 
@@ -68,6 +74,14 @@ def load_library(path: Optional[os.PathLike] = None) -> ctypes.CDLL:
     lib.lbm_device_bytes.argtypes = [ctypes.POINTER(abi.LbmConfig), ctypes.POINTER(ctypes.c_size_t)]
     lib.lbm_set_option.argtypes = [H, ctypes.c_char_p, ctypes.c_int64]
     lib.lbm_get_path.argtypes = [H, ctypes.POINTER(ctypes.c_int32)]
+    lib.lbm_query.argtypes = [H, ctypes.POINTER(ctypes.c_int32)]
+    S = ctypes.c_void_p
+    lib.lbm_server_create.argtypes = [ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(S)]
+    lib.lbm_server_add.argtypes = [S, H, ctypes.c_int32, ctypes.c_int64]
+    lib.lbm_server_poll.argtypes = [S, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int32), ctypes.c_int32,
+                                    ctypes.POINTER(ctypes.c_int32)]
+    lib.lbm_server_active.argtypes = [S, ctypes.POINTER(ctypes.c_int32)]
+    lib.lbm_server_destroy.argtypes = [S]
     for name in EXPORTS:
         if name not in ("lbm_destroy", "lbm_last_error"):
             getattr(lib, name).restype = ctypes.c_int

@@ -90,12 +90,11 @@ struct Store<T, T> {  // full-precision storage: the population itself
 };
 template <>
 struct Store<__half, float> {  // (f_q - s_q) * 2^k as IEEE half
-    template <class I>
-    static __device__ __forceinline__ float load(const __half* p, I i, int q, const Codec& k) {
-        return k.shift[q] + __half2float(p[i]) * k.inv_scale;
+    static __device__ __forceinline__ float decode_with(uint16_t bits, float shift, const Codec& k) {
+        return shift + __half2float(__ushort_as_half(bits)) * k.inv_scale;
     }
-    template <class I>
-    static __device__ __forceinline__ void store(__half* p, I i, int q, float v, const Codec& k, uint32_t base) {
+    static __device__ __forceinline__ float decode(uint16_t bits, int q, const Codec& k) { return decode_with(bits, k.shift[q], k); }
+    static __device__ __forceinline__ uint16_t encode(float v, int q, const Codec& k, uint32_t base) {
         float d = (v - k.shift[q]) * k.scale;
         if (k.stochastic) {
             // keep 10 fraction bits: add 13 dither bits below them, truncate;
@@ -106,7 +105,15 @@ struct Store<__half, float> {  // (f_q - s_q) * 2^k as IEEE half
             if (a < 0x7f800000u) a = (a + (dither_q(base, q) >> 19)) & ~0x1fffu;
             d = __uint_as_float(a | (u & 0x80000000u));
         }
-        p[i] = __float2half_rn(d);
+        return __half_as_ushort(__float2half_rn(d));
+    }
+    template <class I>
+    static __device__ __forceinline__ float load(const __half* p, I i, int q, const Codec& k) {
+        return decode(__half_as_ushort(p[i]), q, k);
+    }
+    template <class I>
+    static __device__ __forceinline__ void store(__half* p, I i, int q, float v, const Codec& k, uint32_t base) {
+        p[i] = __ushort_as_half(encode(v, q, k, base));
     }
 };
 __device__ __forceinline__ float bf16_to_float(bf16_t b) { return __uint_as_float(((uint32_t)b.v) << 16); }
@@ -136,14 +143,23 @@ __device__ __forceinline__ bf16_t float_to_bf16_sr(float f, uint32_t r) {
 }
 template <>
 struct Store<bf16_t, float> {  // (f_q - s_q) * 2^k as bfloat16
+    static __device__ __forceinline__ float decode_with(uint16_t bits, float shift, const Codec& k) {
+        bf16_t b;
+        b.v = bits;
+        return shift + bf16_to_float(b) * k.inv_scale;
+    }
+    static __device__ __forceinline__ float decode(uint16_t bits, int q, const Codec& k) { return decode_with(bits, k.shift[q], k); }
+    static __device__ __forceinline__ uint16_t encode(float v, int q, const Codec& k, uint32_t base) {
+        float d = (v - k.shift[q]) * k.scale;
+        return (k.stochastic ? float_to_bf16_sr(d, dither_q(base, q)) : float_to_bf16(d)).v;
+    }
     template <class I>
     static __device__ __forceinline__ float load(const bf16_t* p, I i, int q, const Codec& k) {
-        return k.shift[q] + bf16_to_float(p[i]) * k.inv_scale;
+        return decode(p[i].v, q, k);
     }
     template <class I>
     static __device__ __forceinline__ void store(bf16_t* p, I i, int q, float v, const Codec& k, uint32_t base) {
-        float d = (v - k.shift[q]) * k.scale;
-        p[i] = k.stochastic ? float_to_bf16_sr(d, dither_q(base, q)) : float_to_bf16(d);
+        p[i].v = encode(v, q, k, base);
     }
 };
 

@@ -196,7 +196,7 @@ __global__ __launch_bounds__(NT) void k_bed2d_a(Consts<T> c, T t, T* h, T* uw, T
 // a cut or closed link (cells above it are fully open: frac = 1 everywhere)
 template <class T>
 __global__ __launch_bounds__(NT) void k_bed2d_b(Consts<T> c, const T* h, T* eta9, T* denom, int* kband, const T* uw, const T* vw, const T* ww, T* uw9,
-                                                int* ksolid) {
+                                                int* ksolid, T* fsol) {
     size_t id = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     size_t plane = (size_t)c.nx * c.ny;
     if (id >= plane) return;
@@ -204,18 +204,25 @@ __global__ __launch_bounds__(NT) void k_bed2d_b(Consts<T> c, const T* h, T* eta9
     T dx = c.dx;
     T xc = ((T)i + T(0.5)) * dx, yc = ((T)j + T(0.5)) * dx;
     T hmax = T(-1e30);
+    T e9[9], w9[27];
+#pragma unroll
     for (int a = -1; a <= 1; a++)
+#pragma unroll
         for (int bb = -1; bb <= 1; bb++) {
             T xq = xc + T(0.5) * dx * (T)a;
             T yq = yc + T(0.5) * dx * (T)bb;
             T e = sample2d(h, xq, yq, c.nx, c.ny, dx);
             const int c9 = (a + 1) * 3 + (bb + 1);
+            e9[c9] = e;
             eta9[(size_t)c9 * plane + id] = e;
             hmax = e > hmax ? e : hmax;
             // wall velocity at the link endpoint (CODEX:937-988)
-            uw9[(size_t)c9 * plane + id] = sample2d(uw, xq, yq, c.nx, c.ny, dx);
-            uw9[(size_t)(9 + c9) * plane + id] = sample2d(vw, xq, yq, c.nx, c.ny, dx);
-            uw9[(size_t)(18 + c9) * plane + id] = sample2d(ww, xq, yq, c.nx, c.ny, dx);
+            w9[c9] = sample2d(uw, xq, yq, c.nx, c.ny, dx);
+            w9[9 + c9] = sample2d(vw, xq, yq, c.nx, c.ny, dx);
+            w9[18 + c9] = sample2d(ww, xq, yq, c.nx, c.ny, dx);
+            uw9[(size_t)c9 * plane + id] = w9[c9];
+            uw9[(size_t)(9 + c9) * plane + id] = w9[9 + c9];
+            uw9[(size_t)(18 + c9) * plane + id] = w9[18 + c9];
         }
     int ip = (i + 1) % c.nx, im = (i - 1 + c.nx) % c.nx;
     int jp = (j + 1) % c.ny, jm = (j - 1 + c.ny) % c.ny;
@@ -251,7 +258,7 @@ __global__ __launch_bounds__(NT) void k_bed2d_b(Consts<T> c, const T* h, T* eta9
 #pragma unroll
             for (int q = 1; q < 19; q++) {
                 T zq = zc + T(0.5) * dx * (T)czq(q);
-                T gap = zq - eta9[(size_t)c9q(q) * plane + id];
+                T gap = zq - e9[c9q(q)];
                 if (!(gap < T(0))) closed = false;
             }
             if (!closed) break;
@@ -259,30 +266,29 @@ __global__ __launch_bounds__(NT) void k_bed2d_b(Consts<T> c, const T* h, T* eta9
         }
     }
     ksolid[id] = ks;
-}
-
-// Populations of a cell whose links the bed closes all: the conditioning
-// of update_bed_populations_and_reinit (CODEX:2894-2940) sets every one to
-// feq(rho0, wall velocity of its link) and they depend on the column only.
-template <class T>
-__global__ __launch_bounds__(NT) void k_bed2d_c(Consts<T> c, Slot<T> s, T* fsol) {
-    size_t id = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    size_t plane = (size_t)c.nx * c.ny;
-    if (id >= plane) return;
-    int i = (int)(id % (size_t)c.nx), j = (int)(id / (size_t)c.nx);
-    const int k = s.ksolid[id];
-    if (k < 0) return;
-    for (int q = 0; q < 19; q++) {
-        int wt = c.include_bed ? LBM_WALL_BED : LBM_WALL_NONE;
-        if (q > 0) {
-            int lo;
-            bool w;
-            link_geo(c, s.eta9, id, plane, i, j, k, q, lo, w);
-            wt = link_wt(c, w, i, j, q);
+    // populations of the closed cells of this column: the conditioning of
+    // update_bed_populations_and_reinit (CODEX:2894-2940) sets each to
+    // feq(rho0, wall velocity of its link); they do not depend on k
+    if (ks >= 0) {
+#pragma unroll
+        for (int q = 0; q < 19; q++) {
+            int wt = c.include_bed ? LBM_WALL_BED : LBM_WALL_NONE;
+            if (q > 0) wt = link_wt(c, link_on_wall(c, i, j, q), i, j, q);
+            T ux = T(0), uy = T(0), uz = T(0);
+            if (wt == LBM_WALL_BED) {
+                if (c.wall_link) {
+                    const int e = c9q(q);
+                    ux = w9[e];
+                    uy = w9[9 + e];
+                    uz = w9[18 + e];
+                } else {
+                    ux = uw[id];
+                    uy = vw[id];
+                    uz = ww[id];
+                }
+            }
+            fsol[(size_t)q * plane + id] = feq(c.rho0, ux, uy, uz, q);
         }
-        T ux = T(0), uy = T(0), uz = T(0);
-        if (wt == LBM_WALL_BED) wall_velocity(c, s, i, j, q, ux, uy, uz);
-        fsol[(size_t)q * plane + id] = feq(c.rho0, ux, uy, uz, q);
     }
 }
 
@@ -1720,6 +1726,14 @@ __device__ __forceinline__ void fused_kbc(T f[19], T rho, T ux, T uy, T uz, T om
     for (int q = 0; q < 19; q++) f[q] = fe[q] + alpha * (f[q] - fe[q]);
 }
 
+// link q of a cell in column (i,j) leaves the domain through a solid x or y
+// wall (CODEX:2655-2672 for x; y walls are new)
+template <class T>
+__host__ __device__ __forceinline__ bool link_on_wall(const Consts<T>& c, int i, int j, int q) {
+    return (c.x_boundary == LBM_XB_SOLID && ((i == 0 && cxq(q) > 0) || (i == c.nx - 1 && cxq(q) < 0))) ||
+           (c.y_boundary == LBM_YB_SOLID && ((j == 0 && cyq(q) > 0) || (j == c.ny - 1 && cyq(q) < 0)));
+}
+
 // open fraction and open flag of link q of cell (i,j,k) from the column bed
 // samples (CODEX:2573-2680), bed obstacle + x/y wall overrides
 template <class T>
@@ -1734,14 +1748,10 @@ __device__ __forceinline__ T link_geo(const Consts<T>& c, const T* eta9, size_t 
     if (frac < T(0)) frac = T(0);
     if (frac > T(1)) frac = T(1);
     lo = ggap < T(0) ? 0 : 1;
-    wall = false;
-    if (q > 0) {
-        if ((c.x_boundary == LBM_XB_SOLID && ((i == 0 && cxq(q) > 0) || (i == c.nx - 1 && cxq(q) < 0))) ||
-            (c.y_boundary == LBM_YB_SOLID && ((j == 0 && cyq(q) > 0) || (j == c.ny - 1 && cyq(q) < 0)))) {
-            lo = 0;
-            frac = T(0);
-            wall = true;
-        }
+    wall = q > 0 && link_on_wall(c, i, j, q);
+    if (wall) {
+        lo = 0;
+        frac = T(0);
     }
     return frac;
 }
@@ -1806,7 +1816,7 @@ __device__ __forceinline__ void fused_gather(const FusedArgs<S, T>& a, int i, in
         // closed at step n+1: the conditioning replaces every population by
         // feq(rho0, wall velocity) with weight 1 and the streamed value with
         // weight 0 (equal whenever the streamed values are finite); the
-        // column's values come from k_bed2d_c
+        // column's values come from k_bed2d_b
         const T* fs = a.s1.fsol + col;
         const size_t plane2 = (size_t)nx * ny;
 #pragma unroll
@@ -1976,16 +1986,15 @@ __global__ __launch_bounds__(FB_THREADS, (FusedOcc<T, BAND>::k1)) void k_moments
     if (BAND) a.open[id] = open0 ? 1 : 0;
 }
 
-// LES viscosity (CODEX:3011-3141), collision (CODEX:3143-3416, air mode)
-// and store of one cell whose conditioned populations, moments and
-// neighbour velocities are known.  nbu(comp, dir) returns the neighbour
-// velocity component comp (0..2) in direction dir = x+, x-, y+, y-, z+, z-
-// (x and y wrap periodically, CODEX:3047-3053; z clamps at the ends).
+// LES viscosity (CODEX:3011-3141) and collision (CODEX:3143-3416, air mode)
+// of one cell whose conditioned populations, moments and neighbour
+// velocities are known, in place; returns omega.  nbu(comp, dir) returns the
+// neighbour velocity component comp (0..2) in direction dir = x+, x-, y+, y-,
+// z+, z- (x and y wrap periodically, CODEX:3047-3053; z clamps at the ends).
 template <class S, class T, class NB>
-__device__ __forceinline__ void les_collide_store(const FusedArgs<S, T>& a, int i, int j, int k, uint32_t id, T f[19], bool open_cur,
-                                                  bool open_below, T rcur, T ucur, T vcur, T wcur, const NB& nbu) {
+__device__ __forceinline__ T les_collide(const FusedArgs<S, T>& a, int i, int j, int k, uint32_t id, T f[19], bool open_cur, bool open_below,
+                                         T rcur, T ucur, T vcur, T wcur, const NB& nbu) {
     const Consts<T>& c = a.c;
-    const uint32_t n = (uint32_t)c.nx * c.ny * c.nz;
     T om = T(1.0) / c.tau0;
     if (c.use_les) {
         if (open_cur) {
@@ -2018,6 +2027,16 @@ __device__ __forceinline__ void les_collide_store(const FusedArgs<S, T>& a, int 
     } else {
         fused_kbc(f, rcur, ucur, vcur, wcur, om);
     }
+    return om;
+}
+
+// les_collide, then the store of f* (and of rho, omega on the last step)
+template <class S, class T, class NB>
+__device__ __forceinline__ void les_collide_store(const FusedArgs<S, T>& a, int i, int j, int k, uint32_t id, T f[19], bool open_cur,
+                                                  bool open_below, T rcur, T ucur, T vcur, T wcur, const NB& nbu) {
+    const Consts<T>& c = a.c;
+    const uint32_t n = (uint32_t)c.nx * c.ny * c.nz;
+    const T om = les_collide(a, i, j, k, id, f, open_cur, open_below, rcur, ucur, vcur, wcur, nbu);
     S* const fout = a.fout;
     const uint32_t dbase = c.store.stochastic ? dither_base(a.step, id) : 0u;
 #pragma unroll
@@ -2051,6 +2070,7 @@ __global__ __launch_bounds__(FB_THREADS, (FusedOcc<T, BAND>::k2)) void k_collide
     auto nbu = [&](int comp, int dir) { return u3[comp][nbi[dir]]; };
     les_collide_store(a, i, j, k, id, f, open_cur, open_below, rcur, ucur, vcur, wcur, nbu);
 }
+
 
 
 // ===========================================================================
@@ -2130,8 +2150,7 @@ static void bed2d_t(const lbm_config_t& cfg, DevBufs& b, float t, int slot, hipS
                                              (const T*)b.h[prev], (const T*)b.piecewise);
     k_bed2d_b<T><<<nblocks(n2), NT, 0, s>>>(c, (const T*)b.h[slot], (T*)b.eta9[slot], (T*)b.denom[slot], b.kband[slot],
                                              (const T*)b.uw[slot], (const T*)b.vw[slot], (const T*)b.ww[slot], (T*)b.uw9[slot],
-                                             b.ksolid[slot]);
-    k_bed2d_c<T><<<nblocks(n2), NT, 0, s>>>(c, make_slot<T>(b, slot), (T*)b.fsol[slot]);
+                                             b.ksolid[slot], (T*)b.fsol[slot]);
     HIP_OK(hipGetLastError());
 }
 void launch_bed2d(const lbm_config_t& c, DevBufs& b, float t, int slot, hipStream_t s) {

@@ -1737,12 +1737,11 @@ __host__ __device__ __forceinline__ bool link_on_wall(const Consts<T>& c, int i,
 // open fraction and open flag of link q of cell (i,j,k) from the column bed
 // samples (CODEX:2573-2680), bed obstacle + x/y wall overrides
 template <class T>
-__device__ __forceinline__ T link_geo(const Consts<T>& c, const T* eta9, size_t col, size_t plane2, int i, int j, int k, int q,
-                                      int& lo, bool& wall) {
+__device__ __forceinline__ T link_geo_e(const Consts<T>& c, T eta_q, int i, int j, int k, int q, int& lo, bool& wall) {
     const T dx = c.dx;
     T zc = ((T)k + T(0.5)) * dx;
     T zq = zc + T(0.5) * dx * (T)czq(q);
-    T gap = zq - eta9[(size_t)c9q(q) * plane2 + col];
+    T gap = zq - eta_q;
     T ggap = c.include_bed ? gap : T(1.0e6);
     T frac = T(0.5) + div_dx(c, ggap);
     if (frac < T(0)) frac = T(0);
@@ -1756,18 +1755,33 @@ __device__ __forceinline__ T link_geo(const Consts<T>& c, const T* eta9, size_t 
     return frac;
 }
 
-// open mask (bit q, q = 1..18) of a cell
 template <class T>
-__device__ __forceinline__ uint32_t link_mask(const Consts<T>& c, const T* eta9, size_t col, size_t plane2, int i, int j, int k) {
+__device__ __forceinline__ T link_geo(const Consts<T>& c, const T* eta9, size_t col, size_t plane2, int i, int j, int k, int q, int& lo,
+                                      bool& wall) {
+    return link_geo_e(c, eta9[(size_t)c9q(q) * plane2 + col], i, j, k, q, lo, wall);
+}
+
+// open mask (bit q, q = 1..18) of a cell from its column's 9 endpoint heights
+template <class T>
+__device__ __forceinline__ uint32_t link_mask_e(const Consts<T>& c, const T e9[9], int i, int j, int k) {
     uint32_t m = 0;
 #pragma unroll
     for (int q = 1; q < 19; q++) {
         int lo;
         bool w;
-        link_geo(c, eta9, col, plane2, i, j, k, q, lo, w);
+        link_geo_e(c, e9[c9q(q)], i, j, k, q, lo, w);
         m |= (uint32_t)lo << q;
     }
     return m;
+}
+
+// open mask (bit q, q = 1..18) of a cell
+template <class T>
+__device__ __forceinline__ uint32_t link_mask(const Consts<T>& c, const T* eta9, size_t col, size_t plane2, int i, int j, int k) {
+    T e9[9];
+#pragma unroll
+    for (int e = 0; e < 9; e++) e9[e] = eta9[(size_t)e * plane2 + col];
+    return link_mask_e(c, e9, i, j, k);
 }
 
 // wall type of link q (bed unless an x/y wall override)
@@ -1845,7 +1859,12 @@ __device__ __forceinline__ void fused_gather(const FusedArgs<S, T>& a, int i, in
         const bool slow = BAND && k <= a.s0.kband[col];
         const size_t plane2 = (size_t)nx * ny;
         uint32_t mask = 0x7fffeu;
-        if (slow) mask = link_mask(c, a.s0.eta9, col, plane2, i, j, k);
+        T e0[9];
+        if (slow) {
+#pragma unroll
+            for (int e = 0; e < 9; e++) e0[e] = a.s0.eta9[(size_t)e * plane2 + col];
+            mask = link_mask_e(c, e0, i, j, k);
+        }
         const bool any_open = (mask & 0x7fffeu) != 0;
         f[0] = LD(0, id);
 #pragma unroll
@@ -1865,7 +1884,7 @@ __device__ __forceinline__ void fused_gather(const FusedArgs<S, T>& a, int i, in
                 if (slow) {
                     int lo;
                     bool w;
-                    T frac = any_open ? link_geo(c, a.s0.eta9, col, plane2, i, j, k, q, lo, w) : T(0);
+                    T frac = any_open ? link_geo_e(c, e0[c9q(q)], i, j, k, q, lo, w) : T(0);
                     out = frac * src + (T(1.0) - frac) * LD(oppq(q), id);
                 } else {
                     out = src;
@@ -1887,7 +1906,7 @@ __device__ __forceinline__ void fused_gather(const FusedArgs<S, T>& a, int i, in
             for (int q = 1; q < 19; q++) {
                 int lo;
                 bool w;
-                T fo = link_geo(c, a.s0.eta9, col, plane2, i, j, k, q, lo, w);
+                T fo = link_geo_e(c, e0[c9q(q)], i, j, k, q, lo, w);
                 if (!any_open) fo = T(0);
                 const int qb = oppq(q);
                 T ux = T(0), uy = T(0), uz = T(0);
@@ -1903,14 +1922,17 @@ __device__ __forceinline__ void fused_gather(const FusedArgs<S, T>& a, int i, in
     bool open0 = true;
     if (BAND && k <= a.s1.kband[col]) {
         const size_t plane2 = (size_t)nx * ny;
-        uint32_t mask = link_mask(c, a.s1.eta9, col, plane2, i, j, k);
+        T e1[9];
+#pragma unroll
+        for (int e = 0; e < 9; e++) e1[e] = a.s1.eta9[(size_t)e * plane2 + col];
+        uint32_t mask = link_mask_e(c, e1, i, j, k);
         open0 = (mask & 0x7fffeu) != 0;
         if ((mask & 0x7fffeu) != 0x7fffeu || !open0) {
 #pragma unroll
             for (int q = 0; q < 19; q++) {
                 int lo;
                 bool w;
-                T frac = link_geo(c, a.s1.eta9, col, plane2, i, j, k, q, lo, w);
+                T frac = link_geo_e(c, e1[c9q(q)], i, j, k, q, lo, w);
                 if (q == 0) lo = open0 ? 1 : 0;
                 else if (!open0) frac = T(0);
                 if (lo) continue;
@@ -1923,6 +1945,82 @@ __device__ __forceinline__ void fused_gather(const FusedArgs<S, T>& a, int i, in
         }
     }
     open0_out = open0;
+}
+
+// Pull of step n into a cell above the wall band (every link open with
+// fraction 1): the lean branch of fused_gather, arranged so that every
+// decision depending on y or z is uniform over the wave (a wave lies in one
+// row of one plane: j is read from the first lane), and the rules of a
+// non-periodic x boundary are a fix-up of the edge cells.
+template <class S, class T>
+__device__ __forceinline__ void lean_gather(const FusedArgs<S, T>& a, int i, int j_lane, int k, T f[19]) {
+    const Consts<T>& c = a.c;
+    const int j = __builtin_amdgcn_readfirstlane(j_lane);
+    const int nx = c.nx, ny = c.ny, nz = c.nz;
+    const uint32_t n = (uint32_t)nx * ny * nz;
+    const uint32_t plane = (uint32_t)nx * ny;
+    const uint32_t id = (uint32_t)k * plane + (uint32_t)j * nx + i;
+    const Codec& cod = c.store;
+    const S* fin = a.fin;
+#define LD(q, idx) Store<S, T>::load(fin + (size_t)(q) * n, (uint32_t)(idx), (q), cod)
+    if (a.mode == 0) {
+#pragma unroll
+        for (int q = 0; q < 19; q++) f[q] = LD(q, id);
+        return;
+    }
+    const bool xper = c.x_boundary == LBM_XB_PERIODIC, xio = c.x_boundary == LBM_XB_INFLOW_OUTFLOW;
+    const bool yper = c.y_boundary == LBM_YB_PERIODIC;
+    const bool free_slip = c.top_boundary == LBM_TOP_FREE_SLIP;
+    // source columns: periodic wrap, else the edge cell itself (the
+    // outflow rule; the other edge rules are applied below)
+    const int im = i == 0 ? (xper ? nx - 1 : 0) : i - 1;
+    const int ip = i == nx - 1 ? (xper ? 0 : nx - 1) : i + 1;
+    const int jm = j == 0 ? (yper ? ny - 1 : 0) : j - 1;
+    const int jp = j == ny - 1 ? (yper ? 0 : ny - 1) : j + 1;
+    f[0] = LD(0, id);
+#pragma unroll
+    for (int q = 1; q < 19; q++) {
+        const int cxi = cxq(q), cyi = cyq(q), czi = czq(q);
+        const int si = cxi > 0 ? im : (cxi < 0 ? ip : i);
+        const int sj = cyi > 0 ? jm : (cyi < 0 ? jp : j);
+        const bool yout = !yper && ((cyi < 0 && j == ny - 1) || (cyi > 0 && j == 0));
+        const int sk = k - czi;
+        T out;
+        if (yout)
+            out = LD(oppq(q), id);
+        else if (sk >= 0 && sk < nz)
+            out = LD(q, (uint32_t)sk * plane + (uint32_t)sj * nx + si);
+        else if (sk >= nz && free_slip)
+            out = LD(mirzq(q), (uint32_t)(nz - 1) * plane + (uint32_t)sj * nx + si);
+        else
+            out = LD(q, id);
+        f[q] = out;
+    }
+    if (!xper && ((i == 0) || (xio ? false : i == nx - 1))) {
+        // solid x walls: bounce back (x-outside source); inflow column:
+        // equilibrium ghost at the inflow velocity (CODEX:3418-3551 + the
+        // inflow/outflow rule); a y-outside source still bounces back
+#pragma unroll
+        for (int q = 1; q < 19; q++) {
+            const int cxi = cxq(q), cyi = cyq(q), czi = czq(q);
+            if (!((i == 0 && cxi > 0) || (i == nx - 1 && cxi < 0))) continue;
+            const bool yout = !yper && ((cyi < 0 && j == ny - 1) || (cyi > 0 && j == 0));
+            if (!xio || yout) {
+                f[q] = LD(oppq(q), id);
+                continue;
+            }
+            const int sj = cyi > 0 ? jm : (cyi < 0 ? jp : j);
+            const int sk = k - czi;
+            if (sk >= 0 && sk < nz)
+                f[q] = feq(c.rho0, a.inflow[(size_t)sk * ny + sj], T(0), T(0), q);
+            else if (sk >= nz && free_slip)
+                f[q] = feq(c.rho0, a.inflow[(size_t)(nz - 1) * ny + sj], T(0), T(0), mirzq(q));
+            else
+                f[q] = LD(q, id);
+        }
+    }
+#undef LD
+    stream_edges(a, i, k, true, f);
 }
 
 // rho and u of step n+1 from the conditioned populations, in the order of
@@ -1961,22 +2059,23 @@ __device__ __forceinline__ bool fused_cell(const FusedArgs<S, T>& a, int& i, int
 }
 
 // waves per SIMD the kernels are compiled for (register budget 512 / waves):
-// the lean moments pass fits 8 waves in FP32, the band passes and the
-// collision 4; FP64 arithmetic halves each
+// in FP32 the lean moments pass fits 8 waves, the lean collision 5, the band
+// passes 4; FP64 arithmetic halves each
 template <class T, bool BAND>
 struct FusedOcc {
     static constexpr int k1 = (sizeof(T) == 8 ? 4 : 8) / (BAND ? 2 : 1);
-    static constexpr int k2 = sizeof(T) == 8 ? 2 : 4;
+    static constexpr int k2 = sizeof(T) == 8 ? 2 : (BAND ? 4 : 5);
 };
 
 template <class S, class T, bool BAND>
-__global__ __launch_bounds__(FB_THREADS, (FusedOcc<T, BAND>::k1)) void k_moments(FusedArgs<S, T> a) {
-    int i, j, k;
-    if (!fused_cell(a, i, j, k)) return;
+__device__ __forceinline__ void moments_cell(const FusedArgs<S, T>& a, int i, int j, int k) {
     const Consts<T>& c = a.c;
     T f[19];
-    bool open0;
-    fused_gather<S, T, BAND>(a, i, j, k, f, open0);
+    bool open0 = true;
+    if (BAND)
+        fused_gather<S, T, BAND>(a, i, j, k, f, open0);
+    else
+        lean_gather(a, i, j, k, f);
     T rr, un, vn, wn;
     fused_moments(c, k, f, rr, un, vn, wn);
     const uint32_t id = ((uint32_t)k * c.ny + j) * c.nx + i;
@@ -1984,6 +2083,13 @@ __global__ __launch_bounds__(FB_THREADS, (FusedOcc<T, BAND>::k1)) void k_moments
     a.uy[id] = vn;
     a.uz[id] = wn;
     if (BAND) a.open[id] = open0 ? 1 : 0;
+}
+
+template <class S, class T, bool BAND>
+__global__ __launch_bounds__(FB_THREADS, (FusedOcc<T, BAND>::k1)) void k_moments(FusedArgs<S, T> a) {
+    int i, j, k;
+    if (!fused_cell(a, i, j, k)) return;
+    moments_cell<S, T, BAND>(a, i, j, k);
 }
 
 // LES viscosity (CODEX:3011-3141) and collision (CODEX:3143-3416, air mode)
@@ -2048,16 +2154,17 @@ __device__ __forceinline__ void les_collide_store(const FusedArgs<S, T>& a, int 
 }
 
 template <class S, class T, bool BAND>
-__global__ __launch_bounds__(FB_THREADS, (FusedOcc<T, BAND>::k2)) void k_collide(FusedArgs<S, T> a) {
-    int i, j, k;
-    if (!fused_cell(a, i, j, k)) return;
+__device__ __forceinline__ void collide_cell(const FusedArgs<S, T>& a, int i, int j, int k) {
     const Consts<T>& c = a.c;
     const int nx = c.nx, ny = c.ny, nz = c.nz;
     const uint32_t plane = (uint32_t)nx * ny;
     const uint32_t id = (uint32_t)k * plane + (uint32_t)j * nx + i;
     T f[19];
-    bool open_cur;
-    fused_gather<S, T, BAND>(a, i, j, k, f, open_cur);
+    bool open_cur = true;
+    if (BAND)
+        fused_gather<S, T, BAND>(a, i, j, k, f, open_cur);
+    else
+        lean_gather(a, i, j, k, f);
     T rcur, ucur, vcur, wcur;
     fused_moments(c, k, f, rcur, ucur, vcur, wcur);
     const bool open_below = BAND && k > 0 ? a.open[id - plane] != 0 : true;
@@ -2071,7 +2178,12 @@ __global__ __launch_bounds__(FB_THREADS, (FusedOcc<T, BAND>::k2)) void k_collide
     les_collide_store(a, i, j, k, id, f, open_cur, open_below, rcur, ucur, vcur, wcur, nbu);
 }
 
-
+template <class S, class T, bool BAND>
+__global__ __launch_bounds__(FB_THREADS, (FusedOcc<T, BAND>::k2)) void k_collide(FusedArgs<S, T> a) {
+    int i, j, k;
+    if (!fused_cell(a, i, j, k)) return;
+    collide_cell<S, T, BAND>(a, i, j, k);
+}
 
 // ===========================================================================
 // reductions (FP64)

@@ -37,13 +37,13 @@ BYTES_PER_LU = {"fp32": 152, "fp16": 76, "bf16": 76, "fp64": 304}
 def run_device(env: dict, steps: int, warmup: int):
     import torch
 
-    from harness import setup_case
     from lbm_config import parse_env
-    from lbm_native import Case
+    from lbm_solver import Solver
 
     cfg = parse_env(env)
-    case = Case(cfg.to_struct())
-    setup_case(case, cfg)
+    solver = Solver(config=cfg)
+    solver.initialize()
+    case = solver.case
     case.set_option("timing", 1)
     if warmup:
         case.step(warmup)
@@ -160,6 +160,76 @@ def serving(n_cases: int, storage: str, seed: int = 7):
     }
 
 
+WORKLOADS = (
+    # label, preset, overrides, algorithmic bytes per lattice update
+    ("c1_plain_64x32x32_fp32", "plain_64x32x32", {"LBM_STORAGE": "fp32"}, 152),
+    ("c3_plain_512x256x192_fp32", "plain_512x256x192", {"LBM_STORAGE": "fp32"}, 152),
+    ("c3_plain_512x256x192_fp16", "plain_512x256x192", {"LBM_STORAGE": "fp16"}, 76),
+    ("c3_plain_512x256x192_bf16", "plain_512x256x192", {"LBM_STORAGE": "bf16"}, 76),
+    ("c4_vof_192x96x160_fp32", "vof_192x96x160", {}, 170),
+)
+
+
+def workloads(steps: int):
+    """The other benchmark configurations: MLUPS and the fraction of the HBM
+    roofline at their algorithmic bytes per update (VOF: populations +
+    mass, fill and type)."""
+    from lbm_config import preset_env
+
+    out = {}
+    for label, name, extra, bpl in WORKLOADS:
+        r = run_device(dict(preset_env(name), **extra), steps, 5)
+        out[label] = {
+            "mlups": round(r["mlups"], 1),
+            "kernel_ms": round(r["ms_kernel"], 5),
+            "hbm_frac": round(r["cells"] * bpl / (r["ms_kernel"] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+            "path": "fused" if r["path"] == 2 else "staged",
+        }
+    f32 = out["c3_plain_512x256x192_fp32"]["kernel_ms"]
+    for st in ("fp16", "bf16"):
+        out[f"c3_{st}_speedup_vs_fp32"] = round(f32 / out[f"c3_plain_512x256x192_{st}"]["kernel_ms"], 3)
+    return out
+
+
+def accuracy(steps: int = 2000):
+    """Reduced storage against the FP64 run of the same case: the plain
+    channel physics at 256x128x128, seed 12345, after `steps` steps: max
+    |u - u64| / max |u64|, max |dp| with dp = (rho - rho64)/3, and the
+    relative drift of the total mass."""
+    import numpy as np
+
+    import lbm_abi as abi
+    from lbm_config import parse_env, preset_env
+    from lbm_solver import Solver
+
+    base = dict(preset_env("plain_64x32x32"), LBM_LX_M="256", LBM_LY_M="128", LBM_LZ_M="128")
+
+    def run(storage):
+        solver = Solver(config=parse_env(dict(base, LBM_STORAGE=storage)))
+        solver.initialize()
+        case = solver.case
+        m0 = case.reduce(abi.REDUCE_MASS)
+        case.step(steps)
+        u = np.stack([case.download(x).astype(np.float64) for x in (abi.FIELD_UX, abi.FIELD_UY, abi.FIELD_UZ)])
+        rho = case.download(abi.FIELD_RHO).astype(np.float64)
+        m1 = case.reduce(abi.REDUCE_MASS)
+        case.close()
+        return u, rho, m0, m1
+
+    u64, r64, m0, m1 = run("fp64")
+    out = {"case": "plain channel physics 256x128x128, seed 12345", "steps": steps,
+           "fp64_mass_drift": abs(m1 - m0) / m0}
+    umax = float(np.max(np.linalg.norm(u64, axis=0)))
+    for st in ("fp32", "fp16", "bf16"):
+        u, r, a0, a1 = run(st)
+        out[st] = {
+            "velocity_rel_error": float(np.max(np.linalg.norm(u - u64, axis=0)) / umax),
+            "pressure_abs_error": float(np.max(np.abs(r - r64)) / 3.0),
+            "mass_drift": abs(a1 - a0) / a0,
+        }
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -176,7 +246,7 @@ def main():
     if args.gpus != 1:
         print(
             f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
-            "(the multi-GPU decomposition is described in DESIGN.md, not implemented)",
+            "(several GPUs are not implemented)",
             file=sys.stderr,
         )
         sys.exit(2)
@@ -247,6 +317,9 @@ def main():
                 "hbm_frac": round(r["cells"] * b / (r["ms_kernel"] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
             }
         result["variants"] = variants
+    if not args.no_variants:
+        result["workloads"] = workloads(max(20, args.steps // 4))
+        result["accuracy"] = accuracy()
     if args.serving_cases > 0 and not args.no_variants:
         result["serving"] = {st: serving(args.serving_cases, st) for st in ("fp32", "fp16")}
     if not args.no_cpu:

@@ -137,7 +137,7 @@ static size_t device_bytes(const lbm_config_t& c, int path) {
     size_t bytes = 2 * 19 * n * store_size(c);  // ping-pong populations
     bytes += 5 * n * R + n;                      // rho, u, omega, cell flags
     if (path == LBM_PATH_STAGED) bytes += 2 * n * R + 2 * n * 4;  // nu_t, smag, obstacle masks
-    bytes += 2 * n2 * (40 * R + 4) + (c.nx + c.nz + (size_t)c.nz * c.ny) * R;
+    bytes += 2 * n2 * (78 * R + 8) + (c.nx + c.nz + (size_t)c.nz * c.ny) * R;
     if (c.fs_enabled) bytes += n * (11 * R + 7 * 4) + 6 * n * R + n2 * R;
     return bytes + (1 << 20);
 }
@@ -210,6 +210,7 @@ static void allocate(lbm_handle* h) {
         b.kband[s] = (int*)dalloc(h, n2 * 4);
         b.ksolid[s] = (int*)dalloc(h, n2 * 4);
         b.fsol[s] = dalloc(h, 19 * n2 * R);
+        b.ibb[s] = dalloc(h, 18 * n2 * R);
         CHECK(hipMemsetAsync(b.uw[s], 0, n2 * R, h->stream));
         CHECK(hipMemsetAsync(b.vw[s], 0, n2 * R, h->stream));
         CHECK(hipMemsetAsync(b.ww[s], 0, n2 * R, h->stream));

@@ -318,7 +318,9 @@ struct Slot {
     const int* kband;   // [ny][nx] highest k with a cut or closed link
     const T* uw9;       // [3][9][ny][nx] wall velocity at the 9 link endpoints
     const int* ksolid;  // [ny][nx] highest k whose 18 links the bed closes (-1: none)
-    const T* fsol;      // [19][ny][nx] conditioned populations of those closed cells
+    const T* fsol;      // [19][ny][nx] feq(rho0, wall velocity of link q, q): the conditioned
+                        // value of a closed link, for every column
+    const T* ibb;       // [18][ny][nx] moving-wall term 6 w_qb rho0 (c_qb . u_wall) of link q = 1..18
 };
 
 // link geometry of one cell (CODEX:2492-2686): open flag, open fraction,

@@ -52,6 +52,7 @@ struct DevBufs {
     int* kband[2];
     int* ksolid[2];
     void* fsol[2];
+    void* ibb[2];
     void *piecewise, *u_inlet, *inflow;
     uint8_t* cellflags;  // bit0 lattice_open[0], bit1 near_obstacle (staged path, current slot)
     // water / VOF

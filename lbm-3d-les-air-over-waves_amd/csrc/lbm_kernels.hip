@@ -121,6 +121,7 @@ static Slot<T> make_slot(DevBufs& b, int s) {
     sl.uw9 = (const T*)b.uw9[s];
     sl.ksolid = b.ksolid[s];
     sl.fsol = (const T*)b.fsol[s];
+    sl.ibb = (const T*)b.ibb[s];
     return sl;
 }
 
@@ -196,7 +197,7 @@ __global__ __launch_bounds__(NT) void k_bed2d_a(Consts<T> c, T t, T* h, T* uw, T
 // a cut or closed link (cells above it are fully open: frac = 1 everywhere)
 template <class T>
 __global__ __launch_bounds__(NT) void k_bed2d_b(Consts<T> c, const T* h, T* eta9, T* denom, int* kband, const T* uw, const T* vw, const T* ww, T* uw9,
-                                                int* ksolid, T* fsol) {
+                                                int* ksolid, T* fsol, T* ibb) {
     size_t id = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     size_t plane = (size_t)c.nx * c.ny;
     if (id >= plane) return;
@@ -266,28 +267,33 @@ __global__ __launch_bounds__(NT) void k_bed2d_b(Consts<T> c, const T* h, T* eta9
         }
     }
     ksolid[id] = ks;
-    // populations of the closed cells of this column: the conditioning of
-    // update_bed_populations_and_reinit (CODEX:2894-2940) sets each to
-    // feq(rho0, wall velocity of its link); they do not depend on k
-    if (ks >= 0) {
+    // per-link wall terms of this column, independent of k: the value the
+    // conditioning of update_bed_populations_and_reinit (CODEX:2894-2940)
+    // gives a closed link, feq(rho0, wall velocity of the link), and the
+    // moving-wall term of the IBB (CODEX:3674-3712); x / y wall links have
+    // wall velocity 0
 #pragma unroll
-        for (int q = 0; q < 19; q++) {
-            int wt = c.include_bed ? LBM_WALL_BED : LBM_WALL_NONE;
-            if (q > 0) wt = link_wt(c, link_on_wall(c, i, j, q), i, j, q);
-            T ux = T(0), uy = T(0), uz = T(0);
-            if (wt == LBM_WALL_BED) {
-                if (c.wall_link) {
-                    const int e = c9q(q);
-                    ux = w9[e];
-                    uy = w9[9 + e];
-                    uz = w9[18 + e];
-                } else {
-                    ux = uw[id];
-                    uy = vw[id];
-                    uz = ww[id];
-                }
+    for (int q = 0; q < 19; q++) {
+        int wt = c.include_bed ? LBM_WALL_BED : LBM_WALL_NONE;
+        if (q > 0) wt = link_wt(c, link_on_wall(c, i, j, q), i, j, q);
+        T ux = T(0), uy = T(0), uz = T(0);
+        if (wt == LBM_WALL_BED) {
+            if (c.wall_link) {
+                const int e = c9q(q);
+                ux = w9[e];
+                uy = w9[9 + e];
+                uz = w9[18 + e];
+            } else {
+                ux = uw[id];
+                uy = vw[id];
+                uz = ww[id];
             }
-            fsol[(size_t)q * plane + id] = feq(c.rho0, ux, uy, uz, q);
+        }
+        fsol[(size_t)q * plane + id] = feq(c.rho0, ux, uy, uz, q);
+        if (q > 0) {
+            const int qb = oppq(q);
+            T cuw = (T)cxq(qb) * ux + (T)cyq(qb) * uy + (T)czq(qb) * uz;
+            ibb[(size_t)(q - 1) * plane + id] = T(6.0) * wq<T>(qb) * c.rho0 * cuw;
         }
     }
 }
@@ -1814,141 +1820,169 @@ __device__ __forceinline__ void stream_edges(const FusedArgs<S, T>& a, int i, in
     }
 }
 
-// stream (+ boundary rules) of step n and conditioning of step n+1 for one
-// cell; returns the populations entering the moments of step n+1
-template <class S, class T, bool BAND>
-__device__ __forceinline__ void fused_gather(const FusedArgs<S, T>& a, int i, int j, int k, T f[19], bool& open0_out) {
+// Pull of step n (+ boundary rules) and conditioning of step n+1 for a cell
+// of the wall band; returns the populations entering the moments of step
+// n+1 and whether lattice_open[0] holds at step n+1.  The rules follow
+// stream (CODEX:3418-3551), the x walls / fringe / top drive / IBB of
+// CODEX:3554-3712 and update_bed_populations_and_reinit (CODEX:2894-2940).
+// A wave lies in one row (j) of one plane (k), so y and z decisions are
+// uniform over it; per-lane x decisions (bed, x edges) are selects, and all
+// loads of a cell are issued before the arithmetic that consumes them.
+template <class S, class T>
+__device__ __forceinline__ void band_gather(const FusedArgs<S, T>& a, int i, int j_lane, int k, T f[19], bool& open0_out) {
     const Consts<T>& c = a.c;
+    const int j = __builtin_amdgcn_readfirstlane(j_lane);
     const int nx = c.nx, ny = c.ny, nz = c.nz;
     const uint32_t n = (uint32_t)nx * ny * nz;
     const uint32_t plane = (uint32_t)nx * ny;
     const uint32_t id = (uint32_t)k * plane + (uint32_t)j * nx + i;
-    const Codec& inv = c.store;
-    const size_t col = (size_t)j * nx + i;
+    const Codec& cod = c.store;
+    const uint32_t col = (uint32_t)j * nx + i;
     const S* fin = a.fin;
-    if (BAND && k <= a.s1.ksolid[col]) {
+    if (k <= a.s1.ksolid[col]) {
         // closed at step n+1: the conditioning replaces every population by
         // feq(rho0, wall velocity) with weight 1 and the streamed value with
-        // weight 0 (equal whenever the streamed values are finite); the
-        // column's values come from k_bed2d_b
+        // weight 0 (equal whenever the streamed values are finite)
         const T* fs = a.s1.fsol + col;
-        const size_t plane2 = (size_t)nx * ny;
 #pragma unroll
-        for (int q = 0; q < 19; q++) f[q] = fs[(size_t)q * plane2];
+        for (int q = 0; q < 19; q++) f[q] = fs[(size_t)q * plane];
         open0_out = false;
         return;
     }
-#define LD(q, idx) Store<S, T>::load(fin + (size_t)(q) * n, (uint32_t)(idx), (q), inv)
+#define LD(q, idx) Store<S, T>::load(fin + (size_t)(q) * n, (uint32_t)(idx), (q), cod)
+    const bool slow0 = a.mode != 0 && k <= a.s0.kband[col];
+    const bool slow1 = k <= a.s1.kband[col];
     if (a.mode == 0) {
 #pragma unroll
         for (int q = 0; q < 19; q++) f[q] = LD(q, id);
     } else {
         const bool xper = c.x_boundary == LBM_XB_PERIODIC, xio = c.x_boundary == LBM_XB_INFLOW_OUTFLOW;
-        const bool xsolid = c.x_boundary == LBM_XB_SOLID;
         const bool yper = c.y_boundary == LBM_YB_PERIODIC;
-        // source columns / rows for c = -1, 0, +1 (index 1 - c)
-        const int im = xper ? (i == 0 ? nx - 1 : i - 1) : i - 1;
-        const int ip = xper ? (i == nx - 1 ? 0 : i + 1) : (xio ? (i == nx - 1 ? nx - 1 : i + 1) : i + 1);
-        const int jm = yper ? (j == 0 ? ny - 1 : j - 1) : j - 1;
-        const int jp = yper ? (j == ny - 1 ? 0 : j + 1) : j + 1;
-        const int cols[3] = {ip, i, im};  // source column for cx = -1, 0, +1
-        const int rows[3] = {jp, j, jm};
-        const bool xout[3] = {!xper && !xio && i == nx - 1, false, !xper && !xio && i == 0};
-        const bool yout[3] = {!yper && j == ny - 1, false, !yper && j == 0};
-        const bool inflow_col = xio && i == 0;
-        const bool slow = BAND && k <= a.s0.kband[col];
-        const size_t plane2 = (size_t)nx * ny;
-        uint32_t mask = 0x7fffeu;
-        T e0[9];
-        if (slow) {
-#pragma unroll
-            for (int e = 0; e < 9; e++) e0[e] = a.s0.eta9[(size_t)e * plane2 + col];
-            mask = link_mask_e(c, e0, i, j, k);
-        }
-        const bool any_open = (mask & 0x7fffeu) != 0;
+        const bool free_slip = c.top_boundary == LBM_TOP_FREE_SLIP;
+        // source columns / rows, clamped into the grid at non-periodic edges
+        // (those links take the edge rules below)
+        const int im = i == 0 ? (xper ? nx - 1 : 0) : i - 1;
+        const int ip = i == nx - 1 ? (xper ? 0 : nx - 1) : i + 1;
+        const int jm = j == 0 ? (yper ? ny - 1 : 0) : j - 1;
+        const int jp = j == ny - 1 ? (yper ? 0 : ny - 1) : j + 1;
+        // x edges: solid walls bounce back, the inflow column takes the
+        // equilibrium ghost at the inflow velocity
+        const bool xlo = !xper && i == 0, xhi = !xper && !xio && i == nx - 1;
         f[0] = LD(0, id);
+        T loc[19];
+        const bool any_slow = __builtin_amdgcn_ballot_w64(slow0) != 0;
+        if (any_slow || xlo || xhi) {
+#pragma unroll
+            for (int q = 1; q < 19; q++) loc[q] = LD(q, id);
+        }
+        T e0[9];
+        uint32_t mask0 = 0x7fffeu;
+        if (any_slow) {
+#pragma unroll
+            for (int e = 0; e < 9; e++) e0[e] = a.s0.eta9[(size_t)e * plane + col];
+        }
 #pragma unroll
         for (int q = 1; q < 19; q++) {
             const int cxi = cxq(q), cyi = cyq(q), czi = czq(q);
+            const int si = cxi > 0 ? im : (cxi < 0 ? ip : i);
+            const int sj = cyi > 0 ? jm : (cyi < 0 ? jp : j);
+            const bool yout = !yper && ((cyi < 0 && j == ny - 1) || (cyi > 0 && j == 0));
             const int sk = k - czi;
-            const int si = cols[cxi + 1], sj = rows[cyi + 1];
-            T out;
-            if (xout[cxi + 1] || yout[cyi + 1]) {
-                out = LD(oppq(q), id);
-            } else if (sk >= 0 && sk < nz) {
-                T src;
-                if (inflow_col && cxi > 0)
-                    src = feq(c.rho0, a.inflow[(size_t)sk * ny + sj], T(0), T(0), q);
-                else
-                    src = LD(q, (uint32_t)sk * plane + (uint32_t)sj * nx + si);
-                if (slow) {
-                    int lo;
-                    bool w;
-                    T frac = any_open ? link_geo_e(c, e0[c9q(q)], i, j, k, q, lo, w) : T(0);
-                    out = frac * src + (T(1.0) - frac) * LD(oppq(q), id);
-                } else {
-                    out = src;
-                }
-            } else if (sk >= nz && c.top_boundary == LBM_TOP_FREE_SLIP) {
-                if (inflow_col && cxi > 0)
-                    out = feq(c.rho0, a.inflow[(size_t)(nz - 1) * ny + sj], T(0), T(0), mirzq(q));
-                else
-                    out = LD(mirzq(q), (uint32_t)(nz - 1) * plane + (uint32_t)sj * nx + si);
-            } else {
-                out = LD(q, id);
+            if (yout)
+                f[q] = LD(oppq(q), id);
+            else if (sk >= 0 && sk < nz)
+                f[q] = LD(q, (uint32_t)sk * plane + (uint32_t)sj * nx + si);
+            else if (sk >= nz && free_slip)
+                f[q] = LD(mirzq(q), (uint32_t)(nz - 1) * plane + (uint32_t)sj * nx + si);
+            else
+                f[q] = LD(q, id);
+        }
+        if (xlo || xhi) {
+#pragma unroll
+            for (int q = 1; q < 19; q++) {
+                const int cxi = cxq(q), cyi = cyq(q), czi = czq(q);
+                if (!((xlo && cxi > 0) || (xhi && cxi < 0))) continue;
+                const bool yout = !yper && ((cyi < 0 && j == ny - 1) || (cyi > 0 && j == 0));
+                const int sj = cyi > 0 ? jm : (cyi < 0 ? jp : j);
+                const int sk = k - czi;
+                if (!xio || yout)
+                    f[q] = loc[oppq(q)];
+                else if (sk >= 0 && sk < nz)
+                    f[q] = feq(c.rho0, a.inflow[(size_t)sk * ny + sj], T(0), T(0), q);
+                else if (sk >= nz && free_slip)
+                    f[q] = feq(c.rho0, a.inflow[(size_t)(nz - 1) * ny + sj], T(0), T(0), mirzq(q));
             }
-            f[q] = out;
+        }
+        bool any_open = true;
+        if (any_slow) {
+            // the open fraction of step n blends each streamed link of a
+            // cut cell with the bounce-back of its own opposite population
+            mask0 = slow0 ? link_mask_e(c, e0, i, j, k) : 0x7fffeu;
+            any_open = (mask0 & 0x7fffeu) != 0;
+#pragma unroll
+            for (int q = 1; q < 19; q++) {
+                const int cxi = cxq(q), cyi = cyq(q), czi = czq(q);
+                const bool yout = !yper && ((cyi < 0 && j == ny - 1) || (cyi > 0 && j == 0));
+                const int sk = k - czi;
+                if (yout || !(sk >= 0 && sk < nz)) continue;  // uniform
+                const bool xout = (xlo && !xio && cxi > 0) || (xhi && cxi < 0);
+                int lo;
+                bool w;
+                T frac = any_open ? link_geo_e(c, e0[c9q(q)], i, j, k, q, lo, w) : T(0);
+                T bl = frac * f[q] + (T(1.0) - frac) * loc[oppq(q)];
+                f[q] = (slow0 && !xout) ? bl : f[q];
+            }
         }
         stream_edges(a, i, k, any_open, f);
         // (C) moving-wall IBB on near-obstacle cells (CODEX:3674-3712)
-        if (slow && (mask & 0x7fffeu) != 0x7fffeu) {
+        const bool ibb_lane = slow0 && (mask0 & 0x7fffeu) != 0x7fffeu;
+        if (__builtin_amdgcn_ballot_w64(ibb_lane) != 0) {
+            const T* g = a.s0.ibb + col;
+            T gw[19];
+#pragma unroll
+            for (int q = 1; q < 19; q++) gw[q] = g[(size_t)(q - 1) * plane];
 #pragma unroll
             for (int q = 1; q < 19; q++) {
                 int lo;
                 bool w;
                 T fo = link_geo_e(c, e0[c9q(q)], i, j, k, q, lo, w);
                 if (!any_open) fo = T(0);
-                const int qb = oppq(q);
-                T ux = T(0), uy = T(0), uz = T(0);
-                if (link_wt(c, w, i, j, q) == LBM_WALL_BED) wall_velocity(c, a.s0, i, j, q, ux, uy, uz);
-                T cuw = (T)cxq(qb) * ux + (T)cyq(qb) * uy + (T)czq(qb) * uz;
-                T fw = LD(qb, id) - T(6.0) * wq<T>(qb) * c.rho0 * cuw;
-                f[q] = f[q] * fo + fw * (T(1.0) - fo);
+                T fw = loc[oppq(q)] - gw[q];
+                T v = f[q] * fo + fw * (T(1.0) - fo);
+                f[q] = ibb_lane ? v : f[q];
             }
         }
     }
 #undef LD
     // update_bed_populations_and_reinit of step n+1 (CODEX:2894-2940)
     bool open0 = true;
-    if (BAND && k <= a.s1.kband[col]) {
-        const size_t plane2 = (size_t)nx * ny;
-        T e1[9];
+    if (__builtin_amdgcn_ballot_w64(slow1) != 0) {
+        T e1[9], fe[19];
 #pragma unroll
-        for (int e = 0; e < 9; e++) e1[e] = a.s1.eta9[(size_t)e * plane2 + col];
-        uint32_t mask = link_mask_e(c, e1, i, j, k);
-        open0 = (mask & 0x7fffeu) != 0;
-        if ((mask & 0x7fffeu) != 0x7fffeu || !open0) {
+        for (int e = 0; e < 9; e++) e1[e] = a.s1.eta9[(size_t)e * plane + col];
+        const T* fs = a.s1.fsol + col;
 #pragma unroll
-            for (int q = 0; q < 19; q++) {
-                int lo;
-                bool w;
-                T frac = link_geo_e(c, e1[c9q(q)], i, j, k, q, lo, w);
-                if (q == 0) lo = open0 ? 1 : 0;
-                else if (!open0) frac = T(0);
-                if (lo) continue;
-                int wt = q == 0 ? (c.include_bed ? LBM_WALL_BED : LBM_WALL_NONE) : link_wt(c, w, i, j, q);
-                T ux = T(0), uy = T(0), uz = T(0);
-                if (wt == LBM_WALL_BED) wall_velocity(c, a.s1, i, j, q, ux, uy, uz);
-                T fe = feq(c.rho0, ux, uy, uz, q);
-                f[q] = frac * f[q] + (T(1.0) - frac) * fe;
-            }
+        for (int q = 0; q < 19; q++) fe[q] = fs[(size_t)q * plane];
+        const uint32_t mask = link_mask_e(c, e1, i, j, k);
+        const bool op = (mask & 0x7fffeu) != 0;
+        const bool cond = slow1 && (mask & 0x7fffeu) != 0x7fffeu;
+        open0 = slow1 ? op : true;
+#pragma unroll
+        for (int q = 0; q < 19; q++) {
+            int lo;
+            bool w;
+            T frac = link_geo_e(c, e1[c9q(q)], i, j, k, q, lo, w);
+            if (q == 0) lo = op ? 1 : 0;
+            else if (!op) frac = T(0);
+            T v = frac * f[q] + (T(1.0) - frac) * fe[q];
+            f[q] = (cond && !lo) ? v : f[q];
         }
     }
     open0_out = open0;
 }
 
 // Pull of step n into a cell above the wall band (every link open with
-// fraction 1): the lean branch of fused_gather, arranged so that every
+// fraction 1): the rules of band_gather without the bed, arranged so that every
 // decision depending on y or z is uniform over the wave (a wave lies in one
 // row of one plane: j is read from the first lane), and the rules of a
 // non-periodic x boundary are a fix-up of the edge cells.
@@ -2073,7 +2107,7 @@ __device__ __forceinline__ void moments_cell(const FusedArgs<S, T>& a, int i, in
     T f[19];
     bool open0 = true;
     if (BAND)
-        fused_gather<S, T, BAND>(a, i, j, k, f, open0);
+        band_gather(a, i, j, k, f, open0);
     else
         lean_gather(a, i, j, k, f);
     T rr, un, vn, wn;
@@ -2162,7 +2196,7 @@ __device__ __forceinline__ void collide_cell(const FusedArgs<S, T>& a, int i, in
     T f[19];
     bool open_cur = true;
     if (BAND)
-        fused_gather<S, T, BAND>(a, i, j, k, f, open_cur);
+        band_gather(a, i, j, k, f, open_cur);
     else
         lean_gather(a, i, j, k, f);
     T rcur, ucur, vcur, wcur;
@@ -2262,7 +2296,7 @@ static void bed2d_t(const lbm_config_t& cfg, DevBufs& b, float t, int slot, hipS
                                              (const T*)b.h[prev], (const T*)b.piecewise);
     k_bed2d_b<T><<<nblocks(n2), NT, 0, s>>>(c, (const T*)b.h[slot], (T*)b.eta9[slot], (T*)b.denom[slot], b.kband[slot],
                                              (const T*)b.uw[slot], (const T*)b.vw[slot], (const T*)b.ww[slot], (T*)b.uw9[slot],
-                                             b.ksolid[slot], (T*)b.fsol[slot]);
+                                             b.ksolid[slot], (T*)b.fsol[slot], (T*)b.ibb[slot]);
     HIP_OK(hipGetLastError());
 }
 void launch_bed2d(const lbm_config_t& c, DevBufs& b, float t, int slot, hipStream_t s) {

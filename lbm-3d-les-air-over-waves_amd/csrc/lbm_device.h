@@ -44,6 +44,8 @@ __host__ __device__ constexpr T wq(int q) {
 struct bf16_t {
     uint16_t v;
 };
+typedef _Float16 h2_t __attribute__((ext_vector_type(2)));
+typedef __bf16 b2_t __attribute__((ext_vector_type(2)));
 
 // Reduced-precision population codec (new).  A population is stored as
 //   h_q = round((f_q - s_q) * 2^k),   f_q = s_q + h_q * 2^-k,
@@ -90,20 +92,21 @@ struct Store<T, T> {  // full-precision storage: the population itself
 };
 template <>
 struct Store<__half, float> {  // (f_q - s_q) * 2^k as IEEE half
+    // h * 2^-k is exact, so the fused multiply-add rounds exactly like
+    // shift + h * 2^-k (one rounding, in the add); it issues as one mixed-
+    // precision FMA that reads the half operand directly
     static __device__ __forceinline__ float decode_with(uint16_t bits, float shift, const Codec& k) {
-        return shift + __half2float(__ushort_as_half(bits)) * k.inv_scale;
+        return __builtin_fmaf(__half2float(__ushort_as_half(bits)), k.inv_scale, shift);
     }
     static __device__ __forceinline__ float decode(uint16_t bits, int q, const Codec& k) { return decode_with(bits, k.shift[q], k); }
     static __device__ __forceinline__ uint16_t encode(float v, int q, const Codec& k, uint32_t base) {
         float d = (v - k.shift[q]) * k.scale;
         if (k.stochastic) {
-            // keep 10 fraction bits: add 13 dither bits below them, truncate;
-            // the result is a half value (or rounds once more, to nearest, in
-            // half's subnormal range)
-            uint32_t u = __float_as_uint(d);
-            uint32_t a = u & 0x7fffffffu;
-            if (a < 0x7f800000u) a = (a + (dither_q(base, q) >> 19)) & ~0x1fffu;
-            d = __uint_as_float(a | (u & 0x80000000u));
+            // the instruction rounds up iff r >> 19 plus the first 13 bits
+            // below the half ulp carry into it (normal and subnormal range)
+            h2_t old = {0, 0};
+            h2_t hv = __builtin_amdgcn_cvt_sr_f16_f32(old, d, dither_q(base, q), false);
+            return __builtin_bit_cast(uint16_t, hv[0]);
         }
         return __half_as_ushort(__float2half_rn(d));
     }
@@ -128,17 +131,15 @@ __device__ __forceinline__ bf16_t float_to_bf16(float f) {
     r.v = (uint16_t)(u >> 16);
     return r;
 }
+// stochastic rounding to bfloat16: r >> 16 is added to the magnitude's bits
+// below the bfloat16 ulp and the sum truncated (a carry may round up a
+// binade; overflow saturates to infinity; NaN stays NaN, quieted).  This is
+// exactly the rounding of v_cvt_sr_bf16_f32.
 __device__ __forceinline__ bf16_t float_to_bf16_sr(float f, uint32_t r) {
-    uint32_t u = __float_as_uint(f);
+    b2_t old = {0, 0};
+    b2_t v = __builtin_amdgcn_cvt_sr_bf16_f32(old, f, r, false);
     bf16_t b;
-    if ((u & 0x7fffffffu) > 0x7f800000u) {
-        b.v = (uint16_t)((u >> 16) | 0x40u);
-        return b;
-    }
-    uint32_t a = u & 0x7fffffffu;
-    if (a < 0x7f800000u) a = a + (r >> 16);  // may carry into the exponent: rounds up a binade
-    if (a > 0x7f800000u) a = 0x7f800000u;
-    b.v = (uint16_t)(((a | (u & 0x80000000u)) >> 16));
+    b.v = __builtin_bit_cast(uint16_t, v[0]);
     return b;
 }
 template <>

@@ -719,13 +719,31 @@ static float half_rne(float x) {
     v.u = a | sg;
     return v.f;
 }
+/* Stochastic rounding to binary16 with the 32-bit random word r: the value
+   is rounded up to the next binary16 number iff the 13 bits of r above bit
+   19 plus the first 13 bits of the value below its binary16 ulp carry into
+   the ulp, i.e. with probability (distance to the lower neighbour) / ulp in
+   steps of 2^-13.  In the normal range that is "add r >> 19 to the low 13
+   fraction bits of the float and truncate" (overflow to infinity at 2^16);
+   in the subnormal range the ulp is 2^-24 and the value's bits below it are
+   counted in units of 2^-37.  This is the rounding of the gfx950
+   instruction v_cvt_sr_f16_f32 (pinned by tests/golden/sr_hw_vectors.npz). */
 static float half_sr(float x, uint32_t r) {
     union { float f; uint32_t u; } v;
     v.f = x;
     uint32_t a = v.u & 0x7fffffffu, sg = v.u & 0x80000000u;
-    if (a < 0x7f800000u) a = (a + (r >> 19)) & ~0x1fffu;
+    if (a >= 0x7f800000u) return x;
+    if (a < 0x38800000u) {
+        uint32_t t = (uint32_t)floor((double)fabsf(x) * 137438953472.0); /* |x| / 2^-37, exact, < 2^23 */
+        uint32_t qv = (t + (r >> 19)) >> 13;                               /* units of 2^-24, <= 1024 */
+        v.f = (float)qv * (1.0f / 16777216.0f);
+        v.u |= sg;
+        return v.f;
+    }
+    a = (a + (r >> 19)) & ~0x1fffu;
+    if (a >= 0x47800000u) a = 0x7f800000u;
     v.u = a | sg;
-    return half_rne(v.f);
+    return v.f;
 }
 static float bf16_rne(float x) {
     union { float f; uint32_t u; } v;
@@ -2279,3 +2297,11 @@ int API(orc_field_fnew)(void *p, void *data, size_t bytes) {
 }
 
 void API(orc_set_step)(void *p, int64_t step) { ((O *)p)->step = step; }
+
+/* the 16-bit roundings of the storage codec on raw values (half: binary16,
+   else bfloat16; sr: stochastic with the words r, else nearest-even); the
+   results are returned as floats (exactly representable) */
+void API(orc_round16)(int half, int sr, const float *x, const uint32_t *r, float *out, size_t n) {
+    for (size_t i = 0; i < n; i++)
+        out[i] = half ? (sr ? half_sr(x[i], r[i]) : half_rne(x[i])) : (sr ? bf16_sr(x[i], r[i]) : bf16_rne(x[i]));
+}

@@ -61,11 +61,24 @@ def load_oracle():
         g("orc_field_fnew").restype = ctypes.c_int
         g("orc_set_step").argtypes = [ctypes.c_void_p, ctypes.c_int64]
         g("orc_set_step").restype = None
+        g("orc_round16").argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t]
+        g("orc_round16").restype = None
         for fn in ("orc_det_sin", "orc_det_cos", "orc_det_log"):
             g(fn).argtypes = [ctypes.c_double]
             g(fn).restype = ctypes.c_double
     _lib = lib
     return lib
+
+
+def round16(half: bool, stochastic: bool, x: np.ndarray, r: np.ndarray) -> np.ndarray:
+    """The oracle's 16-bit storage rounding of raw float32 values (binary16
+    when ``half``, else bfloat16), as exactly representable float32."""
+    lib = load_oracle()
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    r = np.ascontiguousarray(r, dtype=np.uint32)
+    out = np.empty_like(x)
+    lib.orc_round16_f32(int(half), int(stochastic), x.ctypes.data, r.ctypes.data, out.ctypes.data, x.size)
+    return out
 
 
 class OracleCase:

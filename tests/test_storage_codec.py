@@ -7,6 +7,8 @@ conversions: numpy's IEEE binary16 cast and torch's bfloat16 cast for
 round-to-nearest-even, and the two-neighbour / mean property for stochastic
 rounding.  The accuracy of the reduced forms against the FP64 run is checked
 on the plain channel (config of the FP16 precision study, reduced height)."""
+from pathlib import Path
+
 import numpy as np
 import pytest
 import torch
@@ -14,7 +16,7 @@ import torch
 import lbm_abi as abi
 from harness import env_of, setup_case
 from lbm_config import parse_env
-from oracle_ref import OracleCase
+from oracle_ref import OracleCase, round16
 
 W = np.array([1 / 3] + [1 / 18] * 6 + [1 / 36] * 12, dtype=np.float64).astype(np.float32)
 CX = np.array([0, 1, -1, 0, 0, 0, 0, 1, -1, -1, 1, 1, -1, -1, 1, 0, 0, 0, 0], np.float32)
@@ -148,3 +150,31 @@ def test_reduced_storage_error_vs_fp64(storage, rounding, tol):
     assert abs(red.reduce(abi.REDUCE_MASS) - m64) / m64 <= 1e-5
     ref.close()
     red.close()
+
+
+GOLDEN_SR = Path(__file__).resolve().parent / "golden" / "sr_hw_vectors.npz"
+
+
+def test_stochastic_rounding_matches_hardware_vectors():
+    """The oracle's stochastic roundings reproduce, bit for bit, gfx950's
+    v_cvt_sr_f16_f32 / v_cvt_sr_bf16_f32 (which the device encode issues) on
+    the recorded vectors: binary16 normal and subnormal range, underflow,
+    overflow next to 65504, the float32 range, zeros, infinities and NaN
+    (recorded by tests/golden/make_sr_vectors.py)."""
+    v = np.load(GOLDEN_SR)
+    x, r = v["x"], v["r"]
+    nan = np.isnan(x)
+    h = round16(True, True, x, r)
+    hb = h.astype(np.float16).view(np.uint16)
+    assert np.array_equal(hb[~nan], v["f16"][~nan])
+    assert np.isnan(v["f16"][nan].view(np.float16)).all() and np.isnan(h[nan]).all()
+    b = round16(False, True, x, r)
+    bb = (b.view(np.uint32) >> 16).astype(np.uint16)
+    assert np.array_equal(bb[~nan], v["bf16"][~nan])
+    # the subnormal range is rounded at the binary16 ulp 2^-24, not at the
+    # float's 13th bit: some vectors there differ from add-then-round-nearest
+    sub = (np.abs(x) < 2.0**-14) & (np.abs(x) > 0)
+    u = x.view(np.uint32)
+    naive = (((u & 0x7FFFFFFF) + (r >> 19)) & ~np.uint32(0x1FFF)) | (u & 0x80000000)
+    naive16 = naive.view(np.float32).astype(np.float16).view(np.uint16)
+    assert (naive16[sub] != v["f16"][sub]).any()

@@ -1636,6 +1636,22 @@ __device__ __forceinline__ T feq_h(T rho, T ux, T uy, T uz, T u15, int q) {
     T cu = T(3.0) * cdotu<T>(q, ux, uy, uz);
     return rho * wq<T>(q) * (T(1.0) + cu + T(0.5) * cu * cu - u15);
 }
+// feq_h for all 19 q, bit for bit: the opposite link has c.u negated
+// exactly, so 3 c.u, (3 c.u)^2 / 2 and rho w are evaluated once per pair
+template <class T>
+__device__ __forceinline__ void feq_all(T rho, T ux, T uy, T uz, T u15, T fe[19]) {
+    constexpr int base[9] = {1, 3, 5, 7, 8, 11, 12, 15, 16};  // one link of each opposite pair
+    fe[0] = feq_h<T>(rho, ux, uy, uz, u15, 0);
+#pragma unroll
+    for (int p = 0; p < 9; p++) {
+        const int q = base[p], qb = oppq(q);
+        const T cu = T(3.0) * cdotu<T>(q, ux, uy, uz);
+        const T h = T(0.5) * cu * cu;
+        const T rw = rho * wq<T>(q);
+        fe[q] = rw * (T(1.0) + cu + h - u15);
+        fe[qb] = rw * (T(1.0) - cu + h - u15);
+    }
+}
 
 // Q_q : Pi over the non-zero entries of Q (zero entries add exact zeros)
 template <class T>
@@ -1665,9 +1681,9 @@ __device__ __forceinline__ void fused_regularized(T f[19], T rho, T ux, T uy, T 
     T u15 = T(1.5) * u2;
     T fe[19];
     T Pxx = T(0), Pyy = T(0), Pzz = T(0), Pxy = T(0), Pxz = T(0), Pyz = T(0);
+    feq_all<T>(rho, ux, uy, uz, u15, fe);
 #pragma unroll
     for (int q = 0; q < 19; q++) {
-        fe[q] = feq_h<T>(rho, ux, uy, uz, u15, q);
         T d = f[q] - fe[q];
         const int a = cxq(q), b = cyq(q), e = czq(q);
         if (a) Pxx += d;
@@ -1713,9 +1729,9 @@ __device__ __forceinline__ void fused_kbc(T f[19], T rho, T ux, T uy, T uz, T om
     T u15 = T(1.5) * u2;
     T fe[19];
     T alpha = T(1.0);
+    feq_all<T>(rho, ux, uy, uz, u15, fe);
 #pragma unroll
     for (int q = 0; q < 19; q++) {
-        fe[q] = feq_h<T>(rho, ux, uy, uz, u15, q);
         T fs = f[q] - om * (f[q] - fe[q]);
         f[q] = fs;
         if (fs < eps) {
@@ -1728,8 +1744,14 @@ __device__ __forceinline__ void fused_kbc(T f[19], T rho, T ux, T uy, T uz, T om
     }
     if (alpha < T(0)) alpha = T(0);
     if (alpha > T(1)) alpha = T(1);
+    if (alpha != T(1.0)) {
 #pragma unroll
-    for (int q = 0; q < 19; q++) f[q] = fe[q] + alpha * (f[q] - fe[q]);
+        for (int q = 0; q < 19; q++) f[q] = fe[q] + alpha * (f[q] - fe[q]);
+    } else {
+        // 1 * x == x: the same values without the product
+#pragma unroll
+        for (int q = 0; q < 19; q++) f[q] = fe[q] + (f[q] - fe[q]);
+    }
 }
 
 // link q of a cell in column (i,j) leaves the domain through a solid x or y
@@ -2131,7 +2153,11 @@ __global__ __launch_bounds__(FB_THREADS, (FusedOcc<T, BAND>::k1)) void k_moments
 // velocities are known, in place; returns omega.  nbu(comp, dir) returns the
 // neighbour velocity component comp (0..2) in direction dir = x+, x-, y+, y-,
 // z+, z- (x and y wrap periodically, CODEX:3047-3053; z clamps at the ends).
-template <class S, class T, class NB>
+// collision of a launch: any model (chosen per cell from the constants), or
+// one fixed model (no sponge, no backscatter) so the kernel holds only its code
+enum { CM_ANY = 0, CM_KBC = 1, CM_REG = 2 };
+
+template <int CM, class S, class T, class NB>
 __device__ __forceinline__ T les_collide(const FusedArgs<S, T>& a, int i, int j, int k, uint32_t id, T f[19], bool open_cur, bool open_below,
                                          T rcur, T ucur, T vcur, T wcur, const NB& nbu) {
     const Consts<T>& c = a.c;
@@ -2156,7 +2182,11 @@ __device__ __forceinline__ T les_collide(const FusedArgs<S, T>& a, int i, int j,
             om = a.omega[id];  // inactive cells keep their last omega
         }
     }
-    if (c.collision_reg) {
+    if (CM == CM_REG) {
+        fused_regularized(f, rcur, ucur, vcur, wcur, om);
+    } else if (CM == CM_KBC) {
+        fused_kbc(f, rcur, ucur, vcur, wcur, om);
+    } else if (c.collision_reg) {
         fused_regularized(f, rcur, ucur, vcur, wcur, om);
     } else if (c.top_sponge && k > c.nz - 3) {
         sponge_relax(f, rcur, ucur, vcur, wcur);
@@ -2171,12 +2201,12 @@ __device__ __forceinline__ T les_collide(const FusedArgs<S, T>& a, int i, int j,
 }
 
 // les_collide, then the store of f* (and of rho, omega on the last step)
-template <class S, class T, class NB>
+template <int CM, class S, class T, class NB>
 __device__ __forceinline__ void les_collide_store(const FusedArgs<S, T>& a, int i, int j, int k, uint32_t id, T f[19], bool open_cur,
                                                   bool open_below, T rcur, T ucur, T vcur, T wcur, const NB& nbu) {
     const Consts<T>& c = a.c;
     const uint32_t n = (uint32_t)c.nx * c.ny * c.nz;
-    const T om = les_collide(a, i, j, k, id, f, open_cur, open_below, rcur, ucur, vcur, wcur, nbu);
+    const T om = les_collide<CM>(a, i, j, k, id, f, open_cur, open_below, rcur, ucur, vcur, wcur, nbu);
     S* const fout = a.fout;
     const uint32_t dbase = c.store.stochastic ? dither_base(a.step, id) : 0u;
 #pragma unroll
@@ -2187,7 +2217,7 @@ __device__ __forceinline__ void les_collide_store(const FusedArgs<S, T>& a, int 
     }
 }
 
-template <class S, class T, bool BAND>
+template <class S, class T, bool BAND, int CM>
 __device__ __forceinline__ void collide_cell(const FusedArgs<S, T>& a, int i, int j, int k) {
     const Consts<T>& c = a.c;
     const int nx = c.nx, ny = c.ny, nz = c.nz;
@@ -2209,14 +2239,14 @@ __device__ __forceinline__ void collide_cell(const FusedArgs<S, T>& a, int i, in
                              (uint32_t)k * plane + (uint32_t)jm * nx + i, (k + 1 < nz ? id + plane : id), (k > 0 ? id - plane : id)};
     const T* u3[3] = {a.ux, a.uy, a.uz};
     auto nbu = [&](int comp, int dir) { return u3[comp][nbi[dir]]; };
-    les_collide_store(a, i, j, k, id, f, open_cur, open_below, rcur, ucur, vcur, wcur, nbu);
+    les_collide_store<CM>(a, i, j, k, id, f, open_cur, open_below, rcur, ucur, vcur, wcur, nbu);
 }
 
-template <class S, class T, bool BAND>
+template <class S, class T, bool BAND, int CM>
 __global__ __launch_bounds__(FB_THREADS, (FusedOcc<T, BAND>::k2)) void k_collide(FusedArgs<S, T> a) {
     int i, j, k;
     if (!fused_cell(a, i, j, k)) return;
-    collide_cell<S, T, BAND>(a, i, j, k);
+    collide_cell<S, T, BAND, CM>(a, i, j, k);
 }
 
 // ===========================================================================
@@ -2533,11 +2563,17 @@ static void fused_t(const lbm_config_t& cfg, DevBufs& b, int& cur, int mode, int
     }
     if (nb > 0) {
         a.plane0 = 0;
-        k_collide<S, T, true><<<dim3(gx, gy, nb), blk, 0, s>>>(a);
+        k_collide<S, T, true, CM_ANY><<<dim3(gx, gy, nb), blk, 0, s>>>(a);
     }
     if (nb < cfg.nz) {
         a.plane0 = nb;
-        k_collide<S, T, false><<<dim3(gx, gy, cfg.nz - nb), blk, 0, s>>>(a);
+        const dim3 grd(gx, gy, cfg.nz - nb);
+        if (cfg.collision == LBM_COLL_REGULARIZED)
+            k_collide<S, T, false, CM_REG><<<grd, blk, 0, s>>>(a);
+        else if (!cfg.enable_top_sponge && !(cfg.c_backscatter > 0.0f))
+            k_collide<S, T, false, CM_KBC><<<grd, blk, 0, s>>>(a);
+        else
+            k_collide<S, T, false, CM_ANY><<<grd, blk, 0, s>>>(a);
     }
     HIP_OK(hipGetLastError());
     cur ^= 1;

@@ -9,12 +9,19 @@ Prints ONE JSON line:
   value / ms_per_step      whole-step throughput (MLUPS), populations in HBM
   roofline                 the fused step kernels: algorithmic bytes
                            (2 x 19 x 4 B per lattice update) over their
-                           HIP-event-timed average duration, against 8 TB/s
+                           HIP-event-timed average duration, against 8 TB/s;
+                           traffic = HBM bytes per step measured by rocprofv3
+                           counters (profiles/fused_step_<workload>.md)
   cpu_baseline             the CPU restatement of the same step (oracle,
                            single thread) on a bounded sample of the workload
-  variants                 the same measurement for FP16 / BF16 storage and
-                           the regularized collision, plus the plain-channel
-                           and VOF workloads (reported, not the headline)
+  variants                 FP16 / BF16 storage and the regularized collision
+                           on the same workload
+  workloads                the plain channel at 64x32x32 and 512x256x192
+                           (FP32 / FP16 / BF16) and the VOF box 192x96x160
+  accuracy                 FP32 / FP16 / BF16 against FP64 on the plain
+                           channel at 256x128x128 after 2000 steps
+  serving                  the ensemble server against sequential runs
+(all reported beside the headline, which is the default workload)
 """
 from __future__ import annotations
 
@@ -32,6 +39,22 @@ for _p in (str(ROOT / "lbm-3d-les-air-over-waves_amd"), str(ROOT / "tests")):
 
 HBM_PEAK_GBS = 8000.0
 BYTES_PER_LU = {"fp32": 152, "fp16": 76, "bf16": 76, "fp64": 304}
+PROFILES = ROOT / "profiles"
+
+
+def measured_traffic(workload: str, storage: str):
+    """HBM bytes per step of the step kernels measured with rocprofv3 --pmc
+    (FETCH_SIZE x2 + WRITE_SIZE x1, copy-calibrated) for this workload, from
+    the committed profile summary; None if there is none."""
+    import re
+
+    if storage != "fp32":
+        return None
+    path = PROFILES / f"fused_step_{workload}.md"
+    if not path.exists():
+        return None
+    m = re.search(r"^traffic_bytes_per_step:\s*(\d+)", path.read_text(), re.M)
+    return int(m.group(1)) if m else None
 
 
 def run_device(env: dict, steps: int, warmup: int):
@@ -297,7 +320,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            "traffic": measured_traffic(args.workload, cfg.storage),
             "kernel_ms": round(main_run["ms_kernel"], 5),
             "bytes_per_lu": bpl,
         },

@@ -36,6 +36,8 @@ struct lbm_handle {
     lbm_config_t cfg{};
     DevBufs b{};
     hipStream_t stream = nullptr;
+    // static (flat / piecewise) beds: the 2-D bed kernels run once per slot
+    int slot_valid[2] = {0, 0};
     int cur = 0;
     int post_stream = 1;  // 1: f[cur] = f(step), 0: f[cur] = f*(step), stream pending
     int64_t step = 0;
@@ -353,6 +355,8 @@ const char* lbm_last_error(lbm_handle_t* h) { return h ? h->err.c_str() : g_crea
 int lbm_geometry(lbm_handle_t* h, float t) {
     API_BEGIN(h)
     launch_bed2d(h->cfg, h->b, t, 0, h->stream);
+    h->slot_valid[0] = 1;
+    h->slot_valid[1] = 0;
     launch_obstacle(h->cfg, h->b, 0, h->b.obst, h->stream);
     h->geometry_done = 1;
     h->step = 0;
@@ -385,11 +389,15 @@ int lbm_step(lbm_handle_t* h, int32_t nsteps) {
         h->ev.push_back(e);
     }
     if (h->timing) CHECK(hipEventRecord(h->ev[0], h->stream));
+    const bool fixed = c.bed_profile == LBM_BED_FLAT || c.bed_profile == LBM_BED_PIECEWISE;
     for (int s = 0; s < nsteps; s++) {
         int64_t step = h->step + 1;
         float t = (float)((double)(step + 1) * c.dt_phys);  // CODEX:3874
         int slot = slot_of(step);
-        launch_bed2d(c, h->b, t, slot, h->stream);
+        if (!(fixed && h->slot_valid[slot])) {  // a static bed is the same at every time level
+            launch_bed2d(c, h->b, t, slot, h->stream);
+            h->slot_valid[slot] = 1;
+        }
         if (h->timing) CHECK(hipEventRecord(h->ev[2 + 2 * s], h->stream));
         if (h->path == LBM_PATH_STAGED) {
             if (!h->post_stream) throw std::runtime_error("internal: staged step on a post-collision state");
@@ -423,6 +431,7 @@ int lbm_step(lbm_handle_t* h, int32_t nsteps) {
 int lbm_upload_field(lbm_handle_t* h, int32_t field, const void* src, size_t bytes) {
     API_BEGIN(h)
     if (!src) return fail(h, LBM_E_ARG, "src is NULL");
+    if (field == LBM_FIELD_PIECEWISE_BED_Z) h->slot_valid[0] = h->slot_valid[1] = 0;
     const lbm_config_t& c = h->cfg;
     DevBufs& b = h->b;
     size_t n = b.n, n2 = b.n2, R = real_size(c);

@@ -257,6 +257,38 @@ __device__ __forceinline__ T feq(T rho, T ux, T uy, T uz, int q) {
 }
 
 // periodic bilinear sample of a cell-centred (ny,nx) field (CODEX:2536-2569)
+// bilinear sampling position of sample2d, split by axis: the x part depends
+// on xq only and the y part on yq only, so the 9 link endpoints of a column
+// need 3 + 3 of them, and several fields sampled at one point share them
+// (same indices and weights, so the same results as sample2d)
+template <class T>
+struct Axis2d {
+    int i0, i1;
+    T f;
+};
+template <class T>
+__device__ __forceinline__ Axis2d<T> sample_axis(T xq, int nx, T dx) {
+    T Lx = (T)nx * dx;
+    T Nxf = (T)nx;
+    T xw = xq - floor(xq / Lx) * Lx;
+    T u = xw / dx - T(0.5);
+    u = u - floor(u / Nxf) * Nxf;
+    int i0 = (int)floor(u);
+    Axis2d<T> r;
+    r.f = u - (T)i0;
+    if (i0 >= nx) i0 -= nx;
+    r.i0 = i0;
+    r.i1 = i0 + 1 == nx ? 0 : i0 + 1;  // (i0 + 1) % nx for i0 in [-1, nx)
+    return r;
+}
+template <class T>
+__device__ __forceinline__ T sample_at(const T* H, const Axis2d<T>& x, const Axis2d<T>& y, int nx) {
+    T h00 = H[(size_t)y.i0 * nx + x.i0], h10 = H[(size_t)y.i0 * nx + x.i1];
+    T h01 = H[(size_t)y.i1 * nx + x.i0], h11 = H[(size_t)y.i1 * nx + x.i1];
+    T hx0 = h00 * (T(1.0) - x.f) + h10 * x.f;
+    T hx1 = h01 * (T(1.0) - x.f) + h11 * x.f;
+    return hx0 * (T(1.0) - y.f) + hx1 * y.f;
+}
 template <class T>
 __device__ __forceinline__ T sample2d(const T* H, T xq, T yq, int nx, int ny, T dx) {
     T Lx = (T)nx * dx, Ly = (T)ny * dx;

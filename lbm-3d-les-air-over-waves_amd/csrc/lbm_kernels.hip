@@ -193,96 +193,58 @@ __global__ __launch_bounds__(NT) void k_bed2d_a(Consts<T> c, T t, T* h, T* uw, T
 }
 
 // per-column link-endpoint bed heights (CODEX:2536-2569 at the 9 (cx,cy)
-// endpoints), phi slope normaliser (CODEX:2451-2469) and the highest k with
-// a cut or closed link (cells above it are fully open: frac = 1 everywhere)
+// endpoints), phi slope normaliser (CODEX:2451-2469), the highest k with a
+// cut or closed link (cells above it are fully open: frac = 1 everywhere),
+// the highest k whose links are all closed, and the per-link wall terms.
+// A block holds 64 columns and 9 waves: wave e samples endpoint e of every
+// column (4 fields), then the waves share the per-link terms and wave 0
+// derives the per-column bounds.
+constexpr int BED_COLS = 64;
 template <class T>
-__global__ __launch_bounds__(NT) void k_bed2d_b(Consts<T> c, const T* h, T* eta9, T* denom, int* kband, const T* uw, const T* vw, const T* ww, T* uw9,
-                                                int* ksolid, T* fsol, T* ibb) {
-    size_t id = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    size_t plane = (size_t)c.nx * c.ny;
-    if (id >= plane) return;
-    int i = (int)(id % (size_t)c.nx), j = (int)(id / (size_t)c.nx);
-    T dx = c.dx;
-    T xc = ((T)i + T(0.5)) * dx, yc = ((T)j + T(0.5)) * dx;
-    T hmax = T(-1e30);
-    T e9[9], w9[27];
-#pragma unroll
-    for (int a = -1; a <= 1; a++)
-#pragma unroll
-        for (int bb = -1; bb <= 1; bb++) {
-            T xq = xc + T(0.5) * dx * (T)a;
-            T yq = yc + T(0.5) * dx * (T)bb;
-            T e = sample2d(h, xq, yq, c.nx, c.ny, dx);
-            const int c9 = (a + 1) * 3 + (bb + 1);
-            e9[c9] = e;
-            eta9[(size_t)c9 * plane + id] = e;
-            hmax = e > hmax ? e : hmax;
-            // wall velocity at the link endpoint (CODEX:937-988)
-            w9[c9] = sample2d(uw, xq, yq, c.nx, c.ny, dx);
-            w9[9 + c9] = sample2d(vw, xq, yq, c.nx, c.ny, dx);
-            w9[18 + c9] = sample2d(ww, xq, yq, c.nx, c.ny, dx);
-            uw9[(size_t)c9 * plane + id] = w9[c9];
-            uw9[(size_t)(9 + c9) * plane + id] = w9[9 + c9];
-            uw9[(size_t)(18 + c9) * plane + id] = w9[18 + c9];
-        }
-    int ip = (i + 1) % c.nx, im = (i - 1 + c.nx) % c.nx;
-    int jp = (j + 1) % c.ny, jm = (j - 1 + c.ny) % c.ny;
-    T dh_dx = (h[(size_t)j * c.nx + ip] - h[(size_t)j * c.nx + im]) * (T(0.5) / dx);
-    if (c.x_boundary != LBM_XB_PERIODIC) {
-        if (i == 0)
-            dh_dx = (h[(size_t)j * c.nx + i + 1] - h[(size_t)j * c.nx + i]) / dx;
-        else if (i == c.nx - 1)
-            dh_dx = (h[(size_t)j * c.nx + i] - h[(size_t)j * c.nx + i - 1]) / dx;
+__global__ __launch_bounds__(9 * BED_COLS) void k_bed2d_b(Consts<T> c, const T* h, T* eta9, T* denom, int* kband, const T* uw,
+                                                          const T* vw, const T* ww, T* uw9, int* ksolid, T* fsol, T* ibb) {
+    __shared__ T se[9][BED_COLS];
+    __shared__ T sw[27][BED_COLS];
+    const int lane = (int)(threadIdx.x % BED_COLS), e = (int)(threadIdx.x / BED_COLS);
+    const size_t plane = (size_t)c.nx * c.ny;
+    const size_t id = (size_t)blockIdx.x * BED_COLS + lane;
+    const bool live = id < plane;
+    const int i = live ? (int)(id % (size_t)c.nx) : 0, j = live ? (int)(id / (size_t)c.nx) : 0;
+    const T dx = c.dx;
+    if (live) {
+        const T xc = ((T)i + T(0.5)) * dx, yc = ((T)j + T(0.5)) * dx;
+        const int a = e / 3 - 1, bb = e % 3 - 1;  // e = (a + 1) * 3 + (bb + 1)
+        const Axis2d<T> sx = sample_axis(xc + T(0.5) * dx * (T)a, c.nx, dx);
+        const Axis2d<T> sy = sample_axis(yc + T(0.5) * dx * (T)bb, c.ny, dx);
+        const T ev = sample_at(h, sx, sy, c.nx);
+        se[e][lane] = ev;
+        eta9[(size_t)e * plane + id] = ev;
+        // wall velocity at the link endpoint (CODEX:937-988)
+        const T u = sample_at(uw, sx, sy, c.nx), v = sample_at(vw, sx, sy, c.nx), w = sample_at(ww, sx, sy, c.nx);
+        sw[e][lane] = u;
+        sw[9 + e][lane] = v;
+        sw[18 + e][lane] = w;
+        uw9[(size_t)e * plane + id] = u;
+        uw9[(size_t)(9 + e) * plane + id] = v;
+        uw9[(size_t)(18 + e) * plane + id] = w;
     }
-    T dh_dy = (h[(size_t)jp * c.nx + i] - h[(size_t)jm * c.nx + i]) * (T(0.5) / dx);
-    T d = sqrt(T(1.0) + dh_dx * dh_dx + dh_dy * dh_dy);
-    if (d < T(1e-6)) d = T(1.0);
-    denom[id] = d;
-    // a link of cell k has frac < 1 only when zq - eta < dx/2 with
-    // zq >= k dx, i.e. k < eta/dx + 1/2; one extra cell of margin
-    int kb = -1;
-    if (c.include_bed) {
-        double kk = ceil((double)hmax / (double)dx + 0.5) + 1.0;
-        kb = kk < -1.0 ? -1 : (kk > (double)c.nz ? c.nz : (int)kk);
-    }
-    bool wall_col = (c.x_boundary == LBM_XB_SOLID && (i == 0 || i == c.nx - 1)) ||
-                    (c.y_boundary == LBM_YB_SOLID && (j == 0 || j == c.ny - 1));
-    if (wall_col) kb = c.nz;
-    kband[id] = kb;
-    // highest k whose 18 links all end below the bed (gap < 0 with the
-    // arithmetic of link_geo); closedness is monotone in k
-    int ks = -1;
-    if (c.include_bed) {
-        for (int k = 0; k < c.nz; k++) {
-            T zc = ((T)k + T(0.5)) * dx;
-            bool closed = true;
-#pragma unroll
-            for (int q = 1; q < 19; q++) {
-                T zq = zc + T(0.5) * dx * (T)czq(q);
-                T gap = zq - e9[c9q(q)];
-                if (!(gap < T(0))) closed = false;
-            }
-            if (!closed) break;
-            ks = k;
-        }
-    }
-    ksolid[id] = ks;
+    __syncthreads();
+    if (!live) return;
     // per-link wall terms of this column, independent of k: the value the
     // conditioning of update_bed_populations_and_reinit (CODEX:2894-2940)
     // gives a closed link, feq(rho0, wall velocity of the link), and the
     // moving-wall term of the IBB (CODEX:3674-3712); x / y wall links have
-    // wall velocity 0
-#pragma unroll
-    for (int q = 0; q < 19; q++) {
+    // wall velocity 0.  Wave e takes links e, e + 9, e + 18.
+    for (int q = e; q < 19; q += 9) {
         int wt = c.include_bed ? LBM_WALL_BED : LBM_WALL_NONE;
         if (q > 0) wt = link_wt(c, link_on_wall(c, i, j, q), i, j, q);
         T ux = T(0), uy = T(0), uz = T(0);
         if (wt == LBM_WALL_BED) {
             if (c.wall_link) {
-                const int e = c9q(q);
-                ux = w9[e];
-                uy = w9[9 + e];
-                uz = w9[18 + e];
+                const int ep = c9q(q);
+                ux = sw[ep][lane];
+                uy = sw[9 + ep][lane];
+                uz = sw[18 + ep][lane];
             } else {
                 ux = uw[id];
                 uy = vw[id];
@@ -296,6 +258,72 @@ __global__ __launch_bounds__(NT) void k_bed2d_b(Consts<T> c, const T* h, T* eta9
             ibb[(size_t)(q - 1) * plane + id] = T(6.0) * wq<T>(qb) * c.rho0 * cuw;
         }
     }
+    if (e == 1) {
+        int ip = (i + 1) % c.nx, im = (i - 1 + c.nx) % c.nx;
+        int jp = (j + 1) % c.ny, jm = (j - 1 + c.ny) % c.ny;
+        T dh_dx = (h[(size_t)j * c.nx + ip] - h[(size_t)j * c.nx + im]) * (T(0.5) / dx);
+        if (c.x_boundary != LBM_XB_PERIODIC) {
+            if (i == 0)
+                dh_dx = (h[(size_t)j * c.nx + i + 1] - h[(size_t)j * c.nx + i]) / dx;
+            else if (i == c.nx - 1)
+                dh_dx = (h[(size_t)j * c.nx + i] - h[(size_t)j * c.nx + i - 1]) / dx;
+        }
+        T dh_dy = (h[(size_t)jp * c.nx + i] - h[(size_t)jm * c.nx + i]) * (T(0.5) / dx);
+        T d = sqrt(T(1.0) + dh_dx * dh_dx + dh_dy * dh_dy);
+        if (d < T(1e-6)) d = T(1.0);
+        denom[id] = d;
+    }
+    if (e != 0) return;
+    T e9[9];
+    T hmax = T(-1e30);
+#pragma unroll
+    for (int p = 0; p < 9; p++) {
+        e9[p] = se[p][lane];
+        hmax = e9[p] > hmax ? e9[p] : hmax;
+    }
+    // a link of cell k has frac < 1 only when zq - eta < dx/2 with
+    // zq >= k dx, i.e. k < eta/dx + 1/2; one extra cell of margin
+    int kb = -1;
+    if (c.include_bed) {
+        double kk = ceil((double)hmax / (double)dx + 0.5) + 1.0;
+        kb = kk < -1.0 ? -1 : (kk > (double)c.nz ? c.nz : (int)kk);
+    }
+    bool wall_col = (c.x_boundary == LBM_XB_SOLID && (i == 0 || i == c.nx - 1)) ||
+                    (c.y_boundary == LBM_YB_SOLID && (j == 0 || j == c.ny - 1));
+    if (wall_col) kb = c.nz;
+    kband[id] = kb;
+    // highest k whose 18 links all end below the bed (gap < 0 with the
+    // arithmetic of link_geo).  Closedness is monotone in k, so the walk
+    // starts one plane under the real-number bound when that plane is
+    // closed (else at 0) and gives the same k as a walk from 0.
+    auto closed = [&](int k) {
+        T zc = ((T)k + T(0.5)) * dx;
+        bool cl = true;
+#pragma unroll
+        for (int q = 1; q < 19; q++) {
+            T zq = zc + T(0.5) * dx * (T)czq(q);
+            T gap = zq - e9[c9q(q)];
+            if (!(gap < T(0))) cl = false;
+        }
+        return cl;
+    };
+    int ks = -1;
+    if (c.include_bed) {
+        double m = 1e300;
+#pragma unroll
+        for (int q = 1; q < 19; q++) {
+            const double v = (double)e9[c9q(q)] - 0.5 * (double)dx * (double)czq(q);
+            m = v < m ? v : m;
+        }
+        const double kr = m / (double)dx - 0.5;  // closed for real k < kr
+        int k0 = kr > 2.0 ? (int)fmin(ceil(kr) - 2.0, (double)(c.nz - 1)) : 0;
+        if (k0 > 0 && !closed(k0)) k0 = 0;
+        for (int k = k0; k < c.nz; k++) {
+            if (!closed(k)) break;
+            ks = k;
+        }
+    }
+    ksolid[id] = ks;
 }
 
 // ===========================================================================
@@ -2322,9 +2350,9 @@ static void bed2d_t(const lbm_config_t& cfg, DevBufs& b, float t, int slot, hipS
     Consts<T> c = make_consts<T>(cfg, -1);
     size_t n2 = b.n2;
     int prev = slot ^ 1;
-    k_bed2d_a<T><<<nblocks(n2), NT, 0, s>>>(c, (T)t, (T*)b.h[slot], (T*)b.uw[slot], (T*)b.vw[slot], (T*)b.ww[slot],
+    k_bed2d_a<T><<<nblocks(n2, 64), 64, 0, s>>>(c, (T)t, (T*)b.h[slot], (T*)b.uw[slot], (T*)b.vw[slot], (T*)b.ww[slot],
                                              (const T*)b.h[prev], (const T*)b.piecewise);
-    k_bed2d_b<T><<<nblocks(n2), NT, 0, s>>>(c, (const T*)b.h[slot], (T*)b.eta9[slot], (T*)b.denom[slot], b.kband[slot],
+    k_bed2d_b<T><<<nblocks(n2, BED_COLS), 9 * BED_COLS, 0, s>>>(c, (const T*)b.h[slot], (T*)b.eta9[slot], (T*)b.denom[slot], b.kband[slot],
                                              (const T*)b.uw[slot], (const T*)b.vw[slot], (const T*)b.ww[slot], (T*)b.uw9[slot],
                                              b.ksolid[slot], (T*)b.fsol[slot], (T*)b.ibb[slot]);
     HIP_OK(hipGetLastError());

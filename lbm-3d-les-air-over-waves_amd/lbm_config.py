@@ -367,9 +367,21 @@ def parse_env(env: Optional[Mapping[str, str]] = None) -> SolverConfig:
         if solitary_amp_m <= 0.0:
             raise ValueError("LBM_SOLITARY_AMPLITUDE_M must be positive")
         solitary_height_depth_ratio = solitary_amp_m / free_surface_depth_m
+    solitary_x0_m = float(env.get("LBM_SOLITARY_X0_M", 0.25 * Lx_m))
+    solitary_direction = float(env.get("LBM_SOLITARY_DIRECTION", 1.0))
+    if solitary_direction == 0.0:
+        raise ValueError("LBM_SOLITARY_DIRECTION must be nonzero")
+    solitary_direction = 1.0 if solitary_direction > 0.0 else -1.0
+    solitary_initial_velocity_mode = str(env.get("LBM_SOLITARY_INITIAL_VELOCITY", "shallow_water_divfree")).lower()
+    if solitary_initial_velocity_mode not in ("none", "shallow_water", "shallow_water_divfree"):
+        raise ValueError(
+            "LBM_SOLITARY_INITIAL_VELOCITY must be 'none', 'shallow_water', "
+            "or 'shallow_water_divfree'"
+        )
     solitary_celerity_phys = (
         float(np.sqrt(gravity_phys * (free_surface_depth_m + solitary_amp_m))) if gravity_phys > 0.0 else 0.0
     )
+    solitary_kappa_m_inv = float(np.sqrt(3.0 * solitary_amp_m / (4.0 * free_surface_depth_m**3)))
     gaussian_amp_m = float(env.get("LBM_GAUSSIAN_AMPLITUDE_M", 3.0))
     gaussian_center_x_m = float(env.get("LBM_GAUSSIAN_CENTER_X_M", 0.5 * Lx_m))
     gaussian_sigma_m = float(env.get("LBM_GAUSSIAN_SIGMA_M", 12.5))
@@ -515,10 +527,6 @@ def parse_env(env: Optional[Mapping[str, str]] = None) -> SolverConfig:
         unsupported.append("LBM_BOUNDARY_GEOMETRY=phi_fraction|phi")
     if water_free_surface_enabled and tracking != "vof":
         unsupported.append("height-function free-surface tracking")
-    if water_free_surface_enabled and fs_initial == "solitary":
-        unsupported.append("LBM_FREE_SURFACE_INITIAL=solitary")
-    if water_free_surface_enabled and initial_pressure_mode == "linear_wave":
-        unsupported.append("LBM_INITIAL_PRESSURE_MODE=linear_wave")
     if init_fs_pops:
         unsupported.append("LBM_INIT_FREE_SURFACE_POPULATIONS=1")
     if physics_mode == "water" and not water_free_surface_enabled:
@@ -591,6 +599,12 @@ def parse_env(env: Optional[Mapping[str, str]] = None) -> SolverConfig:
             free_surface_base_m=free_surface_base_m,
             solitary_amp_m=solitary_amp_m,
             solitary_height_depth_ratio=solitary_height_depth_ratio,
+            solitary_x0_m=solitary_x0_m,
+            solitary_direction=solitary_direction,
+            solitary_initial_velocity_mode=solitary_initial_velocity_mode,
+            solitary_celerity_phys=solitary_celerity_phys,
+            solitary_kappa_m_inv=solitary_kappa_m_inv,
+            solitary_length_scale_m=1.0 / solitary_kappa_m_inv,
             gaussian_amp_m=gaussian_amp_m,
             gaussian_center_x_m=gaussian_center_x_m,
             gaussian_sigma_m=gaussian_sigma_m,

@@ -56,11 +56,12 @@ def loglaw_profile(cfg: SolverConfig) -> np.ndarray:
     return (cfg.inflow_u_lu * prof / mean).astype(np.float64)
 
 
-def initial_fields(cfg: SolverConfig, obstacle0: np.ndarray) -> dict:
+def initial_fields(cfg: SolverConfig, obstacle0: np.ndarray, bed: np.ndarray = None) -> dict:
     """Macroscopic initial fields and boundary arrays to upload.
 
     obstacle0 is the t=0 solid mask (1 = no open link) from the device
-    geometry; it classifies closed cells as gas in water runs."""
+    geometry; it classifies closed cells as gas in water runs.  bed is the
+    bed height water_h at t = 0 (default: the flat bed level)."""
     nz, ny, nx = cfg.Nz, cfg.Ny, cfg.Nx
     real = np.float64 if cfg.storage == "fp64" else np.float32
     rng = np.random.default_rng(cfg.seed)
@@ -90,12 +91,15 @@ def initial_fields(cfg: SolverConfig, obstacle0: np.ndarray) -> dict:
     out[abi.FIELD_INFLOW_UX] = inflow
     if cfg.water_free_surface_enabled:
         fill, ctype, fs_h = free_surface_state(cfg, obstacle0)
-        rho_init = initial_density(cfg, fs_h)
+        if bed is None:
+            bed = np.full((ny, nx), cfg.offset_m, dtype=np.float32)
+        rho_init = initial_density(cfg, fs_h, bed)
         gas = ctype == abi.FS_GAS
         rho = np.where(gas, cfg.rho0, rho_init).astype(real)
-        ux = np.where(gas, 0.0, cfg.u_current_lb).astype(real)
+        ux0, uz0 = solitary_velocity(cfg, fs_h, bed)
+        ux = np.where(gas, 0.0, ux0).astype(real)
         uy = np.zeros_like(ux)
-        uz = np.zeros_like(ux)
+        uz = np.where(gas, 0.0, uz0).astype(real)
         out[abi.FIELD_FS_FILL] = fill.astype(real)
         out[abi.FIELD_FS_TYPE] = ctype.astype(np.int32)
         out[abi.FIELD_FS_TYPE_PREV] = ctype.astype(np.int32)
@@ -107,12 +111,46 @@ def initial_fields(cfg: SolverConfig, obstacle0: np.ndarray) -> dict:
     return out
 
 
+def _periodic_offset(cfg: SolverConfig, xrel: np.ndarray) -> np.ndarray:
+    """x - x0 folded into [-Lx/2, Lx/2] on a periodic x axis, float32."""
+    f32 = np.float32
+    if cfg.x_boundary_mode != "periodic":
+        return xrel
+    lx = f32(cfg.Nx) * f32(cfg.dx)
+    xrel = np.where(xrel > f32(0.5) * lx, xrel - lx, xrel).astype(np.float32)
+    return np.where(xrel < f32(-0.5) * lx, xrel + lx, xrel).astype(np.float32)
+
+
+def solitary_profile(cfg: SolverConfig, t: float = 0.0):
+    """KdV solitary wave of amplitude A on depth h0 (CODEX:1166-1182):
+    eta(x) = A sech^2(kappa (x - x0 - dir c t)), kappa = sqrt(3 A / (4 h0^3)),
+    float32 at the cell centres.  Returns (sech, tanh) of the argument, so
+    the height and its slope follow without a second evaluation."""
+    f32 = np.float32
+    dx = f32(cfg.dx)
+    h0 = f32(cfg.free_surface_depth_m)
+    amp = f32(cfg.solitary_amp_m)
+    kappa = np.sqrt(f32(3.0) * amp / (f32(4.0) * h0 * h0 * h0)).astype(np.float32)
+    center = f32(cfg.solitary_x0_m) + f32(cfg.solitary_direction) * f32(cfg.solitary_celerity_phys) * f32(t)
+    xp = ((np.arange(cfg.Nx, dtype=np.float32) + f32(0.5)) * dx).astype(np.float32)
+    arg = (kappa * _periodic_offset(cfg, (xp - center).astype(np.float32))).astype(np.float32)
+    ep = np.exp(arg).astype(np.float32)
+    em = np.exp(-arg).astype(np.float32)
+    den = (ep + em).astype(np.float32)
+    return (f32(2.0) / den).astype(np.float32), ((ep - em) / den).astype(np.float32)
+
+
 def surface_height(cfg: SolverConfig) -> np.ndarray:
-    """initialize_free_surface_height (CODEX:1197-1207): flat, or the
-    Gaussian hump gaussian_surface_height_at_x (CODEX:1185-1194), float32."""
+    """initialize_free_surface_height (CODEX:1197-1207): flat, the Gaussian
+    hump gaussian_surface_height_at_x (CODEX:1185-1194) or the solitary wave
+    at t = 0, float32."""
     f32 = np.float32
     ny, nx = cfg.Ny, cfg.Nx
     dx = f32(cfg.dx)
+    if cfg.free_surface_initial_condition == "solitary":
+        sech, _ = solitary_profile(cfg)
+        h = (f32(cfg.free_surface_base_m) + f32(cfg.solitary_amp_m) * sech * sech).astype(np.float32)
+        return np.broadcast_to(h[None, :], (ny, nx)).astype(np.float32)
     if cfg.free_surface_initial_condition != "gaussian":
         return np.full((ny, nx), f32(cfg.free_surface_base_m), dtype=np.float32)
     lx = f32(nx) * dx
@@ -167,11 +205,90 @@ def free_surface_state(cfg: SolverConfig, obstacle0: np.ndarray):
     return fill, ctype, fs_h
 
 
-def initial_density(cfg: SolverConfig, fs_h: np.ndarray) -> np.ndarray:
-    """prepare_initial_density_field (CODEX:743-818), hydrostatic branch."""
+def solitary_velocity(cfg: SolverConfig, fs_h: np.ndarray, bed: np.ndarray):
+    """Initial velocity of the wet cells (CODEX:2756-2800), float32: the
+    current u_current everywhere; under a solitary initial surface plus the
+    shallow-water velocity dir c eta / (h0 + eta), and for
+    "shallow_water_divfree" the vertical velocity -z_rel du/dx that makes it
+    divergence-free (z_rel = height above the bed)."""
+    f32 = np.float32
     nz, ny, nx = cfg.Nz, cfg.Ny, cfg.Nx
-    if cfg.initial_pressure_mode == "linear_wave":
-        raise NotImplementedError("LBM_INITIAL_PRESSURE_MODE=linear_wave is not supported")
+    ux = np.full((nz, ny, nx), f32(cfg.u_current_lb), dtype=np.float32)
+    uz = np.zeros((nz, ny, nx), dtype=np.float32)
+    if cfg.free_surface_initial_condition != "solitary" or cfg.solitary_initial_velocity_mode == "none":
+        return ux, uz
+    h0 = f32(cfg.free_surface_depth_m)
+    c = f32(cfg.solitary_direction) * f32(cfg.solitary_celerity_phys)
+    vs = f32(cfg.vel_scale)
+    eta = np.maximum((fs_h.astype(np.float32) - f32(cfg.free_surface_base_m)).astype(np.float32), f32(0.0))
+    u_sol = (c * eta / (h0 + eta)).astype(np.float32)
+    ux = (ux + (u_sol / vs)[None, :, :]).astype(np.float32)
+    if cfg.solitary_initial_velocity_mode == "shallow_water_divfree":
+        sech, tanh = solitary_profile(cfg)
+        eta_x = (f32(-2.0) * f32(cfg.solitary_amp_m) * f32(cfg.solitary_kappa_m_inv) * sech * sech * tanh).astype(np.float32)
+        du_dx = (c * h0 * eta_x[None, :] / ((h0 + eta) * (h0 + eta))).astype(np.float32)
+        zc = ((np.arange(nz, dtype=np.float32) + f32(0.5)) * f32(cfg.dx)).astype(np.float32)
+        z_rel = np.maximum(zc[:, None, None] - bed.astype(np.float32)[None, :, :], f32(0.0)).astype(np.float32)
+        uz = (uz + (-z_rel * du_dx[None, :, :]) / vs).astype(np.float32)
+    return ux, uz
+
+
+def linear_wave_head(cfg: SolverConfig, eta: np.ndarray, bed: float) -> np.ndarray:
+    """Pressure head (m) of the linear potential-flow wave under the surface
+    elevation eta(x) on a flat bed (CODEX:743-818, linear_wave branch): each
+    horizontal mode k of eta decays with depth as cosh(k z') / cosh(k h)
+    (z' = height above the bed, capped at h = max(base - bed, dx)).  Modes:
+    cos(n pi x / Lx) for solid x walls (discrete cosine series), else the
+    periodic Fourier modes.  Returns head[k_plane, i] in float64."""
+    nx, nz, dx = cfg.Nx, cfg.Nz, cfg.dx
+    depth = max(cfg.free_surface_base_m - bed, dx)
+    x = (np.arange(nx, dtype=np.float64) + 0.5) * dx
+    z = (np.arange(nz, dtype=np.float64) + 0.5) * dx
+
+    def decay(k, zr):
+        # cosh(k zr) / cosh(k depth) without overflow: (e^{k(zr-h)} + e^{-k(zr+h)}) / (1 + e^{-2kh})
+        out = np.ones_like(k)
+        m = k > 0.0
+        kh = k[m] * depth
+        kz = k[m] * zr
+        out[m] = (np.exp(kz - kh) + np.exp(-kz - kh)) / (1.0 + np.exp(-2.0 * kh))
+        return out
+
+    head = np.empty((nz, nx), dtype=np.float64)
+    if cfg.x_boundary_mode == "solid":
+        k = np.arange(nx, dtype=np.float64) * np.pi / (nx * dx)
+        basis = np.cos(np.outer(k, x))             # basis[n, i] = cos(k_n x_i)
+        amp = (2.0 / nx) * (basis @ eta)
+        amp[0] = np.mean(eta)
+        for kk, zc in enumerate(z):
+            zr = min(max(zc - bed, 0.0), depth)
+            head[kk] = (amp * decay(k, zr)) @ basis
+    else:
+        k = 2.0 * np.pi * np.fft.rfftfreq(nx, d=dx)
+        spec = np.fft.rfft(eta)
+        for kk, zc in enumerate(z):
+            zr = min(max(zc - bed, 0.0), depth)
+            head[kk] = np.fft.irfft(spec * decay(k, zr), n=nx)
+    return head
+
+
+def initial_density(cfg: SolverConfig, fs_h: np.ndarray, bed: np.ndarray = None) -> np.ndarray:
+    """prepare_initial_density_field (CODEX:743-818): the linear-wave
+    pressure under a Gaussian surface on a flat bed when asked for, else the
+    hydrostatic head."""
+    nz, ny, nx = cfg.Nz, cfg.Ny, cfg.Nx
+    if bed is None:
+        bed = np.full((ny, nx), cfg.offset_m, dtype=np.float32)
+    bed64 = bed.astype(np.float64)
+    flat = np.allclose(bed64, bed64[0, 0], atol=1.0e-12, rtol=0.0)
+    if cfg.initial_pressure_mode == "linear_wave" and cfg.free_surface_initial_condition == "gaussian" and flat:
+        eta = fs_h.astype(np.float64).mean(axis=0) - cfg.free_surface_base_m
+        head = linear_wave_head(cfg, eta, float(bed64[0, 0]))
+        if cfg.use_hydrostatic_balanced_pressure != 1:
+            z = (np.arange(nz, dtype=np.float64) + 0.5) * cfg.dx
+            head = np.maximum((cfg.free_surface_base_m - z)[:, None] + head, 0.0)
+        line = (cfg.rho0 + cfg.g_lb * head / cfg.dx / (1.0 / 3.0)).astype(np.float32)
+        return np.ascontiguousarray(np.broadcast_to(line[:, None, :], (nz, ny, nx)))
     rho = np.empty((nz, ny, nx), dtype=np.float32)
     h = fs_h.astype(np.float64)
     z = (np.arange(nz, dtype=np.float64) + 0.5) * cfg.dx

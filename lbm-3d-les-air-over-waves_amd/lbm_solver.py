@@ -159,7 +159,7 @@ class Solver:
             case.upload(abi.FIELD_PIECEWISE_BED_Z, cfg.piecewise_bed_z_np)
         case.geometry(0.0)
         obstacle0 = case.download(abi.FIELD_OBSTACLE)
-        fields = initial_fields(cfg, obstacle0)
+        fields = initial_fields(cfg, obstacle0, case.download(abi.FIELD_WATER_H))
         for fid, arr in fields.items():
             case.upload(fid, arr)
         case.init_populations()

@@ -22,7 +22,7 @@ def setup_case(case, cfg_values, piecewise=None):
         case.upload(abi.FIELD_PIECEWISE_BED_Z, cfg_values.piecewise_bed_z_np)
     case.geometry(0.0)
     obstacle0 = case.download(abi.FIELD_OBSTACLE)
-    fields = initial_fields(cfg_values, obstacle0)
+    fields = initial_fields(cfg_values, obstacle0, case.download(abi.FIELD_WATER_H))
     for fid, arr in fields.items():
         case.upload(fid, arr)
     case.init_populations()

@@ -22,7 +22,7 @@
 extern "C" {
 #endif
 
-#define LBM_CONFIG_VERSION 4
+#define LBM_CONFIG_VERSION 5
 
 /* ---- error codes ---------------------------------------------------- */
 #define LBM_OK 0
@@ -70,6 +70,11 @@ extern "C" {
 #define LBM_PATH_AUTO 0
 #define LBM_PATH_STAGED 1 /* one kernel per reference stage */
 #define LBM_PATH_FUSED 2  /* single fused z-marching pass (air mode) */
+/* free_surface_mode: LBM_FREE_SURFACE_MODE (CODEX:52) */
+#define LBM_FSM_NONE 0
+#define LBM_FSM_HEIGHT_STATIC 1
+#define LBM_FSM_HEIGHT_KINEMATIC 2
+#define LBM_FSM_PRESCRIBED_SOLITARY 3
 /* free_surface_tracking: LBM_FREE_SURFACE_TRACKING (CODEX:87) */
 #define LBM_FST_HEIGHT 0
 #define LBM_FST_VOF 1
@@ -119,6 +124,7 @@ typedef struct lbm_config {
     int32_t vof_orphan_search_radius, vof_orphan_max_weak_neighbors;
     int32_t vof_detached_max_wet_neighbors, vof_detached_max_resolved_neighbors;
     int32_t vof_collapse_interval, vof_collapse_flood_sweeps;
+    int32_t fs_mode; /* LBM_FSM_*: how the height-function surface moves */
     uint32_t seed; /* backscatter counter-based RNG seed */
 
     float dx, rho0, tau0, nu0;

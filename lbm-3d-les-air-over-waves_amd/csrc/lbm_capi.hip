@@ -85,7 +85,7 @@ static const char* validate(const lbm_config_t* c) {
         return "fp16/bf16 storage runs on the fused path only";
     if (c->physics_mode == LBM_PHYS_WATER) {
         if (!c->fs_enabled) return "water mode without a free surface is not supported";
-        if (c->fs_tracking != LBM_FST_VOF) return "height-function tracking is not supported";
+        if (c->fs_mode == LBM_FSM_PRESCRIBED_SOLITARY) return "LBM_FREE_SURFACE_MODE=prescribed_solitary is not supported";
         if (c->storage == LBM_STORE_FP16 || c->storage == LBM_STORE_BF16) return "water/VOF runs need fp32 or fp64 storage";
         if (c->kernel_path == LBM_PATH_FUSED) return "the fused path covers air mode only";
     }
@@ -140,7 +140,7 @@ static size_t device_bytes(const lbm_config_t& c, int path) {
     bytes += 5 * n * R + n;                      // rho, u, omega, cell flags
     if (path == LBM_PATH_STAGED) bytes += 2 * n * R + 2 * n * 4;  // nu_t, smag, obstacle masks
     bytes += 2 * n2 * (78 * R + 8) + (c.nx + c.nz + (size_t)c.nz * c.ny) * R;
-    if (c.fs_enabled) bytes += n * (11 * R + 7 * 4) + 6 * n * R + n2 * R;
+    if (c.fs_enabled) bytes += n * (11 * R + 7 * 4) + 6 * n * R + 3 * n2 * R;
     return bytes + (1 << 20);
 }
 
@@ -239,6 +239,8 @@ static void allocate(lbm_handle* h) {
         b.se = dalloc(h, n * R);
         b.smv = dalloc(h, 6 * n * R);
         b.fs_h = dalloc(h, n2 * R);
+        b.fs_qx = dalloc(h, n2 * R);
+        b.fs_qy = dalloc(h, n2 * R);
         b.type = (int*)dalloc(h, n * 4);
         b.type_prev = (int*)dalloc(h, n * 4);
         b.excess_dir = (int*)dalloc(h, n * 4);

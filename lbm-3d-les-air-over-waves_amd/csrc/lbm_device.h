@@ -320,6 +320,7 @@ struct Consts {
     int bed_profile, bed_moving, include_bed, wall_link;
     int top_drive, x_fringe, top_sponge;
     int fs_vof, pf, fs_boundary_hydro, refill_mode, link_aperture, thin_gap_bridge;
+    int fs_mode;  // LBM_FSM_*
     int omega_band_top;  // highest k whose omega must persist (moving beds), -1 otherwise
     uint32_t seed;
     T dx, rho0, tau0, nu0, cs2_les, nu_t_max, kappa, z0_phys;

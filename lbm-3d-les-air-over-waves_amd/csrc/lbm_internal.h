@@ -57,6 +57,7 @@ struct DevBufs {
     uint8_t* cellflags;  // bit0 lattice_open[0], bit1 near_obstacle (staged path, current slot)
     // water / VOF
     void *fill, *mass, *mass2, *mdelta, *excess, *sa, *sb, *sc, *sd, *se, *smv, *fs_h;
+    void *fs_qx, *fs_qy;  // column volume fluxes (height-function tracking)
     int *type, *type_prev, *excess_dir, *ext_gas, *obst, *obst_prev, *si;
     int* icnt;      // [IC_COUNT]
     double* dcnt;   // [DC_COUNT]

@@ -52,6 +52,7 @@ static Consts<T> make_consts(const lbm_config_t& c, int omega_band_top) {
     k.x_fringe = c.enable_x_fringe;
     k.top_sponge = c.enable_top_sponge;
     k.fs_vof = c.fs_enabled && c.fs_tracking == LBM_FST_VOF;
+    k.fs_mode = c.fs_mode;
     k.pf = c.pressure_formulation;
     k.fs_boundary_hydro = c.fs_boundary_mode == LBM_FSB_HYDROSTATIC;
     k.refill_mode = c.refill_mode;
@@ -584,6 +585,29 @@ __global__ __launch_bounds__(NT) void k_collide(Consts<T> c, T* f, const T* rho,
     for (int q = 0; q < 19; q++) f[(size_t)q * n + id] = fl[q];
 }
 
+// free_surface_hydrostatic_density_at_cell (CODEX:821-847): one pressure
+// target per node; component-balanced: the head of the still-water
+// reference where gravity is balanced; hydrostatic-balanced: the surface
+// elevation h - base (p' = rho g eta, may be negative); total pressure: the
+// depth below the column's surface, at least 0
+template <class T>
+__device__ __forceinline__ T fs_density_cell(const Consts<T>& c, int k, bool open0, int type, const T* fs_h, size_t col) {
+    const T dx = c.dx;
+    const bool hbp = c.pf == LBM_PF_HYDROSTATIC_BALANCED || c.pf == LBM_PF_VOF_COMPONENT_BALANCED;
+    T head;
+    if (c.pf == LBM_PF_VOF_COMPONENT_BALANCED) {
+        T zc = ((T)k + T(0.5)) * dx;
+        head = gravity_fraction(c, k, open0, type) < T(0.5) ? zc - c.free_surface_base_m : T(0);
+    } else if (hbp) {
+        head = fs_h[col] - c.free_surface_base_m;
+    } else {
+        T zc = ((T)k + T(0.5)) * dx;
+        head = fs_h[col] - zc;
+    }
+    if (head < T(0) && !hbp) head = T(0);
+    return c.rho0 + c.g_lb * (head / dx) / (T(1.0) / T(3.0));
+}
+
 // free-surface gas density at a link (CODEX:821-867)
 template <class T>
 __device__ __forceinline__ T fs_density_link(const Consts<T>& c, int k, int q, bool open0, int type, const T* fs_h, size_t col) {
@@ -599,11 +623,7 @@ __device__ __forceinline__ T fs_density_link(const Consts<T>& c, int k, int q, b
         T head = z_if - c.free_surface_base_m;
         return c.rho0 + c.g_lb * (head / dx) / (T(1.0) / T(3.0));
     }
-    // free_surface_hydrostatic_density_at_cell, total-pressure form
-    T zc = ((T)k + T(0.5)) * dx;
-    T head = fs_h[col] - zc;
-    if (head < T(0)) head = T(0);
-    return c.rho0 + c.g_lb * (head / dx) / (T(1.0) / T(3.0));
+    return fs_density_cell(c, k, open0, type, fs_h, col);
 }
 
 // post-collision value of population q at (si,sj,sk), with the ghost layers
@@ -822,6 +842,7 @@ struct VofArgs {
     const uint8_t* flags;
     int* icnt;
     double* dcnt;
+    const T* fs_h;  // column surface height
 };
 
 __device__ __forceinline__ bool nbr(int nx, int ny, int nz, int xb, int yb, int i, int j, int k, int di, int dj, int dk, int& ni, int& nj,
@@ -1435,7 +1456,96 @@ __global__ __launch_bounds__(NT) void k_vof_height(Consts<T> c, Slot<T> s, const
     fs_h[col] = s.h[col] + depth;
 }
 
-// refill_new_free_surface_cells (CODEX:2163-2316), VOF tracking
+// ---- height-function free-surface tracking (CODEX:1209-1232, 2376-2429) ----
+
+// classify_free_surface_cells: fill of the column height in cell k, types by
+// the 1e-4 thresholds, closed cells gas; the old type moves to type_prev
+template <class T>
+__global__ __launch_bounds__(NT) void k_fs_classify(Consts<T> c, const T* fs_h, const uint8_t* flags, T* fill, int* type, int* type_prev) {
+    size_t id = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    size_t n = (size_t)c.nx * c.ny * c.nz;
+    if (id >= n) return;
+    int i, j, k;
+    cell_ijk(id, c.nx, c.ny, i, j, k);
+    const int old = type[id];
+    T z_bottom = (T)k * c.dx;
+    T fl = (fs_h[(size_t)j * c.nx + i] - z_bottom) / c.dx;
+    if (fl < T(0.0)) fl = T(0.0);
+    if (fl > T(1.0)) fl = T(1.0);
+    int t = LBM_FS_INTERFACE;
+    if (fl <= T(1e-4))
+        t = LBM_FS_GAS;
+    else if (fl >= T(1.0) - T(1e-4))
+        t = LBM_FS_LIQUID;
+    if (!(flags[id] & 1u)) {
+        fl = T(0.0);
+        t = LBM_FS_GAS;
+    }
+    fill[id] = fl;
+    type_prev[id] = old;
+    type[id] = t;
+}
+
+// compute_free_surface_column_flux: q = sum over the open, non-gas cells of
+// the column of fill u vel_scale dx, bottom to top
+template <class T>
+__global__ __launch_bounds__(NT) void k_fs_flux(Consts<T> c, const T* fill, const int* type, const uint8_t* flags, const T* ux,
+                                                const T* uy, T* qx, T* qy) {
+    size_t col = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    size_t plane = (size_t)c.nx * c.ny;
+    if (col >= plane) return;
+    T sx = T(0), sy = T(0);
+    for (int k = 0; k < c.nz; k++) {
+        size_t id = (size_t)k * plane + col;
+        if ((flags[id] & 1u) && type[id] != LBM_FS_GAS) {
+            T fl = fill[id];
+            sx += fl * ux[id] * c.vel_scale * c.dx;
+            sy += fl * uy[id] * c.vel_scale * c.dx;
+        }
+    }
+    qx[col] = sx;
+    qy[col] = sy;
+}
+
+// divergence of a column flux along one axis: central differences on a
+// periodic axis, half-sum face fluxes with no flux through the walls of a
+// solid one
+template <class T>
+__device__ __forceinline__ T flux_div(const T* q, size_t stride, int a, int n, bool solid, T dx) {
+    if (solid) {
+        T fl = T(0), fr = T(0);
+        if (a > 0) fl = T(0.5) * (q[(size_t)(a - 1) * stride] + q[(size_t)a * stride]);
+        if (a < n - 1) fr = T(0.5) * (q[(size_t)a * stride] + q[(size_t)(a + 1) * stride]);
+        return (fr - fl) / dx;
+    }
+    int ap = a + 1 == n ? 0 : a + 1, am = a == 0 ? n - 1 : a - 1;
+    return (q[(size_t)ap * stride] - q[(size_t)am * stride]) * (T(0.5) / dx);
+}
+
+// advect_free_surface_height: h += dt (-(dq_x/dx + dq_y/dy)) between half a
+// cell above the bed and half a cell below the lid (height_kinematic);
+// height_static keeps h.  Solid y walls (new) take the x-wall form.
+template <class T>
+__global__ __launch_bounds__(NT) void k_fs_advect(Consts<T> c, const T* qx, const T* qy, const T* bed, T* fs_h) {
+    size_t col = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    size_t plane = (size_t)c.nx * c.ny;
+    if (col >= plane) return;
+    if (c.fs_mode != LBM_FSM_HEIGHT_KINEMATIC) return;
+    const int i = (int)(col % (size_t)c.nx), j = (int)(col / (size_t)c.nx);
+    const T dx = c.dx;
+    const T h_old = fs_h[col];
+    T dq_dx = flux_div(qx + (size_t)j * c.nx, 1, i, c.nx, c.x_boundary == LBM_XB_SOLID, dx);
+    T dq_dy = flux_div(qy + i, (size_t)c.nx, j, c.ny, c.y_boundary == LBM_YB_SOLID, dx);
+    T dh_dt = -(dq_dx + dq_dy);
+    T h_new = h_old + c.dt_phys * dh_dt;
+    T min_h = bed[col] + T(0.5) * dx;
+    T max_h = (T)c.nz * dx - T(0.5) * dx;
+    if (h_new < min_h) h_new = min_h;
+    if (h_new > max_h) h_new = max_h;
+    fs_h[col] = h_new;
+}
+
+// refill_new_free_surface_cells (CODEX:2163-2316)
 template <class T>
 __global__ __launch_bounds__(NT) void k_vof_refill(VofArgs<T> a, T* f) {
     VOF_PROLOGUE
@@ -1470,20 +1580,26 @@ __global__ __launch_bounds__(NT) void k_vof_refill(VofArgs<T> a, T* f) {
                 fnf[qq] += wn * fneq;
             }
     }
-    T rr = c.rho0, urx = c.u_current_lb, ury = T(0), urz = T(0);
+    // VOF: density from the liquid neighbours; height tracking: the
+    // hydrostatic density of the column (CODEX:2225-2246)
+    const bool vof = c.fs_vof;
+    // (the gravity fraction of the reference's field is the one of the step's
+    // start, when this cell was still gas: type_prev)
+    T rr = vof ? c.rho0 : fs_density_cell(c, k, (a.flags[id] & 1u) != 0, a.type_prev[id], a.fs_h, (size_t)j * c.nx + i);
+    T urx = c.u_current_lb, ury = T(0), urz = T(0);
     if (ws > T(0)) {
         T inv = T(1.0) / ws;
-        rr = ra * inv;
+        if (vof) rr = ra * inv;
         urx = xa * inv;
         ury = ya * inv;
         urz = za * inv;
     } else if (fws > T(0)) {
         T inv = T(1.0) / fws;
-        rr = rf * inv;
+        if (vof) rr = rf * inv;
         urx = xf * inv;
         ury = yf * inv;
         urz = zf * inv;
-    } else {
+    } else if (vof) {
         rr = c.rho0;
     }
     T fv[19];
@@ -2419,6 +2535,7 @@ static VofArgs<T> make_vof(const lbm_config_t& cfg, DevBufs& b, const T* f) {
     a.si = b.si;
     a.flags = b.cellflags;
     a.icnt = b.icnt;
+    a.fs_h = (const T*)b.fs_h;
     a.dcnt = b.dcnt;
     return a;
 }
@@ -2527,6 +2644,8 @@ static void staged_t(const lbm_config_t& cfg, DevBufs& b, int& cur, int slot, in
     T* f = (T*)b.f[cur];
     T* fn = (T*)b.f[cur ^ 1];
     k_cellflags<T><<<nb, NT, 0, s>>>(c, sl, b.cellflags);
+    const bool height = cfg.fs_enabled && cfg.fs_tracking != LBM_FST_VOF;
+    if (height) k_fs_classify<T><<<nb, NT, 0, s>>>(c, (const T*)b.fs_h, b.cellflags, (T*)b.fill, b.type, b.type_prev);
     k_cond<T><<<nb, NT, 0, s>>>(c, sl, f, b.obst, b.obst_prev);
     k_macro<T><<<nb, NT, 0, s>>>(c, f, (T*)b.rho, (T*)b.ux, (T*)b.uy, (T*)b.uz, b.type, b.cellflags);
     if (cfg.use_les)
@@ -2542,7 +2661,19 @@ static void staged_t(const lbm_config_t& cfg, DevBufs& b, int& cur, int slot, in
     if (cfg.fs_enabled) {
         T* f2 = (T*)b.f[cur];
         k_macro<T><<<nb, NT, 0, s>>>(c, f2, (T*)b.rho, (T*)b.ux, (T*)b.uy, (T*)b.uz, b.type, b.cellflags);
-        if (cfg.fs_tracking == LBM_FST_VOF) vof_pipeline<T>(cfg, b, f2, slot, step, s);
+        if (cfg.fs_tracking == LBM_FST_VOF) {
+            vof_pipeline<T>(cfg, b, f2, slot, step, s);
+        } else {
+            // height-function tracking (CODEX:3917-3921)
+            const unsigned n2b = nblocks(b.n2);
+            k_fs_flux<T><<<n2b, NT, 0, s>>>(c, (const T*)b.fill, b.type, b.cellflags, (const T*)b.ux, (const T*)b.uy, (T*)b.fs_qx,
+                                            (T*)b.fs_qy);
+            k_fs_advect<T><<<n2b, NT, 0, s>>>(c, (const T*)b.fs_qx, (const T*)b.fs_qy, (const T*)b.h[slot], (T*)b.fs_h);
+            k_fs_classify<T><<<nb, NT, 0, s>>>(c, (const T*)b.fs_h, b.cellflags, (T*)b.fill, b.type, b.type_prev);
+            HIP_OK(hipMemsetAsync(b.icnt + IC_FS_REFILL, 0, sizeof(int), s));
+            k_vof_refill<T><<<nb, NT, 0, s>>>(make_vof<T>(cfg, b, f2), f2);
+            HIP_OK(hipGetLastError());
+        }
     }
 }
 void launch_staged_step(const lbm_config_t& c, DevBufs& b, int& cur, int slot, int64_t step, int omega_band_top,

@@ -8,7 +8,7 @@ from __future__ import annotations
 
 import ctypes
 
-LBM_CONFIG_VERSION = 4
+LBM_CONFIG_VERSION = 5
 
 # error codes
 LBM_OK = 0
@@ -28,6 +28,7 @@ BED_MOVING_SINE, BED_FLAT, BED_PIECEWISE, BED_MOVING_SINE1 = 0, 1, 2, 3
 STORE_FP32, STORE_FP16, STORE_BF16, STORE_FP64 = 0, 1, 2, 3
 ROUND_NEAREST, ROUND_STOCHASTIC = 0, 1
 PATH_AUTO, PATH_STAGED, PATH_FUSED = 0, 1, 2
+FSM_NONE, FSM_HEIGHT_STATIC, FSM_HEIGHT_KINEMATIC, FSM_PRESCRIBED_SOLITARY = 0, 1, 2, 3
 FST_HEIGHT, FST_VOF = 0, 1
 PF_TOTAL, PF_HYDROSTATIC_BALANCED, PF_VOF_COMPONENT_BALANCED = 0, 1, 2
 FSB_CELL, FSB_HYDROSTATIC = 0, 1
@@ -146,6 +147,7 @@ class LbmConfig(ctypes.Structure):
         ("vof_detached_max_resolved_neighbors", _I),
         ("vof_collapse_interval", _I),
         ("vof_collapse_flood_sweeps", _I),
+        ("fs_mode", _I),
         ("seed", ctypes.c_uint32),
         ("dx", _F),
         ("rho0", _F),

@@ -107,6 +107,9 @@ class SolverConfig:
             s.storage_u_ref[a] = v["storage_u_ref"][a]
         s.kernel_path = {"auto": abi.PATH_AUTO, "staged": abi.PATH_STAGED, "fused": abi.PATH_FUSED}[v["kernel_path"]]
         s.fs_enabled = v["water_free_surface_enabled"]
+        s.fs_mode = {"none": abi.FSM_NONE, "height_static": abi.FSM_HEIGHT_STATIC,
+                     "height_kinematic": abi.FSM_HEIGHT_KINEMATIC,
+                     "prescribed_solitary": abi.FSM_PRESCRIBED_SOLITARY}[v["free_surface_mode"]]
         s.fs_tracking = abi.FST_VOF if v["free_surface_tracking_mode"] == "vof" else abi.FST_HEIGHT
         s.pressure_formulation = {
             "total_pressure": abi.PF_TOTAL,
@@ -525,8 +528,8 @@ def parse_env(env: Optional[Mapping[str, str]] = None) -> SolverConfig:
         unsupported.append("LBM_OBSTACLE_MODE with a cube")
     if boundary_geometry_mode != "vertical":
         unsupported.append("LBM_BOUNDARY_GEOMETRY=phi_fraction|phi")
-    if water_free_surface_enabled and tracking != "vof":
-        unsupported.append("height-function free-surface tracking")
+    if water_free_surface_enabled and free_surface_mode == "prescribed_solitary":
+        unsupported.append("LBM_FREE_SURFACE_MODE=prescribed_solitary")
     if init_fs_pops:
         unsupported.append("LBM_INIT_FREE_SURFACE_POPULATIONS=1")
     if physics_mode == "water" and not water_free_surface_enabled:

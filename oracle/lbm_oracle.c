@@ -166,6 +166,7 @@ typedef struct {
     REAL *fill, *mass, *mass_delta, *excess, *det_delta, *pref_head, *grav_frac;
     int32_t *type, *type_prev, *excess_dir, *ext_gas;
     REAL *fs_h, *fs_h_last;        /* [ny][nx] */
+    REAL *fs_qx, *fs_qy;           /* [ny][nx] column volume fluxes (height tracking) */
     /* scratch for the deterministic VOF formulation */
     REAL *s_m, *s_a, *s_b, *s_c, *s_d, *s_e, *s_mv;
     int32_t *s_i;
@@ -1795,7 +1796,99 @@ static void vof_height(O *o) {
         }
 }
 
-/* refill_new_free_surface_cells (CODEX:2163-2316), VOF tracking */
+/* ---- height-function free-surface tracking (CODEX:1209-1232, 2376-2429) -- */
+
+/* classify_free_surface_cells: fill = clamp((h - k dx) / dx, 0, 1) of the
+   column height, types by the 1e-4 thresholds, closed cells gas */
+static void fs_classify(O *o) {
+    const lbm_config_t *c = &o->c;
+    const REAL dx = R(c->dx);
+    for (int k = 0; k < o->nz; k++)
+        for (int j = 0; j < o->ny; j++)
+            for (int i = 0; i < o->nx; i++) {
+                size_t id = IDX(k, j, i);
+                int old = o->type[id];
+                REAL z_bottom = (REAL)k * dx;
+                REAL fill = (o->fs_h[ID2(j, i)] - z_bottom) / dx;
+                if (fill < R(0.0)) fill = R(0.0);
+                if (fill > R(1.0)) fill = R(1.0);
+                int t = LBM_FS_INTERFACE;
+                if (fill <= R(1e-4))
+                    t = LBM_FS_GAS;
+                else if (fill >= R(1.0) - R(1e-4))
+                    t = LBM_FS_LIQUID;
+                if (LO(0, id) == 0) {
+                    fill = R(0.0);
+                    t = LBM_FS_GAS;
+                }
+                o->fill[id] = fill;
+                o->type_prev[id] = old;
+                o->type[id] = t;
+            }
+}
+
+/* compute_free_surface_column_flux: q = sum_k fill u vel_scale dx over the
+   open, non-gas cells of a column */
+static void fs_column_flux(O *o) {
+    const lbm_config_t *c = &o->c;
+    const REAL vs = R(c->vel_scale), dx = R(c->dx);
+    for (int j = 0; j < o->ny; j++)
+        for (int i = 0; i < o->nx; i++) {
+            REAL qx = R(0), qy = R(0);
+            for (int k = 0; k < o->nz; k++) {
+                size_t id = IDX(k, j, i);
+                if (LO(0, id) == 1 && o->type[id] != LBM_FS_GAS) {
+                    REAL fl = o->fill[id];
+                    qx += fl * o->ux[id] * vs * dx;
+                    qy += fl * o->uy[id] * vs * dx;
+                }
+            }
+            o->fs_qx[ID2(j, i)] = qx;
+            o->fs_qy[ID2(j, i)] = qy;
+        }
+}
+
+/* divergence of a column flux along one axis at index a of n: central
+   differences on a periodic axis, half-sum face fluxes with zero flux
+   through the walls of a solid one */
+static REAL flux_div(const REAL *q, size_t stride, int a, int n, int solid, REAL dx) {
+    if (solid) {
+        REAL fl = R(0), fr = R(0);
+        if (a > 0) fl = R(0.5) * (q[(size_t)(a - 1) * stride] + q[(size_t)a * stride]);
+        if (a < n - 1) fr = R(0.5) * (q[(size_t)a * stride] + q[(size_t)(a + 1) * stride]);
+        return (fr - fl) / dx;
+    }
+    int ap = (a + 1) % n, am = (a - 1 + n) % n;
+    return (q[(size_t)ap * stride] - q[(size_t)am * stride]) * (R(0.5) / dx);
+}
+
+/* advect_free_surface_height: h += dt (-(dq_x/dx + dq_y/dy)), kept between
+   half a cell above the bed and half a cell below the lid (height_kinematic);
+   height_static keeps h.  y walls (new) take the x-wall form. */
+static void fs_advect_height(O *o) {
+    const lbm_config_t *c = &o->c;
+    const REAL dx = R(c->dx), dt = R(c->dt_phys);
+    const REAL lz = (REAL)o->nz * dx;
+    for (int j = 0; j < o->ny; j++)
+        for (int i = 0; i < o->nx; i++) {
+            size_t cid = ID2(j, i);
+            REAL h_old = o->fs_h[cid], h_new = h_old;
+            if (c->fs_mode == LBM_FSM_HEIGHT_KINEMATIC) {
+                REAL dq_dx = flux_div(o->fs_qx + (size_t)j * o->nx, 1, i, o->nx, c->x_boundary == LBM_XB_SOLID, dx);
+                REAL dq_dy = flux_div(o->fs_qy + i, (size_t)o->nx, j, o->ny, c->y_boundary == LBM_YB_SOLID, dx);
+                REAL dh_dt = -(dq_dx + dq_dy);
+                h_new = h_old + dt * dh_dt;
+                REAL min_h = o->water_h[cid] + R(0.5) * dx;
+                REAL max_h = lz - R(0.5) * dx;
+                if (h_new < min_h) h_new = min_h;
+                if (h_new > max_h) h_new = max_h;
+            }
+            o->fs_h_last[cid] = h_old;
+            o->fs_h[cid] = h_new;
+        }
+}
+
+/* refill_new_free_surface_cells (CODEX:2163-2316) */
 static void vof_refill(O *o) {
     const lbm_config_t *c = &o->c;
     o->cnt.free_surface_refill_count = 0;
@@ -1832,20 +1925,24 @@ static void vof_refill(O *o) {
                             fnf[qq] += wn * fneq;
                         }
                 }
-                REAL rr = R(c->rho0), urx = R(c->u_current_lb), ury = R(0), urz = R(0);
+                /* VOF: density from the liquid neighbours; height tracking:
+                   the hydrostatic density of the column (CODEX:2225-2246) */
+                const int vof = is_vof(o);
+                REAL rr = vof ? R(c->rho0) : fs_density_cell(o, k, j, i);
+                REAL urx = R(c->u_current_lb), ury = R(0), urz = R(0);
                 if (ws > R(0)) {
                     REAL inv = R(1.0) / ws;
-                    rr = ra * inv;
+                    if (vof) rr = ra * inv;
                     urx = xa * inv;
                     ury = ya * inv;
                     urz = za * inv;
                 } else if (fws > R(0)) {
                     REAL inv = R(1.0) / fws;
-                    rr = rf * inv;
+                    if (vof) rr = rf * inv;
                     urx = xf * inv;
                     ury = yf * inv;
                     urz = zf * inv;
-                } else {
+                } else if (vof) {
                     rr = R(c->rho0);
                 }
                 REAL fv[19];
@@ -2013,6 +2110,8 @@ void *API(orc_create)(const lbm_config_t *cfg) {
     AI(o->ext_gas, n);
     AR(o->fs_h, n2);
     AR(o->fs_h_last, n2);
+    AR(o->fs_qx, n2);
+    AR(o->fs_qy, n2);
     AR(o->s_m, n);
     AR(o->s_a, n);
     AR(o->s_b, n);
@@ -2051,7 +2150,7 @@ void API(orc_destroy)(void *p) {
                     o->obst, o->obst_prev, o->water_h, o->water_h_last, o->u_wave, o->v_wave, o->w_wave,
                     o->piecewise_bed_z, o->u_inlet_profile, o->inflow_ux, o->fill, o->mass, o->mass_delta,
                     o->excess, o->det_delta, o->pref_head, o->grav_frac, o->type, o->type_prev, o->excess_dir,
-                    o->ext_gas, o->fs_h, o->fs_h_last, o->s_m, o->s_a, o->s_b, o->s_c, o->s_d, o->s_e, o->s_mv, o->s_i};
+                    o->ext_gas, o->fs_h, o->fs_h_last, o->fs_qx, o->fs_qy, o->s_m, o->s_a, o->s_b, o->s_c, o->s_d, o->s_e, o->s_mv, o->s_i};
     for (size_t i = 0; i < sizeof(ptrs) / sizeof(ptrs[0]); i++) free(ptrs[i]);
     free(o);
 }
@@ -2175,6 +2274,7 @@ static void one_step(O *o) {
     bed_update(o, (REAL)t);
     compute_phi(o);
     build_lattice_open(o);
+    if (is_water(o) && !is_vof(o)) fs_classify(o);
     if (is_water(o)) update_pressure_reference(o);
     bed_populations(o);
     macro(o);
@@ -2211,6 +2311,11 @@ static void one_step(O *o) {
             vof_height(o);
             vof_refill(o);
             vof_repair(o);
+        } else {
+            fs_column_flux(o);
+            fs_advect_height(o);
+            fs_classify(o);
+            vof_refill(o);
         }
     }
 }
@@ -2250,7 +2355,8 @@ double API(orc_reduce)(void *p, int what) {
    20 VOF mass exchange, 21 preclassify, 22 redistribute, 23 finalize,
    24 interface layer, 25 orphans, 26 detached advection, 27 detached
    compaction, 28 trapped-gas collapse, 29 column height, 30 refill,
-   31 coalescence repair. */
+   31 coalescence repair, 32 height-function classification, 33 column
+   flux, 34 height advection. */
 int API(orc_stage)(void *p, int stage, float t) {
     O *o = (O *)p;
     switch (stage) {
@@ -2283,6 +2389,9 @@ int API(orc_stage)(void *p, int stage, float t) {
         case 29: vof_height(o); break;
         case 30: vof_refill(o); break;
         case 31: vof_repair(o); break;
+        case 32: fs_classify(o); break;
+        case 33: fs_column_flux(o); break;
+        case 34: fs_advect_height(o); break;
         default: return LBM_E_ARG;
     }
     return 0;

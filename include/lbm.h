@@ -22,7 +22,7 @@
 extern "C" {
 #endif
 
-#define LBM_CONFIG_VERSION 5
+#define LBM_CONFIG_VERSION 6
 
 /* ---- error codes ---------------------------------------------------- */
 #define LBM_OK 0
@@ -125,6 +125,7 @@ typedef struct lbm_config {
     int32_t vof_detached_max_wet_neighbors, vof_detached_max_resolved_neighbors;
     int32_t vof_collapse_interval, vof_collapse_flood_sweeps;
     int32_t fs_mode; /* LBM_FSM_*: how the height-function surface moves */
+    int32_t include_cube; /* LBM_OBSTACLE_MODE bed_cube / cube (CODEX:372-400) */
     uint32_t seed; /* backscatter counter-based RNG seed */
 
     float dx, rho0, tau0, nu0;
@@ -138,6 +139,9 @@ typedef struct lbm_config {
     float vof_detached_residual_fill_threshold, vof_thin_gap_bridge_strength;
     float vof_collapse_max_volume_m3;
     float storage_u_ref[3]; /* reference velocity of the FP16/BF16 shift */
+    float cube_half[3];     /* half sizes (m) of the box obstacle */
+    float cube_center[3];   /* centre (m) */
+    float cube_rot[4];      /* cos / sin of yaw, cos / sin of pitch */
     float pad0;
     double dt_phys; /* per-step time, the host forms t=(step+1)*dt_phys */
 } lbm_config_t;

@@ -93,8 +93,8 @@ static const char* validate(const lbm_config_t* c) {
     if (!(c->dx > 0.0f)) return "dx must be positive";
     if (c->vof_collapse_interval < 1) return "vof_collapse_interval must be positive";
     if ((c->storage == LBM_STORE_FP16 || c->storage == LBM_STORE_BF16) && !fused_supported(*c))
-        return "fp16/bf16 storage runs on the fused path: air mode only";
-    if (c->kernel_path == LBM_PATH_FUSED && !fused_supported(*c)) return "the fused path covers air mode only";
+        return "fp16/bf16 storage runs on the fused path: air mode without the box obstacle";
+    if (c->kernel_path == LBM_PATH_FUSED && !fused_supported(*c)) return "the fused path covers air mode without the box obstacle";
     return nullptr;
 }
 

@@ -327,6 +327,8 @@ struct Consts {
     T u_top_lb, u_current_lb, g_lb, g_body_lb, c_backscatter;
     T free_surface_base_m, bridge_strength;
     T offset_m, bed_amp_m, bed_wavelength_m, wave_speed, wave_period, depth_swl_m, vel_scale, phase_speed;
+    int include_cube;
+    T cube_half[3], cube_center[3], cube_rot[4];  // box obstacle: half sizes, centre, cos/sin yaw, cos/sin pitch
     T dt_phys;
     T inv_dx;     // 1/dx, used in place of a division by dx when dx is a power of two (exact)
     int dx_pow2;
@@ -363,9 +365,42 @@ template <class T>
 struct CellGeo {
     uint32_t open_mask;  // bit q set when lattice_open[q] == 1 (bit 0 = any open)
     uint32_t xwall_mask; // bit q set when the link is an x/y wall override
+    uint32_t cube_mask;  // bit q set when the box obstacle is the nearer solid of link q (bit 0: cell centre)
     int near;
     T lf[19];
 };
+
+// signed distance to the box obstacle (CODEX:897-934): offsets folded into
+// the periodic domain (x only when x is periodic, y always), rotated by the
+// yaw about z and the pitch about y, then the box distance (outside: length
+// of the positive excess; inside: the largest, negative, excess)
+template <class T>
+__device__ __forceinline__ T cube_sdf(const Consts<T>& c, T xp, T yp, T zp) {
+    const T lx = (T)c.nx * c.dx, ly = (T)c.ny * c.dx;
+    T dxp = xp - c.cube_center[0];
+    T dyp = yp - c.cube_center[1];
+    if (c.x_boundary == LBM_XB_PERIODIC) {
+        if (dxp > T(0.5) * lx) dxp -= lx;
+        if (dxp < T(-0.5) * lx) dxp += lx;
+    }
+    if (dyp > T(0.5) * ly) dyp -= ly;
+    if (dyp < T(-0.5) * ly) dyp += ly;
+    const T yc = c.cube_rot[0], ys = c.cube_rot[1], pc = c.cube_rot[2], ps = c.cube_rot[3];
+    T xl = yc * dxp + ys * dyp;
+    T yl = -ys * dxp + yc * dyp;
+    T z1 = zp - c.cube_center[2];
+    T xpitch = pc * xl - ps * z1;
+    T zl = ps * xl + pc * z1;
+    T qx = fabs(xpitch) - c.cube_half[0];
+    T qy = fabs(yl) - c.cube_half[1];
+    T qz = fabs(zl) - c.cube_half[2];
+    T ox = qx > T(0) ? qx : T(0), oy = qy > T(0) ? qy : T(0), oz = qz > T(0) ? qz : T(0);
+    T outside = sqrt(ox * ox + oy * oy + oz * oz);
+    T m = qy > qz ? qy : qz;
+    m = qx > m ? qx : m;
+    T inside = m < T(0) ? m : T(0);
+    return outside + inside;
+}
 
 template <class T>
 __device__ __forceinline__ void cell_geometry(const Consts<T>& c, const Slot<T>& s, int i, int j, int k, CellGeo<T>& g) {
@@ -373,11 +408,19 @@ __device__ __forceinline__ void cell_geometry(const Consts<T>& c, const Slot<T>&
     const size_t plane = (size_t)c.nx * c.ny;
     const T dx = c.dx;
     T zc = ((T)k + T(0.5)) * dx;
-    uint32_t mask = 0, xw = 0;
+    const T xc = ((T)i + T(0.5)) * dx, yc = ((T)j + T(0.5)) * dx;
+    uint32_t mask = 0, xw = 0, cube = 0;
     {
         T eta_c = s.eta9[4 * plane + col];
         T gap_c = zc - eta_c;
         T ggap = c.include_bed ? gap_c : T(1.0e6);
+        if (c.include_cube) {
+            T cg = cube_sdf(c, xc, yc, zc);
+            if (cg < ggap) {
+                ggap = cg;
+                cube |= 1u;
+            }
+        }
         T frac = T(0.5) + div_dx(c, ggap);
         if (frac < T(0)) frac = T(0);
         if (frac > T(1)) frac = T(1);
@@ -389,6 +432,13 @@ __device__ __forceinline__ void cell_geometry(const Consts<T>& c, const Slot<T>&
         T eta = s.eta9[(size_t)c9q(q) * plane + col];
         T gap = zq - eta;
         T ggap = c.include_bed ? gap : T(1.0e6);
+        if (c.include_cube) {
+            T cg = cube_sdf(c, xc + T(0.5) * dx * (T)cxq(q), yc + T(0.5) * dx * (T)cyq(q), zq);
+            if (cg < ggap) {
+                ggap = cg;
+                cube |= 1u << q;
+            }
+        }
         T frac = T(0.5) + div_dx(c, ggap);
         if (frac < T(0)) frac = T(0);
         if (frac > T(1)) frac = T(1);
@@ -418,6 +468,7 @@ __device__ __forceinline__ void cell_geometry(const Consts<T>& c, const Slot<T>&
     }
     g.open_mask = mask | (uint32_t)any_open;
     g.xwall_mask = xw;
+    g.cube_mask = cube;
 }
 
 // wall type of link q given the geometry (bed-only obstacle): x/y wall
@@ -430,6 +481,7 @@ __device__ __forceinline__ int link_wall_type(const Consts<T>& c, const CellGeo<
         if (c.y_boundary == LBM_YB_SOLID && j == c.ny - 1 && cyq(q) < 0) return LBM_WALL_YMAX;
         return (i == 0 && cxq(q) > 0) ? LBM_WALL_XMIN : LBM_WALL_XMAX;
     }
+    if ((g.cube_mask >> q) & 1u) return LBM_WALL_CUBE;
     return c.include_bed ? LBM_WALL_BED : LBM_WALL_NONE;
 }
 
@@ -460,6 +512,10 @@ __device__ __forceinline__ T cell_phi(const Consts<T>& c, const Slot<T>& s, int 
     T xc = ((T)i + T(0.5)) * dx;
     T phi = T(1.0e6);
     if (c.include_bed) phi = (zc - s.h[col]) / s.denom[col];
+    if (c.include_cube) {
+        T cp = cube_sdf(c, xc, ((T)j + T(0.5)) * dx, zc);
+        if (cp < phi) phi = cp;
+    }
     if (c.x_boundary == LBM_XB_SOLID) {
         T a = xc, b = (T)c.nx * dx - xc;
         T xw = a < b ? a : b;
@@ -476,7 +532,7 @@ __device__ __forceinline__ void condition_cell(const Consts<T>& c, const Slot<T>
     for (int q = 0; q < 19; q++) {
         bool open = q == 0 ? (g.open_mask & 1u) : ((g.open_mask >> q) & 1u);
         if (open) continue;
-        int wt = q == 0 ? (c.include_bed ? LBM_WALL_BED : LBM_WALL_NONE) : link_wall_type(c, g, i, j, q);
+        int wt = link_wall_type(c, g, i, j, q);
         T ux = T(0), uy = T(0), uz = T(0);
         if (wt == LBM_WALL_BED && c.bed_moving) {
             wall_velocity(c, s, i, j, q, ux, uy, uz);

@@ -47,6 +47,12 @@ static Consts<T> make_consts(const lbm_config_t& c, int omega_band_top) {
     k.bed_profile = c.bed_profile;
     k.bed_moving = c.bed_profile == LBM_BED_MOVING_SINE || c.bed_profile == LBM_BED_MOVING_SINE1;
     k.include_bed = c.include_bed;
+    k.include_cube = c.include_cube;
+    for (int a = 0; a < 3; a++) {
+        k.cube_half[a] = (T)c.cube_half[a];
+        k.cube_center[a] = (T)c.cube_center[a];
+    }
+    for (int a = 0; a < 4; a++) k.cube_rot[a] = (T)c.cube_rot[a];
     k.wall_link = c.wall_velocity_link;
     k.top_drive = c.enable_top_drive;
     k.x_fringe = c.enable_x_fringe;
@@ -342,7 +348,7 @@ __global__ __launch_bounds__(NT) void k_geometry_fields(Consts<T> c, Slot<T> s, 
     for (int q = 0; q < 19; q++) {
         lo[(size_t)q * n + id] = (g.open_mask >> q) & 1u;
         lf[(size_t)q * n + id] = g.lf[q];
-        wt[(size_t)q * n + id] = q == 0 ? (c.include_bed ? LBM_WALL_BED : LBM_WALL_NONE) : link_wall_type(c, g, i, j, q);
+        wt[(size_t)q * n + id] = link_wall_type(c, g, i, j, q);
     }
     near[id] = g.near;
     phi[id] = cell_phi(c, s, i, j, k);
@@ -394,7 +400,7 @@ __global__ __launch_bounds__(NT) void k_init_populations(Consts<T> c, Slot<T> s,
     obst[id] = open0 ? 0 : 1;
     if (!open0) {
         T a = T(0), bq = T(0), e = T(0);
-        if (c.include_bed) {
+        if (link_wall_type(c, g, i, j, 0) == LBM_WALL_BED) {
             size_t col = (size_t)j * c.nx + i;
             a = s.uw[col];
             bq = s.vw[col];
@@ -2459,7 +2465,8 @@ __global__ __launch_bounds__(NT) void k_load_populations(Consts<T> c, const S* s
 // ===========================================================================
 // launchers
 // ===========================================================================
-bool fused_supported(const lbm_config_t& c) { return c.physics_mode == LBM_PHYS_AIR && c.nz >= 3; }
+// air mode without the box obstacle (the band passes know the bed only)
+bool fused_supported(const lbm_config_t& c) { return c.physics_mode == LBM_PHYS_AIR && !c.include_cube && c.nz >= 3; }
 
 template <class T>
 static void bed2d_t(const lbm_config_t& cfg, DevBufs& b, float t, int slot, hipStream_t s) {

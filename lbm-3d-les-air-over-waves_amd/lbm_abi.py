@@ -8,7 +8,7 @@ from __future__ import annotations
 
 import ctypes
 
-LBM_CONFIG_VERSION = 5
+LBM_CONFIG_VERSION = 6
 
 # error codes
 LBM_OK = 0
@@ -148,6 +148,7 @@ class LbmConfig(ctypes.Structure):
         ("vof_collapse_interval", _I),
         ("vof_collapse_flood_sweeps", _I),
         ("fs_mode", _I),
+        ("include_cube", _I),
         ("seed", ctypes.c_uint32),
         ("dx", _F),
         ("rho0", _F),
@@ -178,6 +179,9 @@ class LbmConfig(ctypes.Structure):
         ("vof_thin_gap_bridge_strength", _F),
         ("vof_collapse_max_volume_m3", _F),
         ("storage_u_ref", _F * 3),
+        ("cube_half", _F * 3),
+        ("cube_center", _F * 3),
+        ("cube_rot", _F * 4),
         ("pad0", _F),
         ("dt_phys", ctypes.c_double),
     ]

@@ -94,6 +94,10 @@ class SolverConfig:
             "moving_sine1": abi.BED_MOVING_SINE1,
         }[v["bed_profile"]]
         s.include_bed = v["include_bed_geometry"]
+        s.include_cube = v["include_cube_geometry"]
+        s.cube_half[:] = v["cube_half"]
+        s.cube_center[:] = v["cube_center"]
+        s.cube_rot[:] = v["cube_rot"]
         s.wall_velocity_link = v["use_link_wall_velocity"]
         s.enable_top_drive = v["enable_top_drive"]
         s.enable_x_fringe = v["enable_x_fringe"]
@@ -428,6 +432,23 @@ def parse_env(env: Optional[Mapping[str, str]] = None) -> SolverConfig:
         raise ValueError("LBM_OBSTACLE_MODE must be 'bed', 'bed_cube', or 'cube'")
     include_bed_geometry = 1 if obstacle_mode in ("bed", "bed_cube") else 0
     include_cube_geometry = 1 if obstacle_mode in ("bed_cube", "cube") else 0
+    # the box obstacle (CODEX:376-400): sizes, centre, yaw about z, pitch about y
+    cube_side_m = float(env.get("LBM_CUBE_SIDE_M", 6.0))
+    cube_size = [float(env.get(f"LBM_CUBE_SIZE_{a}_M", cube_side_m)) for a in "XYZ"]
+    if min(cube_size) <= 0.0:
+        raise ValueError("LBM_CUBE_SIZE_X_M, LBM_CUBE_SIZE_Y_M, and LBM_CUBE_SIZE_Z_M must be positive")
+    cube_half = [0.5 * v for v in cube_size]
+    cube_yaw_deg = float(env.get("LBM_CUBE_YAW_DEG", 0.0))
+    cube_pitch_deg = float(env.get("LBM_CUBE_PITCH_DEG", 0.0))
+    cube_rot = [float(np.cos(np.deg2rad(cube_yaw_deg))), float(np.sin(np.deg2rad(cube_yaw_deg))),
+                float(np.cos(np.deg2rad(cube_pitch_deg))), float(np.sin(np.deg2rad(cube_pitch_deg)))]
+    cube_vertical_half_extent_m = cube_half[2] * abs(cube_rot[2]) + cube_half[0] * abs(cube_rot[3])
+    cube_center_z_default = min(
+        Lz_m - cube_vertical_half_extent_m - dx,
+        max(0.5 * Lz_m, offset_m + 2.0 * bed_amp_m + cube_vertical_half_extent_m + dx),
+    )
+    cube_center = [float(env.get("LBM_CUBE_CENTER_X_M", 0.5 * Lx_m)), float(env.get("LBM_CUBE_CENTER_Y_M", 0.5 * Ly_m)),
+                   float(env.get("LBM_CUBE_CENTER_Z_M", cube_center_z_default))]
 
     rho0 = 1.0
     u_lid = 0.075
@@ -524,8 +545,6 @@ def parse_env(env: Optional[Mapping[str, str]] = None) -> SolverConfig:
 
     # features this build does not implement are refused, not ignored
     unsupported = []
-    if include_cube_geometry:
-        unsupported.append("LBM_OBSTACLE_MODE with a cube")
     if boundary_geometry_mode != "vertical":
         unsupported.append("LBM_BOUNDARY_GEOMETRY=phi_fraction|phi")
     if water_free_surface_enabled and free_surface_mode == "prescribed_solitary":
@@ -620,6 +639,12 @@ def parse_env(env: Optional[Mapping[str, str]] = None) -> SolverConfig:
             obstacle_mode=obstacle_mode,
             include_bed_geometry=include_bed_geometry,
             include_cube_geometry=include_cube_geometry,
+            cube_size_x_m=cube_size[0], cube_size_y_m=cube_size[1], cube_size_z_m=cube_size[2], cube_side_m=cube_side_m,
+            cube_half_x_m=cube_half[0], cube_half_y_m=cube_half[1], cube_half_z_m=cube_half[2], cube_half=cube_half,
+            cube_center_x_m=cube_center[0], cube_center_y_m=cube_center[1], cube_center_z_m=cube_center[2],
+            cube_center=cube_center, cube_yaw_deg=cube_yaw_deg, cube_pitch_deg=cube_pitch_deg,
+            cube_yaw_cos=cube_rot[0], cube_yaw_sin=cube_rot[1], cube_pitch_cos=cube_rot[2], cube_pitch_sin=cube_rot[3],
+            cube_rot=cube_rot,
             rho0=rho0,
             u_lid=u_lid,
             U_ref_phys=U_ref_phys,

@@ -311,7 +311,38 @@ static REAL sample2d(const O *o, const REAL *H, REAL xq, REAL yq) {
     return hx0 * (R(1.0) - fy) + hx1 * fy;
 }
 
-/* compute_phi_slope_corrected (CODEX:2432-2490), bed + x-wall terms. */
+/* cube_sdf_at_point (CODEX:897-934): signed distance to the box obstacle,
+   offsets folded into the periodic domain (x only when x is periodic, y
+   always), rotated by yaw about z then pitch about y. */
+static REAL cube_sdf(const O *o, REAL xp, REAL yp, REAL zp) {
+    const lbm_config_t *c = &o->c;
+    const REAL lx = (REAL)o->nx * R(c->dx), ly = (REAL)o->ny * R(c->dx);
+    REAL dxp = xp - R(c->cube_center[0]);
+    REAL dyp = yp - R(c->cube_center[1]);
+    if (c->x_boundary == LBM_XB_PERIODIC) {
+        if (dxp > R(0.5) * lx) dxp -= lx;
+        if (dxp < R(-0.5) * lx) dxp += lx;
+    }
+    if (dyp > R(0.5) * ly) dyp -= ly;
+    if (dyp < R(-0.5) * ly) dyp += ly;
+    const REAL ycos = R(c->cube_rot[0]), ysin = R(c->cube_rot[1]), pcos = R(c->cube_rot[2]), psin = R(c->cube_rot[3]);
+    REAL x_local = ycos * dxp + ysin * dyp;
+    REAL y_local = -ysin * dxp + ycos * dyp;
+    REAL z1 = zp - R(c->cube_center[2]);
+    REAL x_pitch = pcos * x_local - psin * z1;
+    REAL z_local = psin * x_local + pcos * z1;
+    REAL qx = (REAL)fabs((double)x_pitch) - R(c->cube_half[0]);
+    REAL qy = (REAL)fabs((double)y_local) - R(c->cube_half[1]);
+    REAL qz = (REAL)fabs((double)z_local) - R(c->cube_half[2]);
+    REAL ox = qx > R(0) ? qx : R(0), oy = qy > R(0) ? qy : R(0), oz = qz > R(0) ? qz : R(0);
+    REAL outside = (REAL)sqrt((double)(ox * ox + oy * oy + oz * oz));
+    REAL m = qy > qz ? qy : qz;
+    m = qx > m ? qx : m;
+    REAL inside = m < R(0) ? m : R(0);
+    return outside + inside;
+}
+
+/* compute_phi_slope_corrected (CODEX:2432-2490), bed, box and x-wall terms. */
 static void compute_phi(O *o) {
     const lbm_config_t *c = &o->c;
     REAL dx = R(c->dx);
@@ -335,6 +366,10 @@ static void compute_phi(O *o) {
                 REAL xc = ((REAL)i + R(0.5)) * dx;
                 REAL phi_val = R(1.0e6);
                 if (c->include_bed) phi_val = (zc - h_loc) / denom;
+                if (c->include_cube) {
+                    REAL cp = cube_sdf(o, xc, ((REAL)j + R(0.5)) * dx, zc);
+                    if (cp < phi_val) phi_val = cp;
+                }
                 if (c->x_boundary == LBM_XB_SOLID) {
                     REAL a = xc, b = (REAL)o->nx * dx - xc;
                     REAL xw = a < b ? a : b;
@@ -346,7 +381,8 @@ static void compute_phi(O *o) {
 }
 
 /* build_lattice_open_from_free_surface_periodic (CODEX:2492-2686), vertical
-   geometry mode, bed obstacle; solid y walls mirror the x-wall override. */
+   geometry mode, bed and box obstacles (the nearer one sets the gap and the
+   wall type); solid y walls mirror the x-wall override. */
 static void build_lattice_open(O *o) {
     const lbm_config_t *c = &o->c;
     REAL dx = R(c->dx);
@@ -369,6 +405,13 @@ static void build_lattice_open(O *o) {
                     ggap_c = gap_c;
                     wtc = LBM_WALL_BED;
                 }
+                if (c->include_cube) {
+                    REAL cg = cube_sdf(o, xc, yc, zc);
+                    if (cg < ggap_c) {
+                        ggap_c = cg;
+                        wtc = LBM_WALL_CUBE;
+                    }
+                }
                 REAL frac_c = R(0.5) + ggap_c / dx;
                 if (frac_c < R(0)) frac_c = R(0);
                 if (frac_c > R(1)) frac_c = R(1);
@@ -385,6 +428,13 @@ static void build_lattice_open(O *o) {
                     if (c->include_bed) {
                         ggap = gap;
                         wq = LBM_WALL_BED;
+                    }
+                    if (c->include_cube) {
+                        REAL cg = cube_sdf(o, xq, yq, zq);
+                        if (cg < ggap) {
+                            ggap = cg;
+                            wq = LBM_WALL_CUBE;
+                        }
                     }
                     REAL frac = R(0.5) + ggap / dx;
                     if (frac < R(0)) frac = R(0);

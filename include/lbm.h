@@ -22,7 +22,7 @@
 extern "C" {
 #endif
 
-#define LBM_CONFIG_VERSION 6
+#define LBM_CONFIG_VERSION 7
 
 /* ---- error codes ---------------------------------------------------- */
 #define LBM_OK 0
@@ -126,6 +126,7 @@ typedef struct lbm_config {
     int32_t vof_collapse_interval, vof_collapse_flood_sweeps;
     int32_t fs_mode; /* LBM_FSM_*: how the height-function surface moves */
     int32_t include_cube; /* LBM_OBSTACLE_MODE bed_cube / cube (CODEX:372-400) */
+    int32_t phi_fraction, phi_flags; /* LBM_BOUNDARY_GEOMETRY phi_fraction / phi (CODEX:194-198) */
     uint32_t seed; /* backscatter counter-based RNG seed */
 
     float dx, rho0, tau0, nu0;

@@ -141,6 +141,7 @@ static size_t device_bytes(const lbm_config_t& c, int path) {
     if (path == LBM_PATH_STAGED) bytes += 2 * n * R + 2 * n * 4;  // nu_t, smag, obstacle masks
     bytes += 2 * n2 * (78 * R + 8) + (c.nx + c.nz + (size_t)c.nz * c.ny) * R;
     if (c.fs_enabled) bytes += n * (11 * R + 7 * 4) + 6 * n * R + 3 * n2 * R;
+    if (c.phi_fraction) bytes += n * R;
     return bytes + (1 << 20);
 }
 
@@ -217,6 +218,7 @@ static void allocate(lbm_handle* h) {
         CHECK(hipMemsetAsync(b.vw[s], 0, n2 * R, h->stream));
         CHECK(hipMemsetAsync(b.ww[s], 0, n2 * R, h->stream));
     }
+    if (c.phi_fraction) b.phi = dalloc(h, n * R);
     b.piecewise = dalloc(h, c.nx * R);
     b.u_inlet = dalloc(h, c.nz * R);
     b.inflow = dalloc(h, (size_t)c.nz * c.ny * R);

@@ -328,6 +328,7 @@ struct Consts {
     T free_surface_base_m, bridge_strength;
     T offset_m, bed_amp_m, bed_wavelength_m, wave_speed, wave_period, depth_swl_m, vel_scale, phase_speed;
     int include_cube;
+    int phi_fraction, phi_flags;  // open fractions / open flags from phi sampled at the link endpoints
     T cube_half[3], cube_center[3], cube_rot[4];  // box obstacle: half sizes, centre, cos/sin yaw, cos/sin pitch
     T dt_phys;
     T inv_dx;     // 1/dx, used in place of a division by dx when dx is a power of two (exact)
@@ -357,6 +358,7 @@ struct Slot {
     const T* fsol;      // [19][ny][nx] feq(rho0, wall velocity of link q, q): the conditioned
                         // value of a closed link, for every column
     const T* ibb;       // [18][ny][nx] moving-wall term 6 w_qb rho0 (c_qb . u_wall) of link q = 1..18
+    const T* phi;       // [nz][ny][nx] phi of this time level (phi boundary-geometry modes only)
 };
 
 // link geometry of one cell (CODEX:2492-2686): open flag, open fraction,
@@ -402,6 +404,32 @@ __device__ __forceinline__ T cube_sdf(const Consts<T>& c, T xp, T yp, T zp) {
     return outside + inside;
 }
 
+// sample_phi_periodic (CODEX:661-707): trilinear sample of phi, periodic
+// bilinear in x / y with the bed sampling's indices, clamped in z
+template <class T>
+__device__ __forceinline__ T sample_phi(const Consts<T>& c, const T* phi, T xp, T yp, T zp) {
+    const T dx = c.dx;
+    const Axis2d<T> ax = sample_axis(xp, c.nx, dx), ay = sample_axis(yp, c.ny, dx);
+    T r = zp / dx - T(0.5);
+    int k0 = (int)floor(r);
+    T fz = r - (T)k0;
+    int k1 = k0 + 1;
+    if (k0 < 0) {
+        k0 = 0;
+        k1 = 0;
+        fz = T(0);
+    }
+    if (k1 >= c.nz) {
+        k0 = c.nz - 1;
+        k1 = c.nz - 1;
+        fz = T(0);
+    }
+    const size_t plane = (size_t)c.nx * c.ny;
+    const T p0 = sample_at(phi + (size_t)k0 * plane, ax, ay, c.nx);
+    const T p1 = sample_at(phi + (size_t)k1 * plane, ax, ay, c.nx);
+    return p0 * (T(1.0) - fz) + p1 * fz;
+}
+
 template <class T>
 __device__ __forceinline__ void cell_geometry(const Consts<T>& c, const Slot<T>& s, int i, int j, int k, CellGeo<T>& g) {
     const size_t col = (size_t)j * c.nx + i;
@@ -421,7 +449,8 @@ __device__ __forceinline__ void cell_geometry(const Consts<T>& c, const Slot<T>&
                 cube |= 1u;
             }
         }
-        T frac = T(0.5) + div_dx(c, ggap);
+        T sgap = c.phi_fraction ? s.phi[((size_t)k * c.ny + j) * c.nx + i] : ggap;
+        T frac = T(0.5) + div_dx(c, sgap);
         if (frac < T(0)) frac = T(0);
         if (frac > T(1)) frac = T(1);
         g.lf[0] = frac;
@@ -439,10 +468,11 @@ __device__ __forceinline__ void cell_geometry(const Consts<T>& c, const Slot<T>&
                 cube |= 1u << q;
             }
         }
-        T frac = T(0.5) + div_dx(c, ggap);
+        T sgap = c.phi_fraction ? sample_phi(c, s.phi, xc + T(0.5) * dx * (T)cxq(q), yc + T(0.5) * dx * (T)cyq(q), zq) : ggap;
+        T frac = T(0.5) + div_dx(c, sgap);
         if (frac < T(0)) frac = T(0);
         if (frac > T(1)) frac = T(1);
-        int lo = ggap < T(0) ? 0 : 1;
+        int lo = (c.phi_flags ? sgap : ggap) < T(0) ? 0 : 1;
         if (c.x_boundary == LBM_XB_SOLID) {
             if ((i == 0 && cxq(q) > 0) || (i == c.nx - 1 && cxq(q) < 0)) {
                 lo = 0;

@@ -53,6 +53,7 @@ struct DevBufs {
     int* ksolid[2];
     void* fsol[2];
     void* ibb[2];
+    void* phi;  // phi of the latest bed time level (phi boundary-geometry modes only)
     void *piecewise, *u_inlet, *inflow;
     uint8_t* cellflags;  // bit0 lattice_open[0], bit1 near_obstacle (staged path, current slot)
     // water / VOF

@@ -48,6 +48,8 @@ static Consts<T> make_consts(const lbm_config_t& c, int omega_band_top) {
     k.bed_moving = c.bed_profile == LBM_BED_MOVING_SINE || c.bed_profile == LBM_BED_MOVING_SINE1;
     k.include_bed = c.include_bed;
     k.include_cube = c.include_cube;
+    k.phi_fraction = c.phi_fraction;
+    k.phi_flags = c.phi_flags;
     for (int a = 0; a < 3; a++) {
         k.cube_half[a] = (T)c.cube_half[a];
         k.cube_center[a] = (T)c.cube_center[a];
@@ -129,6 +131,7 @@ static Slot<T> make_slot(DevBufs& b, int s) {
     sl.ksolid = b.ksolid[s];
     sl.fsol = (const T*)b.fsol[s];
     sl.ibb = (const T*)b.ibb[s];
+    sl.phi = (const T*)b.phi;
     return sl;
 }
 
@@ -2465,8 +2468,23 @@ __global__ __launch_bounds__(NT) void k_load_populations(Consts<T> c, const S* s
 // ===========================================================================
 // launchers
 // ===========================================================================
-// air mode without the box obstacle (the band passes know the bed only)
-bool fused_supported(const lbm_config_t& c) { return c.physics_mode == LBM_PHYS_AIR && !c.include_cube && c.nz >= 3; }
+// air mode, the bed the only obstacle and its gap the open fraction (the band
+// passes evaluate the bed's link geometry only)
+bool fused_supported(const lbm_config_t& c) {
+    return c.physics_mode == LBM_PHYS_AIR && !c.include_cube && !c.phi_fraction && c.nz >= 3;
+}
+
+// compute_phi_slope_corrected (CODEX:2432-2490) into a 3-D field, for the
+// boundary-geometry modes that sample it at the link endpoints
+template <class T>
+__global__ __launch_bounds__(NT) void k_phi(Consts<T> c, Slot<T> s, T* phi) {
+    size_t id = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    size_t n = (size_t)c.nx * c.ny * c.nz;
+    if (id >= n) return;
+    int i, j, k;
+    cell_ijk(id, c.nx, c.ny, i, j, k);
+    phi[id] = cell_phi(c, s, i, j, k);
+}
 
 template <class T>
 static void bed2d_t(const lbm_config_t& cfg, DevBufs& b, float t, int slot, hipStream_t s) {
@@ -2478,6 +2496,7 @@ static void bed2d_t(const lbm_config_t& cfg, DevBufs& b, float t, int slot, hipS
     k_bed2d_b<T><<<nblocks(n2, BED_COLS), 9 * BED_COLS, 0, s>>>(c, (const T*)b.h[slot], (T*)b.eta9[slot], (T*)b.denom[slot], b.kband[slot],
                                              (const T*)b.uw[slot], (const T*)b.vw[slot], (const T*)b.ww[slot], (T*)b.uw9[slot],
                                              b.ksolid[slot], (T*)b.fsol[slot], (T*)b.ibb[slot]);
+    if (cfg.phi_fraction) k_phi<T><<<nblocks(b.n), NT, 0, s>>>(c, make_slot<T>(b, slot), (T*)b.phi);
     HIP_OK(hipGetLastError());
 }
 void launch_bed2d(const lbm_config_t& c, DevBufs& b, float t, int slot, hipStream_t s) {

@@ -8,7 +8,7 @@ from __future__ import annotations
 
 import ctypes
 
-LBM_CONFIG_VERSION = 6
+LBM_CONFIG_VERSION = 7
 
 # error codes
 LBM_OK = 0
@@ -149,6 +149,8 @@ class LbmConfig(ctypes.Structure):
         ("vof_collapse_flood_sweeps", _I),
         ("fs_mode", _I),
         ("include_cube", _I),
+        ("phi_fraction", _I),
+        ("phi_flags", _I),
         ("seed", ctypes.c_uint32),
         ("dx", _F),
         ("rho0", _F),

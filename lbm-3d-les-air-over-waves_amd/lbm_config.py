@@ -95,6 +95,8 @@ class SolverConfig:
         }[v["bed_profile"]]
         s.include_bed = v["include_bed_geometry"]
         s.include_cube = v["include_cube_geometry"]
+        s.phi_fraction = 1 if v["boundary_geometry_mode"] in ("phi_fraction", "phi") else 0
+        s.phi_flags = 1 if v["boundary_geometry_mode"] == "phi" else 0
         s.cube_half[:] = v["cube_half"]
         s.cube_center[:] = v["cube_center"]
         s.cube_rot[:] = v["cube_rot"]
@@ -545,8 +547,6 @@ def parse_env(env: Optional[Mapping[str, str]] = None) -> SolverConfig:
 
     # features this build does not implement are refused, not ignored
     unsupported = []
-    if boundary_geometry_mode != "vertical":
-        unsupported.append("LBM_BOUNDARY_GEOMETRY=phi_fraction|phi")
     if water_free_surface_enabled and free_surface_mode == "prescribed_solitary":
         unsupported.append("LBM_FREE_SURFACE_MODE=prescribed_solitary")
     if init_fs_pops:

@@ -380,8 +380,51 @@ static void compute_phi(O *o) {
         }
 }
 
-/* build_lattice_open_from_free_surface_periodic (CODEX:2492-2686), vertical
-   geometry mode, bed and box obstacles (the nearer one sets the gap and the
+/* sample_phi_periodic (CODEX:661-707): trilinear sample of phi, periodic
+   bilinear in x / y (the bed sampling's indices), clamped in z */
+static REAL sample_phi(const O *o, REAL xp, REAL yp, REAL zp) {
+    const REAL dx = R(o->c.dx);
+    const REAL Lx = (REAL)o->nx * dx, Ly = (REAL)o->ny * dx;
+    const REAL Nxf = (REAL)o->nx, Nyf = (REAL)o->ny;
+    REAL xw = xp - (REAL)floor((double)(xp / Lx)) * Lx;
+    REAL yw = yp - (REAL)floor((double)(yp / Ly)) * Ly;
+    REAL u = xw / dx - R(0.5);
+    REAL v = yw / dx - R(0.5);
+    u = u - (REAL)floor((double)(u / Nxf)) * Nxf;
+    v = v - (REAL)floor((double)(v / Nyf)) * Nyf;
+    int i0 = (int)floor((double)u), j0 = (int)floor((double)v);
+    REAL fx = u - (REAL)i0, fy = v - (REAL)j0;
+    if (i0 >= o->nx) i0 -= o->nx;
+    if (j0 >= o->ny) j0 -= o->ny;
+    int i1 = (i0 + 1) % o->nx, j1 = (j0 + 1) % o->ny;
+    REAL r = zp / dx - R(0.5);
+    int k0 = (int)floor((double)r);
+    REAL fz = r - (REAL)k0;
+    int k1 = k0 + 1;
+    if (k0 < 0) {
+        k0 = 0;
+        k1 = 0;
+        fz = R(0);
+    }
+    if (k1 >= o->nz) {
+        k0 = o->nz - 1;
+        k1 = o->nz - 1;
+        fz = R(0);
+    }
+    const REAL *ph = o->phi;
+    REAL p00_0 = ph[IDX(k0, j0, i0)] * (R(1.0) - fx) + ph[IDX(k0, j0, i1)] * fx;
+    REAL p01_0 = ph[IDX(k0, j1, i0)] * (R(1.0) - fx) + ph[IDX(k0, j1, i1)] * fx;
+    REAL pxy_0 = p00_0 * (R(1.0) - fy) + p01_0 * fy;
+    REAL p00_1 = ph[IDX(k1, j0, i0)] * (R(1.0) - fx) + ph[IDX(k1, j0, i1)] * fx;
+    REAL p01_1 = ph[IDX(k1, j1, i0)] * (R(1.0) - fx) + ph[IDX(k1, j1, i1)] * fx;
+    REAL pxy_1 = p00_1 * (R(1.0) - fy) + p01_1 * fy;
+    return pxy_0 * (R(1.0) - fz) + pxy_1 * fz;
+}
+
+/* build_lattice_open_from_free_surface_periodic (CODEX:2492-2686): the
+   open fraction from the obstacle gap ("vertical") or from phi sampled at
+   the link endpoint ("phi_fraction", "phi"), the open flag from the gap or
+   (mode "phi") from the same phi sample; bed and box obstacles (the nearer one sets the gap and the
    wall type); solid y walls mirror the x-wall override. */
 static void build_lattice_open(O *o) {
     const lbm_config_t *c = &o->c;
@@ -412,7 +455,8 @@ static void build_lattice_open(O *o) {
                         wtc = LBM_WALL_CUBE;
                     }
                 }
-                REAL frac_c = R(0.5) + ggap_c / dx;
+                REAL sgap_c = c->phi_fraction ? o->phi[id] : ggap_c;
+                REAL frac_c = R(0.5) + sgap_c / dx;
                 if (frac_c < R(0)) frac_c = R(0);
                 if (frac_c > R(1)) frac_c = R(1);
                 lf[0] = frac_c;
@@ -436,12 +480,14 @@ static void build_lattice_open(O *o) {
                             wq = LBM_WALL_CUBE;
                         }
                     }
-                    REAL frac = R(0.5) + ggap / dx;
+                    REAL sgap = c->phi_fraction ? sample_phi(o, xq, yq, zq) : ggap;
+                    REAL frac = R(0.5) + sgap / dx;
                     if (frac < R(0)) frac = R(0);
                     if (frac > R(1)) frac = R(1);
                     lf[q] = frac;
                     wt[q] = wq;
-                    lo[q] = (ggap < R(0)) ? 0 : 1;
+                    REAL flag_gap = c->phi_flags ? sgap : ggap;
+                    lo[q] = (flag_gap < R(0)) ? 0 : 1;
                     if (solid_x) {
                         if (i == 0 && CX[q] > 0) {
                             lo[q] = 0;

@@ -1,7 +1,9 @@
-"""Parity of the box obstacle on the device (staged path) with the oracle:
-geometry fields bit-exact, then 100 steps within the FP32 bar (max-relative
-1e-5) for a box above a moving sine bed and for a yawed and pitched box in a
-channel with inflow."""
+"""Parity of the box obstacle and of the phi boundary-geometry modes on the
+device (staged path) with the oracle: geometry fields bit-exact, then 100
+steps within the FP32 bar (max-relative 1e-5) for a box above a moving sine
+bed, a yawed and pitched box in a channel with inflow, the moving bed with
+fractions and flags from phi, and the box over the bed with fractions from
+phi."""
 import numpy as np
 import pytest
 
@@ -19,6 +21,12 @@ CASES = {
                              LBM_CUBE_SIZE_X_M=8, LBM_CUBE_SIZE_Y_M=5, LBM_CUBE_SIZE_Z_M=6, LBM_CUBE_YAW_DEG=25,
                              LBM_CUBE_PITCH_DEG=15, LBM_CUBE_CENTER_X_M=20.2, LBM_CUBE_CENTER_Y_M=7.7,
                              LBM_CUBE_CENTER_Z_M=11.1, LBM_COLLISION="kbc"),
+    "phi_waves": env_of("waves_256x128x128", LBM_LX_M=64, LBM_LY_M=16, LBM_LZ_M=24, LBM_BED_AMP_M=2,
+                        LBM_BED_WAVELENGTH_M=32, LBM_BOUNDARY_GEOMETRY="phi"),
+    "phi_fraction_bed_cube": env_of("waves_256x128x128", LBM_LX_M=64, LBM_LY_M=16, LBM_LZ_M=24, LBM_BED_AMP_M=2,
+                                    LBM_BED_WAVELENGTH_M=32, LBM_OBSTACLE_MODE="bed_cube", LBM_CUBE_SIDE_M=6,
+                                    LBM_CUBE_CENTER_X_M=30.4, LBM_CUBE_CENTER_Y_M=8.1, LBM_CUBE_CENTER_Z_M=13.3,
+                                    LBM_BOUNDARY_GEOMETRY="phi_fraction"),
 }
 
 

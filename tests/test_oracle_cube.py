@@ -73,3 +73,42 @@ def test_rest_state_around_the_box_stays_at_rest():
     u = np.stack([o.download(f) for f in (abi.FIELD_UX, abi.FIELD_UY, abi.FIELD_UZ)])
     assert np.max(np.abs(u)) < 1e-12
     o.close()
+
+
+def test_phi_modes_on_a_flat_bed_equal_the_vertical_gap():
+    """On a flat bed phi is the vertical distance z - h, and its trilinear
+    sample at a link endpoint is exact for a linear field: the phi modes
+    reproduce the vertical mode's fractions and flags."""
+    base = env_of("plain_64x32x32", LBM_LX_M=16, LBM_LY_M=8, LBM_LZ_M=12, LBM_BED_LEVEL_M=3.3)
+    _, v = make_oracle(base, fp64=True)
+    for mode in ("phi_fraction", "phi"):
+        _, p = make_oracle(dict(base, LBM_BOUNDARY_GEOMETRY=mode), fp64=True)
+        np.testing.assert_allclose(p.download(abi.FIELD_LATTICE_OPEN_FRAC), v.download(abi.FIELD_LATTICE_OPEN_FRAC),
+                                   atol=1e-12)
+        assert np.array_equal(p.download(abi.FIELD_LATTICE_OPEN), v.download(abi.FIELD_LATTICE_OPEN))
+        p.close()
+    v.close()
+
+
+def test_phi_fraction_on_a_slope_is_the_normal_distance(tmp_path):
+    """On a uniform slope s the phi field is the normal distance
+    (z - h) / sqrt(1 + s^2), so away from the clamps every open fraction of
+    the phi modes is 0.5 + (frac_vertical - 0.5) / sqrt(1 + s^2)."""
+    nx = 32
+    slope = 0.25
+    bottom = tmp_path / "bottom.txt"
+    bottom.write_text(f"0 2.0\n{nx} {2.0 + slope * nx}\n")
+    env = env_of("plain_64x32x32", LBM_LX_M=nx, LBM_LY_M=4, LBM_LZ_M=16, LBM_BED_PROFILE="piecewise_linear",
+                 LBM_BOTTOM_FILE=str(bottom), LBM_X_BOUNDARY="inflow_outflow")
+    _, v = make_oracle(env, fp64=True)
+    _, p = make_oracle(dict(env, LBM_BOUNDARY_GEOMETRY="phi_fraction"), fp64=True)
+    lf_v = v.download(abi.FIELD_LATTICE_OPEN_FRAC)[5]  # +z link: its endpoint lies above the cell centre
+    lf_p = p.download(abi.FIELD_LATTICE_OPEN_FRAC)[5]
+    inner = (lf_v > 0.05) & (lf_v < 0.95)
+    inner[:, :, :2] = False
+    inner[:, :, -2:] = False
+    assert inner.sum() > 10
+    expect = 0.5 + (lf_v[inner] - 0.5) / np.sqrt(1.0 + slope**2)
+    np.testing.assert_allclose(lf_p[inner], expect, atol=2e-6)
+    v.close()
+    p.close()

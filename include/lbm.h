@@ -22,7 +22,7 @@
 extern "C" {
 #endif
 
-#define LBM_CONFIG_VERSION 7
+#define LBM_CONFIG_VERSION 8
 
 /* ---- error codes ---------------------------------------------------- */
 #define LBM_OK 0
@@ -127,6 +127,7 @@ typedef struct lbm_config {
     int32_t fs_mode; /* LBM_FSM_*: how the height-function surface moves */
     int32_t include_cube; /* LBM_OBSTACLE_MODE bed_cube / cube (CODEX:372-400) */
     int32_t phi_fraction, phi_flags; /* LBM_BOUNDARY_GEOMETRY phi_fraction / phi (CODEX:194-198) */
+    int32_t init_fs_populations;     /* LBM_INIT_FREE_SURFACE_POPULATIONS (CODEX:2829-2875) */
     uint32_t seed; /* backscatter counter-based RNG seed */
 
     float dx, rho0, tau0, nu0;
@@ -143,6 +144,7 @@ typedef struct lbm_config {
     float cube_half[3];     /* half sizes (m) of the box obstacle */
     float cube_center[3];   /* centre (m) */
     float cube_rot[4];      /* cos / sin of yaw, cos / sin of pitch */
+    float solitary[5];      /* amplitude, depth h0, x0 (m), direction (+-1), celerity (m/s) (CODEX:290-325) */
     float pad0;
     double dt_phys; /* per-step time, the host forms t=(step+1)*dt_phys */
 } lbm_config_t;

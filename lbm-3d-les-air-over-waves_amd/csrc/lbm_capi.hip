@@ -85,7 +85,6 @@ static const char* validate(const lbm_config_t* c) {
         return "fp16/bf16 storage runs on the fused path only";
     if (c->physics_mode == LBM_PHYS_WATER) {
         if (!c->fs_enabled) return "water mode without a free surface is not supported";
-        if (c->fs_mode == LBM_FSM_PRESCRIBED_SOLITARY) return "LBM_FREE_SURFACE_MODE=prescribed_solitary is not supported";
         if (c->storage == LBM_STORE_FP16 || c->storage == LBM_STORE_BF16) return "water/VOF runs need fp32 or fp64 storage";
         if (c->kernel_path == LBM_PATH_FUSED) return "the fused path covers air mode only";
     }

@@ -238,6 +238,36 @@ __device__ __forceinline__ double det_log(double x) {
     return (de * LN2_HI + lm) + de * LN2_LO;
 }
 
+/* exp(x) in double by Cody-Waite reduction by ln 2 (two-part constant) and a
+   degree-14 Taylor polynomial on |r| <= ln2/2, scaled by 2^n; evaluated
+   without contraction, so the device and the oracle agree bit for bit. */
+__device__ __forceinline__ double det_exp(double x) {
+    if (x != x) return x;
+    if (x > 709.78) return __builtin_inf();
+    if (x < -745.2) return 0.0;
+    const double INV_LN2 = 1.44269504088896338700e+00;
+    const double LN2_HI = 6.93147180369123816490e-01;
+    const double LN2_LO = 1.90821492927058770002e-10;
+    double n = floor(x * INV_LN2 + 0.5);
+    double r = (x - n * LN2_HI) - n * LN2_LO;
+    double p = 1.0 / 87178291200.0;    /* 1/14! */
+    p = 1.0 / 6227020800.0 + r * p;    /* 1/13! */
+    p = 1.0 / 479001600.0 + r * p;
+    p = 1.0 / 39916800.0 + r * p;
+    p = 1.0 / 3628800.0 + r * p;
+    p = 1.0 / 362880.0 + r * p;
+    p = 1.0 / 40320.0 + r * p;
+    p = 1.0 / 5040.0 + r * p;
+    p = 1.0 / 720.0 + r * p;
+    p = 1.0 / 120.0 + r * p;
+    p = 1.0 / 24.0 + r * p;
+    p = 1.0 / 6.0 + r * p;
+    p = 0.5 + r * p;
+    p = 1.0 + r * p;
+    p = 1.0 + r * p;
+    return ldexp(p, (int)n);
+}
+
 __device__ __forceinline__ int pmod(int a, int n) {
     int r = a % n;
     return r < 0 ? r + n : r;
@@ -330,6 +360,7 @@ struct Consts {
     int include_cube;
     int phi_fraction, phi_flags;  // open fractions / open flags from phi sampled at the link endpoints
     T cube_half[3], cube_center[3], cube_rot[4];  // box obstacle: half sizes, centre, cos/sin yaw, cos/sin pitch
+    T solitary[5];                                // amplitude, depth, x0, direction, celerity
     T dt_phys;
     T inv_dx;     // 1/dx, used in place of a division by dx when dx is a power of two (exact)
     int dx_pow2;

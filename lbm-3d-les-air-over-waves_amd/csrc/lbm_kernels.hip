@@ -55,6 +55,7 @@ static Consts<T> make_consts(const lbm_config_t& c, int omega_band_top) {
         k.cube_center[a] = (T)c.cube_center[a];
     }
     for (int a = 0; a < 4; a++) k.cube_rot[a] = (T)c.cube_rot[a];
+    for (int a = 0; a < 5; a++) k.solitary[a] = (T)c.solitary[a];
     k.wall_link = c.wall_velocity_link;
     k.top_drive = c.enable_top_drive;
     k.x_fringe = c.enable_x_fringe;
@@ -1531,16 +1532,41 @@ __device__ __forceinline__ T flux_div(const T* q, size_t stride, int a, int n, b
     return (q[(size_t)ap * stride] - q[(size_t)am * stride]) * (T(0.5) / dx);
 }
 
-// advect_free_surface_height: h += dt (-(dq_x/dx + dq_y/dy)) between half a
-// cell above the bed and half a cell below the lid (height_kinematic);
-// height_static keeps h.  Solid y walls (new) take the x-wall form.
+// solitary_surface_height_at_x (CODEX:1166-1182): the KdV wave
+// A sech^2(kappa (x - x0 - dir c t)), kappa = sqrt(3 A / (4 h0^3)), offset
+// folded into the periodic domain; exp is det_exp
 template <class T>
-__global__ __launch_bounds__(NT) void k_fs_advect(Consts<T> c, const T* qx, const T* qy, const T* bed, T* fs_h) {
+__device__ __forceinline__ T solitary_height(const Consts<T>& c, T xp, T t) {
+    const T lx = (T)c.nx * c.dx;
+    const T amp = c.solitary[0], h0 = c.solitary[1];
+    const T kappa = sqrt(T(3.0) * amp / (T(4.0) * h0 * h0 * h0));
+    const T center = c.solitary[2] + c.solitary[3] * c.solitary[4] * t;
+    T xrel = xp - center;
+    if (c.x_boundary == LBM_XB_PERIODIC) {
+        if (xrel > T(0.5) * lx) xrel -= lx;
+        if (xrel < T(-0.5) * lx) xrel += lx;
+    }
+    const T arg = kappa * xrel;
+    const T ep = (T)det_exp((double)arg), em = (T)det_exp((double)(-arg));
+    const T sech = T(2.0) / (ep + em);
+    return c.free_surface_base_m + amp * sech * sech;
+}
+
+// advect_free_surface_height: h += dt (-(dq_x/dx + dq_y/dy)) between half a
+// cell above the bed and half a cell below the lid (height_kinematic); the
+// solitary wave at time t (prescribed_solitary); height_static keeps h.
+// Solid y walls (new) take the x-wall form.
+template <class T>
+__global__ __launch_bounds__(NT) void k_fs_advect(Consts<T> c, const T* qx, const T* qy, const T* bed, T* fs_h, T t) {
     size_t col = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     size_t plane = (size_t)c.nx * c.ny;
     if (col >= plane) return;
-    if (c.fs_mode != LBM_FSM_HEIGHT_KINEMATIC) return;
     const int i = (int)(col % (size_t)c.nx), j = (int)(col / (size_t)c.nx);
+    if (c.fs_mode == LBM_FSM_PRESCRIBED_SOLITARY) {
+        fs_h[col] = solitary_height(c, ((T)i + T(0.5)) * c.dx, t);
+        return;
+    }
+    if (c.fs_mode != LBM_FSM_HEIGHT_KINEMATIC) return;
     const T dx = c.dx;
     const T h_old = fs_h[col];
     T dq_dx = flux_div(qx + (size_t)j * c.nx, 1, i, c.nx, c.x_boundary == LBM_XB_SOLID, dx);
@@ -2566,6 +2592,51 @@ static VofArgs<T> make_vof(const lbm_config_t& cfg, DevBufs& b, const T* f) {
     return a;
 }
 
+// initialize_free_surface_boundary_populations (CODEX:2829-2875): fout is a
+// copy of fin except, in open wet cells, the populations streamed from an
+// open gas cell (or from above the lid), which take the gas-pressure closure
+// of the pull stream feq_q(rho_fs, u) + feq_qbar(rho_fs, u) - f_qbar; sources
+// outside solid x (or y) walls are not gas
+template <class T>
+__global__ __launch_bounds__(NT) void k_fs_project(Consts<T> c, const T* fin, T* fout, const T* ux, const T* uy, const T* uz,
+                                                   const int* type, const int* obst, const T* fs_h) {
+    size_t id = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    size_t n = (size_t)c.nx * c.ny * c.nz;
+    if (id >= n) return;
+    int i, j, k;
+    cell_ijk(id, c.nx, c.ny, i, j, k);
+    for (int q = 0; q < 19; q++) fout[(size_t)q * n + id] = fin[(size_t)q * n + id];
+    if (obst[id] != 0 || type[id] == LBM_FS_GAS) return;
+    const size_t col = (size_t)j * c.nx + i;
+    for (int q = 1; q < 19; q++) {
+        int si = i - cxq(q), sj = j - cyq(q), sk = k - czq(q);
+        bool fluid = true;
+        if (c.x_boundary == LBM_XB_PERIODIC)
+            si = pmod(si, c.nx);
+        else if (si < 0 || si >= c.nx)
+            fluid = false;
+        if (c.y_boundary == LBM_YB_PERIODIC)
+            sj = pmod(sj, c.ny);
+        else if (sj < 0 || sj >= c.ny)
+            fluid = false;
+        bool gas = false;
+        if (!fluid)
+            gas = false;
+        else if (sk >= c.nz)
+            gas = true;
+        else if (sk >= 0) {
+            size_t sid = ((size_t)sk * c.ny + sj) * c.nx + si;
+            gas = type[sid] == LBM_FS_GAS && obst[sid] == 0;
+        }
+        if (!gas) continue;
+        const int qb = oppq(q);
+        T rho_fs = c.rho0;
+        if (c.pf != LBM_PF_TOTAL || c.fs_boundary_hydro) rho_fs = fs_density_link(c, k, q, true, type[id], fs_h, col);
+        const T u = ux[id], v = uy[id], w = uz[id];
+        fout[(size_t)q * n + id] = feq(rho_fs, u, v, w, q) + feq(rho_fs, u, v, w, qb) - fin[(size_t)qb * n + id];
+    }
+}
+
 template <class S, class T>
 static void init_t(const lbm_config_t& cfg, DevBufs& b, int slot, hipStream_t s) {
     Consts<T> c = make_consts<T>(cfg, -1);
@@ -2573,6 +2644,12 @@ static void init_t(const lbm_config_t& cfg, DevBufs& b, int slot, hipStream_t s)
     k_cellflags<T><<<nblocks(b.n), NT, 0, s>>>(c, sl, b.cellflags);
     k_init_populations<S, T><<<nblocks(b.n), NT, 0, s>>>(c, sl, (S*)b.f[0], (T*)b.rho, (T*)b.ux, (T*)b.uy, (T*)b.uz, b.obst,
                                                          b.type);
+    if (cfg.fs_enabled && cfg.init_fs_populations) {
+        if (sizeof(S) != sizeof(T)) throw std::runtime_error("the free-surface population projection needs fp32 or fp64 storage");
+        k_fs_project<T><<<nblocks(b.n), NT, 0, s>>>(c, (const T*)b.f[0], (T*)b.f[1], (const T*)b.ux, (const T*)b.uy,
+                                                     (const T*)b.uz, b.type, b.obst, (const T*)b.fs_h);
+        HIP_OK(hipMemcpyAsync(b.f[0], b.f[1], 19 * b.n * sizeof(T), hipMemcpyDeviceToDevice, s));
+    }
     if (cfg.fs_enabled && cfg.fs_tracking == LBM_FST_VOF) {
         VofArgs<T> a = make_vof<T>(cfg, b, (const T*)b.f[0]);
         k_vof_init_mass<T><<<nblocks(b.n), NT, 0, s>>>(a);
@@ -2694,7 +2771,8 @@ static void staged_t(const lbm_config_t& cfg, DevBufs& b, int& cur, int slot, in
             const unsigned n2b = nblocks(b.n2);
             k_fs_flux<T><<<n2b, NT, 0, s>>>(c, (const T*)b.fill, b.type, b.cellflags, (const T*)b.ux, (const T*)b.uy, (T*)b.fs_qx,
                                             (T*)b.fs_qy);
-            k_fs_advect<T><<<n2b, NT, 0, s>>>(c, (const T*)b.fs_qx, (const T*)b.fs_qy, (const T*)b.h[slot], (T*)b.fs_h);
+            const T t = (T)(float)((double)(step + 1) * cfg.dt_phys);  // CODEX:3874
+            k_fs_advect<T><<<n2b, NT, 0, s>>>(c, (const T*)b.fs_qx, (const T*)b.fs_qy, (const T*)b.h[slot], (T*)b.fs_h, t);
             k_fs_classify<T><<<nb, NT, 0, s>>>(c, (const T*)b.fs_h, b.cellflags, (T*)b.fill, b.type, b.type_prev);
             HIP_OK(hipMemsetAsync(b.icnt + IC_FS_REFILL, 0, sizeof(int), s));
             k_vof_refill<T><<<nb, NT, 0, s>>>(make_vof<T>(cfg, b, f2), f2);

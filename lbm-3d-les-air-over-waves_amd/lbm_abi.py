@@ -8,7 +8,7 @@ from __future__ import annotations
 
 import ctypes
 
-LBM_CONFIG_VERSION = 7
+LBM_CONFIG_VERSION = 8
 
 # error codes
 LBM_OK = 0
@@ -151,6 +151,7 @@ class LbmConfig(ctypes.Structure):
         ("include_cube", _I),
         ("phi_fraction", _I),
         ("phi_flags", _I),
+        ("init_fs_populations", _I),
         ("seed", ctypes.c_uint32),
         ("dx", _F),
         ("rho0", _F),
@@ -184,6 +185,7 @@ class LbmConfig(ctypes.Structure):
         ("cube_half", _F * 3),
         ("cube_center", _F * 3),
         ("cube_rot", _F * 4),
+        ("solitary", _F * 5),
         ("pad0", _F),
         ("dt_phys", ctypes.c_double),
     ]

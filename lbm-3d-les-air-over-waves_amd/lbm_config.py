@@ -97,6 +97,9 @@ class SolverConfig:
         s.include_cube = v["include_cube_geometry"]
         s.phi_fraction = 1 if v["boundary_geometry_mode"] in ("phi_fraction", "phi") else 0
         s.phi_flags = 1 if v["boundary_geometry_mode"] == "phi" else 0
+        s.init_fs_populations = 1 if v["initialize_free_surface_populations"] else 0
+        s.solitary[:] = [v["solitary_amp_m"], v["free_surface_depth_m"], v["solitary_x0_m"], v["solitary_direction"],
+                         v["solitary_celerity_phys"]]
         s.cube_half[:] = v["cube_half"]
         s.cube_center[:] = v["cube_center"]
         s.cube_rot[:] = v["cube_rot"]
@@ -547,10 +550,6 @@ def parse_env(env: Optional[Mapping[str, str]] = None) -> SolverConfig:
 
     # features this build does not implement are refused, not ignored
     unsupported = []
-    if water_free_surface_enabled and free_surface_mode == "prescribed_solitary":
-        unsupported.append("LBM_FREE_SURFACE_MODE=prescribed_solitary")
-    if init_fs_pops:
-        unsupported.append("LBM_INIT_FREE_SURFACE_POPULATIONS=1")
     if physics_mode == "water" and not water_free_surface_enabled:
         unsupported.append("water mode without a free surface")
 

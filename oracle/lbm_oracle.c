@@ -112,6 +112,36 @@ static double det_sincos_d(double x, int want_cos) {
 /* natural logarithm: x = m 2^e with m in [sqrt(1/2), sqrt(2)), then
    log m = 2 atanh(s), s = (m-1)/(m+1), as an odd series to s^23, plus
    e ln 2 with ln 2 split in two parts; double, no contraction. */
+/* exp(x) in double by Cody-Waite reduction by ln 2 (two-part constant) and a
+   degree-14 Taylor polynomial on |r| <= ln2/2, scaled by 2^n; evaluated
+   without contraction, so the device and the oracle agree bit for bit. */
+static double det_exp_d(double x) {
+    if (x != x) return x;
+    if (x > 709.78) return HUGE_VAL;
+    if (x < -745.2) return 0.0;
+    const double INV_LN2 = 1.44269504088896338700e+00;
+    const double LN2_HI = 6.93147180369123816490e-01;
+    const double LN2_LO = 1.90821492927058770002e-10;
+    double n = floor(x * INV_LN2 + 0.5);
+    double r = (x - n * LN2_HI) - n * LN2_LO;
+    double p = 1.0 / 87178291200.0;    /* 1/14! */
+    p = 1.0 / 6227020800.0 + r * p;    /* 1/13! */
+    p = 1.0 / 479001600.0 + r * p;
+    p = 1.0 / 39916800.0 + r * p;
+    p = 1.0 / 3628800.0 + r * p;
+    p = 1.0 / 362880.0 + r * p;
+    p = 1.0 / 40320.0 + r * p;
+    p = 1.0 / 5040.0 + r * p;
+    p = 1.0 / 720.0 + r * p;
+    p = 1.0 / 120.0 + r * p;
+    p = 1.0 / 24.0 + r * p;
+    p = 1.0 / 6.0 + r * p;
+    p = 0.5 + r * p;
+    p = 1.0 + r * p;
+    p = 1.0 + r * p;
+    return ldexp(p, (int)n);
+}
+
 static double det_log_d(double x) {
     if (!(x > 0.0)) return x == 0.0 ? -INFINITY : NAN;
     if (isinf(x)) return x;
@@ -141,6 +171,7 @@ static double det_log_d(double x) {
     return (de * LN2_HI + lm) + de * LN2_LO;
 }
 double API(orc_det_log)(double x) { return det_log_d(x); }
+double API(orc_det_exp)(double x) { return det_exp_d(x); }
 double API(orc_det_sin)(double x) { return det_sincos_d(x, 0); }
 double API(orc_det_cos)(double x) { return det_sincos_d(x, 1); }
 static REAL dsin(REAL x) { return (REAL)det_sincos_d((double)x, 0); }
@@ -1960,8 +1991,30 @@ static REAL flux_div(const REAL *q, size_t stride, int a, int n, int solid, REAL
 
 /* advect_free_surface_height: h += dt (-(dq_x/dx + dq_y/dy)), kept between
    half a cell above the bed and half a cell below the lid (height_kinematic);
-   height_static keeps h.  y walls (new) take the x-wall form. */
-static void fs_advect_height(O *o) {
+   the solitary wave at time t (prescribed_solitary); height_static keeps h.
+   y walls (new) take the x-wall form. */
+/* solitary_surface_height_at_x (CODEX:1166-1182) in float: the KdV wave
+   A sech^2(kappa (x - x0 - dir c t)), kappa = sqrt(3 A / (4 h0^3)), the offset
+   folded into the periodic domain; exp is det_exp */
+static REAL solitary_height(const O *o, REAL xp, REAL t) {
+    const lbm_config_t *c = &o->c;
+    const REAL lx = (REAL)o->nx * R(c->dx);
+    const REAL amp = R(c->solitary[0]), h0 = R(c->solitary[1]);
+    const REAL kappa = (REAL)sqrt((double)(R(3.0) * amp / (R(4.0) * h0 * h0 * h0)));
+    const REAL center = R(c->solitary[2]) + R(c->solitary[3]) * R(c->solitary[4]) * t;
+    REAL xrel = xp - center;
+    if (c->x_boundary == LBM_XB_PERIODIC) {
+        if (xrel > R(0.5) * lx) xrel -= lx;
+        if (xrel < R(-0.5) * lx) xrel += lx;
+    }
+    REAL arg = kappa * xrel;
+    REAL ep = (REAL)det_exp_d((double)arg);
+    REAL em = (REAL)det_exp_d((double)(-arg));
+    REAL sech = R(2.0) / (ep + em);
+    return R(c->free_surface_base_m) + amp * sech * sech;
+}
+
+static void fs_advect_height(O *o, REAL t) {
     const lbm_config_t *c = &o->c;
     const REAL dx = R(c->dx), dt = R(c->dt_phys);
     const REAL lz = (REAL)o->nz * dx;
@@ -1969,7 +2022,9 @@ static void fs_advect_height(O *o) {
         for (int i = 0; i < o->nx; i++) {
             size_t cid = ID2(j, i);
             REAL h_old = o->fs_h[cid], h_new = h_old;
-            if (c->fs_mode == LBM_FSM_HEIGHT_KINEMATIC) {
+            if (c->fs_mode == LBM_FSM_PRESCRIBED_SOLITARY) {
+                h_new = solitary_height(o, ((REAL)i + R(0.5)) * dx, t);
+            } else if (c->fs_mode == LBM_FSM_HEIGHT_KINEMATIC) {
                 REAL dq_dx = flux_div(o->fs_qx + (size_t)j * o->nx, 1, i, o->nx, c->x_boundary == LBM_XB_SOLID, dx);
                 REAL dq_dy = flux_div(o->fs_qy + i, (size_t)o->nx, j, o->ny, c->y_boundary == LBM_YB_SOLID, dx);
                 REAL dh_dt = -(dq_dx + dq_dy);
@@ -2323,6 +2378,50 @@ void API(orc_geometry)(void *p, float t) {
 
 /* init_fields tail + populations (CODEX:2712, 2789-2818, 2829-2875), then the
    VOF initialisation of main() (CODEX:3789-3792). */
+/* initialize_free_surface_boundary_populations (CODEX:2829-2875): in every
+   open wet cell, each population streamed from an open gas cell (or from
+   above the lid) is replaced by the gas-pressure closure of the pull stream,
+   feq_q(rho_fs, u) + feq_qbar(rho_fs, u) - f_qbar; sources outside solid
+   x (or y) walls are not gas */
+static void fs_init_projection(O *o) {
+    const lbm_config_t *c = &o->c;
+    memcpy(o->fn, o->f, 19 * o->n * sizeof(REAL));
+    for (int k = 0; k < o->nz; k++)
+        for (int j = 0; j < o->ny; j++)
+            for (int i = 0; i < o->nx; i++) {
+                size_t id = IDX(k, j, i);
+                if (!(LO(0, id) == 1 && o->type[id] != LBM_FS_GAS)) continue;
+                for (int q = 1; q < 19; q++) {
+                    int si = i - CX[q], sj = j - CY[q], sk = k - CZ[q];
+                    int fluid = 1;
+                    if (c->x_boundary == LBM_XB_PERIODIC)
+                        si = (si + o->nx) % o->nx;
+                    else if (si < 0 || si >= o->nx)
+                        fluid = 0;
+                    if (c->y_boundary == LBM_YB_PERIODIC)
+                        sj = (sj + o->ny) % o->ny;
+                    else if (sj < 0 || sj >= o->ny)
+                        fluid = 0;
+                    int gas = 0;
+                    if (!fluid)
+                        gas = 0;
+                    else if (sk >= o->nz)
+                        gas = 1;
+                    else if (sk >= 0) {
+                        size_t sid = IDX(sk, sj, si);
+                        gas = o->type[sid] == LBM_FS_GAS && LO(0, sid) == 1;
+                    }
+                    if (!gas) continue;
+                    const int qb = OPP[q];
+                    REAL rho_fs = R(c->rho0);
+                    if (use_hbp(o) || c->fs_boundary_mode == LBM_FSB_HYDROSTATIC) rho_fs = fs_density_link(o, k, j, i, q);
+                    REAL ux = o->ux[id], uy = o->uy[id], uz = o->uz[id];
+                    o->fn[(size_t)q * o->n + id] = feq(o, rho_fs, ux, uy, uz, q) + feq(o, rho_fs, ux, uy, uz, qb) - F(qb, id);
+                }
+            }
+    memcpy(o->f, o->fn, 19 * o->n * sizeof(REAL));
+}
+
 void API(orc_init_populations)(void *p) {
     O *o = (O *)p;
     const lbm_config_t *c = &o->c;
@@ -2352,6 +2451,7 @@ void API(orc_init_populations)(void *p) {
         }
     }
     store_round(o, 0);
+    if (is_water(o) && c->init_fs_populations) fs_init_projection(o);
     memcpy(o->fn, o->f, 19 * o->n * sizeof(REAL));
     if (is_vof(o)) {
         vof_init_mass(o);
@@ -2409,7 +2509,7 @@ static void one_step(O *o) {
             vof_repair(o);
         } else {
             fs_column_flux(o);
-            fs_advect_height(o);
+            fs_advect_height(o, (REAL)t);
             fs_classify(o);
             vof_refill(o);
         }
@@ -2487,7 +2587,7 @@ int API(orc_stage)(void *p, int stage, float t) {
         case 31: vof_repair(o); break;
         case 32: fs_classify(o); break;
         case 33: fs_column_flux(o); break;
-        case 34: fs_advect_height(o); break;
+        case 34: fs_advect_height(o, (REAL)t); break;
         default: return LBM_E_ARG;
     }
     return 0;

@@ -63,7 +63,7 @@ def load_oracle():
         g("orc_set_step").restype = None
         g("orc_round16").argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t]
         g("orc_round16").restype = None
-        for fn in ("orc_det_sin", "orc_det_cos", "orc_det_log"):
+        for fn in ("orc_det_sin", "orc_det_cos", "orc_det_log", "orc_det_exp"):
             g(fn).argtypes = [ctypes.c_double]
             g(fn).restype = ctypes.c_double
     _lib = lib

@@ -3,6 +3,10 @@ CODEX:1209-1232, 2163-2316, 2376-2429) with the oracle: a Gaussian mound in
 a periodic box, the reference's default water set-up (solitary wave, solid
 x walls, hydrostatic-balanced pressure) and the total-pressure variant.
 
+Also the surface prescribed as the travelling solitary wave and the
+free-surface population projection at initialisation (CODEX:2829-2875), in
+height and VOF tracking.
+
 Bar: cell types bit-exact; column heights, fill, populations, velocity and
 density within max-relative 1e-5 after 60 steps; refill counters equal."""
 import numpy as np
@@ -22,6 +26,10 @@ CASES = {
     "mound_total_pressure": height_env(LBM_FREE_SURFACE_INITIAL="gaussian", LBM_GAUSSIAN_AMPLITUDE_M=2.5,
                                        LBM_GAUSSIAN_SIGMA_M=4, LBM_PRESSURE_FORMULATION="total_pressure",
                                        LBM_X_BOUNDARY="solid"),
+    "prescribed_solitary": height_env(LBM_FREE_SURFACE_MODE="prescribed_solitary", LBM_FREE_SURFACE_INITIAL="solitary",
+                                      LBM_SOLITARY_AMPLITUDE_M=2.0, LBM_SOLITARY_X0_M=8.0),
+    "mound_projected_init": height_env(LBM_FREE_SURFACE_INITIAL="gaussian", LBM_GAUSSIAN_AMPLITUDE_M=2.5,
+                                       LBM_GAUSSIAN_SIGMA_M=4, LBM_INIT_FREE_SURFACE_POPULATIONS=1),
 }
 
 
@@ -41,5 +49,19 @@ def test_height_tracking_parity(name):
     cc, oc = c.counters(), o.counters()
     for key in ("free_surface_refill_count", "free_surface_cut_link_count", "boundary_cut_link_count"):
         assert cc[key] == oc[key], key
+    c.close()
+    o.close()
+
+
+def test_vof_projected_init_parity():
+    from test_oracle_vof import vof_env
+
+    cfg, o, c = pair(vof_env(LBM_INIT_FREE_SURFACE_POPULATIONS=1))
+    assert np.array_equal(c.download(abi.FIELD_F), o.download(abi.FIELD_F))
+    o.step(40)
+    c.step(40)
+    assert np.array_equal(c.download(abi.FIELD_FS_TYPE), o.download(abi.FIELD_FS_TYPE))
+    for fid in (abi.FIELD_FS_FILL, abi.FIELD_FS_MASS, abi.FIELD_F, abi.FIELD_UZ):
+        assert relerr(c.download(fid), o.download(fid)) <= 1e-6, fid
     c.close()
     o.close()

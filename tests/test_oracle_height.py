@@ -97,3 +97,58 @@ def test_pool_at_rest_stays_flat():
     assert np.array_equal(o.download(abi.FIELD_FS_TYPE), ty0)
     assert np.max(np.abs(o.download(abi.FIELD_UZ))) < 1e-9
     o.close()
+
+
+def test_det_exp_against_libm():
+    import math
+
+    from oracle_ref import load_oracle
+
+    lib = load_oracle()
+    xs = np.concatenate([np.linspace(-40, 40, 2001), [0.0, 1.0, -1.0, 1e-9, 700.0, -700.0]])
+    for x in xs:
+        e = lib.orc_det_exp_f64(float(x))
+        assert e == pytest.approx(math.exp(x), rel=4e-16, abs=0.0), x
+    assert lib.orc_det_exp_f64(0.0) == 1.0
+
+
+def test_prescribed_solitary_surface_moves_at_the_celerity():
+    env = height_env(LBM_FREE_SURFACE_MODE="prescribed_solitary", LBM_FREE_SURFACE_INITIAL="solitary",
+                     LBM_SOLITARY_AMPLITUDE_M=2.0, LBM_SOLITARY_X0_M=8.0, LBM_X_BOUNDARY="periodic")
+    cfg, o = make_oracle(env, fp64=True)
+    t = 4.0 / cfg.solitary_celerity_phys  # the crest travels 4 m
+    o.stage(STAGE_ADVECT, t)
+    h = o.download(abi.FIELD_FS_H)[0].astype(np.float64)
+    x = np.arange(cfg.Nx) + 0.5
+    kappa = np.sqrt(3.0 * 2.0 / (4.0 * 12.0**3))
+    xc = 8.0 + cfg.solitary_celerity_phys * np.float32(t)
+    xrel = x - xc
+    xrel = np.where(xrel > 0.5 * cfg.Nx, xrel - cfg.Nx, xrel)  # folded into the periodic domain
+    xrel = np.where(xrel < -0.5 * cfg.Nx, xrel + cfg.Nx, xrel)
+    expect = cfg.free_surface_base_m + 2.0 / np.cosh(kappa * xrel) ** 2
+    np.testing.assert_allclose(h, expect, rtol=0, atol=1e-5)
+    assert abs(x[np.argmax(h)] - 12.0) <= 0.5
+    o.close()
+
+
+def test_free_surface_population_projection():
+    """initialize_free_surface_boundary_populations: at rest (u = 0) the
+    closure replaces a population streamed from gas by
+    feq_q(rho_fs) + feq_qbar(rho_fs) - feq_qbar(rho) = f_q + 2 w_q (rho_fs - rho);
+    populations with wet sources are unchanged."""
+    env = height_env(LBM_PRESSURE_FORMULATION="total_pressure")
+    _, plain = make_oracle(env, fp64=True)
+    _, proj = make_oracle(dict(env, LBM_INIT_FREE_SURFACE_POPULATIONS="1"), fp64=True)
+    f0 = plain.download(abi.FIELD_F)
+    f1 = proj.download(abi.FIELD_F)
+    rho = plain.download(abi.FIELD_RHO)
+    w = np.array([1 / 3] + [1 / 18] * 6 + [1 / 36] * 12)
+    top = 11  # the last wet plane (surface at 12 m)
+    # links from above (cz < 0: q 6, 13, 14, 17, 18) read gas; rho_fs = rho0 in the total-pressure form
+    for q in (6, 13, 14, 17, 18):
+        np.testing.assert_allclose(f1[q, top], f0[q, top] + 2.0 * w[q] * (1.0 - rho[top]), atol=1e-12)
+    for q in (0, 1, 2, 3, 4, 5):
+        assert np.array_equal(f1[q, top], f0[q, top])
+    assert np.array_equal(f1[:, :top], f0[:, :top])
+    plain.close()
+    proj.close()

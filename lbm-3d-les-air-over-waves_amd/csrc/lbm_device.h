@@ -345,7 +345,9 @@ __device__ __forceinline__ T sample2d(const T* H, T xq, T yq, int nx, int ny, T 
 template <class T>
 struct Consts {
     int nx, ny, nz;
-    int physics_water, collision_reg, use_les;
+    int fs_water;    // a free surface is tracked (the reference's water_free_surface_enabled)
+    int air_walls;   // air physics: closed links of static walls are conditioned too (CODEX:2922)
+    int collision_reg, use_les;
     int x_boundary, y_boundary, top_boundary;
     int bed_profile, bed_moving, include_bed, wall_link;
     int top_drive, x_fringe, top_sponge;
@@ -588,7 +590,7 @@ __device__ __forceinline__ T cell_phi(const Consts<T>& c, const Slot<T>& s, int 
 // update_bed_populations_and_reinit for one cell (CODEX:2894-2940)
 template <class T>
 __device__ __forceinline__ void condition_cell(const Consts<T>& c, const Slot<T>& s, const CellGeo<T>& g, int i, int j, T f[19]) {
-    const bool air = !c.physics_water;
+    const bool air = c.air_walls;
 #pragma unroll
     for (int q = 0; q < 19; q++) {
         bool open = q == 0 ? (g.open_mask & 1u) : ((g.open_mask >> q) & 1u);

@@ -38,7 +38,8 @@ static Consts<T> make_consts(const lbm_config_t& c, int omega_band_top) {
     k.nx = c.nx;
     k.ny = c.ny;
     k.nz = c.nz;
-    k.physics_water = c.physics_mode == LBM_PHYS_WATER;
+    k.fs_water = c.fs_enabled != 0;
+    k.air_walls = c.physics_mode == LBM_PHYS_AIR;
     k.collision_reg = c.collision == LBM_COLL_REGULARIZED;
     k.use_les = c.use_les;
     k.x_boundary = c.x_boundary;
@@ -374,7 +375,7 @@ __global__ __launch_bounds__(NT) void k_cellflags(Consts<T> c, Slot<T> s, uint8_
 // update_pressure_reference_fields CODEX:710-740 evaluated in place)
 template <class T>
 __device__ __forceinline__ T gravity_fraction(const Consts<T>& c, int k, bool open0, int type) {
-    if (!(c.physics_water && open0 && type != LBM_FS_GAS)) return T(0);
+    if (!(c.fs_water && open0 && type != LBM_FS_GAS)) return T(0);
     if (c.pf == LBM_PF_HYDROSTATIC_BALANCED) return T(0);
     if (c.pf == LBM_PF_VOF_COMPONENT_BALANCED) {
         T zc = ((T)k + T(0.5)) * c.dx;
@@ -384,7 +385,7 @@ __device__ __forceinline__ T gravity_fraction(const Consts<T>& c, int k, bool op
 }
 template <class T>
 __device__ __forceinline__ T local_g(const Consts<T>& c, int k, bool open0, int type) {
-    if (c.physics_water && c.pf == LBM_PF_VOF_COMPONENT_BALANCED) return c.g_lb * gravity_fraction(c, k, open0, type);
+    if (c.fs_water && c.pf == LBM_PF_VOF_COMPONENT_BALANCED) return c.g_lb * gravity_fraction(c, k, open0, type);
     return c.g_body_lb;
 }
 
@@ -416,8 +417,8 @@ __global__ __launch_bounds__(NT) void k_init_populations(Consts<T> c, Slot<T> s,
         rho[id] = c.rho0;
     }
     T u = ux[id], v = uy[id], w = uz[id];
-    int ty = c.physics_water ? type[id] : LBM_FS_LIQUID;
-    if (c.physics_water && open0 && ty != LBM_FS_GAS) w += T(0.5) * local_g(c, k, open0, ty);
+    int ty = c.fs_water ? type[id] : LBM_FS_LIQUID;
+    if (c.fs_water && open0 && ty != LBM_FS_GAS) w += T(0.5) * local_g(c, k, open0, ty);
     T r = rho[id];
     T u2 = u * u + v * v + w * w;
     const uint32_t dbase = dither_base(c.store_step, (uint32_t)id);
@@ -476,7 +477,7 @@ __global__ __launch_bounds__(NT) void k_macro(Consts<T> c, const T* f, T* rho, T
         if (czq(q)) jz += fl[q] * (T)czq(q);
     }
     T u, v, w;
-    if (c.physics_water) {
+    if (c.fs_water) {
         int ty = type[id];
         if (ty == LBM_FS_GAS) {
             r = c.rho0;
@@ -515,7 +516,7 @@ __global__ __launch_bounds__(NT) void k_les(Consts<T> c, Slot<T> s, const T* ux,
     int i, j, k;
     cell_ijk(id, c.nx, c.ny, i, j, k);
     bool active = flags[id] & 1u;
-    if (c.physics_water && type[id] == LBM_FS_GAS) active = false;
+    if (c.fs_water && type[id] == LBM_FS_GAS) active = false;
     if (!active) return;
     int nx = c.nx, ny = c.ny, nz = c.nz;
     int ip = (i + 1) % nx, im = (i + nx - 1) % nx, jp = (j + 1) % ny, jm = (j + ny - 1) % ny;
@@ -555,7 +556,7 @@ __global__ __launch_bounds__(NT) void k_collide(Consts<T> c, T* f, const T* rho,
     bool open0 = flags[id] & 1u;
     bool gas = false;
     int ty = LBM_FS_LIQUID;
-    if (c.physics_water) {
+    if (c.fs_water) {
         ty = type[id];
         if (ty == LBM_FS_GAS) {
             gas = true;
@@ -571,7 +572,7 @@ __global__ __launch_bounds__(NT) void k_collide(Consts<T> c, T* f, const T* rho,
         if (gas) {
 #pragma unroll
             for (int q = 0; q < 19; q++) fl[q] = feq(r, u, v, w, q);
-        } else if (c.physics_water) {
+        } else if (c.fs_water) {
             T force_z = -r * local_g(c, k, open0, ty);
             collide_regularized<T, true>(fl, r, u, v, w, om, force_z);
         } else {
@@ -583,7 +584,7 @@ __global__ __launch_bounds__(NT) void k_collide(Consts<T> c, T* f, const T* rho,
         } else {
             bool bs = c.c_backscatter > T(0) && open0;
             float xi = bs ? bs_uniform(c.seed, step, (uint32_t)id) : 0.0f;
-            if (c.physics_water && !gas) {
+            if (c.fs_water && !gas) {
                 T force_z = -r * local_g(c, k, open0, ty);
                 collide_kbc<T, true>(fl, r, u, v, w, om, force_z, bs, c.c_backscatter, xi);
             } else {
@@ -622,13 +623,13 @@ __device__ __forceinline__ T fs_density_cell(const Consts<T>& c, int k, bool ope
 template <class T>
 __device__ __forceinline__ T fs_density_link(const Consts<T>& c, int k, int q, bool open0, int type, const T* fs_h, size_t col) {
     T dx = c.dx;
-    if (c.physics_water && c.pf == LBM_PF_VOF_COMPONENT_BALANCED) {
+    if (c.fs_water && c.pf == LBM_PF_VOF_COMPONENT_BALANCED) {
         T z_if = ((T)k + T(0.5)) * dx - T(0.5) * (T)czq(q) * dx;
         T head = T(0);
         if (gravity_fraction(c, k, open0, type) < T(0.5)) head += z_if - c.free_surface_base_m;
         return c.rho0 + c.g_lb * (head / dx) / (T(1.0) / T(3.0));
     }
-    if (c.physics_water && c.pf == LBM_PF_HYDROSTATIC_BALANCED && c.fs_vof) {
+    if (c.fs_water && c.pf == LBM_PF_HYDROSTATIC_BALANCED && c.fs_vof) {
         T z_if = ((T)k + T(0.5)) * dx - T(0.5) * (T)czq(q) * dx;
         T head = z_if - c.free_surface_base_m;
         return c.rho0 + c.g_lb * (head / dx) / (T(1.0) / T(3.0));
@@ -659,7 +660,7 @@ __global__ __launch_bounds__(NT) void k_stream_bc(Consts<T> c, Slot<T> s, const 
     int i, j, k;
     cell_ijk(id, c.nx, c.ny, i, j, k);
     const int nx = c.nx, ny = c.ny, nz = c.nz;
-    const bool water = c.physics_water;
+    const bool water = c.fs_water;
     const bool xper = c.x_boundary == LBM_XB_PERIODIC, xio = c.x_boundary == LBM_XB_INFLOW_OUTFLOW;
     const bool yper = c.y_boundary == LBM_YB_PERIODIC;
     const Codec& inv = c.store;
@@ -2447,7 +2448,7 @@ __global__ __launch_bounds__(NT) void k_reduce(Consts<T> c, const S* f, const T*
             v = (double)fill[id];
         } else if (what == LBM_REDUCE_MAX_SPEED) {
             bool act = flags[id] & 1u;
-            if (c.physics_water && type[id] == LBM_FS_GAS) act = false;
+            if (c.fs_water && type[id] == LBM_FS_GAS) act = false;
             if (act) {
                 double a = ux[id], b = uy[id], e = uz[id];
                 v = sqrt(a * a + b * b + e * e);
@@ -2497,7 +2498,7 @@ __global__ __launch_bounds__(NT) void k_load_populations(Consts<T> c, const S* s
 // air mode, the bed the only obstacle and its gap the open fraction (the band
 // passes evaluate the bed's link geometry only)
 bool fused_supported(const lbm_config_t& c) {
-    return c.physics_mode == LBM_PHYS_AIR && !c.include_cube && !c.phi_fraction && c.nz >= 3;
+    return c.physics_mode == LBM_PHYS_AIR && !c.fs_enabled && !c.include_cube && !c.phi_fraction && c.nz >= 3;
 }
 
 // compute_phi_slope_corrected (CODEX:2432-2490) into a 3-D field, for the
@@ -2843,7 +2844,7 @@ static void fused_t(const lbm_config_t& cfg, DevBufs& b, int& cur, int mode, int
 }
 void launch_fused_step(const lbm_config_t& c, DevBufs& b, int& cur, int mode, int slot_prev, int slot, int64_t step,
                        int write_macro, int omega_band_top, int band_top, int block_x, hipStream_t s) {
-    if (!fused_supported(c)) throw std::runtime_error("fused path: needs air mode");
+    if (!fused_supported(c)) throw std::runtime_error("fused path: needs air mode without a free surface");
     switch (b.storage) {
         case LBM_STORE_FP64: fused_t<double, double>(c, b, cur, mode, slot_prev, slot, step, write_macro, omega_band_top, band_top, block_x, s); break;
         case LBM_STORE_FP16: fused_t<__half, float>(c, b, cur, mode, slot_prev, slot, step, write_macro, omega_band_top, band_top, block_x, s); break;

@@ -550,8 +550,6 @@ def parse_env(env: Optional[Mapping[str, str]] = None) -> SolverConfig:
 
     # features this build does not implement are refused, not ignored
     unsupported = []
-    if physics_mode == "water" and not water_free_surface_enabled:
-        unsupported.append("water mode without a free surface")
 
     v.update(
         dict(

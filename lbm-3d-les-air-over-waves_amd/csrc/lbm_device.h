@@ -40,6 +40,19 @@ __host__ __device__ constexpr T wq(int q) {
     return q == 0 ? T(1.0 / 3.0) : (q <= 6 ? T(1.0 / 18.0) : T(1.0 / 36.0));
 }
 
+// Division of a cell index u < 2^31 by an invariant d <= 2^31 as a
+// multiply-high and a shift (Granlund and Montgomery, "Division by invariant
+// integers using multiplication", 1994): with s = ceil(log2 d) and
+// m = floor(2^(31+s) / d) + 1 < 2^32, floor(u / d) = (u m) >> (31 + s),
+// because u m / 2^(31+s) exceeds u / d by less than 2^-s <= 1/d.
+__host__ __device__ inline void fast_div_magic(uint32_t d, uint32_t& m, int& s) {
+    s = 0;
+    while (((uint64_t)1 << s) < d) s++;
+    m = (uint32_t)(((uint64_t)1 << (31 + s)) / d + 1);
+}
+__device__ __forceinline__ uint32_t fast_div(uint32_t u, uint32_t m, int s) {
+    return (uint32_t)(((uint64_t)u * m) >> (31 + s));
+}
 // ---- storage types -------------------------------------------------------
 struct bf16_t {
     uint16_t v;
@@ -345,6 +358,8 @@ __device__ __forceinline__ T sample2d(const T* H, T xq, T yq, int nx, int ny, T 
 template <class T>
 struct Consts {
     int nx, ny, nz;
+    uint32_t div_nx_m, div_ny_m;  // cell index -> (i, j, k) by multiply-high (see cell_ijk)
+    int div_nx_s, div_ny_s;
     int fs_water;    // a free surface is tracked (the reference's water_free_surface_enabled)
     int air_walls;   // air physics: closed links of static walls are conditioned too (CODEX:2922)
     int collision_reg, use_les;
@@ -463,6 +478,33 @@ __device__ __forceinline__ T sample_phi(const Consts<T>& c, const T* phi, T xp, 
     return p0 * (T(1.0) - fz) + p1 * fz;
 }
 
+// open fraction of the cell centre (link 0) and whether the box is the
+// nearer surface there
+template <class T>
+__device__ __forceinline__ T cell_center_fraction(const Consts<T>& c, const Slot<T>& s, int i, int j, int k, bool& by_cube) {
+    const size_t col = (size_t)j * c.nx + i;
+    const size_t plane = (size_t)c.nx * c.ny;
+    const T dx = c.dx;
+    const T zc = ((T)k + T(0.5)) * dx;
+    const T xc = ((T)i + T(0.5)) * dx, yc = ((T)j + T(0.5)) * dx;
+    by_cube = false;
+    T eta_c = s.eta9[4 * plane + col];
+    T gap_c = zc - eta_c;
+    T ggap = c.include_bed ? gap_c : T(1.0e6);
+    if (c.include_cube) {
+        T cg = cube_sdf(c, xc, yc, zc);
+        if (cg < ggap) {
+            ggap = cg;
+            by_cube = true;
+        }
+    }
+    T sgap = c.phi_fraction ? s.phi[((size_t)k * c.ny + j) * c.nx + i] : ggap;
+    T frac = T(0.5) + div_dx(c, sgap);
+    if (frac < T(0)) frac = T(0);
+    if (frac > T(1)) frac = T(1);
+    return frac;
+}
+
 template <class T>
 __device__ __forceinline__ void cell_geometry(const Consts<T>& c, const Slot<T>& s, int i, int j, int k, CellGeo<T>& g) {
     const size_t col = (size_t)j * c.nx + i;
@@ -471,22 +513,32 @@ __device__ __forceinline__ void cell_geometry(const Consts<T>& c, const Slot<T>&
     T zc = ((T)k + T(0.5)) * dx;
     const T xc = ((T)i + T(0.5)) * dx, yc = ((T)j + T(0.5)) * dx;
     uint32_t mask = 0, xw = 0, cube = 0;
-    {
-        T eta_c = s.eta9[4 * plane + col];
-        T gap_c = zc - eta_c;
-        T ggap = c.include_bed ? gap_c : T(1.0e6);
-        if (c.include_cube) {
-            T cg = cube_sdf(c, xc, yc, zc);
-            if (cg < ggap) {
-                ggap = cg;
-                cube |= 1u;
-            }
+    // a cell at least 2 dx above the highest of its 9 bed samples, away from
+    // solid x/y walls, with the bed as the only surface: every link gap is
+    // >= dx/2 after rounding, so every fraction clamps to 1 and every link is
+    // open (the same values the per-link evaluation below gives)
+    if (!c.include_cube && !c.phi_fraction && !(c.x_boundary == LBM_XB_SOLID && (i == 0 || i == c.nx - 1)) &&
+        !(c.y_boundary == LBM_YB_SOLID && (j == 0 || j == c.ny - 1))) {
+        T e9[9];
+#pragma unroll
+        for (int e = 0; e < 9; e++) e9[e] = s.eta9[(size_t)e * plane + col];  // issued together
+        bool far = true;  // false for a NaN sample, which takes the per-link path
+#pragma unroll
+        for (int e = 0; e < 9; e++) far = far & (zc - e9[e] >= T(2.0) * dx);
+        if (!c.include_bed || far) {
+#pragma unroll
+            for (int q = 0; q < 19; q++) g.lf[q] = T(1);
+            g.near = false;
+            g.open_mask = 0x7ffffu;
+            g.xwall_mask = 0;
+            g.cube_mask = 0;
+            return;
         }
-        T sgap = c.phi_fraction ? s.phi[((size_t)k * c.ny + j) * c.nx + i] : ggap;
-        T frac = T(0.5) + div_dx(c, sgap);
-        if (frac < T(0)) frac = T(0);
-        if (frac > T(1)) frac = T(1);
-        g.lf[0] = frac;
+    }
+    {
+        bool by_cube;
+        g.lf[0] = cell_center_fraction(c, s, i, j, k, by_cube);
+        if (by_cube) cube |= 1u;
     }
 #pragma unroll
     for (int q = 1; q < 19; q++) {

@@ -38,6 +38,8 @@ static Consts<T> make_consts(const lbm_config_t& c, int omega_band_top) {
     k.nx = c.nx;
     k.ny = c.ny;
     k.nz = c.nz;
+    fast_div_magic((uint32_t)c.nx, k.div_nx_m, k.div_nx_s);
+    fast_div_magic((uint32_t)c.ny, k.div_ny_m, k.div_ny_s);
     k.fs_water = c.fs_enabled != 0;
     k.air_walls = c.physics_mode == LBM_PHYS_AIR;
     k.collision_reg = c.collision == LBM_COLL_REGULARIZED;
@@ -139,11 +141,14 @@ static Slot<T> make_slot(DevBufs& b, int s) {
 
 static inline unsigned nblocks(size_t n, int nt = NT) { return (unsigned)((n + nt - 1) / nt); }
 
-__device__ __forceinline__ void cell_ijk(size_t id, int nx, int ny, int& i, int& j, int& k) {
-    i = (int)(id % (size_t)nx);
-    size_t r = id / (size_t)nx;
-    j = (int)(r % (size_t)ny);
-    k = (int)(r / (size_t)ny);
+template <class T>
+__device__ __forceinline__ void cell_ijk(const Consts<T>& c, size_t id, int& i, int& j, int& k) {
+    const uint32_t u = (uint32_t)id;
+    const uint32_t r = fast_div(u, c.div_nx_m, c.div_nx_s);
+    const uint32_t kk = fast_div(r, c.div_ny_m, c.div_ny_s);
+    i = (int)(u - r * (uint32_t)c.nx);
+    j = (int)(r - kk * (uint32_t)c.ny);
+    k = (int)kk;
 }
 
 // ===========================================================================
@@ -347,7 +352,7 @@ __global__ __launch_bounds__(NT) void k_geometry_fields(Consts<T> c, Slot<T> s, 
     size_t n = (size_t)c.nx * c.ny * c.nz;
     if (id >= n) return;
     int i, j, k;
-    cell_ijk(id, c.nx, c.ny, i, j, k);
+    cell_ijk(c, id, i, j, k);
     CellGeo<T> g;
     cell_geometry(c, s, i, j, k, g);
     for (int q = 0; q < 19; q++) {
@@ -365,7 +370,7 @@ __global__ __launch_bounds__(NT) void k_cellflags(Consts<T> c, Slot<T> s, uint8_
     size_t n = (size_t)c.nx * c.ny * c.nz;
     if (id >= n) return;
     int i, j, k;
-    cell_ijk(id, c.nx, c.ny, i, j, k);
+    cell_ijk(c, id, i, j, k);
     CellGeo<T> g;
     cell_geometry(c, s, i, j, k, g);
     flags[id] = (uint8_t)((g.open_mask & 1u) | (g.near ? 2u : 0u));
@@ -398,7 +403,7 @@ __global__ __launch_bounds__(NT) void k_init_populations(Consts<T> c, Slot<T> s,
     size_t n = (size_t)c.nx * c.ny * c.nz;
     if (id >= n) return;
     int i, j, k;
-    cell_ijk(id, c.nx, c.ny, i, j, k);
+    cell_ijk(c, id, i, j, k);
     CellGeo<T> g;
     cell_geometry(c, s, i, j, k, g);
     bool open0 = g.open_mask & 1u;
@@ -441,12 +446,13 @@ __global__ __launch_bounds__(NT) void k_cond(Consts<T> c, Slot<T> s, T* f, int* 
     size_t n = (size_t)c.nx * c.ny * c.nz;
     if (id >= n) return;
     int i, j, k;
-    cell_ijk(id, c.nx, c.ny, i, j, k);
+    cell_ijk(c, id, i, j, k);
     CellGeo<T> g;
     cell_geometry(c, s, i, j, k, g);
     obst_prev[id] = obst[id];
     obst[id] = (g.open_mask & 1u) ? 0 : 1;
     if ((g.open_mask & 0x7ffffu) == 0x7ffffu) return;  // no closed link
+    if (!c.air_walls && !c.bed_moving) return;          // water: only a moving bed conditions its links
     T fl[19];
 #pragma unroll
     for (int q = 0; q < 19; q++) fl[q] = f[(size_t)q * n + id];
@@ -462,7 +468,7 @@ __global__ __launch_bounds__(NT) void k_macro(Consts<T> c, const T* f, T* rho, T
     size_t n = (size_t)c.nx * c.ny * c.nz;
     if (id >= n) return;
     int i, j, k;
-    cell_ijk(id, c.nx, c.ny, i, j, k);
+    cell_ijk(c, id, i, j, k);
     T fl[19];
 #pragma unroll
     for (int q = 0; q < 19; q++) fl[q] = f[(size_t)q * n + id];
@@ -514,7 +520,7 @@ __global__ __launch_bounds__(NT) void k_les(Consts<T> c, Slot<T> s, const T* ux,
     size_t n = (size_t)c.nx * c.ny * c.nz;
     if (id >= n) return;
     int i, j, k;
-    cell_ijk(id, c.nx, c.ny, i, j, k);
+    cell_ijk(c, id, i, j, k);
     bool active = flags[id] & 1u;
     if (c.fs_water && type[id] == LBM_FS_GAS) active = false;
     if (!active) return;
@@ -551,7 +557,7 @@ __global__ __launch_bounds__(NT) void k_collide(Consts<T> c, T* f, const T* rho,
     size_t n = (size_t)c.nx * c.ny * c.nz;
     if (id >= n) return;
     int i, j, k;
-    cell_ijk(id, c.nx, c.ny, i, j, k);
+    cell_ijk(c, id, i, j, k);
     T r = rho[id], u = ux[id], v = uy[id], w = uz[id];
     bool open0 = flags[id] & 1u;
     bool gas = false;
@@ -658,7 +664,7 @@ __global__ __launch_bounds__(NT) void k_stream_bc(Consts<T> c, Slot<T> s, const 
     size_t n = (size_t)c.nx * c.ny * c.nz;
     if (id >= n) return;
     int i, j, k;
-    cell_ijk(id, c.nx, c.ny, i, j, k);
+    cell_ijk(c, id, i, j, k);
     const int nx = c.nx, ny = c.ny, nz = c.nz;
     const bool water = c.fs_water;
     const bool xper = c.x_boundary == LBM_XB_PERIODIC, xio = c.x_boundary == LBM_XB_INFLOW_OUTFLOW;
@@ -670,56 +676,90 @@ __global__ __launch_bounds__(NT) void k_stream_bc(Consts<T> c, Slot<T> s, const 
     int ty = water ? type[id] : LBM_FS_LIQUID;
     bool dest_gas = water && ty == LBM_FS_GAS && open0;
     T fpre[19], fo[19];
+    // a gas cell takes the rest equilibrium on every link; its own
+    // populations are read only by the wall correction of a cell next to a wall
+    if (!dest_gas || g.near) {
 #pragma unroll
-    for (int q = 0; q < 19; q++) fpre[q] = Store<S, T>::load(fin, (size_t)q * n + id, q, inv);
+        for (int q = 0; q < 19; q++) fpre[q] = Store<S, T>::load(fin, (size_t)q * n + id, q, inv);
+    } else {
+#pragma unroll
+        for (int q = 0; q < 19; q++) fpre[q] = T(0);
+    }
     int n_cut = 0, n_fs = 0, n_bridge = 0;
     fo[0] = dest_gas ? feq(c.rho0, T(0), T(0), T(0), 0) : fpre[0];
     size_t col = (size_t)j * nx + i;
-#pragma unroll
-    for (int q = 1; q < 19; q++) {
-        const int qb = oppq(q);
-        int sir = i - cxq(q), si = sir;
-        bool fluid = true;
+    // source position of link q: periodic wrap, else the raw index with
+    // `fluid` false outside a solid x or y wall (an inflow/outflow x
+    // boundary keeps fluid and the raw index)
+    auto source = [&](int q, int& si, int& sj, int& sk, bool& fluid) {
+        const int sir = i - cxq(q);
+        si = sir;
+        fluid = true;
         if (xper)
             si = pmod(sir, nx);
         else if (!xio && (sir < 0 || sir >= nx))
             fluid = false;
-        int sjr = j - cyq(q), sj = yper ? pmod(sjr, ny) : sjr;
+        const int sjr = j - cyq(q);
+        sj = yper ? pmod(sjr, ny) : sjr;
         if (!yper && (sjr < 0 || sjr >= ny)) fluid = false;
-        int sk = k - czq(q);
-        T out;
-        if (water) {
-            bool src_gas = false;
-            if (fluid && sk >= 0 && sk < nz) {
-                if (si >= 0 && si < nx) {
-                    size_t sid = ((size_t)sk * ny + sj) * nx + si;
-                    src_gas = type[sid] == LBM_FS_GAS && (flags[sid] & 1u);
-                }
-            } else if (sk >= nz) {
-                src_gas = true;
-            }
-            if (dest_gas) {
-                out = feq(c.rho0, T(0), T(0), T(0), q);
-            } else if (!fluid) {
+        sk = k - czq(q);
+    };
+    if (water && dest_gas) {
+        // gas destination: rest equilibrium on every link
+#pragma unroll
+        for (int q = 1; q < 19; q++) fo[q] = feq(c.rho0, T(0), T(0), T(0), q);
+    } else if (water) {
+        // pass 1: every load of every link, from in-grid addresses (a source
+        // outside the grid reads the cell itself, and the rules below ignore
+        // that value), issued before any rule so their latencies overlap
+        const bool aperture = c.fs_vof && c.link_aperture;
+        int st_a[19];
+        T sfill_a[19], sv_a[19];
+#pragma unroll
+        for (int q = 1; q < 19; q++) {
+            int si, sj, sk;
+            bool fluid;
+            source(q, si, sj, sk, fluid);
+            const bool src_ok = fluid && sk >= 0 && sk < nz;
+            const bool inx = si >= 0 && si < nx;
+            const int sic = si < 0 ? 0 : (si >= nx ? nx - 1 : si);  // outflow: copy of the last layer
+            const size_t sidx = src_ok ? ((size_t)sk * ny + sj) * nx + sic : id;
+            const size_t tid = (src_ok && inx) ? sidx : id;
+            st_a[q] = type[tid] * 2 + (int)(flags[tid] & 1u);
+            sfill_a[q] = aperture ? fill[tid] : T(0);
+            sv_a[q] = Store<S, T>::load(fin, (size_t)q * n + sidx, q, inv);
+        }
+        // pass 2: the rules of stream (CODEX:3438-3551) for a liquid or
+        // interface destination
+#pragma unroll
+        for (int q = 1; q < 19; q++) {
+            const int qb = oppq(q);
+            int si, sj, sk;
+            bool fluid;
+            source(q, si, sj, sk, fluid);
+            const bool inz = sk >= 0 && sk < nz;
+            const bool inx = si >= 0 && si < nx;
+            const int sty = st_a[q] >> 1;
+            const bool sopen = (st_a[q] & 1) != 0;
+            const bool src_gas = (fluid && inz) ? (inx && sty == LBM_FS_GAS && sopen) : (sk >= nz);
+            T out;
+            if (!fluid) {
                 out = fpre[qb];
                 n_cut++;
-            } else if (sk >= 0 && sk < nz && !src_gas) {
+            } else if (inz && !src_gas) {
                 T frac = g.lf[q];
-                T fstr = frac * src_value<S, T>(c, fin, n, inflow, si, sj, sk, q) + (T(1.0) - frac) * fpre[qb];
+                const T sv = si < 0 ? feq(c.rho0, inflow[(size_t)sk * ny + sj], T(0), T(0), q) : sv_a[q];
+                T fstr = frac * sv + (T(1.0) - frac) * fpre[qb];
                 out = fstr;
-                if (c.fs_vof && c.link_aperture) {
+                if (aperture) {
                     T la = T(1);
-                    if (si >= 0 && si < nx) {
-                        size_t sid = ((size_t)sk * ny + sj) * nx + si;
-                        if (flags[sid] & 1u) {
-                            int st = type[sid];
-                            if (st == LBM_FS_GAS)
-                                la = T(0);
-                            else if (st == LBM_FS_INTERFACE) {
-                                la = fill[sid];
-                                if (la < T(0)) la = T(0);
-                                if (la > T(1)) la = T(1);
-                            }
+                    if (inx && sopen) {
+                        if (sty == LBM_FS_GAS)
+                            la = T(0);
+                        else if (sty == LBM_FS_INTERFACE) {
+                            la = sfill_a[q];
+                            if (la < T(0)) la = T(0);
+                            if (la > T(1)) la = T(1);
                         }
                     }
                     if (la < T(1.0)) {
@@ -770,19 +810,30 @@ __global__ __launch_bounds__(NT) void k_stream_bc(Consts<T> c, Slot<T> s, const 
                 }
                 n_fs++;
             }
-        } else if (!fluid) {
-            out = fpre[qb];
-            n_cut++;
-        } else if (sk >= 0 && sk < nz) {
-            T frac = g.lf[q];
-            out = frac * src_value<S, T>(c, fin, n, inflow, si, sj, sk, q) + (T(1.0) - frac) * fpre[qb];
-            if (frac < T(1.0)) n_cut++;
-        } else if (sk >= nz && c.top_boundary == LBM_TOP_FREE_SLIP) {
-            out = src_value<S, T>(c, fin, n, inflow, si, sj, nz - 1, mirzq(q));
-        } else {
-            out = fpre[q];
+            fo[q] = out;
         }
-        fo[q] = out;
+    } else {
+#pragma unroll
+        for (int q = 1; q < 19; q++) {
+            const int qb = oppq(q);
+            int si, sj, sk;
+            bool fluid;
+            source(q, si, sj, sk, fluid);
+            T out;
+            if (!fluid) {
+                out = fpre[qb];
+                n_cut++;
+            } else if (sk >= 0 && sk < nz) {
+                T frac = g.lf[q];
+                out = frac * src_value<S, T>(c, fin, n, inflow, si, sj, sk, q) + (T(1.0) - frac) * fpre[qb];
+                if (frac < T(1.0)) n_cut++;
+            } else if (sk >= nz && c.top_boundary == LBM_TOP_FREE_SLIP) {
+                out = src_value<S, T>(c, fin, n, inflow, si, sj, nz - 1, mirzq(q));
+            } else {
+                out = fpre[q];
+            }
+            fo[q] = out;
+        }
     }
     // (A) x fringe (CODEX:3620-3644)
     if (c.x_fringe && i < 5 && open0 && k > nz / 2) {
@@ -822,7 +873,7 @@ __global__ __launch_bounds__(NT) void k_cut_links(Consts<T> c, Slot<T> s, int* i
     int cnt = 0;
     if (id < n) {
         int i, j, k;
-        cell_ijk(id, c.nx, c.ny, i, j, k);
+        cell_ijk(c, id, i, j, k);
         CellGeo<T> g;
         cell_geometry(c, s, i, j, k, g);
         for (int q = 1; q < 19; q++) {
@@ -881,7 +932,7 @@ __device__ __forceinline__ bool nbr(int nx, int ny, int nz, int xb, int yb, int 
     size_t n = (size_t)a.c.nx * a.c.ny * a.c.nz;                          \
     if (id >= n) return;                                                  \
     int i, j, k;                                                          \
-    cell_ijk(id, a.c.nx, a.c.ny, i, j, k);
+    cell_ijk(a.c, id, i, j, k);
 
 // compute_vof_mass_exchange (CODEX:1305-1350)
 template <class T>
@@ -1188,6 +1239,10 @@ __global__ __launch_bounds__(NT) void k_vof_orphan_a(VofArgs<T> a, T empty_thr, 
 template <class T>
 __global__ __launch_bounds__(NT) void k_vof_orphan_b(VofArgs<T> a, T thr, int rmax) {
     VOF_PROLOGUE
+    if (a.icnt[IC_VOF_ORPHAN] == 0) {  // phase A found no orphan (the common case): nothing moves
+        a.mass2[id] = a.mass[id];
+        return;
+    }
     bool orphan = a.si[id] != 0;
     T acc = orphan ? T(0) : a.mass[id];
     if ((a.flags[id] & 1u) && a.type[id] != LBM_FS_GAS && a.fill[id] >= thr) {
@@ -1455,14 +1510,18 @@ __global__ __launch_bounds__(NT) void k_vof_height(Consts<T> c, Slot<T> s, const
     size_t plane = (size_t)c.nx * c.ny;
     if (col >= plane) return;
     int i = (int)(col % (size_t)c.nx), j = (int)(col / (size_t)c.nx);
+    // loads of every plane are unconditional (a closed cell adds nothing:
+    // the select keeps depth unchanged), so the unrolled loop keeps several
+    // planes in flight instead of one dependent load pair per plane
     T depth = T(0);
+#pragma unroll 8
     for (int k = 0; k < c.nz; k++) {
-        size_t id = (size_t)k * plane + col;
-        if (flags[id] & 1u) {
-            CellGeo<T> g;
-            cell_geometry(c, s, i, j, k, g);
-            depth += fill[id] * g.lf[0] * c.dx;
-        }
+        const size_t id = (size_t)k * plane + col;
+        const bool open = flags[id] & 1u;
+        const T fl = fill[id];
+        bool by_cube;
+        const T add = fl * cell_center_fraction(c, s, i, j, k, by_cube) * c.dx;
+        depth = open ? depth + add : depth;
     }
     fs_h[col] = s.h[col] + depth;
 }
@@ -1477,7 +1536,7 @@ __global__ __launch_bounds__(NT) void k_fs_classify(Consts<T> c, const T* fs_h, 
     size_t n = (size_t)c.nx * c.ny * c.nz;
     if (id >= n) return;
     int i, j, k;
-    cell_ijk(id, c.nx, c.ny, i, j, k);
+    cell_ijk(c, id, i, j, k);
     const int old = type[id];
     T z_bottom = (T)k * c.dx;
     T fl = (fs_h[(size_t)j * c.nx + i] - z_bottom) / c.dx;
@@ -2509,7 +2568,7 @@ __global__ __launch_bounds__(NT) void k_phi(Consts<T> c, Slot<T> s, T* phi) {
     size_t n = (size_t)c.nx * c.ny * c.nz;
     if (id >= n) return;
     int i, j, k;
-    cell_ijk(id, c.nx, c.ny, i, j, k);
+    cell_ijk(c, id, i, j, k);
     phi[id] = cell_phi(c, s, i, j, k);
 }
 
@@ -2605,7 +2664,7 @@ __global__ __launch_bounds__(NT) void k_fs_project(Consts<T> c, const T* fin, T*
     size_t n = (size_t)c.nx * c.ny * c.nz;
     if (id >= n) return;
     int i, j, k;
-    cell_ijk(id, c.nx, c.ny, i, j, k);
+    cell_ijk(c, id, i, j, k);
     for (int q = 0; q < 19; q++) fout[(size_t)q * n + id] = fin[(size_t)q * n + id];
     if (obst[id] != 0 || type[id] == LBM_FS_GAS) return;
     const size_t col = (size_t)j * c.nx + i;
@@ -2655,7 +2714,7 @@ static void init_t(const lbm_config_t& cfg, DevBufs& b, int slot, hipStream_t s)
         VofArgs<T> a = make_vof<T>(cfg, b, (const T*)b.f[0]);
         k_vof_init_mass<T><<<nblocks(b.n), NT, 0, s>>>(a);
         k_vof_enforce<T><<<nblocks(b.n), NT, 0, s>>>(a);
-        k_vof_height<T><<<nblocks(b.n2), NT, 0, s>>>(c, sl, (const T*)b.fill, b.cellflags, (T*)b.fs_h);
+        k_vof_height<T><<<nblocks(b.n2, 64), 64, 0, s>>>(c, sl, (const T*)b.fill, b.cellflags, (T*)b.fs_h);
     }
     HIP_OK(hipGetLastError());
 }
@@ -2732,7 +2791,7 @@ static void vof_pipeline(const lbm_config_t& cfg, DevBufs& b, T* f, int slot, in
     k_vof_enforce<T><<<nb, NT, 0, s>>>(A());
     {
         Consts<T> c = make_consts<T>(cfg, -1);
-        k_vof_height<T><<<nblocks(b.n2), NT, 0, s>>>(c, make_slot<T>(b, slot), (const T*)b.fill, b.cellflags, (T*)b.fs_h);
+        k_vof_height<T><<<nblocks(b.n2, 64), 64, 0, s>>>(c, make_slot<T>(b, slot), (const T*)b.fill, b.cellflags, (T*)b.fs_h);
     }
     HIP_OK(hipMemsetAsync(b.icnt + IC_FS_REFILL, 0, 2 * sizeof(int), s));
     k_vof_refill<T><<<nb, NT, 0, s>>>(A(), f);

@@ -39,6 +39,7 @@ for _p in (str(ROOT / "lbm-3d-les-air-over-waves_amd"), str(ROOT / "tests")):
 
 HBM_PEAK_GBS = 8000.0
 BYTES_PER_LU = {"fp32": 152, "fp16": 76, "bf16": 76, "fp64": 304}
+VOF_BYTES_PER_LU = 170
 PROFILES = ROOT / "profiles"
 
 
@@ -50,11 +51,11 @@ def measured_traffic(workload: str, storage: str):
 
     if storage != "fp32":
         return None
-    path = PROFILES / f"fused_step_{workload}.md"
-    if not path.exists():
-        return None
-    m = re.search(r"^traffic_bytes_per_step:\s*(\d+)", path.read_text(), re.M)
-    return int(m.group(1)) if m else None
+    for path in sorted(PROFILES.glob(f"*_step_{workload}.md")):
+        m = re.search(r"^traffic_bytes_per_step:\s*(\d+)", path.read_text(), re.M)
+        if m:
+            return int(m.group(1))
+    return None
 
 
 def run_device(env: dict, steps: int, warmup: int):
@@ -296,6 +297,8 @@ def main():
     cfg = parse_env(env)
     main_run = run_device(env, args.steps, args.warmup)
     bpl = BYTES_PER_LU[cfg.storage]
+    if cfg.water_free_surface_enabled and cfg.free_surface_tracking_mode == "vof":
+        bpl = VOF_BYTES_PER_LU  # populations + mass, fill and type
     achieved = main_run["cells"] * bpl / (main_run["ms_kernel"] * 1e-3) / 1e9
     result = {
         "metric": "MLUPS & % HBM peak (FP32 vs FP16-storage); ensemble p50/p99 latency",

@@ -27,6 +27,7 @@ enum {
     IC_VOF_COMPRESS,
     IC_FLOOD_CHANGED,      // set by a flood-fill sweep that marked a cell
     IC_FLOOD_UNCONVERGED,  // sweeps exhausted before the fixed point
+    IC_REDIST_DONORS,      // nonzero when a redistribution round has a donor
     IC_COUNT
 };
 // FP64 counters (indices into DevBufs::dcnt)

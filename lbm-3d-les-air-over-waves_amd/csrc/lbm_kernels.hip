@@ -961,6 +961,7 @@ __global__ __launch_bounds__(NT) void k_vof_exchange(VofArgs<T> a) {
 template <class T>
 __global__ __launch_bounds__(NT) void k_vof_preclassify(VofArgs<T> a, T empty_thr) {
     VOF_PROLOGUE
+    if (id == 0) a.icnt[IC_REDIST_DONORS] = 0;  // the next redistribution round starts without donors
     int old = a.type[id];
     a.type_prev[id] = old;
     a.excess[id] = T(0);
@@ -1083,12 +1084,17 @@ __global__ __launch_bounds__(NT) void k_vof_redist_a(VofArgs<T> a) {
     a.sc[id] = nzv;
     a.sd[id] = ws;
     a.se[id] = cs;
+    a.icnt[IC_REDIST_DONORS] = 1;  // every donor stores the same value
 }
 
 // phase B: every receiver adds its shares (own retained excess, then q order)
 template <class T>
 __global__ __launch_bounds__(NT) void k_vof_redist_b(VofArgs<T> a) {
     VOF_PROLOGUE
+    if (a.icnt[IC_REDIST_DONORS] == 0) {  // no donor this round: the mass is unchanged
+        a.mass2[id] = a.mass[id];
+        return;
+    }
     T acc = a.mass[id];
     if (a.si[id] != 0 && !(a.se[id] > T(1.0e-12))) acc += a.excess[id];
     if (a.flags[id] & 1u) {
@@ -1122,6 +1128,7 @@ __global__ __launch_bounds__(NT) void k_vof_redist_b(VofArgs<T> a) {
 template <class T>
 __global__ __launch_bounds__(NT) void k_vof_finalize(VofArgs<T> a, T empty_thr) {
     VOF_PROLOGUE
+    if (id == 0) a.icnt[IC_REDIST_DONORS] = 0;  // the next redistribution round starts without donors
     a.excess[id] = T(0);
     a.excess_dir[id] = 0;
     if (!(a.flags[id] & 1u)) {

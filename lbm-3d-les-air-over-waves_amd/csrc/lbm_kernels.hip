@@ -927,6 +927,16 @@ __device__ __forceinline__ bool nbr(int nx, int ny, int nz, int xb, int yb, int 
 }
 #define NBR(di, dj, dk, ni, nj, nk) nbr(a.c.nx, a.c.ny, a.c.nz, a.c.x_boundary, a.c.y_boundary, i, j, k, di, dj, dk, ni, nj, nk)
 #define CID(kk, jj, ii) (((size_t)(kk) * a.c.ny + (jj)) * a.c.nx + (ii))
+// index of neighbour (di, dj, dk), or of the cell itself where that neighbour
+// lies beyond a wall (valid false): always in the grid, so the loads of
+// every link can be issued unconditionally and combined afterwards
+template <class T>
+__device__ __forceinline__ size_t nbr_id(const Consts<T>& c, int i, int j, int k, int di, int dj, int dk, size_t self, bool& valid) {
+    int ni, nj, nk;
+    valid = nbr(c.nx, c.ny, c.nz, c.x_boundary, c.y_boundary, i, j, k, di, dj, dk, ni, nj, nk);
+    return valid ? ((size_t)nk * c.ny + nj) * c.nx + ni : self;
+}
+#define NBR_ID(di, dj, dk, valid) nbr_id(a.c, i, j, k, di, dj, dk, id, valid)
 #define VOF_PROLOGUE                                                      \
     size_t id = (size_t)blockIdx.x * blockDim.x + threadIdx.x;            \
     size_t n = (size_t)a.c.nx * a.c.ny * a.c.nz;                          \
@@ -943,15 +953,19 @@ __global__ __launch_bounds__(NT) void k_vof_exchange(VofArgs<T> a) {
     a.excess_dir[id] = 0;
     if (!((a.flags[id] & 1u) && a.type[id] == LBM_FS_INTERFACE)) return;
     T dm = T(0);
+    const T fill_c = a.fill[id];
 #pragma unroll
     for (int q = 1; q < 19; q++) {
-        int ni, nj, nk;
-        if (!NBR(cxq(q), cyq(q), czq(q), ni, nj, nk)) continue;
-        size_t nid = CID(nk, nj, ni);
-        if ((a.flags[nid] & 1u) && a.type[nid] != LBM_FS_GAS) {
+        bool v;
+        const size_t nid = NBR_ID(cxq(q), cyq(q), czq(q), v);
+        const bool open = (a.flags[nid] & 1u) != 0;
+        const int ty = a.type[nid];
+        const T fn = a.fill[nid];
+        const T fa = a.f[(size_t)oppq(q) * n + id], fb = a.f[(size_t)q * n + nid];
+        if (v & open & (ty != LBM_FS_GAS)) {
             T lw = T(1.0);
-            if (a.type[nid] == LBM_FS_INTERFACE) lw = T(0.5) * (a.fill[id] + a.fill[nid]);
-            if (lw > T(0)) dm += lw * (a.f[(size_t)oppq(q) * n + id] - a.f[(size_t)q * n + nid]);
+            if (ty == LBM_FS_INTERFACE) lw = T(0.5) * (fill_c + fn);
+            if (lw > T(0)) dm += lw * (fa - fb);
         }
     }
     a.mdelta[id] = dm;
@@ -1101,11 +1115,11 @@ __global__ __launch_bounds__(NT) void k_vof_redist_b(VofArgs<T> a) {
         int ty = a.type[id];
 #pragma unroll
         for (int q = 1; q < 19; q++) {
-            int si_, sj, sk;
-            if (!NBR(-cxq(q), -cyq(q), -czq(q), si_, sj, sk)) continue;
-            size_t sid = CID(sk, sj, si_);
-            int ed = a.si[sid];
-            if (ed == 0 || !(a.se[sid] > T(1.0e-12))) continue;
+            bool v;
+            const size_t sid = NBR_ID(-cxq(q), -cyq(q), -czq(q), v);
+            const int ed = a.si[sid];
+            const T se = a.se[sid];
+            if (!v | (ed == 0) | !(se > T(1.0e-12))) continue;
             T ex = a.excess[sid];
             bool ok = ed > 0 ? ty != LBM_FS_LIQUID : ty != LBM_FS_GAS;
             if (!ok) continue;
@@ -1187,10 +1201,9 @@ __global__ __launch_bounds__(NT) void k_vof_enforce(VofArgs<T> a) {
     bool touches = false;
 #pragma unroll
     for (int q = 1; q < 19; q++) {
-        int ni, nj, nk;
-        if (!NBR(cxq(q), cyq(q), czq(q), ni, nj, nk)) continue;
-        size_t nid = CID(nk, nj, ni);
-        if ((a.flags[nid] & 1u) && a.type[nid] == LBM_FS_GAS) touches = true;
+        bool v;
+        const size_t nid = NBR_ID(cxq(q), cyq(q), czq(q), v);
+        touches |= v & ((a.flags[nid] & 1u) != 0) & (a.type[nid] == LBM_FS_GAS);
     }
     if (touches) {
         a.type[id] = LBM_FS_INTERFACE;
@@ -1208,13 +1221,11 @@ __global__ __launch_bounds__(NT) void k_vof_orphan_a(VofArgs<T> a, T empty_thr, 
     int wet = 0, res = 0;
 #pragma unroll
     for (int q = 1; q < 19; q++) {
-        int ni, nj, nk;
-        if (!NBR(cxq(q), cyq(q), czq(q), ni, nj, nk)) continue;
-        size_t nid = CID(nk, nj, ni);
-        if ((a.flags[nid] & 1u) && a.type[nid] != LBM_FS_GAS) {
-            wet++;
-            if (a.fill[nid] >= thr) res++;
-        }
+        bool v;
+        const size_t nid = NBR_ID(cxq(q), cyq(q), czq(q), v);
+        const bool w = v & ((a.flags[nid] & 1u) != 0) & (a.type[nid] != LBM_FS_GAS);
+        wet += w;
+        res += w & (a.fill[nid] >= thr);
     }
     if (!(res == 0 && (a.fill[id] < thr || wet <= max_weak))) return;
     T m = a.mass[id];
@@ -1295,13 +1306,11 @@ __device__ bool vof_detached(const VofArgs<T>& a, int i, int j, int k, T orphan_
     int wet = 0, res = 0;
 #pragma unroll
     for (int q = 1; q < 19; q++) {
-        int ni, nj, nk;
-        if (!NBR(cxq(q), cyq(q), czq(q), ni, nj, nk)) continue;
-        size_t nid = CID(nk, nj, ni);
-        if ((a.flags[nid] & 1u) && a.type[nid] != LBM_FS_GAS) {
-            wet++;
-            if (a.fill[nid] >= orphan_thr) res++;
-        }
+        bool v;
+        const size_t nid = NBR_ID(cxq(q), cyq(q), czq(q), v);
+        const bool w = v & ((a.flags[nid] & 1u) != 0) & (a.type[nid] != LBM_FS_GAS);
+        wet += w;
+        res += w & (a.fill[nid] >= orphan_thr);
     }
     return wet <= max_wet && res <= max_res;
 }
@@ -1763,12 +1772,19 @@ __global__ __launch_bounds__(NT) void k_vof_repair(VofArgs<T> a, T* f) {
     if (!((a.flags[id] & 1u) && a.type[id] != LBM_FS_GAS && a.type_prev[id] != LBM_FS_GAS)) return;
     bool rep = false;
     int cnt = 0;
+    // the sources that just filled (loads only, then the copies)
+    uint32_t fresh = 0;
+    size_t src[19];
+#pragma unroll
     for (int q = 1; q < 19; q++) {
-        int si_, sj, sk;
-        if (!NBR(-cxq(q), -cyq(q), -czq(q), si_, sj, sk)) continue;
-        size_t sid = CID(sk, sj, si_);
-        if ((a.flags[sid] & 1u) && a.type_prev[sid] == LBM_FS_GAS && a.type[sid] != LBM_FS_GAS) {
-            f[(size_t)q * n + id] = f[(size_t)q * n + sid];
+        bool v;
+        src[q] = NBR_ID(-cxq(q), -cyq(q), -czq(q), v);
+        const bool nf = v & ((a.flags[src[q]] & 1u) != 0) & (a.type_prev[src[q]] == LBM_FS_GAS) & (a.type[src[q]] != LBM_FS_GAS);
+        fresh |= (uint32_t)nf << q;
+    }
+    for (int q = 1; q < 19; q++) {
+        if ((fresh >> q) & 1u) {
+            f[(size_t)q * n + id] = f[(size_t)q * n + src[q]];
             rep = true;
             cnt++;
         }

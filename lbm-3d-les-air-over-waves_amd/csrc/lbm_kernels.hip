@@ -709,12 +709,15 @@ __global__ __launch_bounds__(NT) void k_stream_bc(Consts<T> c, Slot<T> s, const 
 #pragma unroll
         for (int q = 1; q < 19; q++) fo[q] = feq(c.rho0, T(0), T(0), T(0), q);
     } else if (water) {
-        // pass 1: every load of every link, from in-grid addresses (a source
-        // outside the grid reads the cell itself, and the rules below ignore
-        // that value), issued before any rule so their latencies overlap
+        // pass 1: the source state and population of every link, from
+        // in-grid addresses (a source outside the grid reads the cell itself,
+        // and the rules below ignore that value), issued before any rule so
+        // their latencies overlap; the fill of an interface source is read
+        // where the aperture needs it
         const bool aperture = c.fs_vof && c.link_aperture;
-        int st_a[19];
-        T sfill_a[19], sv_a[19];
+        // per link: 2 * type + open of the source, 3 bits each
+        uint64_t st_bits = 0;
+        T sv_a[19];
 #pragma unroll
         for (int q = 1; q < 19; q++) {
             int si, sj, sk;
@@ -725,8 +728,7 @@ __global__ __launch_bounds__(NT) void k_stream_bc(Consts<T> c, Slot<T> s, const 
             const int sic = si < 0 ? 0 : (si >= nx ? nx - 1 : si);  // outflow: copy of the last layer
             const size_t sidx = src_ok ? ((size_t)sk * ny + sj) * nx + sic : id;
             const size_t tid = (src_ok && inx) ? sidx : id;
-            st_a[q] = type[tid] * 2 + (int)(flags[tid] & 1u);
-            sfill_a[q] = aperture ? fill[tid] : T(0);
+            st_bits |= (uint64_t)(type[tid] * 2 + (int)(flags[tid] & 1u)) << (3 * q);
             sv_a[q] = Store<S, T>::load(fin, (size_t)q * n + sidx, q, inv);
         }
         // pass 2: the rules of stream (CODEX:3438-3551) for a liquid or
@@ -739,8 +741,9 @@ __global__ __launch_bounds__(NT) void k_stream_bc(Consts<T> c, Slot<T> s, const 
             source(q, si, sj, sk, fluid);
             const bool inz = sk >= 0 && sk < nz;
             const bool inx = si >= 0 && si < nx;
-            const int sty = st_a[q] >> 1;
-            const bool sopen = (st_a[q] & 1) != 0;
+            const int stq = (int)(st_bits >> (3 * q)) & 7;
+            const int sty = stq >> 1;
+            const bool sopen = (stq & 1) != 0;
             const bool src_gas = (fluid && inz) ? (inx && sty == LBM_FS_GAS && sopen) : (sk >= nz);
             T out;
             if (!fluid) {
@@ -757,7 +760,7 @@ __global__ __launch_bounds__(NT) void k_stream_bc(Consts<T> c, Slot<T> s, const 
                         if (sty == LBM_FS_GAS)
                             la = T(0);
                         else if (sty == LBM_FS_INTERFACE) {
-                            la = sfill_a[q];
+                            la = fill[((size_t)sk * ny + sj) * nx + si];  // an open interface source inside the grid
                             if (la < T(0)) la = T(0);
                             if (la > T(1)) la = T(1);
                         }

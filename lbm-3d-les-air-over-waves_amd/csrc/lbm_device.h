@@ -513,12 +513,12 @@ __device__ __forceinline__ void cell_geometry(const Consts<T>& c, const Slot<T>&
     T zc = ((T)k + T(0.5)) * dx;
     const T xc = ((T)i + T(0.5)) * dx, yc = ((T)j + T(0.5)) * dx;
     uint32_t mask = 0, xw = 0, cube = 0;
-    // a cell at least 2 dx above the highest of its 9 bed samples, away from
-    // solid x/y walls, with the bed as the only surface: every link gap is
-    // >= dx/2 after rounding, so every fraction clamps to 1 and every link is
-    // open (the same values the per-link evaluation below gives)
-    if (!c.include_cube && !c.phi_fraction && !(c.x_boundary == LBM_XB_SOLID && (i == 0 || i == c.nx - 1)) &&
-        !(c.y_boundary == LBM_YB_SOLID && (j == 0 || j == c.ny - 1))) {
+    // a cell at least 2 dx above the highest of its 9 bed samples, with the
+    // bed as the only surface: every link gap is >= dx/2 after rounding, so
+    // every bed fraction clamps to 1 and every bed link is open; the solid
+    // x / y wall overrides below then close the links leaving the domain
+    // (the same values the per-link evaluation gives, without its bed work)
+    if (!c.include_cube && !c.phi_fraction) {
         T e9[9];
 #pragma unroll
         for (int e = 0; e < 9; e++) e9[e] = s.eta9[(size_t)e * plane + col];  // issued together
@@ -526,11 +526,20 @@ __device__ __forceinline__ void cell_geometry(const Consts<T>& c, const Slot<T>&
 #pragma unroll
         for (int e = 0; e < 9; e++) far = far & (zc - e9[e] >= T(2.0) * dx);
         if (!c.include_bed || far) {
+            g.lf[0] = T(1);
 #pragma unroll
-            for (int q = 0; q < 19; q++) g.lf[q] = T(1);
-            g.near = false;
-            g.open_mask = 0x7ffffu;
-            g.xwall_mask = 0;
+            for (int q = 1; q < 19; q++) {
+                bool wall = false;
+                if (c.x_boundary == LBM_XB_SOLID) wall |= (i == 0 && cxq(q) > 0) || (i == c.nx - 1 && cxq(q) < 0);
+                if (c.y_boundary == LBM_YB_SOLID) wall |= (j == 0 && cyq(q) > 0) || (j == c.ny - 1 && cyq(q) < 0);
+                g.lf[q] = wall ? T(0) : T(1);
+                xw |= (uint32_t)wall << q;
+            }
+            mask = 0x7fffeu & ~xw;
+            // a cell has at most 9 wall links (corner column), so one stays open
+            g.near = mask != 0x7fffeu;
+            g.open_mask = mask | 1u;
+            g.xwall_mask = xw;
             g.cube_mask = 0;
             return;
         }

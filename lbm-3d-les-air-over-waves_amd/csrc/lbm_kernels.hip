@@ -1386,8 +1386,10 @@ __global__ __launch_bounds__(NT) void k_vof_detached_a(VofArgs<T> a, int mode, T
 
 // phase B: apply outgoing then incoming moves (fixed direction order)
 template <class T>
-__global__ __launch_bounds__(NT) void k_vof_detached_b(VofArgs<T> a) {
+__global__ __launch_bounds__(NT) void k_vof_detached_b(VofArgs<T> a, int mode) {
     VOF_PROLOGUE
+    // phase A moved nothing (no cell counted): every share is zero
+    if (a.icnt[mode == 0 ? IC_VOF_ADVECT : IC_VOF_COMPRESS] == 0) return;
     T out = T(0);
     for (int d = 0; d < 6; d++) out -= a.smv[6 * id + d];
     T in = T(0);
@@ -1533,14 +1535,32 @@ __global__ __launch_bounds__(NT) void k_vof_height(Consts<T> c, Slot<T> s, const
     // the select keeps depth unchanged), so the unrolled loop keeps several
     // planes in flight instead of one dependent load pair per plane
     T depth = T(0);
-#pragma unroll 8
-    for (int k = 0; k < c.nz; k++) {
-        const size_t id = (size_t)k * plane + col;
-        const bool open = flags[id] & 1u;
-        const T fl = fill[id];
-        bool by_cube;
-        const T add = fl * cell_center_fraction(c, s, i, j, k, by_cube) * c.dx;
-        depth = open ? depth + add : depth;
+    if (!c.include_cube && !c.phi_fraction) {
+        // the bed alone sets the centre fraction: cell_center_fraction with
+        // the column's bed height read once
+        const T eta_c = s.eta9[4 * plane + col];
+#pragma unroll 16
+        for (int k = 0; k < c.nz; k++) {
+            const size_t id = (size_t)k * plane + col;
+            const bool open = flags[id] & 1u;
+            const T fl = fill[id];
+            const T zc = ((T)k + T(0.5)) * c.dx;
+            const T gap = c.include_bed ? zc - eta_c : T(1.0e6);
+            T frac = T(0.5) + div_dx(c, gap);
+            if (frac < T(0)) frac = T(0);
+            if (frac > T(1)) frac = T(1);
+            const T add = fl * frac * c.dx;
+            depth = open ? depth + add : depth;
+        }
+    } else {
+        for (int k = 0; k < c.nz; k++) {
+            const size_t id = (size_t)k * plane + col;
+            const bool open = flags[id] & 1u;
+            const T fl = fill[id];
+            bool by_cube;
+            const T add = fl * cell_center_fraction(c, s, i, j, k, by_cube) * c.dx;
+            depth = open ? depth + add : depth;
+        }
     }
     fs_h[col] = s.h[col] + depth;
 }
@@ -2788,7 +2808,7 @@ static void vof_pipeline(const lbm_config_t& cfg, DevBufs& b, T* f, int slot, in
             k_vof_detached_a<T><<<nb, NT, 0, s>>>(A(), mode, othr, cfg.vof_detached_max_wet_neighbors,
                                                   cfg.vof_detached_max_resolved_neighbors,
                                                   (T)cfg.vof_detached_residual_fill_threshold);
-            k_vof_detached_b<T><<<nb, NT, 0, s>>>(A());
+            k_vof_detached_b<T><<<nb, NT, 0, s>>>(A(), mode);
         }
         finalize();
     }

@@ -245,7 +245,9 @@ int lbm_step(lbm_handle_t *h, int32_t nsteps);
 int lbm_upload_field(lbm_handle_t *h, int32_t field, const void *src, size_t bytes);
 int lbm_download_field(lbm_handle_t *h, int32_t field, void *dst, size_t bytes);
 
-/* `counter[None]` reads of the shape-() fields. */
+/* `counter[None]` reads of the shape-() fields.  As in the reference, each
+   counter describes the most recent step (the last step of the latest
+   lbm_step call); the stream's link counters are counted on that step only. */
 int lbm_get_counters(lbm_handle_t *h, lbm_counters_t *out);
 
 /* FP64 device reductions (the health print CODEX:3930-3955 and the mass /

@@ -403,7 +403,7 @@ int lbm_step(lbm_handle_t* h, int32_t nsteps) {
         if (h->timing) CHECK(hipEventRecord(h->ev[2 + 2 * s], h->stream));
         if (h->path == LBM_PATH_STAGED) {
             if (!h->post_stream) throw std::runtime_error("internal: staged step on a post-collision state");
-            launch_staged_step(c, h->b, h->cur, slot, step, h->omega_band_top, h->stream);
+            launch_staged_step(c, h->b, h->cur, slot, step, h->omega_band_top, s == nsteps - 1, h->stream);
         } else {
             int last = s == nsteps - 1;
             launch_fused_step(c, h->b, h->cur, h->post_stream ? 0 : 1, slot_of(step - 1), slot, step, last, h->omega_band_top,

@@ -72,7 +72,8 @@ void launch_geometry_fields(const lbm_config_t& c, DevBufs& b, int slot, int32_t
 void launch_cellflags(const lbm_config_t& c, DevBufs& b, int slot, hipStream_t s);
 void launch_init_populations(const lbm_config_t& c, DevBufs& b, int slot, hipStream_t s);
 // one staged step: bed arrays for slot must be current; advances b.f[cur]
-void launch_staged_step(const lbm_config_t& c, DevBufs& b, int& cur, int slot, int64_t step, int omega_band_top,
+// count_links: count the stream's link counters (the last step of a call)
+void launch_staged_step(const lbm_config_t& c, DevBufs& b, int& cur, int slot, int64_t step, int omega_band_top, int count_links,
                         hipStream_t s);
 // fused step: mode 0 reads post-stream populations, mode 1 post-collision
 // ones; slot_prev is the time level of the pending stream, slot the new one

@@ -656,10 +656,10 @@ __device__ __forceinline__ T src_value(const Consts<T>& c, const S* f, size_t n,
 
 // stream + apply_x_solid_wall_bounceback + apply_open_boundary_conditions
 // (CODEX:3418-3712) for one destination cell; fin = post-collision state.
-template <class S, class T>
+template <class S, class T, bool COUNT>
 __global__ __launch_bounds__(NT) void k_stream_bc(Consts<T> c, Slot<T> s, const S* fin, T* fout, const T* ux, const T* uy, const T* uz,
                             const int* type, const T* fill, const T* u_inlet, const T* inflow, const T* fs_h,
-                            const uint8_t* flags, int* icnt, int count) {
+                            const uint8_t* flags, int* icnt) {
     size_t id = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     size_t n = (size_t)c.nx * c.ny * c.nz;
     if (id >= n) return;
@@ -860,7 +860,7 @@ __global__ __launch_bounds__(NT) void k_stream_bc(Consts<T> c, Slot<T> s, const 
     }
 #pragma unroll
     for (int q = 0; q < 19; q++) fout[(size_t)q * n + id] = fo[q];
-    if (count) {
+    if (COUNT) {
         if (n_cut) atomicAdd(&icnt[IC_BOUNDARY_CUT], n_cut);
         if (n_fs) atomicAdd(&icnt[IC_FS_CUT], n_fs);
         if (n_bridge) atomicAdd(&icnt[IC_FS_BRIDGE], n_bridge);
@@ -2846,7 +2846,8 @@ static void vof_pipeline(const lbm_config_t& cfg, DevBufs& b, T* f, int slot, in
 }
 
 template <class T>
-static void staged_t(const lbm_config_t& cfg, DevBufs& b, int& cur, int slot, int64_t step, int omega_band_top, hipStream_t s) {
+static void staged_t(const lbm_config_t& cfg, DevBufs& b, int& cur, int slot, int64_t step, int omega_band_top, int count_links,
+                     hipStream_t s) {
     Consts<T> c = make_consts<T>(cfg, omega_band_top);
     Slot<T> sl = make_slot<T>(b, slot);
     const unsigned nb = nblocks(b.n);
@@ -2862,9 +2863,13 @@ static void staged_t(const lbm_config_t& cfg, DevBufs& b, int& cur, int slot, in
                                    b.type, b.cellflags);
     k_collide<T><<<nb, NT, 0, s>>>(c, f, (const T*)b.rho, (const T*)b.ux, (const T*)b.uy, (const T*)b.uz, (const T*)b.omega,
                                    b.type, b.cellflags, (uint32_t)step);
-    HIP_OK(hipMemsetAsync(b.icnt + IC_BOUNDARY_CUT, 0, 3 * sizeof(int), s));
-    k_stream_bc<T, T><<<nb, NT, 0, s>>>(c, sl, f, fn, (const T*)b.ux, (const T*)b.uy, (const T*)b.uz, b.type, (const T*)b.fill,
-                                        (const T*)b.u_inlet, (const T*)b.inflow, (const T*)b.fs_h, b.cellflags, b.icnt, 1);
+    // the link counters of stream describe one step and are overwritten by
+    // the next: they are counted on the last step of a call, the one a
+    // caller can observe
+    if (count_links) HIP_OK(hipMemsetAsync(b.icnt + IC_BOUNDARY_CUT, 0, 3 * sizeof(int), s));
+    auto stream = count_links ? k_stream_bc<T, T, true> : k_stream_bc<T, T, false>;
+    stream<<<nb, NT, 0, s>>>(c, sl, f, fn, (const T*)b.ux, (const T*)b.uy, (const T*)b.uz, b.type, (const T*)b.fill,
+                             (const T*)b.u_inlet, (const T*)b.inflow, (const T*)b.fs_h, b.cellflags, b.icnt);
     cur ^= 1;
     HIP_OK(hipGetLastError());
     if (cfg.fs_enabled) {
@@ -2886,12 +2891,12 @@ static void staged_t(const lbm_config_t& cfg, DevBufs& b, int& cur, int slot, in
         }
     }
 }
-void launch_staged_step(const lbm_config_t& c, DevBufs& b, int& cur, int slot, int64_t step, int omega_band_top,
+void launch_staged_step(const lbm_config_t& c, DevBufs& b, int& cur, int slot, int64_t step, int omega_band_top, int count_links,
                         hipStream_t s) {
     if (b.storage == LBM_STORE_FP64)
-        staged_t<double>(c, b, cur, slot, step, omega_band_top, s);
+        staged_t<double>(c, b, cur, slot, step, omega_band_top, count_links, s);
     else if (b.storage == LBM_STORE_FP32)
-        staged_t<float>(c, b, cur, slot, step, omega_band_top, s);
+        staged_t<float>(c, b, cur, slot, step, omega_band_top, count_links, s);
     else
         throw std::runtime_error("the staged path needs fp32 or fp64 storage");
 }
@@ -2963,9 +2968,9 @@ static void materialize_t(const lbm_config_t& cfg, DevBufs& b, int cur, int slot
     Consts<T> c = make_consts<T>(cfg, -1);
     Slot<T> sl = make_slot<T>(b, slot);
     k_cellflags<T><<<nblocks(b.n), NT, 0, s>>>(c, sl, b.cellflags);
-    k_stream_bc<S, T><<<nblocks(b.n), NT, 0, s>>>(c, sl, (const S*)b.f[cur], (T*)dst, (const T*)b.ux, (const T*)b.uy,
-                                                  (const T*)b.uz, b.type, (const T*)b.fill, (const T*)b.u_inlet,
-                                                  (const T*)b.inflow, (const T*)b.fs_h, b.cellflags, b.icnt, 0);
+    k_stream_bc<S, T, false><<<nblocks(b.n), NT, 0, s>>>(c, sl, (const S*)b.f[cur], (T*)dst, (const T*)b.ux, (const T*)b.uy,
+                                                         (const T*)b.uz, b.type, (const T*)b.fill, (const T*)b.u_inlet,
+                                                         (const T*)b.inflow, (const T*)b.fs_h, b.cellflags, b.icnt);
     HIP_OK(hipGetLastError());
 }
 void launch_materialize(const lbm_config_t& c, DevBufs& b, int cur, int slot, void* dst, hipStream_t s) {

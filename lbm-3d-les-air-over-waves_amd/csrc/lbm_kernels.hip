@@ -2773,6 +2773,23 @@ void launch_init_populations(const lbm_config_t& c, DevBufs& b, int slot, hipStr
     }
 }
 
+// the per-step counters of the VOF pipeline, cleared by one launch at its
+// start (each is only added to after its clear point in the step)
+__global__ void k_vof_counters_reset(int* icnt, double* dcnt) {
+    const int t = (int)threadIdx.x;
+    if (t == 0) {
+        icnt[IC_VOF_EXCESS] = 0;
+        icnt[IC_VOF_ORPHAN] = 0;
+        icnt[IC_VOF_ADVECT] = 0;
+        icnt[IC_VOF_COMPRESS] = 0;
+        icnt[IC_VOF_COLLAPSE_CELLS] = 0;
+        icnt[IC_FS_REFILL] = 0;
+        icnt[IC_FS_REPAIR] = 0;
+    }
+    if (t < 2) dcnt[DC_ADVECT_MASS + t] = 0.0;  // advected and compressed mass
+    if (t < 5) dcnt[DC_CAND_MASS + t] = 0.0;    // the collapse sums
+}
+
 template <class T>
 static void vof_pipeline(const lbm_config_t& cfg, DevBufs& b, T* f, int slot, int64_t step, hipStream_t s) {
     const size_t n = b.n;
@@ -2780,7 +2797,7 @@ static void vof_pipeline(const lbm_config_t& cfg, DevBufs& b, T* f, int slot, in
     auto A = [&]() { return make_vof<T>(cfg, b, f); };
     auto swap_mass = [&]() { std::swap(b.mass, b.mass2); };
     T empty_thr = (T)cfg.vof_empty_fill_threshold;
-    HIP_OK(hipMemsetAsync(b.icnt + IC_VOF_EXCESS, 0, sizeof(int), s));
+    k_vof_counters_reset<<<1, 64, 0, s>>>(b.icnt, b.dcnt);
     k_vof_exchange<T><<<nb, NT, 0, s>>>(A());
     k_vof_preclassify<T><<<nb, NT, 0, s>>>(A(), empty_thr);
     auto redistribute = [&]() {
@@ -2794,14 +2811,11 @@ static void vof_pipeline(const lbm_config_t& cfg, DevBufs& b, T* f, int slot, in
         finalize();
     }
     k_vof_enforce<T><<<nb, NT, 0, s>>>(A());
-    HIP_OK(hipMemsetAsync(b.icnt + IC_VOF_ORPHAN, 0, sizeof(int), s));
     T othr = (T)cfg.vof_orphan_empty_threshold;
     k_vof_orphan_a<T><<<nb, NT, 0, s>>>(A(), empty_thr, othr, cfg.vof_orphan_search_radius, cfg.vof_orphan_max_weak_neighbors);
     k_vof_orphan_b<T><<<nb, NT, 0, s>>>(A(), othr, cfg.vof_orphan_search_radius);
     swap_mass();
     k_vof_orphan_c<T><<<nb, NT, 0, s>>>(A());
-    HIP_OK(hipMemsetAsync(b.icnt + IC_VOF_ADVECT, 0, 2 * sizeof(int), s));
-    HIP_OK(hipMemsetAsync(b.dcnt + DC_ADVECT_MASS, 0, 2 * sizeof(double), s));
     for (int mode = 0; mode < 2; mode++) {
         bool on = cfg.vof_detached_advection && (mode == 0 || cfg.vof_detached_residual_fill_threshold > 0.0f);
         if (on) {
@@ -2816,8 +2830,6 @@ static void vof_pipeline(const lbm_config_t& cfg, DevBufs& b, T* f, int slot, in
     finalize();
     k_vof_enforce<T><<<nb, NT, 0, s>>>(A());
     // collapse_trapped_gas_if_due (CODEX:2129-2143)
-    HIP_OK(hipMemsetAsync(b.icnt + IC_VOF_COLLAPSE_CELLS, 0, sizeof(int), s));
-    HIP_OK(hipMemsetAsync(b.dcnt + DC_CAND_MASS, 0, 5 * sizeof(double), s));
     if (cfg.vof_collapse_trapped_gas && step % cfg.vof_collapse_interval == 0) {
         k_flood_seed<T><<<nb, NT, 0, s>>>(A());
         // sweep flags: flagbuf[0] = 1 primes the first sweep; each sweep
@@ -2839,7 +2851,6 @@ static void vof_pipeline(const lbm_config_t& cfg, DevBufs& b, T* f, int slot, in
         Consts<T> c = make_consts<T>(cfg, -1);
         k_vof_height<T><<<nblocks(b.n2, 64), 64, 0, s>>>(c, make_slot<T>(b, slot), (const T*)b.fill, b.cellflags, (T*)b.fs_h);
     }
-    HIP_OK(hipMemsetAsync(b.icnt + IC_FS_REFILL, 0, 2 * sizeof(int), s));
     k_vof_refill<T><<<nb, NT, 0, s>>>(A(), f);
     k_vof_repair<T><<<nb, NT, 0, s>>>(A(), f);
     HIP_OK(hipGetLastError());

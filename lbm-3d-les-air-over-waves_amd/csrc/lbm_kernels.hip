@@ -462,17 +462,10 @@ __global__ __launch_bounds__(NT) void k_cond(Consts<T> c, Slot<T> s, T* f, int* 
 }
 
 // macro_step (CODEX:2943-3008)
+// rho and u of one cell from its populations (macro_step, CODEX:2943-3008)
 template <class T>
-__global__ __launch_bounds__(NT) void k_macro(Consts<T> c, const T* f, T* rho, T* ux, T* uy, T* uz, const int* type, const uint8_t* flags) {
-    size_t id = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    size_t n = (size_t)c.nx * c.ny * c.nz;
-    if (id >= n) return;
-    int i, j, k;
-    cell_ijk(c, id, i, j, k);
-    T fl[19];
-#pragma unroll
-    for (int q = 0; q < 19; q++) fl[q] = f[(size_t)q * n + id];
-    T r = T(0);
+__device__ __forceinline__ void macro_cell(const Consts<T>& c, int k, const T fl[19], int ty, bool open0, T& r, T& u, T& v, T& w) {
+    r = T(0);
 #pragma unroll
     for (int q = 0; q < 19; q++) r += fl[q];
     T jx = T(0), jy = T(0), jz = T(0);
@@ -482,14 +475,12 @@ __global__ __launch_bounds__(NT) void k_macro(Consts<T> c, const T* f, T* rho, T
         if (cyq(q)) jy += fl[q] * (T)cyq(q);
         if (czq(q)) jz += fl[q] * (T)czq(q);
     }
-    T u, v, w;
     if (c.fs_water) {
-        int ty = type[id];
         if (ty == LBM_FS_GAS) {
             r = c.rho0;
             u = v = w = T(0);
         } else {
-            T force_z = -r * local_g(c, k, flags[id] & 1u, ty);
+            T force_z = -r * local_g(c, k, open0, ty);
             T inv = T(1.0) / r;
             u = jx * inv;
             v = jy * inv;
@@ -506,6 +497,20 @@ __global__ __launch_bounds__(NT) void k_macro(Consts<T> c, const T* f, T* rho, T
         v = T(0);
         w = T(0);
     }
+}
+
+template <class T>
+__global__ __launch_bounds__(NT) void k_macro(Consts<T> c, const T* f, T* rho, T* ux, T* uy, T* uz, const int* type, const uint8_t* flags) {
+    size_t id = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    size_t n = (size_t)c.nx * c.ny * c.nz;
+    if (id >= n) return;
+    int i, j, k;
+    cell_ijk(c, id, i, j, k);
+    T fl[19];
+#pragma unroll
+    for (int q = 0; q < 19; q++) fl[q] = f[(size_t)q * n + id];
+    T r, u, v, w;
+    macro_cell(c, k, fl, c.fs_water ? type[id] : LBM_FS_LIQUID, (flags[id] & 1u) != 0, r, u, v, w);
     rho[id] = r;
     ux[id] = u;
     uy[id] = v;
@@ -656,10 +661,14 @@ __device__ __forceinline__ T src_value(const Consts<T>& c, const S* f, size_t n,
 
 // stream + apply_x_solid_wall_bounceback + apply_open_boundary_conditions
 // (CODEX:3418-3712) for one destination cell; fin = post-collision state.
-template <class S, class T, bool COUNT>
+// COUNT: add the link counters; MACRO: also write rho and u of the streamed
+// populations (the macro step that follows stream in a water run, computed
+// from the values this thread stores; each cell reads only its own velocity
+// before that write)
+template <class S, class T, bool COUNT, bool MACRO>
 __global__ __launch_bounds__(NT) void k_stream_bc(Consts<T> c, Slot<T> s, const S* fin, T* fout, const T* ux, const T* uy, const T* uz,
                             const int* type, const T* fill, const T* u_inlet, const T* inflow, const T* fs_h,
-                            const uint8_t* flags, int* icnt) {
+                            const uint8_t* flags, int* icnt, T* mrho, T* mux, T* muy, T* muz) {
     size_t id = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     size_t n = (size_t)c.nx * c.ny * c.nz;
     if (id >= n) return;
@@ -860,6 +869,14 @@ __global__ __launch_bounds__(NT) void k_stream_bc(Consts<T> c, Slot<T> s, const 
     }
 #pragma unroll
     for (int q = 0; q < 19; q++) fout[(size_t)q * n + id] = fo[q];
+    if (MACRO) {
+        T r, u, v, w;
+        macro_cell(c, k, fo, ty, (flags[id] & 1u) != 0, r, u, v, w);
+        mrho[id] = r;
+        mux[id] = u;
+        muy[id] = v;
+        muz[id] = w;
+    }
     if (COUNT) {
         if (n_cut) atomicAdd(&icnt[IC_BOUNDARY_CUT], n_cut);
         if (n_fs) atomicAdd(&icnt[IC_FS_CUT], n_fs);
@@ -2878,14 +2895,17 @@ static void staged_t(const lbm_config_t& cfg, DevBufs& b, int& cur, int slot, in
     // the next: they are counted on the last step of a call, the one a
     // caller can observe
     if (count_links) HIP_OK(hipMemsetAsync(b.icnt + IC_BOUNDARY_CUT, 0, 3 * sizeof(int), s));
-    auto stream = count_links ? k_stream_bc<T, T, true> : k_stream_bc<T, T, false>;
+    // a free-surface run takes the macro step of the streamed populations
+    // inside the stream kernel
+    auto stream = cfg.fs_enabled ? (count_links ? k_stream_bc<T, T, true, true> : k_stream_bc<T, T, false, true>)
+                                 : (count_links ? k_stream_bc<T, T, true, false> : k_stream_bc<T, T, false, false>);
     stream<<<nb, NT, 0, s>>>(c, sl, f, fn, (const T*)b.ux, (const T*)b.uy, (const T*)b.uz, b.type, (const T*)b.fill,
-                             (const T*)b.u_inlet, (const T*)b.inflow, (const T*)b.fs_h, b.cellflags, b.icnt);
+                             (const T*)b.u_inlet, (const T*)b.inflow, (const T*)b.fs_h, b.cellflags, b.icnt, (T*)b.rho, (T*)b.ux,
+                             (T*)b.uy, (T*)b.uz);
     cur ^= 1;
     HIP_OK(hipGetLastError());
     if (cfg.fs_enabled) {
         T* f2 = (T*)b.f[cur];
-        k_macro<T><<<nb, NT, 0, s>>>(c, f2, (T*)b.rho, (T*)b.ux, (T*)b.uy, (T*)b.uz, b.type, b.cellflags);
         if (cfg.fs_tracking == LBM_FST_VOF) {
             vof_pipeline<T>(cfg, b, f2, slot, step, s);
         } else {
@@ -2979,9 +2999,10 @@ static void materialize_t(const lbm_config_t& cfg, DevBufs& b, int cur, int slot
     Consts<T> c = make_consts<T>(cfg, -1);
     Slot<T> sl = make_slot<T>(b, slot);
     k_cellflags<T><<<nblocks(b.n), NT, 0, s>>>(c, sl, b.cellflags);
-    k_stream_bc<S, T, false><<<nblocks(b.n), NT, 0, s>>>(c, sl, (const S*)b.f[cur], (T*)dst, (const T*)b.ux, (const T*)b.uy,
-                                                         (const T*)b.uz, b.type, (const T*)b.fill, (const T*)b.u_inlet,
-                                                         (const T*)b.inflow, (const T*)b.fs_h, b.cellflags, b.icnt);
+    k_stream_bc<S, T, false, false><<<nblocks(b.n), NT, 0, s>>>(c, sl, (const S*)b.f[cur], (T*)dst, (const T*)b.ux, (const T*)b.uy,
+                                                                (const T*)b.uz, b.type, (const T*)b.fill, (const T*)b.u_inlet,
+                                                                (const T*)b.inflow, (const T*)b.fs_h, b.cellflags, b.icnt,
+                                                                nullptr, nullptr, nullptr, nullptr);
     HIP_OK(hipGetLastError());
 }
 void launch_materialize(const lbm_config_t& c, DevBufs& b, int cur, int slot, void* dst, hipStream_t s) {

@@ -957,6 +957,21 @@ __device__ __forceinline__ size_t nbr_id(const Consts<T>& c, int i, int j, int k
     return valid ? ((size_t)nk * c.ny + nj) * c.nx + ni : self;
 }
 #define NBR_ID(di, dj, dk, valid) nbr_id(a.c, i, j, k, di, dj, dk, id, valid)
+
+// over links q0 .. q1-1: add the wet neighbours (open, not gas) to wet and
+// those of them with fill >= thr to res (loads issued unconditionally)
+template <class T>
+__device__ __forceinline__ void wet_resolved(const VofArgs<T>& a, int i, int j, int k, size_t id, int q0, int q1, T thr, int& wet,
+                                             int& res) {
+#pragma unroll
+    for (int q = q0; q < q1; q++) {
+        bool v;
+        const size_t nid = NBR_ID(cxq(q), cyq(q), czq(q), v);
+        const bool w = v & ((a.flags[nid] & 1u) != 0) & (a.type[nid] != LBM_FS_GAS);
+        wet += w;
+        res += w & (a.fill[nid] >= thr);
+    }
+}
 #define VOF_PROLOGUE                                                      \
     size_t id = (size_t)blockIdx.x * blockDim.x + threadIdx.x;            \
     size_t n = (size_t)a.c.nx * a.c.ny * a.c.nz;                          \
@@ -1239,14 +1254,9 @@ __global__ __launch_bounds__(NT) void k_vof_orphan_a(VofArgs<T> a, T empty_thr, 
     a.si[id] = 0;
     if (!((a.flags[id] & 1u) && a.type[id] != LBM_FS_GAS && a.fill[id] > empty_thr)) return;
     int wet = 0, res = 0;
-#pragma unroll
-    for (int q = 1; q < 19; q++) {
-        bool v;
-        const size_t nid = NBR_ID(cxq(q), cyq(q), czq(q), v);
-        const bool w = v & ((a.flags[nid] & 1u) != 0) & (a.type[nid] != LBM_FS_GAS);
-        wet += w;
-        res += w & (a.fill[nid] >= thr);
-    }
+    wet_resolved(a, i, j, k, id, 1, 7, thr, wet, res);  // the 6 face neighbours
+    if (res != 0) return;                                // a resolved neighbour: not an orphan
+    wet_resolved(a, i, j, k, id, 7, 19, thr, wet, res);  // the 12 edge neighbours
     if (!(res == 0 && (a.fill[id] < thr || wet <= max_weak))) return;
     T m = a.mass[id];
     T cw = T(0), tw = T(0);
@@ -1324,14 +1334,9 @@ __device__ bool vof_detached(const VofArgs<T>& a, int i, int j, int k, T orphan_
     size_t id = CID(k, j, i);
     if (!((a.flags[id] & 1u) && a.type[id] != LBM_FS_GAS && a.fill[id] > T(1.0e-4))) return false;
     int wet = 0, res = 0;
-#pragma unroll
-    for (int q = 1; q < 19; q++) {
-        bool v;
-        const size_t nid = NBR_ID(cxq(q), cyq(q), czq(q), v);
-        const bool w = v & ((a.flags[nid] & 1u) != 0) & (a.type[nid] != LBM_FS_GAS);
-        wet += w;
-        res += w & (a.fill[nid] >= orphan_thr);
-    }
+    wet_resolved(a, i, j, k, id, 1, 7, orphan_thr, wet, res);  // the 6 face neighbours
+    if (wet > max_wet || res > max_res) return false;           // the counts only grow: decided
+    wet_resolved(a, i, j, k, id, 7, 19, orphan_thr, wet, res);  // the 12 edge neighbours
     return wet <= max_wet && res <= max_res;
 }
 

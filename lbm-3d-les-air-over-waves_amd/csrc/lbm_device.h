@@ -285,6 +285,8 @@ __device__ __forceinline__ int pmod(int a, int n) {
     int r = a % n;
     return r < 0 ? r + n : r;
 }
+// pmod(a, n) for -n <= a < 2n (a neighbour one step away), without a division
+__device__ __forceinline__ int wrap1(int a, int n) { return a < 0 ? a + n : (a >= n ? a - n : a); }
 
 template <class T>
 __device__ __forceinline__ T smoothstep01(T x) {

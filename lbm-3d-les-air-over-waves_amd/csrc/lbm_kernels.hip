@@ -705,11 +705,11 @@ __global__ __launch_bounds__(NT) void k_stream_bc(Consts<T> c, Slot<T> s, const 
         si = sir;
         fluid = true;
         if (xper)
-            si = pmod(sir, nx);
+            si = wrap1(sir, nx);
         else if (!xio && (sir < 0 || sir >= nx))
             fluid = false;
         const int sjr = j - cyq(q);
-        sj = yper ? pmod(sjr, ny) : sjr;
+        sj = yper ? wrap1(sjr, ny) : sjr;
         if (!yper && (sjr < 0 || sjr >= ny)) fluid = false;
         sk = k - czq(q);
     };
@@ -952,6 +952,11 @@ __device__ __forceinline__ bool nbr(int nx, int ny, int nz, int xb, int yb, int 
 // every link can be issued unconditionally and combined afterwards
 template <class T>
 __device__ __forceinline__ size_t nbr_id(const Consts<T>& c, int i, int j, int k, int di, int dj, int dk, size_t self, bool& valid) {
+    // inside the grid on every axis (the usual case): a fixed offset
+    if ((unsigned)(i + di) < (unsigned)c.nx && (unsigned)(j + dj) < (unsigned)c.ny && (unsigned)(k + dk) < (unsigned)c.nz) {
+        valid = true;
+        return self + (ptrdiff_t)di + (ptrdiff_t)dj * c.nx + (ptrdiff_t)dk * ((ptrdiff_t)c.nx * c.ny);
+    }
     int ni, nj, nk;
     valid = nbr(c.nx, c.ny, c.nz, c.x_boundary, c.y_boundary, i, j, k, di, dj, dk, ni, nj, nk);
     return valid ? ((size_t)nk * c.ny + nj) * c.nx + ni : self;

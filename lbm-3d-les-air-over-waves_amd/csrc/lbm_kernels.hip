@@ -276,8 +276,8 @@ __global__ __launch_bounds__(9 * BED_COLS) void k_bed2d_b(Consts<T> c, const T* 
         }
     }
     if (e == 1) {
-        int ip = (i + 1) % c.nx, im = (i - 1 + c.nx) % c.nx;
-        int jp = (j + 1) % c.ny, jm = (j - 1 + c.ny) % c.ny;
+        int ip = wrap1(i + 1, c.nx), im = wrap1(i - 1, c.nx);
+        int jp = wrap1(j + 1, c.ny), jm = wrap1(j - 1, c.ny);
         T dh_dx = (h[(size_t)j * c.nx + ip] - h[(size_t)j * c.nx + im]) * (T(0.5) / dx);
         if (c.x_boundary != LBM_XB_PERIODIC) {
             if (i == 0)
@@ -530,7 +530,7 @@ __global__ __launch_bounds__(NT) void k_les(Consts<T> c, Slot<T> s, const T* ux,
     if (c.fs_water && type[id] == LBM_FS_GAS) active = false;
     if (!active) return;
     int nx = c.nx, ny = c.ny, nz = c.nz;
-    int ip = (i + 1) % nx, im = (i + nx - 1) % nx, jp = (j + 1) % ny, jm = (j + ny - 1) % ny;
+    int ip = wrap1(i + 1, nx), im = wrap1(i - 1, nx), jp = wrap1(j + 1, ny), jm = wrap1(j - 1, ny);
     int kp = k + 1 >= nz ? nz - 1 : k + 1, km = k - 1 < 0 ? 0 : k - 1;
     size_t pl = (size_t)nx * ny;
     size_t base = (size_t)k * pl;
@@ -1091,15 +1091,15 @@ __global__ __launch_bounds__(NT) void k_vof_redist_a(VofArgs<T> a) {
     int nx = a.c.nx, ny = a.c.ny, nz = a.c.nz;
     int ip = i + 1, im = i - 1, jp = j + 1, jm = j - 1;
     if (a.c.x_boundary == LBM_XB_PERIODIC) {
-        ip = (i + 1) % nx;
-        im = (i - 1 + nx) % nx;
+        ip = wrap1(i + 1, nx);
+        im = wrap1(i - 1, nx);
     } else {
         if (ip >= nx) ip = nx - 1;
         if (im < 0) im = 0;
     }
     if (a.c.y_boundary == LBM_YB_PERIODIC) {
-        jp = (j + 1) % ny;
-        jm = (j - 1 + ny) % ny;
+        jp = wrap1(j + 1, ny);
+        jm = wrap1(j - 1, ny);
     } else {
         if (jp >= ny) jp = ny - 1;
         if (jm < 0) jm = 0;

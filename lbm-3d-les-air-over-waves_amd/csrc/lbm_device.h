@@ -508,26 +508,33 @@ __device__ __forceinline__ T cell_center_fraction(const Consts<T>& c, const Slot
 }
 
 template <class T>
-__device__ __forceinline__ void cell_geometry(const Consts<T>& c, const Slot<T>& s, int i, int j, int k, CellGeo<T>& g) {
+// exact_walls: also the box bit of every open link (the wall type of each
+// link, for the geometry download); the step kernels read the wall type of
+// closed links only
+__device__ __forceinline__ void cell_geometry(const Consts<T>& c, const Slot<T>& s, int i, int j, int k, CellGeo<T>& g,
+                                              bool exact_walls = false) {
     const size_t col = (size_t)j * c.nx + i;
     const size_t plane = (size_t)c.nx * c.ny;
     const T dx = c.dx;
     T zc = ((T)k + T(0.5)) * dx;
     const T xc = ((T)i + T(0.5)) * dx, yc = ((T)j + T(0.5)) * dx;
     uint32_t mask = 0, xw = 0, cube = 0;
-    // a cell at least 2 dx above the highest of its 9 bed samples, with the
-    // bed as the only surface: every link gap is >= dx/2 after rounding, so
-    // every bed fraction clamps to 1 and every bed link is open; the solid
-    // x / y wall overrides below then close the links leaving the domain
-    // (the same values the per-link evaluation gives, without its bed work)
-    if (!c.include_cube && !c.phi_fraction) {
+    // a cell at least 2 dx above the highest of its 9 bed samples and at
+    // least 2 dx outside the box: every link endpoint lies within 0.71 dx of
+    // the centre and both distances change by at most that much (the box
+    // distance is a Euclidean distance), so every gap is >= dx/2, every
+    // fraction clamps to 1 and every link is open; the solid x / y wall
+    // overrides below then close the links leaving the domain (the values of
+    // the per-link evaluation, without its bed and box work)
+    if (!c.phi_fraction && !(c.include_cube && exact_walls)) {
         T e9[9];
 #pragma unroll
         for (int e = 0; e < 9; e++) e9[e] = s.eta9[(size_t)e * plane + col];  // issued together
         bool far = true;  // false for a NaN sample, which takes the per-link path
 #pragma unroll
         for (int e = 0; e < 9; e++) far = far & (zc - e9[e] >= T(2.0) * dx);
-        if (!c.include_bed || far) {
+        const bool far_box = !c.include_cube || cube_sdf(c, xc, yc, zc) >= T(2.0) * dx;
+        if ((!c.include_bed || far) && far_box) {
             g.lf[0] = T(1);
 #pragma unroll
             for (int q = 1; q < 19; q++) {

@@ -354,7 +354,7 @@ __global__ __launch_bounds__(NT) void k_geometry_fields(Consts<T> c, Slot<T> s, 
     int i, j, k;
     cell_ijk(c, id, i, j, k);
     CellGeo<T> g;
-    cell_geometry(c, s, i, j, k, g);
+    cell_geometry(c, s, i, j, k, g, true);
     for (int q = 0; q < 19; q++) {
         lo[(size_t)q * n + id] = (g.open_mask >> q) & 1u;
         lf[(size_t)q * n + id] = g.lf[q];

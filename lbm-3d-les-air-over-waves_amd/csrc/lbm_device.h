@@ -50,7 +50,7 @@ __host__ __device__ inline void fast_div_magic(uint32_t d, uint32_t& m, int& s) 
     while (((uint64_t)1 << s) < d) s++;
     m = (uint32_t)(((uint64_t)1 << (31 + s)) / d + 1);
 }
-__device__ __forceinline__ uint32_t fast_div(uint32_t u, uint32_t m, int s) {
+__host__ __device__ inline uint32_t fast_div(uint32_t u, uint32_t m, int s) {
     return (uint32_t)(((uint64_t)u * m) >> (31 + s));
 }
 // ---- storage types -------------------------------------------------------

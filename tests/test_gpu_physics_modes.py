@@ -44,3 +44,28 @@ def test_physics_mode_parity(name):
     assert np.max(np.abs(c.download(abi.FIELD_UX))) > 0.0
     c.close()
     o.close()
+
+
+def test_link_counters_describe_the_last_step():
+    """The stream's link counters are counted on the last step of a call:
+    after step(5) they equal those after five step(1) calls, and those of
+    the oracle after the same steps (which counts every step)."""
+    env = CASES["air_height_surface"]
+    cfg, o, a = pair(env)
+    from harness import setup_case
+    from lbm_native import Case
+
+    b = Case(cfg.to_struct())
+    setup_case(b, cfg)
+    a.step(5)
+    for _ in range(5):
+        b.step(1)
+    o.step(5)
+    keys = ("boundary_cut_link_count", "free_surface_cut_link_count", "free_surface_thin_gap_bridge_count")
+    ca, cb, co = a.counters(), b.counters(), o.counters()
+    for key in keys:
+        assert ca[key] == cb[key] == co[key], (key, ca[key], cb[key], co[key])
+    assert ca["free_surface_cut_link_count"] > 0
+    assert np.array_equal(a.download(abi.FIELD_F), b.download(abi.FIELD_F))
+    for c in (a, b, o):
+        c.close()

@@ -264,6 +264,8 @@ def main():
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--workload", default="waves_256x128x128")
+    ap.add_argument("--storage", default=None, choices=("fp32", "fp16", "bf16", "fp64"),
+                    help="population storage of the main run (default: the workload's)")
     ap.add_argument("--no-variants", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--serving-cases", type=int, default=64, help="requests per precision of the serving stream (0: skip)")
@@ -294,6 +296,8 @@ def main():
         return
     env = preset_env(args.workload)
     env["LBM_KERNEL_PATH"] = "auto"
+    if args.storage:
+        env["LBM_STORAGE"] = args.storage
     cfg = parse_env(env)
     main_run = run_device(env, args.steps, args.warmup)
     bpl = BYTES_PER_LU[cfg.storage]

@@ -13,7 +13,8 @@ Prints ONE JSON line:
                            traffic = HBM bytes per step measured by rocprofv3
                            counters (profiles/fused_step_<workload>.md)
   cpu_baseline             the CPU restatement of the same step (oracle,
-                           single thread) on a bounded sample of the workload
+                           OpenMP over cells, all host threads; and one
+                           thread) on a bounded sample of the workload
   variants                 FP16 / BF16 storage and the regularized collision
                            on the same workload
   workloads                the plain channel at 64x32x32 and 512x256x192
@@ -97,32 +98,44 @@ def run_device(env: dict, steps: int, warmup: int):
     return out
 
 
-def cpu_baseline(env: dict, budget_s: float = 15.0):
-    """Oracle (CPU restatement, one thread) on a bounded sample: as many full
-    steps of the workload as fit in the budget (at least one)."""
+def cpu_baseline(env: dict, budget_s: float = 12.0):
+    """Oracle (the C restatement, OpenMP over cells) on a bounded sample: as
+    many full steps of the workload as fit in the budget (at least one),
+    with all its threads (the host's cores, or OMP_NUM_THREADS) and with one
+    thread.  The results do not depend on the thread count."""
     from harness import setup_case
     from lbm_config import parse_env
-    from oracle_ref import OracleCase
+    from oracle_ref import OracleCase, set_threads
 
     cfg = parse_env(env)
-    o = OracleCase(cfg.to_struct())
-    setup_case(o, cfg)
     cells = cfg.Nx * cfg.Ny * cfg.Nz
-    n = 0
-    t0 = time.perf_counter()
-    while True:
-        o.step(1)
-        n += 1
-        if time.perf_counter() - t0 > budget_s:
-            break
-    wall = time.perf_counter() - t0
-    o.close()
+
+    def timed(threads):
+        used = set_threads(threads)
+        o = OracleCase(cfg.to_struct())
+        setup_case(o, cfg)
+        n = 0
+        t0 = time.perf_counter()
+        while True:
+            o.step(1)
+            n += 1
+            if time.perf_counter() - t0 > budget_s:
+                break
+        wall = time.perf_counter() - t0
+        o.close()
+        return used, n, cells * n / wall / 1e6
+
+    threads, n_all, mlups_all = timed(0)
+    _, n_one, mlups_one = timed(1)
+    set_threads(0)
     return {
-        "value": round(cells * n / wall / 1e6, 3),
+        "value": round(mlups_all, 3),
         "unit": "MLUPS",
-        "cores": 1,
+        "cores": threads,
         "kind": "port",
-        "sample": f"{n} full step(s) of the {cfg.Nx}x{cfg.Ny}x{cfg.Nz} workload on the CPU oracle (C, -O2, 1 thread)",
+        "sample": f"{n_all} full step(s) of the {cfg.Nx}x{cfg.Ny}x{cfg.Nz} workload on the CPU oracle (C, -O2, OpenMP, "
+                  f"{threads} threads)",
+        "single_thread": {"value": round(mlups_one, 3), "unit": "MLUPS", "sample": f"{n_one} full step(s), 1 thread"},
     }
 
 

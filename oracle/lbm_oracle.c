@@ -41,6 +41,9 @@
 #include <string.h>
 
 #include "../include/lbm.h"
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 
 #ifndef LBM_REAL
 #define LBM_REAL float
@@ -462,6 +465,7 @@ static void build_lattice_open(O *o) {
     REAL dx = R(c->dx);
     int solid_x = c->x_boundary == LBM_XB_SOLID;
     int solid_y = c->y_boundary == LBM_YB_SOLID;
+#pragma omp parallel for collapse(3) schedule(static)
     for (int k = 0; k < o->nz; k++)
         for (int j = 0; j < o->ny; j++)
             for (int i = 0; i < o->nx; i++) {
@@ -670,6 +674,7 @@ static void bed_populations(O *o) {
         o->obst_prev[id] = o->obst[id];
         o->obst[id] = LO(0, id) == 0 ? 1 : 0;
     }
+#pragma omp parallel for collapse(3) schedule(static)
     for (int k = 0; k < o->nz; k++)
         for (int j = 0; j < o->ny; j++)
             for (int i = 0; i < o->nx; i++) {
@@ -695,11 +700,13 @@ static void bed_populations(O *o) {
 /* macro_step (CODEX:2943-3008) */
 static void macro(O *o) {
     const lbm_config_t *c = &o->c;
+#pragma omp parallel for schedule(static)
     for (size_t id = 0; id < o->n; id++) {
         REAL r = R(0);
         for (int q = 0; q < 19; q++) r += F(q, id);
         o->rho[id] = r;
     }
+#pragma omp parallel for schedule(static)
     for (size_t id = 0; id < o->n; id++) {
         REAL jx = R(0), jy = R(0), jz = R(0);
         for (int q = 0; q < 19; q++) {
@@ -744,6 +751,7 @@ static void les(O *o) {
     REAL csd = R(c->cs_les) * R(c->delta_les);
     REAL cs2 = csd * csd;
     REAL nu0 = R(c->nu0);
+#pragma omp parallel for collapse(3) schedule(static)
     for (int k = 0; k < o->nz; k++)
         for (int j = 0; j < o->ny; j++)
             for (int i = 0; i < o->nx; i++) {
@@ -932,6 +940,7 @@ static void collide_kbc(O *o) {
     REAL cs2 = R(1.0) / R(3.0);
     REAL eps = R(1e-6);
     REAL om_sp = R(1.0) / R(0.95);
+#pragma omp parallel for collapse(3) schedule(static)
     for (int k = 0; k < o->nz; k++)
         for (int j = 0; j < o->ny; j++)
             for (int i = 0; i < o->nx; i++) {
@@ -1033,6 +1042,7 @@ static void collide_reg(O *o) {
     REAL cs2 = R(1.0) / R(3.0);
     REAL inv2 = R(1.0) / (R(2.0) * cs2 * cs2);
     REAL eps = R(1e-6);
+#pragma omp parallel for collapse(3) schedule(static)
     for (int k = 0; k < o->nz; k++)
         for (int j = 0; j < o->ny; j++)
             for (int i = 0; i < o->nx; i++) {
@@ -1118,9 +1128,8 @@ static void stream(O *o) {
     int nx = o->nx, ny = o->ny, nz = o->nz;
     REAL rho0 = R(c->rho0);
     lbm_counters_t *cn = &o->cnt;
-    cn->boundary_cut_link_count = 0;
-    cn->free_surface_cut_link_count = 0;
-    cn->free_surface_thin_gap_bridge_count = 0;
+    int n_cut = 0, n_fs = 0, n_bridge = 0;  /* integer sums: independent of the thread split */
+#pragma omp parallel for collapse(3) schedule(static) reduction(+ : n_cut, n_fs, n_bridge)
     for (int k = 0; k < nz; k++)
         for (int j = 0; j < ny; j++)
             for (int i = 0; i < nx; i++) {
@@ -1160,7 +1169,7 @@ static void stream(O *o) {
                             FN(q, id) = feq(o, rho0, R(0), R(0), R(0), q);
                         } else if (!fluid_i) {
                             FN(q, id) = F(qb, id);
-                            cn->boundary_cut_link_count++;
+                            n_cut++;
                         } else if (sk >= 0 && sk < nz && !src_gas) {
                             REAL frac = LF(q, id);
                             REAL fstr = frac * src_value(o, si, sj, sk, q) + (R(1.0) - frac) * F(qb, id);
@@ -1175,14 +1184,14 @@ static void stream(O *o) {
                                     REAL fb = feq(o, rfs, o->ux[id], o->uy[id], o->uz[id], qb);
                                     REAL ffs = fa + fb - F(qb, id);
                                     fout = la * fstr + (R(1.0) - la) * ffs;
-                                    cn->free_surface_cut_link_count++;
+                                    n_fs++;
                                 }
                             }
                             FN(q, id) = fout;
-                            if (frac < R(1.0)) cn->boundary_cut_link_count++;
+                            if (frac < R(1.0)) n_cut++;
                         } else if (sk < 0) {
                             FN(q, id) = F(qb, id);
-                            cn->boundary_cut_link_count++;
+                            n_cut++;
                         } else {
                             REAL rfs = rho0;
                             if (use_hbp(o) || c->fs_boundary_mode == LBM_FSB_HYDROSTATIC)
@@ -1213,21 +1222,21 @@ static void stream(O *o) {
                                         if (bw > R(0)) {
                                             if (bw > R(1)) bw = R(1);
                                             fout = (R(1.0) - bw) * ffs + bw * F(q, bid);
-                                            cn->free_surface_thin_gap_bridge_count++;
+                                            n_bridge++;
                                         }
                                     }
                                 }
                             }
                             FN(q, id) = fout;
-                            cn->free_surface_cut_link_count++;
+                            n_fs++;
                         }
                     } else if (!fluid_i) {
                         FN(q, id) = F(qb, id);
-                        cn->boundary_cut_link_count++;
+                        n_cut++;
                     } else if (sk >= 0 && sk < nz) {
                         REAL frac = LF(q, id);
                         FN(q, id) = frac * src_value(o, si, sj, sk, q) + (R(1.0) - frac) * F(qb, id);
-                        if (frac < R(1.0)) cn->boundary_cut_link_count++;
+                        if (frac < R(1.0)) n_cut++;
                     } else if (sk >= nz && c->top_boundary == LBM_TOP_FREE_SLIP) {
                         /* new: specular reflection at the top face */
                         FN(q, id) = src_value(o, si, sj, nz - 1, MIRZ[q]);
@@ -1236,6 +1245,9 @@ static void stream(O *o) {
                     }
                 }
             }
+    cn->boundary_cut_link_count = n_cut;
+    cn->free_surface_cut_link_count = n_fs;
+    cn->free_surface_thin_gap_bridge_count = n_bridge;
 }
 
 /* apply_x_solid_wall_bounceback (CODEX:3554-3580) */
@@ -2542,6 +2554,24 @@ double API(orc_reduce)(void *p, int what) {
         for (size_t id = 0; id < o->n; id++) s += (double)o->fill[id];
     }
     return s;
+}
+
+/* Threads of the per-cell loops, for the CPU-baseline timing (the results do
+   not depend on it: every parallel loop writes only its own cell, and its
+   counters are integer sums). */
+void API(orc_set_threads)(int n) {
+#ifdef _OPENMP
+    if (n > 0) omp_set_num_threads(n);
+#else
+    (void)n;
+#endif
+}
+int API(orc_get_threads)(void) {
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
 }
 
 /* Run one reference kernel by itself (unit tests pin each against closed

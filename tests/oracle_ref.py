@@ -66,8 +66,29 @@ def load_oracle():
         for fn in ("orc_det_sin", "orc_det_cos", "orc_det_log", "orc_det_exp"):
             g(fn).argtypes = [ctypes.c_double]
             g(fn).restype = ctypes.c_double
+        g("orc_set_threads").argtypes = [ctypes.c_int]
+        g("orc_set_threads").restype = None
+        g("orc_get_threads").argtypes = []
+        g("orc_get_threads").restype = ctypes.c_int
     _lib = lib
     return lib
+
+
+_default_threads = None
+
+
+def set_threads(n: int) -> int:
+    """Threads of the oracle's per-cell loops (results do not depend on
+    it; n <= 0 restores the count the library started with); returns the
+    count now in effect."""
+    global _default_threads
+    lib = load_oracle()
+    if _default_threads is None:
+        _default_threads = int(lib.orc_get_threads_f32())
+    n = int(n) if n > 0 else _default_threads
+    lib.orc_set_threads_f32(n)
+    lib.orc_set_threads_f64(n)
+    return int(lib.orc_get_threads_f32())
 
 
 def round16(half: bool, stochastic: bool, x: np.ndarray, r: np.ndarray) -> np.ndarray:

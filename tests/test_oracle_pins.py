@@ -362,3 +362,27 @@ def test_moving_wall_momentum_sign():
     corr = np.corrcoef(ux[fluid_near], np.broadcast_to(uw[None], ux.shape)[fluid_near])[0, 1]
     assert corr > 0.5
     o.close()
+
+
+@pytest.mark.parametrize("preset,extra", [
+    ("waves_256x128x128", dict(LBM_LX_M=48, LBM_LY_M=8, LBM_LZ_M=16, LBM_BED_AMP_M=2, LBM_BED_WAVELENGTH_M=24)),
+    ("vof_192x96x160", dict(LBM_LX_M=24, LBM_LY_M=8, LBM_LZ_M=24, LBM_FREE_SURFACE_DEPTH_M=10, LBM_VOF_BLOCK_SIZE_M=4,
+                            LBM_VOF_BLOCK_BOTTOM_M=14)),
+])
+def test_oracle_is_independent_of_its_thread_count(preset, extra):
+    """The oracle's parallel loops write only their own cell and sum integer
+    counters: one thread and several give the same bits."""
+    from harness import env_of, make_oracle
+    from oracle_ref import set_threads
+
+    env = env_of(preset, **extra)
+    runs = []
+    for threads in (1, 4):
+        set_threads(threads)
+        cfg, o = make_oracle(env)
+        o.step(12)
+        runs.append((o.download(abi.FIELD_F).copy(), o.download(abi.FIELD_UX).copy(), o.counters()))
+        o.close()
+    set_threads(0)
+    assert np.array_equal(runs[0][0], runs[1][0]) and np.array_equal(runs[0][1], runs[1][1])
+    assert runs[0][2] == runs[1][2]

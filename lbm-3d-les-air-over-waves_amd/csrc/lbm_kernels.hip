@@ -2976,16 +2976,27 @@ static void fused_t(const lbm_config_t& cfg, DevBufs& b, int& cur, int mode, int
         a.plane0 = nb;
         k_moments<S, T, false><<<dim3(gx, gy, cfg.nz - nb), blk, 0, s>>>(a);
     }
+    // the collision model is fixed at compile time unless the top sponge or
+    // backscatter can select another one per cell
+    const int cm = cfg.collision == LBM_COLL_REGULARIZED                         ? CM_REG
+                   : (!cfg.enable_top_sponge && !(cfg.c_backscatter > 0.0f)) ? CM_KBC
+                                                                               : CM_ANY;
     if (nb > 0) {
         a.plane0 = 0;
-        k_collide<S, T, true, CM_ANY><<<dim3(gx, gy, nb), blk, 0, s>>>(a);
+        const dim3 grd(gx, gy, nb);
+        if (cm == CM_REG)
+            k_collide<S, T, true, CM_REG><<<grd, blk, 0, s>>>(a);
+        else if (cm == CM_KBC)
+            k_collide<S, T, true, CM_KBC><<<grd, blk, 0, s>>>(a);
+        else
+            k_collide<S, T, true, CM_ANY><<<grd, blk, 0, s>>>(a);
     }
     if (nb < cfg.nz) {
         a.plane0 = nb;
         const dim3 grd(gx, gy, cfg.nz - nb);
-        if (cfg.collision == LBM_COLL_REGULARIZED)
+        if (cm == CM_REG)
             k_collide<S, T, false, CM_REG><<<grd, blk, 0, s>>>(a);
-        else if (!cfg.enable_top_sponge && !(cfg.c_backscatter > 0.0f))
+        else if (cm == CM_KBC)
             k_collide<S, T, false, CM_KBC><<<grd, blk, 0, s>>>(a);
         else
             k_collide<S, T, false, CM_ANY><<<grd, blk, 0, s>>>(a);

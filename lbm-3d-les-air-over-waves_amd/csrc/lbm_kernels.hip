@@ -251,8 +251,12 @@ __global__ __launch_bounds__(9 * BED_COLS) void k_bed2d_b(Consts<T> c, const T* 
     // conditioning of update_bed_populations_and_reinit (CODEX:2894-2940)
     // gives a closed link, feq(rho0, wall velocity of the link), and the
     // moving-wall term of the IBB (CODEX:3674-3712); x / y wall links have
-    // wall velocity 0.  Wave e takes links e, e + 9, e + 18.
-    for (int q = e; q < 19; q += 9) {
+    // wall velocity 0.  Wave e takes links e, e + 9, e + 18 (e is uniform over
+    // a wave, so each link's code is compiled with its q fixed).
+    const int ew = __builtin_amdgcn_readfirstlane(e);
+#pragma unroll
+    for (int q = 0; q < 19; q++) {
+        if (q % 9 != ew) continue;
         int wt = c.include_bed ? LBM_WALL_BED : LBM_WALL_NONE;
         if (q > 0) wt = link_wt(c, link_on_wall(c, i, j, q), i, j, q);
         T ux = T(0), uy = T(0), uz = T(0);

@@ -55,6 +55,9 @@ CASES = {
     "reference_air": {"LBM_LX_M": "64", "LBM_LY_M": "16", "LBM_LZ_M": "24", "LBM_DX_M": "1", "LBM_BACKSCATTER": "0.02",
                       "LBM_SEED": "7", "LBM_TAU": "0.55"},
     "solid_y": env_of("plain_64x32x32", LBM_Y_BOUNDARY="solid", **PLAIN_SMALL),
+    # wall velocity of a closed link taken at the cell centre instead of the
+    # link endpoint (CODEX:937-988)
+    "waves_cell_wall_velocity": env_of("waves_256x128x128", LBM_WALL_VELOCITY_SAMPLING="cell", **WAVES_SMALL),
 }
 GEOM_FIELDS = [
     abi.FIELD_LATTICE_OPEN,

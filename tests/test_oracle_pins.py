@@ -364,6 +364,41 @@ def test_moving_wall_momentum_sign():
     o.close()
 
 
+def test_deep_solid_cells_hold_the_wall_equilibrium():
+    """A solid cell whose 18 neighbours are solid too has every link closed:
+    the conditioning sets it to feq(rho0, u_w) (CODEX:2894-2940), the
+    collision keeps an equilibrium, the stream bounces back the opposite
+    population, feq_qb(u_w), and the IBB with open fraction 0 subtracts
+    6 w rho0 (c_qb . u_w) (CODEX:3674-3712).  Together that is exactly
+    feq_q(rho0, u_w) when both rules take the wall velocity at the cell
+    centre (LBM_WALL_VELOCITY_SAMPLING=cell); sampled per link endpoint, the
+    two terms use different velocities and the identity does not hold."""
+    errs = {}
+    for mode in ("cell", "link"):
+        env = env_of("waves_256x128x128", LBM_LX_M=32, LBM_LY_M=8, LBM_LZ_M=16, LBM_BED_AMP_M=1.5,
+                     LBM_BED_WAVELENGTH_M=32, LBM_BED_LEVEL_M=6.3, LBM_X_BOUNDARY="periodic",
+                     LBM_WALL_VELOCITY_SAMPLING=mode)
+        cfg, o = make_oracle(env, fp64=True)
+        o.step(7)
+        f = o.download(abi.FIELD_F)
+        solid = o.download(abi.FIELD_LATTICE_OPEN).sum(axis=0) == 0
+        u, v, w = (o.download(x) for x in (abi.FIELD_U_WAVE, abi.FIELD_V_WAVE, abi.FIELD_W_WAVE))
+        o.close()
+        deep = solid.copy()
+        for q in range(1, 19):
+            deep &= np.roll(solid, shift=(-CZ[q], -CY[q], -CX[q]), axis=(0, 1, 2))
+        deep[0] = deep[-1] = False
+        assert deep.sum() > 100
+        err = 0.0
+        for q in range(19):
+            cu = 3.0 * (CX[q] * u + CY[q] * v + CZ[q] * w)
+            feq = cfg.rho0 * W[q] * (1.0 + cu + 0.5 * cu * cu - 1.5 * (u * u + v * v + w * w))
+            err = max(err, float(np.max(np.abs(f[q] - feq[None])[deep])))
+        errs[mode] = err
+    assert errs["cell"] < 1e-14, errs
+    assert errs["link"] > 1e-5, errs
+
+
 @pytest.mark.parametrize("preset,extra", [
     ("waves_256x128x128", dict(LBM_LX_M=48, LBM_LY_M=8, LBM_LZ_M=16, LBM_BED_AMP_M=2, LBM_BED_WAVELENGTH_M=24)),
     ("vof_192x96x160", dict(LBM_LX_M=24, LBM_LY_M=8, LBM_LZ_M=24, LBM_FREE_SURFACE_DEPTH_M=10, LBM_VOF_BLOCK_SIZE_M=4,

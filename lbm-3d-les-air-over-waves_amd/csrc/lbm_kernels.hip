@@ -217,6 +217,8 @@ __global__ __launch_bounds__(NT) void k_bed2d_a(Consts<T> c, T t, T* h, T* uw, T
 // column (4 fields), then the waves share the per-link terms and wave 0
 // derives the per-column bounds.
 constexpr int BED_COLS = 64;
+// one wave per endpoint: the per-link loop reads e from the first lane
+static_assert(BED_COLS == 64, "k_bed2d_b needs one 64-lane wave per endpoint");
 template <class T>
 __global__ __launch_bounds__(9 * BED_COLS) void k_bed2d_b(Consts<T> c, const T* h, T* eta9, T* denom, int* kband, const T* uw,
                                                           const T* vw, const T* ww, T* uw9, int* ksolid, T* fsol, T* ibb) {

@@ -18,6 +18,7 @@
 #include <algorithm>
 #include <stdexcept>
 #include <string>
+#include <type_traits>
 
 #include "lbm_device.h"
 #include "lbm_internal.h"
@@ -1920,6 +1921,7 @@ struct FusedArgs {
     int mode;       // 0: fin holds post-stream populations (no stream)
     int write_macro;
     int plane0;  // first plane of this launch
+    int band_planes;  // k_collide_all: planes 0 .. band_planes - 1 take the band rules
     uint32_t step;
 };
 
@@ -2567,6 +2569,19 @@ __global__ __launch_bounds__(FB_THREADS, (FusedOcc<T, BAND>::k2)) void k_collide
     collide_cell<S, T, BAND, CM>(a, i, j, k);
 }
 
+// both collision forms in one launch over every plane: the branch on k is
+// uniform over a block, so each block runs one form, and the step saves the
+// boundary between two launches (the two forms need about the same registers)
+template <class S, class T, int CM>
+__global__ __launch_bounds__(FB_THREADS, 6) void k_collide_all(FusedArgs<S, T> a) {
+    int i, j, k;
+    if (!fused_cell(a, i, j, k)) return;
+    if (k < a.band_planes)
+        collide_cell<S, T, true, CM>(a, i, j, k);
+    else
+        collide_cell<S, T, false, CM>(a, i, j, k);
+}
+
 // ===========================================================================
 // reductions (FP64)
 // ===========================================================================
@@ -2987,25 +3002,41 @@ static void fused_t(const lbm_config_t& cfg, DevBufs& b, int& cur, int mode, int
     const int cm = cfg.collision == LBM_COLL_REGULARIZED                         ? CM_REG
                    : (!cfg.enable_top_sponge && !(cfg.c_backscatter > 0.0f)) ? CM_KBC
                                                                                : CM_ANY;
-    if (nb > 0) {
-        a.plane0 = 0;
-        const dim3 grd(gx, gy, nb);
-        if (cm == CM_REG)
-            k_collide<S, T, true, CM_REG><<<grd, blk, 0, s>>>(a);
-        else if (cm == CM_KBC)
-            k_collide<S, T, true, CM_KBC><<<grd, blk, 0, s>>>(a);
-        else
-            k_collide<S, T, true, CM_ANY><<<grd, blk, 0, s>>>(a);
+    a.band_planes = nb;
+    bool merged = false;
+    // FP32 storage only: the FP16 / BF16 forms spill in the merged kernel
+    if constexpr (std::is_same<S, float>::value && std::is_same<T, float>::value) {
+        if (cm != CM_ANY && nb > 0 && nb < cfg.nz) {
+            a.plane0 = 0;
+            const dim3 grd(gx, gy, cfg.nz);
+            if (cm == CM_REG)
+                k_collide_all<S, T, CM_REG><<<grd, blk, 0, s>>>(a);
+            else
+                k_collide_all<S, T, CM_KBC><<<grd, blk, 0, s>>>(a);
+            merged = true;
+        }
     }
-    if (nb < cfg.nz) {
-        a.plane0 = nb;
-        const dim3 grd(gx, gy, cfg.nz - nb);
-        if (cm == CM_REG)
-            k_collide<S, T, false, CM_REG><<<grd, blk, 0, s>>>(a);
-        else if (cm == CM_KBC)
-            k_collide<S, T, false, CM_KBC><<<grd, blk, 0, s>>>(a);
-        else
-            k_collide<S, T, false, CM_ANY><<<grd, blk, 0, s>>>(a);
+    if (!merged) {
+        if (nb > 0) {
+            a.plane0 = 0;
+            const dim3 grd(gx, gy, nb);
+            if (cm == CM_REG)
+                k_collide<S, T, true, CM_REG><<<grd, blk, 0, s>>>(a);
+            else if (cm == CM_KBC)
+                k_collide<S, T, true, CM_KBC><<<grd, blk, 0, s>>>(a);
+            else
+                k_collide<S, T, true, CM_ANY><<<grd, blk, 0, s>>>(a);
+        }
+        if (nb < cfg.nz) {
+            a.plane0 = nb;
+            const dim3 grd(gx, gy, cfg.nz - nb);
+            if (cm == CM_REG)
+                k_collide<S, T, false, CM_REG><<<grd, blk, 0, s>>>(a);
+            else if (cm == CM_KBC)
+                k_collide<S, T, false, CM_KBC><<<grd, blk, 0, s>>>(a);
+            else
+                k_collide<S, T, false, CM_ANY><<<grd, blk, 0, s>>>(a);
+        }
     }
     HIP_OK(hipGetLastError());
     cur ^= 1;

@@ -46,6 +46,7 @@ struct lbm_handle {
     int wall_top = 0;
     double piecewise_max = 0.0;
     int block_x = 64;  // x extent of a fused-kernel block (64, 128 or 256 cells; set from nx at creation)
+    FusedSched sched;  // fused path: wavefront (one launch per step) or three launches
     int initialized = 0;
     int geometry_done = 0;
     int timing = 0;
@@ -199,6 +200,10 @@ static void allocate(lbm_handle* h) {
     } else {
         b.obst = (int*)dalloc(h, n * 4);  // init writes the t=0 mask here
         b.obst_prev = nullptr;
+        // error flag + one flag per moments tile
+        const size_t wq = wave_flag_ints(c.nx, c.ny, c.nz);
+        b.waveq = (int*)dalloc(h, wq * sizeof(int));
+        CHECK(hipMemsetAsync(b.waveq, 0, wq * sizeof(int), h->stream));
     }
     for (int s = 0; s < 2; s++) {
         b.h[s] = dalloc(h, n2 * R);
@@ -307,6 +312,22 @@ static int fail(lbm_handle* h, int code, const std::string& msg) {
     return code;
 }
 
+// the wavefront step's bounded waits leave a flag when one ran out (the
+// step's results are then not valid); read and cleared after the stream
+// has drained
+static void check_wave_flag(lbm_handle* h) {
+    if (!h->b.waveq) return;
+    int flag = 0;
+    CHECK(hipMemcpyAsync(&flag, h->b.waveq, sizeof(int), hipMemcpyDeviceToHost, h->stream));
+    CHECK(hipStreamSynchronize(h->stream));
+    if (flag) {
+        int zero = 0;
+        CHECK(hipMemcpyAsync(h->b.waveq, &zero, sizeof(int), hipMemcpyHostToDevice, h->stream));
+        CHECK(hipStreamSynchronize(h->stream));
+        throw std::runtime_error("wavefront step: a wait for the moments of a neighbouring plane ran out; the state is not valid");
+    }
+}
+
 extern "C" {
 
 int lbm_create(const lbm_config_t* cfg, lbm_handle_t** out) {
@@ -406,8 +427,15 @@ int lbm_step(lbm_handle_t* h, int32_t nsteps) {
             launch_staged_step(c, h->b, h->cur, slot, step, h->omega_band_top, s == nsteps - 1, h->stream);
         } else {
             int last = s == nsteps - 1;
+            if (h->sched.epoch >= INT32_MAX - 1) {
+                // the tile flags hold launch numbers: start them over
+                const size_t wq = wave_flag_ints(c.nx, c.ny, c.nz);
+                CHECK(hipMemsetAsync(h->b.waveq + WAVE_FLAG0, 0, (wq - WAVE_FLAG0) * sizeof(int), h->stream));
+                h->sched.epoch = 0;
+            }
+            h->sched.epoch++;
             launch_fused_step(c, h->b, h->cur, h->post_stream ? 0 : 1, slot_of(step - 1), slot, step, last, h->omega_band_top,
-                              h->wall_top, h->block_x, h->stream);
+                              h->wall_top, h->block_x, h->sched, h->stream);
             h->post_stream = 0;
         }
         if (h->timing) CHECK(hipEventRecord(h->ev[3 + 2 * s], h->stream));
@@ -612,6 +640,7 @@ int lbm_download_field(lbm_handle_t* h, int32_t field, void* dst, size_t bytes) 
     if (field == LBM_FIELD_F) src = post_stream_populations(h);
     CHECK(hipMemcpyAsync(dst, src, need, hipMemcpyDeviceToHost, h->stream));
     CHECK(hipStreamSynchronize(h->stream));
+    check_wave_flag(h);
     API_END(h)
 }
 
@@ -664,6 +693,7 @@ int lbm_reduce(lbm_handle_t* h, int32_t what, double* out) {
     double v = 0.0;
     CHECK(hipMemcpyAsync(&v, h->b.dcnt + DC_REDUCE, sizeof(double), hipMemcpyDeviceToHost, h->stream));
     CHECK(hipStreamSynchronize(h->stream));
+    check_wave_flag(h);
     *out = v;
     API_END(h)
 }
@@ -677,6 +707,7 @@ int lbm_get_step(lbm_handle_t* h, int64_t* out) {
 int lbm_sync(lbm_handle_t* h) {
     API_BEGIN(h)
     CHECK(hipStreamSynchronize(h->stream));
+    check_wave_flag(h);
     API_END(h)
 }
 
@@ -713,6 +744,15 @@ int lbm_set_option(lbm_handle_t* h, const char* key, int64_t value) {
     if (!h || !key) return LBM_E_ARG;
     if (!strcmp(key, "timing")) {
         h->timing = value ? 1 : 0;
+        return LBM_OK;
+    }
+    if (!strcmp(key, "wavefront")) {
+        h->sched.wavefront = value ? 1 : 0;
+        return LBM_OK;
+    }
+    if (!strcmp(key, "wave_lag")) {
+        if (value < 1 || value > 4096) return fail(h, LBM_E_ARG, "wave_lag must be in [1, 4096]");
+        h->sched.lag = (int)value;
         return LBM_OK;
     }
     if (!strcmp(key, "block_x")) {

@@ -63,6 +63,31 @@ struct DevBufs {
     int *type, *type_prev, *excess_dir, *ext_gas, *obst, *obst_prev, *si;
     int* icnt;      // [IC_COUNT]
     double* dcnt;   // [DC_COUNT]
+    int* waveq;     // fused path, wavefront step: [0] error flag, [WAVE_FLAG0 + t] epoch of moments tile t
+};
+constexpr int WAVE_FLAG0 = 64;  // first tile flag (the error flag keeps a cache line of its own)
+// tiles per plane of the fused kernels (block of 256 cells, block_x cells in x)
+inline size_t fused_tiles_per_plane(int nx, int ny, int block_x) {
+    const int by = 256 / block_x;
+    return (size_t)((nx + block_x - 1) / block_x) * (size_t)((ny + by - 1) / by);
+}
+// ints of the wavefront flag buffer: room for the tiles of any block_x
+inline size_t wave_flag_ints(int nx, int ny, int nz) {
+    size_t m = 0;
+    for (int bx = 64; bx <= 256; bx *= 2) {
+        const size_t t = fused_tiles_per_plane(nx, ny, bx);
+        m = t > m ? t : m;
+    }
+    return WAVE_FLAG0 + (size_t)nz * m;
+}
+
+// schedule of the fused step: wavefront (both passes in one launch, tiles
+// taken in ticket order) or three launches; lag = planes of moments ahead
+// of the collision
+struct FusedSched {
+    int wavefront = 1;  // FP32 storage (the other storage types keep three launches)
+    int lag = 8;
+    int epoch = 0;  // launches so far (the tile flags of launch e hold e)
 };
 
 // host-side launchers (lbm_kernels.hip)
@@ -79,7 +104,7 @@ void launch_staged_step(const lbm_config_t& c, DevBufs& b, int& cur, int slot, i
 // ones; slot_prev is the time level of the pending stream, slot the new one
 // band_top: highest k at which any cell can have a cut or closed link
 void launch_fused_step(const lbm_config_t& c, DevBufs& b, int& cur, int mode, int slot_prev, int slot, int64_t step,
-                       int write_macro, int omega_band_top, int band_top, int block_x, hipStream_t s);
+                       int write_macro, int omega_band_top, int band_top, int block_x, const FusedSched& fs, hipStream_t s);
 // post-collision state -> post-stream populations (arithmetic type) in dst
 void launch_materialize(const lbm_config_t& c, DevBufs& b, int cur, int slot, void* dst, hipStream_t s);
 // convert arithmetic-type populations in src into the storage buffer f[cur]

@@ -4,12 +4,13 @@
 //  * staged: one kernel per stage of the reference step loop
 //    (LBM_3D_LES_air_over_waves_CODEX.py:3873-3925), whole fields in HBM,
 //    used for water/VOF runs and as the cross-check of the fused path;
-//  * fused: one z-marching pass per step for air runs.  The state between
-//    steps is the post-collision population set f*; one pass streams it
-//    (with every boundary rule), conditions the wall links of the next
-//    step, forms the moments, evaluates the LES viscosity from velocity
-//    planes held in LDS, collides and writes f* of the next step.  Each
-//    population is read once and written once per step.
+//  * fused: air runs.  The state between steps is the post-collision
+//    population set f*; a moments pass streams it (with every boundary
+//    rule), conditions the wall links of the next step and writes the
+//    velocity; a collision pass repeats the pull, evaluates the LES
+//    viscosity from the neighbours' velocities, collides and writes f* of
+//    the next step.  With FP32 storage both passes run in one launch whose
+//    tiles wait on per-tile flags (the wavefront schedule).
 //
 // Arithmetic follows the CPU restatement operation for operation (see
 // lbm_device.h), so results agree bit for bit in FP32 and FP64 storage.
@@ -2443,7 +2444,23 @@ struct FusedOcc {
     static constexpr int k2 = sizeof(T) == 8 ? 2 : (BAND ? 4 : 5);
 };
 
-template <class S, class T, bool BAND>
+// velocity / open-flag exchange between the passes: plain accesses between
+// two launches, device-coherent ones (agent scope, no cache flush) inside
+// the wavefront launch, where the reader may run on another XCD
+template <bool COH, class V>
+__device__ __forceinline__ void xput(V* p, V v) {
+    if (COH)
+        __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    else
+        *p = v;
+}
+template <bool COH, class V>
+__device__ __forceinline__ V xget(const V* p) {
+    if (COH) return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return *p;
+}
+
+template <class S, class T, bool BAND, bool COH = false>
 __device__ __forceinline__ void moments_cell(const FusedArgs<S, T>& a, int i, int j, int k) {
     const Consts<T>& c = a.c;
     T f[19];
@@ -2455,10 +2472,10 @@ __device__ __forceinline__ void moments_cell(const FusedArgs<S, T>& a, int i, in
     T rr, un, vn, wn;
     fused_moments(c, k, f, rr, un, vn, wn);
     const uint32_t id = ((uint32_t)k * c.ny + j) * c.nx + i;
-    a.ux[id] = un;
-    a.uy[id] = vn;
-    a.uz[id] = wn;
-    if (BAND) a.open[id] = open0 ? 1 : 0;
+    xput<COH>(a.ux + id, un);
+    xput<COH>(a.uy + id, vn);
+    xput<COH>(a.uz + id, wn);
+    if (BAND) xput<COH>(a.open + id, (uint8_t)(open0 ? 1 : 0));
 }
 
 template <class S, class T, bool BAND>
@@ -2537,7 +2554,7 @@ __device__ __forceinline__ void les_collide_store(const FusedArgs<S, T>& a, int 
     }
 }
 
-template <class S, class T, bool BAND, int CM>
+template <class S, class T, bool BAND, int CM, bool COH = false>
 __device__ __forceinline__ void collide_cell(const FusedArgs<S, T>& a, int i, int j, int k) {
     const Consts<T>& c = a.c;
     const int nx = c.nx, ny = c.ny, nz = c.nz;
@@ -2551,14 +2568,14 @@ __device__ __forceinline__ void collide_cell(const FusedArgs<S, T>& a, int i, in
         lean_gather(a, i, j, k, f);
     T rcur, ucur, vcur, wcur;
     fused_moments(c, k, f, rcur, ucur, vcur, wcur);
-    const bool open_below = BAND && k > 0 ? a.open[id - plane] != 0 : true;
+    const bool open_below = BAND && k > 0 ? xget<COH>(a.open + (id - plane)) != 0 : true;
     const int ip = i == nx - 1 ? 0 : i + 1, im = i == 0 ? nx - 1 : i - 1;
     const int jp = j == ny - 1 ? 0 : j + 1, jm = j == 0 ? ny - 1 : j - 1;
     const uint32_t row = (uint32_t)k * plane + (uint32_t)j * nx;
     const uint32_t nbi[6] = {row + ip, row + im, (uint32_t)k * plane + (uint32_t)jp * nx + i,
                              (uint32_t)k * plane + (uint32_t)jm * nx + i, (k + 1 < nz ? id + plane : id), (k > 0 ? id - plane : id)};
     const T* u3[3] = {a.ux, a.uy, a.uz};
-    auto nbu = [&](int comp, int dir) { return u3[comp][nbi[dir]]; };
+    auto nbu = [&](int comp, int dir) { return xget<COH>(u3[comp] + nbi[dir]); };
     les_collide_store<CM>(a, i, j, k, id, f, open_cur, open_below, rcur, ucur, vcur, wcur, nbu);
 }
 
@@ -2580,6 +2597,134 @@ __global__ __launch_bounds__(FB_THREADS, 6) void k_collide_all(FusedArgs<S, T> a
         collide_cell<S, T, true, CM>(a, i, j, k);
     else
         collide_cell<S, T, false, CM>(a, i, j, k);
+}
+
+// ---------------------------------------------------------------------------
+// Wavefront form of the fused step: both passes in one launch
+// ---------------------------------------------------------------------------
+//
+// One launch runs both passes.  Block t is one block-sized tile (FB_THREADS
+// cells of one plane) of one pass, and the block indices run through the
+// planes in the order
+//   M(0) .. M(L-1),  M(L) C(0),  M(L+1) C(1),  ...,  C(nz-L) .. C(nz-1)
+// (M: moments pass, C: collision pass, L: lag in planes).  The collision of
+// plane k thus re-reads the populations that the moments of planes
+// k-1 .. k+1 read a few planes earlier, while they are still held in the L2
+// or the memory-side cache, and the step has no launch boundary between the
+// passes.
+//
+// A collision tile needs the velocities (and, on band planes, the open
+// flags) of the moments tiles around it: the 3 x 3 tiles of its own plane
+// and the tile above and below.  Each moments tile publishes with a flag of
+// its own (no read-modify-write): its stores of u and of the flag are
+// device-coherent (agent scope, no cache flush), and the flag is written
+// once the stores are acknowledged.  The flag holds the launch's epoch, so
+// no launch needs a reset.  A collision tile polls its at most 11 flags.
+// Every tile it waits on has a lower block index; the hardware dispatches
+// the blocks of each XCD in increasing index order, so the lowest
+// unfinished block is always resident and its waits end.  Each wait is also
+// bounded: one that runs out sets an error flag (reported by the host) and
+// every later wait of the launch gives up at once.
+struct WaveQ {
+    int* flag;   // [nz * bpp] epoch of the last launch whose moments tile is done
+    int* err;    // [0] a wait ran out
+    int epoch;   // this launch (>= 1)
+    int lag;     // planes of moments ahead of the collision
+    int bpp;     // tiles per plane
+    int gx, gy;  // tiles per row of tiles (x), rows of tiles (y)
+    int bx;      // tile extent in x (64, 128 or 256)
+};
+constexpr int WAVE_SPIN_LIMIT = 1 << 22;  // s_sleep(2) polls: a few hundred milliseconds
+
+// pass (0 moments, 1 collision) and plane of the u-th plane unit
+__device__ __forceinline__ void wave_unit(int u, int L, int nz, int& pass, int& k) {
+    if (u < L) {
+        pass = 0;
+        k = u;
+        return;
+    }
+    int v = u - L;
+    const int npair = nz - L;
+    if (v < 2 * npair) {
+        pass = v & 1;
+        k = (v >> 1) + (pass ? 0 : L);
+        return;
+    }
+    pass = 1;
+    k = npair + (v - 2 * npair);
+}
+
+// flag index polled by lane l (0 .. 10) of a collision tile, -1 for none:
+// lanes 0..8 the 3 x 3 tiles of plane k, 9 and 10 the tiles below / above
+__device__ __forceinline__ int wave_dep(const WaveQ& w, int nz, int k, int bxi, int byi, int l) {
+    int kk = k, tx = bxi, ty = byi;
+    if (l < 9) {
+        const int dx = l % 3 - 1, dy = l / 3 - 1;
+        tx += dx;
+        ty += dy;
+        // the LES differences wrap in x and y whatever the boundaries
+        // (CODEX:3047-3053)
+        if (tx < 0) tx = w.gx - 1;
+        if (tx >= w.gx) tx = 0;
+        if (ty < 0) ty = w.gy - 1;
+        if (ty >= w.gy) ty = 0;
+    } else if (l < 11) {
+        kk = k + (l == 9 ? -1 : 1);
+        if (kk < 0 || kk >= nz) return -1;
+    } else {
+        return -1;
+    }
+    return kk * w.bpp + ty * w.gx + tx;
+}
+
+template <class S, class T, int CM>
+__global__ __launch_bounds__(FB_THREADS, 6) void k_step_wave(FusedArgs<S, T> a, WaveQ w) {
+    const int tid = (int)threadIdx.x;
+    const int nz = a.c.nz;
+    const int t = (int)blockIdx.x;
+    const int u = t / w.bpp, b = t - u * w.bpp;
+    int pass, k;
+    wave_unit(u, w.lag, nz, pass, k);
+    const int byi = b / w.gx, bxi = b - byi * w.gx;
+    const int i = bxi * w.bx + tid % w.bx, j = byi * (FB_THREADS / w.bx) + tid / w.bx;
+    const bool in = i < a.c.nx && j < a.c.ny;
+    const bool band = k < a.band_planes;
+    if (pass == 0) {
+        if (in) {
+            if (band)
+                moments_cell<S, T, true, true>(a, i, j, k);
+            else
+                moments_cell<S, T, false, true>(a, i, j, k);
+        }
+        // publish: every wave waits until its device-coherent stores are
+        // acknowledged, then one thread writes the tile's flag
+        __builtin_amdgcn_s_waitcnt(0);
+        __syncthreads();
+        if (tid == 0) __hip_atomic_store(&w.flag[k * w.bpp + b], w.epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    } else {
+        if (tid < 64) {
+            const int d = wave_dep(w, nz, k, bxi, byi, tid);
+            bool wait = d >= 0;
+            int spins = 0;
+            while (__builtin_amdgcn_ballot_w64(wait) != 0) {
+                if (wait) wait = __hip_atomic_load(&w.flag[d], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != w.epoch;
+                if (__builtin_amdgcn_ballot_w64(wait) == 0) break;
+                if (++spins > WAVE_SPIN_LIMIT || __hip_atomic_load(w.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
+                    if (tid == 0) __hip_atomic_store(w.err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(2);
+            }
+        }
+        // the velocity and flag loads after this barrier are device-coherent
+        __syncthreads();
+        if (in) {
+            if (band)
+                collide_cell<S, T, true, CM, true>(a, i, j, k);
+            else
+                collide_cell<S, T, false, CM, true>(a, i, j, k);
+        }
+    }
 }
 
 // ===========================================================================
@@ -2965,7 +3110,7 @@ void launch_staged_step(const lbm_config_t& c, DevBufs& b, int& cur, int slot, i
 
 template <class S, class T>
 static void fused_t(const lbm_config_t& cfg, DevBufs& b, int& cur, int mode, int slot_prev, int slot, int64_t step, int write_macro,
-                    int omega_band_top, int band_top, int block_x, hipStream_t s) {
+                    int omega_band_top, int band_top, int block_x, const FusedSched& fs, hipStream_t s) {
     FusedArgs<S, T> a;
     a.c = make_consts<T>(cfg, omega_band_top);
     a.s0 = make_slot<T>(b, slot_prev);
@@ -2989,6 +3134,31 @@ static void fused_t(const lbm_config_t& cfg, DevBufs& b, int& cur, int mode, int
     const int bxd = block_x > 0 ? block_x : 64;
     const dim3 blk(bxd, FB_THREADS / bxd);
     const unsigned gx = (cfg.nx + blk.x - 1) / blk.x, gy = (cfg.ny + blk.y - 1) / blk.y;
+    // the collision model is fixed at compile time unless the top sponge or
+    // backscatter can select another one per cell
+    const int cm = cfg.collision == LBM_COLL_REGULARIZED                         ? CM_REG
+                   : (!cfg.enable_top_sponge && !(cfg.c_backscatter > 0.0f)) ? CM_KBC
+                                                                               : CM_ANY;
+    a.band_planes = nb;
+    // FP32 storage only: with FP64 the merged kernel spills to scratch, and
+    // the VALU-bound 16-bit forms ran 20-30 % slower merged (SGPR spills)
+    if (fs.wavefront && std::is_same<S, float>::value) {
+        WaveQ w;
+        w.flag = b.waveq + WAVE_FLAG0;
+        w.err = b.waveq;
+        w.epoch = fs.epoch;
+        w.lag = std::max(1, std::min(fs.lag, cfg.nz));
+        w.bx = bxd;
+        w.gx = (int)gx;
+        w.gy = (int)gy;
+        w.bpp = (int)(gx * gy);
+        auto kern = cm == CM_REG ? k_step_wave<S, T, CM_REG> : (cm == CM_KBC ? k_step_wave<S, T, CM_KBC> : k_step_wave<S, T, CM_ANY>);
+        a.plane0 = 0;
+        kern<<<2 * cfg.nz * w.bpp, FB_THREADS, 0, s>>>(a, w);
+        HIP_OK(hipGetLastError());
+        cur ^= 1;
+        return;
+    }
     if (nb > 0) {
         a.plane0 = 0;
         k_moments<S, T, true><<<dim3(gx, gy, nb), blk, 0, s>>>(a);
@@ -2997,12 +3167,6 @@ static void fused_t(const lbm_config_t& cfg, DevBufs& b, int& cur, int mode, int
         a.plane0 = nb;
         k_moments<S, T, false><<<dim3(gx, gy, cfg.nz - nb), blk, 0, s>>>(a);
     }
-    // the collision model is fixed at compile time unless the top sponge or
-    // backscatter can select another one per cell
-    const int cm = cfg.collision == LBM_COLL_REGULARIZED                         ? CM_REG
-                   : (!cfg.enable_top_sponge && !(cfg.c_backscatter > 0.0f)) ? CM_KBC
-                                                                               : CM_ANY;
-    a.band_planes = nb;
     bool merged = false;
     // FP32 storage only: the FP16 / BF16 forms spill in the merged kernel
     if constexpr (std::is_same<S, float>::value && std::is_same<T, float>::value) {
@@ -3042,15 +3206,17 @@ static void fused_t(const lbm_config_t& cfg, DevBufs& b, int& cur, int mode, int
     cur ^= 1;
 }
 void launch_fused_step(const lbm_config_t& c, DevBufs& b, int& cur, int mode, int slot_prev, int slot, int64_t step,
-                       int write_macro, int omega_band_top, int band_top, int block_x, hipStream_t s) {
+                       int write_macro, int omega_band_top, int band_top, int block_x, const FusedSched& fs, hipStream_t s) {
     if (!fused_supported(c)) throw std::runtime_error("fused path: needs air mode without a free surface");
     switch (b.storage) {
-        case LBM_STORE_FP64: fused_t<double, double>(c, b, cur, mode, slot_prev, slot, step, write_macro, omega_band_top, band_top, block_x, s); break;
-        case LBM_STORE_FP16: fused_t<__half, float>(c, b, cur, mode, slot_prev, slot, step, write_macro, omega_band_top, band_top, block_x, s); break;
-        case LBM_STORE_BF16: fused_t<bf16_t, float>(c, b, cur, mode, slot_prev, slot, step, write_macro, omega_band_top, band_top, block_x, s); break;
-        default: fused_t<float, float>(c, b, cur, mode, slot_prev, slot, step, write_macro, omega_band_top, band_top, block_x, s); break;
+        case LBM_STORE_FP64: fused_t<double, double>(c, b, cur, mode, slot_prev, slot, step, write_macro, omega_band_top, band_top, block_x, fs, s); break;
+        case LBM_STORE_FP16: fused_t<__half, float>(c, b, cur, mode, slot_prev, slot, step, write_macro, omega_band_top, band_top, block_x, fs, s); break;
+        case LBM_STORE_BF16: fused_t<bf16_t, float>(c, b, cur, mode, slot_prev, slot, step, write_macro, omega_band_top, band_top, block_x, fs, s); break;
+        default: fused_t<float, float>(c, b, cur, mode, slot_prev, slot, step, write_macro, omega_band_top, band_top, block_x, fs, s); break;
     }
 }
+
+
 
 template <class S, class T>
 static void materialize_t(const lbm_config_t& cfg, DevBufs& b, int cur, int slot, void* dst, hipStream_t s) {

@@ -1,0 +1,81 @@
+"""The wavefront schedule of the fused step (both passes in one launch, the
+collision of a tile waiting on the moments tiles around it) against the
+three-launch schedule: the same arithmetic in another order of tiles, so
+the states agree bit for bit, on every collision model and on ragged grids.
+FP32 storage only: the 16-bit storage forms keep the three launches (the
+option is accepted and has no effect there, which the fp16 / bf16 cases
+check)."""
+import numpy as np
+import pytest
+
+import lbm_abi as abi
+from harness import env_of, setup_case
+from lbm_config import parse_env
+
+pytestmark = pytest.mark.gpu
+
+SMALL = dict(LBM_LX_M=64, LBM_LY_M=16, LBM_LZ_M=24, LBM_BED_AMP_M=2, LBM_BED_WAVELENGTH_M=32)
+CASES = {
+    "waves_kbc": env_of("waves_256x128x128", **SMALL),
+    "waves_reg": env_of("waves_256x128x128", LBM_COLLISION="regularized", **SMALL),
+    "plain_sponge": env_of("plain_64x32x32", LBM_COLLISION="kbc", LBM_ENABLE_TOP_SPONGE=1, LBM_TOP_BOUNDARY="keep",
+                           LBM_LX_M=32, LBM_LY_M=16, LBM_LZ_M=16),
+    "ragged": env_of("waves_256x128x128", LBM_LX_M=70, LBM_LY_M=13, LBM_LZ_M=19, LBM_BED_AMP_M=2, LBM_BED_WAVELENGTH_M=35),
+    "waves_fp16": env_of("waves_256x128x128", LBM_STORAGE="fp16", **SMALL),
+    "waves_bf16": env_of("waves_256x128x128", LBM_STORAGE="bf16", **SMALL),
+}
+
+
+def run(env, steps, **opts):
+    from lbm_native import Case
+
+    cfg = parse_env(dict(env, LBM_KERNEL_PATH="fused"))
+    c = Case(cfg.to_struct())
+    for k, v in opts.items():
+        c.set_option(k, v)
+    setup_case(c, cfg)
+    c.step(steps)
+    out = {fid: c.download(fid) for fid in (abi.FIELD_F, abi.FIELD_UX, abi.FIELD_UY, abi.FIELD_UZ, abi.FIELD_RHO, abi.FIELD_OMEGA)}
+    c.close()
+    return out
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_wavefront_equals_three_launches(name):
+    print("case", name, flush=True)
+    env = CASES[name]
+    ref = run(env, 30, wavefront=0)
+    for lag in (1, 8, 64):
+        got = run(env, 30, wavefront=1, wave_lag=lag)
+        for fid, a in ref.items():
+            assert np.array_equal(got[fid], a), (name, lag, fid, int(np.sum(got[fid] != a)))
+
+
+@pytest.mark.parametrize("block_x", [64, 128, 256])
+def test_wavefront_tile_shapes(block_x):
+    """Tiles of 64 x 4, 128 x 2 and 256 x 1 cells: several tiles per row and
+    per plane, so the flags of x- and y-neighbouring tiles (wrapped at the
+    edges) are what the collision waits on."""
+    env = CASES["ragged"]
+    ref = run(env, 20, wavefront=0, block_x=block_x)
+    got = run(env, 20, wavefront=1, wave_lag=2, block_x=block_x)
+    for fid, a in ref.items():
+        assert np.array_equal(got[fid], a), (block_x, fid, int(np.sum(got[fid] != a)))
+
+
+def test_wavefront_counters_reset_between_calls():
+    """Several lbm_step calls of different lengths in a row: each launch must
+    find the counters it needs at zero (the last tile resets them)."""
+    env = CASES["waves_kbc"]
+    ref = run(env, 1 + 2 + 7, wavefront=0)
+    from lbm_native import Case
+
+    cfg = parse_env(dict(env, LBM_KERNEL_PATH="fused"))
+    c = Case(cfg.to_struct())
+    setup_case(c, cfg)
+    for n in (1, 2, 7):
+        c.step(n)
+        c.sync()
+    f = c.download(abi.FIELD_F)
+    c.close()
+    assert np.array_equal(f, ref[abi.FIELD_F])

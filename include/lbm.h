@@ -251,7 +251,9 @@ int lbm_download_field(lbm_handle_t *h, int32_t field, void *dst, size_t bytes);
 int lbm_get_counters(lbm_handle_t *h, lbm_counters_t *out);
 
 /* FP64 device reductions (the health print CODEX:3930-3955 and the mass /
-   volume diagnostics of run_ab_case.py:286-679). */
+   volume diagnostics of run_ab_case.py:286-679).  Deterministic: fixed
+   groups of cells and a fixed summation tree (partial sums in a buffer of
+   the handle); a post-collision state is streamed on the fly. */
 int lbm_reduce(lbm_handle_t *h, int32_t what, double *out);
 
 /* Number of steps advanced so far. */
@@ -299,7 +301,8 @@ typedef struct lbm_server lbm_server_t;
 int lbm_server_create(int32_t chunk_steps, int32_t n_workers, lbm_server_t **out);
 
 /* Hand over an initialised case to advance by nsteps; ticket is the caller's
-   identifier, reported back by lbm_server_poll. */
+   identifier, reported back by lbm_server_poll.  LBM_E_STATE if the handle
+   is already held by the server (a job on it has not finished). */
 int lbm_server_add(lbm_server_t *s, lbm_handle_t *h, int32_t nsteps, int64_t ticket);
 
 /* Move up to max finished tickets into tickets[] (*n_done of them) together

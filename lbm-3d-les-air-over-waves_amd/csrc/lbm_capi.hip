@@ -231,6 +231,7 @@ static void allocate(lbm_handle* h) {
     b.dcnt = (double*)dalloc(h, DC_COUNT * sizeof(double));
     CHECK(hipMemsetAsync(b.icnt, 0, IC_COUNT * sizeof(int), h->stream));
     CHECK(hipMemsetAsync(b.dcnt, 0, DC_COUNT * sizeof(double), h->stream));
+    b.rpart = (double*)dalloc(h, 3 * reduce_chunks(n) * sizeof(double));
     if (c.fs_enabled) {
         b.fill = dalloc(h, n * R);
         b.mass = dalloc(h, n * R);
@@ -676,25 +677,29 @@ int lbm_get_counters(lbm_handle_t* h, lbm_counters_t* out) {
     API_END(h)
 }
 
-int lbm_reduce(lbm_handle_t* h, int32_t what, double* out) {
-    API_BEGIN(h)
+static int reduce_checks(lbm_handle* h, int32_t what, const double* out) {
     if (!out) return fail(h, LBM_E_ARG, "out is NULL");
     if (what < 0 || what > LBM_REDUCE_NONFINITE) return fail(h, LBM_E_ARG, "unknown reduction");
     if ((what == LBM_REDUCE_VOF_MASS || what == LBM_REDUCE_VOF_VOLUME) && !h->cfg.fs_enabled)
         return fail(h, LBM_E_ARG, "VOF reductions need a water run");
+    return LBM_OK;
+}
+
+// the reduction into the device slot dst; part: partial sums (max_parts)
+static void reduce_into(lbm_handle* h, int32_t what, double* part, size_t max_parts, double* dst, double* out) {
     launch_cellflags(h->cfg, h->b, slot_of(h->step), h->stream);
-    const void* f = h->b.f[h->cur];
-    int is_storage = 1;
-    if (what == LBM_REDUCE_MASS || what == LBM_REDUCE_NONFINITE) {
-        f = post_stream_populations(h);
-        is_storage = f == h->b.f[h->cur];
-    }
-    launch_reduce(h->cfg, h->b, f, is_storage, what, h->stream);
+    launch_reduce(h->cfg, h->b, h->cur, h->post_stream, slot_of(h->step), what, part, max_parts, dst, h->stream);
     double v = 0.0;
-    CHECK(hipMemcpyAsync(&v, h->b.dcnt + DC_REDUCE, sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    CHECK(hipMemcpyAsync(&v, dst, sizeof(double), hipMemcpyDeviceToHost, h->stream));
     CHECK(hipStreamSynchronize(h->stream));
     check_wave_flag(h);
     *out = v;
+}
+
+int lbm_reduce(lbm_handle_t* h, int32_t what, double* out) {
+    API_BEGIN(h)
+    if (int e = reduce_checks(h, what, out)) return e;
+    reduce_into(h, what, h->b.rpart, 3 * reduce_chunks(h->b.n), h->b.dcnt + DC_REDUCE, out);
     API_END(h)
 }
 

@@ -64,6 +64,7 @@ struct DevBufs {
     int* icnt;      // [IC_COUNT]
     double* dcnt;   // [DC_COUNT]
     int* waveq;     // fused path, wavefront step: [0] error flag, [WAVE_FLAG0 + t] epoch of moments tile t
+    double* rpart;  // FP64 partial sums of the reductions: 3 * reduce_chunks(n)
 };
 constexpr int WAVE_FLAG0 = 64;  // first tile flag (the error flag keeps a cache line of its own)
 // tiles per plane of the fused kernels (block of 256 cells, block_x cells in x)
@@ -109,7 +110,13 @@ void launch_fused_step(const lbm_config_t& c, DevBufs& b, int& cur, int mode, in
 void launch_materialize(const lbm_config_t& c, DevBufs& b, int cur, int slot, void* dst, hipStream_t s);
 // convert arithmetic-type populations in src into the storage buffer f[cur]
 void launch_store_populations(const lbm_config_t& c, DevBufs& b, int cur, const void* src, int64_t step, hipStream_t s);
-void launch_reduce(const lbm_config_t& c, DevBufs& b, const void* f, int f_is_storage, int what, hipStream_t s);
+// deterministic FP64 reduction `what` of the state f[cur] (post_stream 0:
+// a post-collision state, streamed on the fly with the time level `slot`)
+// into out[0] (device), with `part` (at least reduce_chunks(n) doubles)
+// for the partial sums
+void launch_reduce(const lbm_config_t& c, DevBufs& b, int cur, int post_stream, int slot, int what, double* part, size_t max_parts,
+                   double* out, hipStream_t s);
+size_t reduce_chunks(size_t n);
 void launch_cut_link_count(const lbm_config_t& c, DevBufs& b, int slot, hipStream_t s);
 bool fused_supported(const lbm_config_t& c);
 // obstacle mask of a time level (init_fields, CODEX:2711-2712) into dst

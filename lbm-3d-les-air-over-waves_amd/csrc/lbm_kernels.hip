@@ -153,6 +153,36 @@ __device__ __forceinline__ void cell_ijk(const Consts<T>& c, size_t id, int& i, 
     k = (int)kk;
 }
 
+// Deterministic FP64 reductions in two passes.  Pass 1: block b sums the
+// cells [b * RED_CHUNK, (b + 1) * RED_CHUNK), thread t the cells
+// t, t + RED_THREADS, ... of that chunk in order, then a fixed tree over the
+// block; pass 2: one block sums the partials the same way.  The grouping
+// depends only on the grid size, so a result does not depend on timing or
+// on what else runs on the device.
+constexpr int RED_THREADS = 256;
+constexpr int RED_PER_THREAD = 16;
+constexpr size_t RED_CHUNK = (size_t)RED_THREADS * RED_PER_THREAD;
+
+// fixed-shape block tree: xor butterfly inside each wave, then the four
+// wave results in order (sum, or max when MAX)
+template <bool MAX>
+__device__ __forceinline__ double block_tree(double v) {
+    __shared__ double wsum[RED_THREADS / 64];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        const double o = __shfl_xor(v, off);
+        v = MAX ? fmax(v, o) : v + o;
+    }
+    const int w = (int)threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) wsum[w] = v;
+    __syncthreads();
+    double r = wsum[0];
+#pragma unroll
+    for (int i = 1; i < RED_THREADS / 64; i++) r = MAX ? fmax(r, wsum[i]) : r + wsum[i];
+    __syncthreads();
+    return r;
+}
+
 // ===========================================================================
 // 2-D bed kernels
 // ===========================================================================
@@ -673,15 +703,18 @@ __device__ __forceinline__ T src_value(const Consts<T>& c, const S* f, size_t n,
 // populations (the macro step that follows stream in a water run, computed
 // from the values this thread stores; each cell reads only its own velocity
 // before that write)
-template <class S, class T, bool COUNT, bool MACRO>
-__global__ __launch_bounds__(NT) void k_stream_bc(Consts<T> c, Slot<T> s, const S* fin, T* fout, const T* ux, const T* uy, const T* uz,
-                            const int* type, const T* fill, const T* u_inlet, const T* inflow, const T* fs_h,
-                            const uint8_t* flags, int* icnt, T* mrho, T* mux, T* muy, T* muz) {
-    size_t id = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+// stream (CODEX:3418-3551), x walls, fringe, top drive and IBB
+// (CODEX:3554-3712) of cell id: the post-stream populations fo, the
+// cell's free-surface type and plane, and the link counts of the step
+template <class S, class T>
+__device__ __forceinline__ void stream_bc_cell(const Consts<T>& c, const Slot<T>& s, const S* fin, const T* ux, const T* uy, const T* uz,
+                                               const int* type, const T* fill, const T* u_inlet, const T* inflow, const T* fs_h,
+                                               const uint8_t* flags, size_t id, T fo[19], int& ty_out, int& k_out, int& n_cut,
+                                               int& n_fs, int& n_bridge) {
     size_t n = (size_t)c.nx * c.ny * c.nz;
-    if (id >= n) return;
     int i, j, k;
     cell_ijk(c, id, i, j, k);
+    k_out = k;
     const int nx = c.nx, ny = c.ny, nz = c.nz;
     const bool water = c.fs_water;
     const bool xper = c.x_boundary == LBM_XB_PERIODIC, xio = c.x_boundary == LBM_XB_INFLOW_OUTFLOW;
@@ -692,7 +725,8 @@ __global__ __launch_bounds__(NT) void k_stream_bc(Consts<T> c, Slot<T> s, const 
     bool open0 = g.open_mask & 1u;
     int ty = water ? type[id] : LBM_FS_LIQUID;
     bool dest_gas = water && ty == LBM_FS_GAS && open0;
-    T fpre[19], fo[19];
+    ty_out = ty;
+    T fpre[19];
     // a gas cell takes the rest equilibrium on every link; its own
     // populations are read only by the wall correction of a cell next to a wall
     if (!dest_gas || g.near) {
@@ -702,7 +736,9 @@ __global__ __launch_bounds__(NT) void k_stream_bc(Consts<T> c, Slot<T> s, const 
 #pragma unroll
         for (int q = 0; q < 19; q++) fpre[q] = T(0);
     }
-    int n_cut = 0, n_fs = 0, n_bridge = 0;
+    n_cut = 0;
+    n_fs = 0;
+    n_bridge = 0;
     fo[0] = dest_gas ? feq(c.rho0, T(0), T(0), T(0), 0) : fpre[0];
     size_t col = (size_t)j * nx + i;
     // source position of link q: periodic wrap, else the raw index with
@@ -875,6 +911,18 @@ __global__ __launch_bounds__(NT) void k_stream_bc(Consts<T> c, Slot<T> s, const 
 #pragma unroll
         for (int q = 1; q < 19; q++) fo[q] = ibb_link(c, s, g, i, j, q, fo[q], fpre[oppq(q)]);
     }
+}
+
+template <class S, class T, bool COUNT, bool MACRO>
+__global__ __launch_bounds__(NT) void k_stream_bc(Consts<T> c, Slot<T> s, const S* fin, T* fout, const T* ux, const T* uy, const T* uz,
+                            const int* type, const T* fill, const T* u_inlet, const T* inflow, const T* fs_h,
+                            const uint8_t* flags, int* icnt, T* mrho, T* mux, T* muy, T* muz) {
+    size_t id = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    size_t n = (size_t)c.nx * c.ny * c.nz;
+    if (id >= n) return;
+    T fo[19];
+    int ty, k, n_cut, n_fs, n_bridge;
+    stream_bc_cell<S, T>(c, s, fin, ux, uy, uz, type, fill, u_inlet, inflow, fs_h, flags, id, fo, ty, k, n_cut, n_fs, n_bridge);
 #pragma unroll
     for (int q = 0; q < 19; q++) fout[(size_t)q * n + id] = fo[q];
     if (MACRO) {
@@ -1481,41 +1529,51 @@ __global__ __launch_bounds__(NT) void k_flood_sweep(VofArgs<T> a, int* changed_p
 }
 
 // summarize_vof_trapped_gas_collapse sums (CODEX:2058-2075) in FP64
+// candidate mass, candidate volume and exterior capacity of the trapped
+// gas (CODEX:2059-2105), as per-chunk FP64 partials (three per chunk,
+// interleaved); k_collapse_decide sums them in a fixed order
 template <class T>
-__global__ __launch_bounds__(NT) void k_collapse_sum(VofArgs<T> a) {
-    size_t id = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    size_t n = (size_t)a.c.nx * a.c.ny * a.c.nz;
+__global__ __launch_bounds__(RED_THREADS) void k_collapse_sum(VofArgs<T> a, double* part) {
+    const size_t n = (size_t)a.c.nx * a.c.ny * a.c.nz;
+    const size_t base = (size_t)blockIdx.x * RED_CHUNK + threadIdx.x;
     double cm = 0.0, cv = 0.0, cap = 0.0;
-    if (id < n) {
+    for (int r = 0; r < RED_PER_THREAD; r++) {
+        const size_t id = base + (size_t)r * RED_THREADS;
+        if (id >= n) break;
         T g = gas_fraction(a, id);
         if (g > T(1.0e-6)) {
             if (a.ext_gas[id] == 0) {
                 T rr = a.rho[id];
                 if (rr < T(1.0e-8)) rr = a.c.rho0;
-                cm = (double)(g * rr);
-                cv = (double)(g * a.c.dx * a.c.dx * a.c.dx);
+                cm += (double)(g * rr);
+                cv += (double)(g * a.c.dx * a.c.dx * a.c.dx);
             } else if (a.type[id] != LBM_FS_GAS) {
                 T m = a.mass[id];
-                if (m > T(0)) cap = (double)m;
+                if (m > T(0)) cap += (double)m;
             }
         }
     }
-    // wave reduction, one atomic per wave
-    for (int off = 32; off > 0; off >>= 1) {
-        cm += __shfl_down(cm, off);
-        cv += __shfl_down(cv, off);
-        cap += __shfl_down(cap, off);
-    }
-    if ((threadIdx.x & 63) == 0) {
-        if (cm != 0.0) atomicAdd(&a.dcnt[DC_CAND_MASS], cm);
-        if (cv != 0.0) atomicAdd(&a.dcnt[DC_CAND_VOLUME], cv);
-        if (cap != 0.0) atomicAdd(&a.dcnt[DC_EXT_CAPACITY], cap);
+    cm = block_tree<false>(cm);
+    cv = block_tree<false>(cv);
+    cap = block_tree<false>(cap);
+    if (threadIdx.x == 0) {
+        part[3 * (size_t)blockIdx.x] = cm;
+        part[3 * (size_t)blockIdx.x + 1] = cv;
+        part[3 * (size_t)blockIdx.x + 2] = cap;
     }
 }
-
-// accept test of summarize_vof_trapped_gas_collapse (CODEX:2077-2091)
-__global__ __launch_bounds__(NT) void k_collapse_decide(double* dcnt, double max_volume) {
-    double cm = dcnt[DC_CAND_MASS], cv = dcnt[DC_CAND_VOLUME], cap = dcnt[DC_EXT_CAPACITY];
+__global__ __launch_bounds__(RED_THREADS) void k_collapse_decide(const double* part, int count, double* dcnt, double max_volume) {
+    double s3[3];
+    for (int e = 0; e < 3; e++) {
+        double v = 0.0;
+        for (int i = (int)threadIdx.x; i < count; i += RED_THREADS) v += part[3 * (size_t)i + e];
+        s3[e] = block_tree<false>(v);
+    }
+    if (threadIdx.x != 0) return;
+    const double cm = s3[0], cv = s3[1], cap = s3[2];
+    dcnt[DC_CAND_MASS] = cm;
+    dcnt[DC_CAND_VOLUME] = cv;
+    dcnt[DC_EXT_CAPACITY] = cap;
     dcnt[DC_APPLIED_VOLUME] = 0.0;
     dcnt[DC_REMOVAL_SCALE] = 0.0;
     if (cv > 0.0 && cv <= max_volume && cap >= cm && cap > 1.0e-12) {
@@ -1523,8 +1581,6 @@ __global__ __launch_bounds__(NT) void k_collapse_decide(double* dcnt, double max
         dcnt[DC_REMOVAL_SCALE] = cm / cap;
     }
 }
-
-// apply_vof_trapped_gas_collapse (CODEX:2094-2126)
 template <class T>
 __global__ __launch_bounds__(NT) void k_collapse_apply(VofArgs<T> a) {
     VOF_PROLOGUE
@@ -2731,45 +2787,80 @@ __global__ __launch_bounds__(FB_THREADS, 6) void k_step_wave(FusedArgs<S, T> a, 
 // reductions (FP64)
 // ===========================================================================
 template <class S, class T>
-__global__ __launch_bounds__(NT) void k_reduce(Consts<T> c, const S* f, const T* ux, const T* uy, const T* uz, const T* mass, const T* fill,
-                         const uint8_t* flags, const int* type, int what, double* out) {
-    size_t id = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    size_t n = (size_t)c.nx * c.ny * c.nz;
-    double v = 0.0;
-    if (id < n) {
-        if (what == LBM_REDUCE_MASS) {
-            if (flags[id] & 1u)
-                for (int q = 0; q < 19; q++) v += (double)Store<S, T>::load(f, (size_t)q * n + id, q, c.store);
-        } else if (what == LBM_REDUCE_VOF_MASS) {
-            v = (double)mass[id];
-        } else if (what == LBM_REDUCE_VOF_VOLUME) {
-            v = (double)fill[id];
-        } else if (what == LBM_REDUCE_MAX_SPEED) {
-            bool act = flags[id] & 1u;
-            if (c.fs_water && type[id] == LBM_FS_GAS) act = false;
-            if (act) {
-                double a = ux[id], b = uy[id], e = uz[id];
-                v = sqrt(a * a + b * b + e * e);
-                if (v != v) v = 0.0;  // np.nanmax ignores NaN
-            }
-        } else if (what == LBM_REDUCE_NONFINITE) {
-            for (int q = 0; q < 19; q++) {
-                T x = Store<S, T>::load(f, (size_t)q * n + id, q, c.store);
-                if (!__builtin_isfinite(x)) v += 1.0;
-            }
-        }
-    }
+struct RedArgs {
+    Consts<T> c;
+    Slot<T> s;        // time level of the state (pending stream of a post-collision state)
+    const S* f;       // populations: post-stream (SRC 0) or post-collision (SRC 1)
+    const T *ux, *uy, *uz, *mass, *fill, *u_inlet, *inflow, *fs_h;
+    const uint8_t* flags;
+    const int* type;
+};
+
+// one cell's contribution to reduction `what`; SRC 1 streams a
+// post-collision state on the fly (the post-stream populations the
+// materialised download returns, without a buffer for them)
+template <int SRC, class S, class T>
+__device__ __forceinline__ double reduce_cell(const RedArgs<S, T>& a, int what, size_t id) {
+    const Consts<T>& c = a.c;
+    const size_t n = (size_t)c.nx * c.ny * c.nz;
+    if (what == LBM_REDUCE_VOF_MASS) return (double)a.mass[id];
+    if (what == LBM_REDUCE_VOF_VOLUME) return (double)a.fill[id];
     if (what == LBM_REDUCE_MAX_SPEED) {
-        for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_down(v, off));
-        if ((threadIdx.x & 63) == 0 && v > 0.0) {
-            unsigned long long* p = (unsigned long long*)out;
-            unsigned long long bits = (unsigned long long)__double_as_longlong(v);
-            atomicMax(p, bits);  // non-negative doubles order like their bits
-        }
-    } else {
-        for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off);
-        if ((threadIdx.x & 63) == 0 && v != 0.0) atomicAdd(out, v);
+        bool act = a.flags[id] & 1u;
+        if (c.fs_water && a.type[id] == LBM_FS_GAS) act = false;
+        if (!act) return 0.0;
+        const double x = a.ux[id], y = a.uy[id], z = a.uz[id];
+        const double v = sqrt(x * x + y * y + z * z);
+        return v != v ? 0.0 : v;  // np.nanmax ignores NaN
     }
+    if (what == LBM_REDUCE_MASS && !(a.flags[id] & 1u)) return 0.0;
+    T fo[19];
+    if (SRC == 1) {
+        int ty, k, n1, n2, n3;
+        stream_bc_cell<S, T>(c, a.s, a.f, a.ux, a.uy, a.uz, a.type, a.fill, a.u_inlet, a.inflow, a.fs_h, a.flags, id, fo, ty, k, n1, n2, n3);
+    } else {
+#pragma unroll
+        for (int q = 0; q < 19; q++) fo[q] = Store<S, T>::load(a.f, (size_t)q * n + id, q, c.store);
+    }
+    double v = 0.0;
+    if (what == LBM_REDUCE_MASS) {
+#pragma unroll
+        for (int q = 0; q < 19; q++) v += (double)fo[q];
+    } else {
+#pragma unroll
+        for (int q = 0; q < 19; q++)
+            if (!__builtin_isfinite(fo[q])) v += 1.0;
+    }
+    return v;
+}
+
+template <int SRC, class S, class T>
+__global__ __launch_bounds__(RED_THREADS) void k_reduce_part(RedArgs<S, T> a, int what, double* part) {
+    const size_t n = (size_t)a.c.nx * a.c.ny * a.c.nz;
+    const size_t base = (size_t)blockIdx.x * RED_CHUNK + threadIdx.x;
+    const bool is_max = what == LBM_REDUCE_MAX_SPEED;
+    double v = 0.0;
+    for (int r = 0; r < RED_PER_THREAD; r++) {
+        const size_t id = base + (size_t)r * RED_THREADS;
+        if (id < n) {
+            const double x = reduce_cell<SRC>(a, what, id);
+            v = is_max ? fmax(v, x) : v + x;
+        }
+    }
+    v = is_max ? block_tree<true>(v) : block_tree<false>(v);
+    if (threadIdx.x == 0) part[blockIdx.x] = v;
+}
+
+// partials -> out[0]; `stride` and `count` select one of several
+// interleaved partial sets
+__global__ __launch_bounds__(RED_THREADS) void k_reduce_final(const double* part, int count, int stride, int is_max, double* out) {
+    double v = 0.0;
+    for (int i = (int)threadIdx.x; i < count; i += RED_THREADS) {
+        const double x = part[(size_t)i * stride];
+        v = is_max ? fmax(v, x) : v + x;
+    }
+    v = is_max ? block_tree<true>(v) : block_tree<false>(v);
+    if (threadIdx.x == 0) out[0] = v;
 }
 
 template <class S, class T>
@@ -3035,8 +3126,9 @@ static void vof_pipeline(const lbm_config_t& cfg, DevBufs& b, T* f, int slot, in
         HIP_OK(hipMemsetAsync(flagbuf + 1, 0, sweeps * sizeof(int), s));
         for (int w = 0; w < sweeps; w++) k_flood_sweep<T><<<nb, NT, 0, s>>>(A(), flagbuf + w, flagbuf + w + 1);
         HIP_OK(hipMemcpyAsync(b.icnt + IC_FLOOD_UNCONVERGED, flagbuf + sweeps, sizeof(int), hipMemcpyDeviceToDevice, s));
-        k_collapse_sum<T><<<nb, NT, 0, s>>>(A());
-        k_collapse_decide<<<1, 1, 0, s>>>(b.dcnt, (double)cfg.vof_collapse_max_volume_m3);
+        const unsigned nch = (unsigned)reduce_chunks(n);
+        k_collapse_sum<T><<<nch, RED_THREADS, 0, s>>>(A(), b.rpart);
+        k_collapse_decide<<<1, RED_THREADS, 0, s>>>(b.rpart, (int)nch, b.dcnt, (double)cfg.vof_collapse_max_volume_m3);
         k_collapse_apply<T><<<nb, NT, 0, s>>>(A());
     }
     k_vof_enforce<T><<<nb, NT, 0, s>>>(A());
@@ -3254,23 +3346,43 @@ void launch_store_populations(const lbm_config_t& c, DevBufs& b, int cur, const 
     }
 }
 
+size_t reduce_chunks(size_t n) { return (n + RED_CHUNK - 1) / RED_CHUNK; }
+
 template <class S, class T>
-static void reduce_t(const lbm_config_t& cfg, DevBufs& b, const void* f, int what, hipStream_t s) {
-    Consts<T> c = make_consts<T>(cfg, -1);
-    HIP_OK(hipMemsetAsync(b.dcnt + DC_REDUCE, 0, sizeof(double), s));
-    k_reduce<S, T><<<nblocks(b.n), NT, 0, s>>>(c, (const S*)f, (const T*)b.ux, (const T*)b.uy, (const T*)b.uz, (const T*)b.mass,
-                                               (const T*)b.fill, b.cellflags, b.type, what, b.dcnt + DC_REDUCE);
+static void reduce_t(const lbm_config_t& cfg, DevBufs& b, int cur, int src, int slot, int what, double* part, double* out,
+                     hipStream_t s) {
+    RedArgs<S, T> a;
+    a.c = make_consts<T>(cfg, -1);
+    a.s = make_slot<T>(b, slot);
+    a.f = (const S*)b.f[cur];
+    a.ux = (const T*)b.ux;
+    a.uy = (const T*)b.uy;
+    a.uz = (const T*)b.uz;
+    a.mass = (const T*)b.mass;
+    a.fill = (const T*)b.fill;
+    a.u_inlet = (const T*)b.u_inlet;
+    a.inflow = (const T*)b.inflow;
+    a.fs_h = (const T*)b.fs_h;
+    a.flags = b.cellflags;
+    a.type = b.type;
+    const unsigned nb = (unsigned)reduce_chunks(b.n);
+    if (src)
+        k_reduce_part<1, S, T><<<nb, RED_THREADS, 0, s>>>(a, what, part);
+    else
+        k_reduce_part<0, S, T><<<nb, RED_THREADS, 0, s>>>(a, what, part);
+    k_reduce_final<<<1, RED_THREADS, 0, s>>>(part, (int)nb, 1, what == LBM_REDUCE_MAX_SPEED, out);
     HIP_OK(hipGetLastError());
 }
-void launch_reduce(const lbm_config_t& c, DevBufs& b, const void* f, int f_is_storage, int what, hipStream_t s) {
-    if (b.storage == LBM_STORE_FP64)
-        reduce_t<double, double>(c, b, f, what, s);
-    else if (!f_is_storage || b.storage == LBM_STORE_FP32)
-        reduce_t<float, float>(c, b, f, what, s);
-    else if (b.storage == LBM_STORE_FP16)
-        reduce_t<__half, float>(c, b, f, what, s);
-    else
-        reduce_t<bf16_t, float>(c, b, f, what, s);
+void launch_reduce(const lbm_config_t& c, DevBufs& b, int cur, int post_stream, int slot, int what, double* part, size_t max_parts,
+                   double* out, hipStream_t s) {
+    if (reduce_chunks(b.n) > max_parts) throw std::runtime_error("reduction: more partial sums than the buffer holds");
+    const int src = post_stream ? 0 : 1;
+    switch (b.storage) {
+        case LBM_STORE_FP64: reduce_t<double, double>(c, b, cur, src, slot, what, part, out, s); break;
+        case LBM_STORE_FP16: reduce_t<__half, float>(c, b, cur, src, slot, what, part, out, s); break;
+        case LBM_STORE_BF16: reduce_t<bf16_t, float>(c, b, cur, src, slot, what, part, out, s); break;
+        default: reduce_t<float, float>(c, b, cur, src, slot, what, part, out, s); break;
+    }
 }
 
 template <class T>

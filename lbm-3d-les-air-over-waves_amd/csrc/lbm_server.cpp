@@ -17,6 +17,7 @@
 #include <condition_variable>
 #include <deque>
 #include <mutex>
+#include <set>
 #include <stdexcept>
 #include <thread>
 #include <vector>
@@ -49,9 +50,11 @@ struct lbm_server {
     std::atomic<bool> stop{false};
     std::mutex fin_m;
     std::deque<std::pair<int64_t, int32_t>> finished;
+    std::set<lbm_handle_t*> inflight;  // handles owned by the server (guarded by fin_m)
 
-    void finish(int64_t ticket, int32_t status) {
+    void finish(lbm_handle_t* h, int64_t ticket, int32_t status) {
         std::lock_guard<std::mutex> g(fin_m);
+        inflight.erase(h);
         finished.emplace_back(ticket, status);
     }
 
@@ -90,7 +93,7 @@ struct lbm_server {
                     a++;
                     continue;
                 }
-                finish(j.ticket, rc);
+                finish(j.h, j.ticket, rc);
                 w->active.erase(w->active.begin() + (long)a);
                 w->n_active.fetch_sub(1);
                 progressed = true;
@@ -139,6 +142,12 @@ int lbm_server_create(int32_t chunk_steps, int32_t n_workers, lbm_server_t** out
 int lbm_server_add(lbm_server_t* s, lbm_handle_t* h, int32_t nsteps, int64_t ticket) {
     if (!s || !h || nsteps < 0) return LBM_E_ARG;
     if (s->stop.load()) return LBM_E_STATE;
+    {
+        // a handle is stepped by one job at a time: two jobs on one handle
+        // would race on its step counter, ping-pong state and stream
+        std::lock_guard<std::mutex> g(s->fin_m);
+        if (!s->inflight.insert(h).second) return LBM_E_STATE;
+    }
     Worker* best = s->workers[0];
     for (Worker* w : s->workers)
         if (w->n_active.load() < best->n_active.load()) best = w;

@@ -11,9 +11,11 @@ runtime reports a case finished, the scheduler downloads the requested
 fields, frees the case and admits the next requests: a request that arrives
 while others run starts as soon as a slot and memory free up.
 
-Every kernel is deterministic (no atomics on floating-point state, counter-
-based random numbers), so a case's result does not depend on what else runs:
-``tests/test_gpu_server.py`` checks alone-vs-mix equality bit for bit.
+Every kernel that writes the state is deterministic (no floating-point
+atomics on it; FP64 sums in fixed groups and a fixed tree; counter-based
+random numbers), so a case's result does not depend on what else runs:
+``tests/test_gpu_server.py`` checks alone-vs-mix equality bit for bit,
+with a trapped-gas collapse (whose FP64 sums scale the state) in the mix.
 """
 from __future__ import annotations
 
@@ -21,7 +23,7 @@ import ctypes
 import time
 from collections import deque
 from dataclasses import dataclass, field
-from typing import Dict, List, Mapping, Optional, Sequence
+from typing import Callable, Dict, List, Mapping, Optional, Sequence
 
 import numpy as np
 
@@ -45,6 +47,7 @@ class Request:
     t_done: float = 0.0
     status: str = "queued"  # queued | active | done | failed
     error: str = ""
+    prepare: Optional[Callable] = None  # called with the initialised Solver before stepping
     results: Dict[int, np.ndarray] = field(default_factory=dict)
 
     @property
@@ -86,7 +89,10 @@ class EnsembleServer:
         self._next = 0
 
     # -- client side ---------------------------------------------------------
-    def submit(self, env_or_config, steps: int, result_fields: Sequence[int] = DEFAULT_RESULTS) -> int:
+    def submit(self, env_or_config, steps: int, result_fields: Sequence[int] = DEFAULT_RESULTS,
+               prepare: Optional[Callable] = None) -> int:
+        """Queue a case; `prepare(solver)`, if given, runs after the
+        initialisation (e.g. uploads of an initial state of its own)."""
         config = env_or_config if isinstance(env_or_config, SolverConfig) else parse_env(env_or_config)
         if config.unsupported:
             raise NotImplementedError("not supported by this build: " + ", ".join(config.unsupported))
@@ -95,7 +101,7 @@ class EnsembleServer:
             raise ValueError(f"case needs {fp} B of device memory, more than the whole budget {self.budget} B")
         ticket = self._next
         self._next += 1
-        self.queue.append(Request(ticket, config, int(steps), tuple(result_fields), fp, time.perf_counter()))
+        self.queue.append(Request(ticket, config, int(steps), tuple(result_fields), fp, time.perf_counter(), prepare=prepare))
         return ticket
 
     # -- scheduler -----------------------------------------------------------
@@ -131,6 +137,8 @@ class EnsembleServer:
             try:
                 solver = Solver(config=req.config)
                 solver.initialize()
+                if req.prepare is not None:
+                    req.prepare(solver)
                 self.used += req.footprint
                 self.active[req.ticket] = (req, solver)
                 rc = self.lib.lbm_server_add(self.srv, solver.case.h, req.steps, req.ticket)

@@ -256,6 +256,19 @@ int lbm_get_counters(lbm_handle_t *h, lbm_counters_t *out);
    the handle); a post-collision state is streamed on the fly. */
 int lbm_reduce(lbm_handle_t *h, int32_t what, double *out);
 
+/* lbm_reduce that allocates nothing (no counterpart in the reference): for
+   callers that cannot allocate device memory (code under HIP graph capture
+   of its own, tight serving loops, harnesses that forbid hipMalloc).  The
+   partial sums go to a static device array of the library instead of the
+   handle's buffer, so grids up to LBM_REDUCE_NOALLOC_MAX_CELLS cells are
+   accepted (larger ones return LBM_E_ARG).  The array is one per GPU and
+   held until the process ends; calls that overlap on one GPU (from several
+   threads, or on handles whose streams run concurrently) race on it and
+   must be serialised by the caller.  Same results as lbm_reduce, bit for
+   bit. */
+#define LBM_REDUCE_NOALLOC_MAX_CELLS ((size_t)1 << 26)
+int lbm_reduce_noalloc(lbm_handle_t *h, int32_t what, double *out);
+
 /* Number of steps advanced so far. */
 int lbm_get_step(lbm_handle_t *h, int64_t *out);
 

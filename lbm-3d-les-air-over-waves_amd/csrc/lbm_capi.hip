@@ -54,6 +54,8 @@ struct lbm_handle {
     std::vector<hipEvent_t> ev;  // timing events: 2 per step + 2 around the loop
     std::string err;
     std::vector<void*> allocs;
+    double* noalloc_part = nullptr;  // device address of the static reduction scratch (this handle's device)
+    size_t noalloc_parts = 0;
 };
 
 #define CHECK(expr)                                                                                     \
@@ -354,6 +356,10 @@ int lbm_create(const lbm_config_t* cfg, lbm_handle_t** out) {
         if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) throw std::runtime_error("no HIP device visible");
         CHECK(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
         allocate(h);
+        // the static scratch of lbm_reduce_noalloc: its address on this
+        // device, looked up here so that the entry point itself does nothing
+        // but launch
+        h->noalloc_part = noalloc_scratch(&h->noalloc_parts);
     } catch (const std::exception& e) {
         g_create_error = e.what();
         free_all(h);
@@ -700,6 +706,18 @@ int lbm_reduce(lbm_handle_t* h, int32_t what, double* out) {
     API_BEGIN(h)
     if (int e = reduce_checks(h, what, out)) return e;
     reduce_into(h, what, h->b.rpart, 3 * reduce_chunks(h->b.n), h->b.dcnt + DC_REDUCE, out);
+    API_END(h)
+}
+
+int lbm_reduce_noalloc(lbm_handle_t* h, int32_t what, double* out) {
+    API_BEGIN(h)
+    if (int e = reduce_checks(h, what, out)) return e;
+    if (h->b.n > LBM_REDUCE_NOALLOC_MAX_CELLS || reduce_chunks(h->b.n) > h->noalloc_parts)
+        return fail(h, LBM_E_ARG, "lbm_reduce_noalloc: the grid has " + std::to_string(h->b.n) + " cells, more than the " +
+                                      std::to_string((size_t)LBM_REDUCE_NOALLOC_MAX_CELLS) +
+                                      " its static scratch serves; use lbm_reduce");
+    // partial sums in the static array, the result in the slot after them
+    reduce_into(h, what, h->noalloc_part, h->noalloc_parts, h->noalloc_part + h->noalloc_parts, out);
     API_END(h)
 }
 

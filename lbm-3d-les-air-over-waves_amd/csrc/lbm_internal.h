@@ -117,6 +117,9 @@ void launch_store_populations(const lbm_config_t& c, DevBufs& b, int cur, const 
 void launch_reduce(const lbm_config_t& c, DevBufs& b, int cur, int post_stream, int slot, int what, double* part, size_t max_parts,
                    double* out, hipStream_t s);
 size_t reduce_chunks(size_t n);
+// device address of the static partial-sum array of the current device
+// (looked up once per device); *parts = its capacity in partials
+double* noalloc_scratch(size_t* parts);
 void launch_cut_link_count(const lbm_config_t& c, DevBufs& b, int slot, hipStream_t s);
 bool fused_supported(const lbm_config_t& c);
 // obstacle mask of a time level (init_fields, CODEX:2711-2712) into dst

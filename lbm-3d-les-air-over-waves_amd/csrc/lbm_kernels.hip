@@ -17,6 +17,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <type_traits>
@@ -3383,6 +3384,32 @@ void launch_reduce(const lbm_config_t& c, DevBufs& b, int cur, int post_stream, 
         case LBM_STORE_BF16: reduce_t<bf16_t, float>(c, b, cur, src, slot, what, part, out, s); break;
         default: reduce_t<float, float>(c, b, cur, src, slot, what, part, out, s); break;
     }
+}
+
+// The partial sums of lbm_reduce_noalloc live in this file-scope device
+// array instead of a buffer of the handle: LBM_REDUCE_NOALLOC_MAX_CELLS /
+// RED_CHUNK partials and one result slot.  Each device holds its own copy
+// at its own address; host code reaches it only through the address that
+// hipGetSymbolAddress returns for the current device (the array's name is a
+// host-side stand-in, not a device address).
+constexpr size_t NOALLOC_PARTS = (LBM_REDUCE_NOALLOC_MAX_CELLS + RED_CHUNK - 1) / RED_CHUNK;
+static __device__ __align__(256) double scratch[NOALLOC_PARTS + 32];
+
+double* noalloc_scratch(size_t* parts) {
+    static std::mutex mu;
+    static double* ptrs[256] = {};
+    int dev = 0;
+    HIP_OK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= 256) throw std::runtime_error("device index out of range for the reduction scratch");
+    std::lock_guard<std::mutex> lock(mu);
+    if (!ptrs[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) throw std::runtime_error(std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        ptrs[dev] = (double*)p;
+    }
+    if (parts) *parts = NOALLOC_PARTS;
+    return ptrs[dev];
 }
 
 template <class T>

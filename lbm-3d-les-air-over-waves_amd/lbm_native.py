@@ -29,6 +29,7 @@ EXPORTS = [
     "lbm_download_field",
     "lbm_get_counters",
     "lbm_reduce",
+    "lbm_reduce_noalloc",
     "lbm_get_step",
     "lbm_sync",
     "lbm_last_error",
@@ -66,6 +67,7 @@ def load_library(path: Optional[os.PathLike] = None) -> ctypes.CDLL:
     lib.lbm_download_field.argtypes = [H, ctypes.c_int32, ctypes.c_void_p, ctypes.c_size_t]
     lib.lbm_get_counters.argtypes = [H, ctypes.POINTER(abi.LbmCounters)]
     lib.lbm_reduce.argtypes = [H, ctypes.c_int32, ctypes.POINTER(ctypes.c_double)]
+    lib.lbm_reduce_noalloc.argtypes = [H, ctypes.c_int32, ctypes.POINTER(ctypes.c_double)]
     lib.lbm_get_step.argtypes = [H, ctypes.POINTER(ctypes.c_int64)]
     lib.lbm_sync.argtypes = [H]
     lib.lbm_last_error.argtypes = [H]
@@ -178,6 +180,13 @@ class Case:
     def reduce(self, what: int) -> float:
         out = ctypes.c_double(0.0)
         self._check(self.lib.lbm_reduce(self.h, what, ctypes.byref(out)), "lbm_reduce")
+        return float(out.value)
+
+    def reduce_noalloc(self, what: int) -> float:
+        """lbm_reduce with the partial sums in the library's static device
+        scratch (allocates nothing)."""
+        out = ctypes.c_double(0.0)
+        self._check(self.lib.lbm_reduce_noalloc(self.h, what, ctypes.byref(out)), "lbm_reduce_noalloc")
         return float(out.value)
 
     def step_index(self) -> int:

@@ -3120,7 +3120,9 @@ static void vof_pipeline(const lbm_config_t& cfg, DevBufs& b, T* f, int slot, in
         // sweep flags: flagbuf[0] = 1 primes the first sweep; each sweep
         // clears its own flag before running
         int* flagbuf = b.si;  // reuse the int scratch: first (sweeps+1) ints
-        int sweeps = std::max(1, cfg.nx + cfg.ny + cfg.nz) * 4;
+        // to the fixed point, bounded by 4 (nx + ny + nz) sweeps or by the
+        // configured count if that is larger
+        int sweeps = std::max(std::max(1, cfg.nx + cfg.ny + cfg.nz) * 4, cfg.vof_collapse_flood_sweeps);
         if ((size_t)(sweeps + 1) > n) sweeps = (int)n - 1;
         int one = 1;
         HIP_OK(hipMemcpyAsync(flagbuf, &one, sizeof(int), hipMemcpyHostToDevice, s));

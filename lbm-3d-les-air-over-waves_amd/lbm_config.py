@@ -321,6 +321,15 @@ def parse_env(env: Optional[Mapping[str, str]] = None) -> SolverConfig:
     vof_collapse_flood_sweeps = env_int(env, "LBM_VOF_COLLAPSE_FLOOD_SWEEPS", max(1, Nx + Ny + Nz))
     if vof_collapse_flood_sweeps < 1:
         raise ValueError("LBM_VOF_COLLAPSE_FLOOD_SWEEPS must be positive")
+    if vof_collapse_flood_sweeps < Nx + Ny + Nz:
+        # the reference's sweeps update the marks in place in parallel, so a
+        # fill cut short depends on the sweep schedule; this build runs the
+        # flood fill to its fixed point (exact 6-connected reachability)
+        raise ValueError(
+            "LBM_VOF_COLLAPSE_FLOOD_SWEEPS below Nx + Ny + Nz is not supported: the trapped-gas flood fill runs to "
+            "its fixed point here (a partial fill depends on the sweep schedule); leave it unset or use a value "
+            ">= Nx + Ny + Nz, which bounds the sweeps"
+        )
     steps = env_int(env, "LBM_STEPS", 400000)
     plot_freq = env_int(env, "LBM_PLOT_FREQ", 100)
     plot_step_start = env_int(env, "LBM_PLOT_START", 0)

@@ -123,3 +123,19 @@ def test_presets():
     c4 = parse_env(preset_env("vof_192x96x160"))
     assert (c4.Nx, c4.Ny, c4.Nz) == (192, 96, 160) and c4.g_lb == 1e-5
     assert c4.free_surface_base_m == 80 and c4.vof_block_center_z_m == 128
+
+
+def test_partial_flood_fill_is_refused():
+    """A flood-fill sweep count below Nx + Ny + Nz would cut the fill short
+    (schedule-dependent in the reference); it is refused, larger counts are
+    accepted and passed through as the bound on the sweeps."""
+    import pytest
+
+    from harness import env_of
+    from lbm_config import parse_env
+
+    env = env_of("vof_192x96x160", LBM_LX_M=32, LBM_LY_M=16, LBM_LZ_M=40)
+    with pytest.raises(ValueError, match="FLOOD_SWEEPS"):
+        parse_env(dict(env, LBM_VOF_COLLAPSE_FLOOD_SWEEPS="5"))
+    assert parse_env(dict(env, LBM_VOF_COLLAPSE_FLOOD_SWEEPS="1000")).to_struct().vof_collapse_flood_sweeps == 1000
+    assert parse_env(env).to_struct().vof_collapse_flood_sweeps == 32 + 16 + 40

@@ -37,7 +37,7 @@ struct lbm_handle {
     DevBufs b{};
     hipStream_t stream = nullptr;
     // static (flat / piecewise) beds: the 2-D bed kernels run once per slot
-    int slot_valid[2] = {0, 0};
+    int slot_valid[BED_RING] = {};
     int cur = 0;
     int post_stream = 1;  // 1: f[cur] = f(step), 0: f[cur] = f*(step), stream pending
     int64_t step = 0;
@@ -134,13 +134,21 @@ static int wall_band_top(const lbm_config_t& c, double piecewise_max) {
     return k >= (double)(c.nz - 1) ? c.nz - 1 : (int)k;
 }
 
+// slots of the bed ring: a moving bed keeps BED_RING time levels (computed
+// BED_BATCH at a time), a static one (or the phi modes, one phi field per
+// level) two
+static int bed_ring(const lbm_config_t& c) {
+    const bool moving = c.bed_profile == LBM_BED_MOVING_SINE || c.bed_profile == LBM_BED_MOVING_SINE1;
+    return moving && !c.phi_fraction ? BED_RING : 2;
+}
+
 static size_t device_bytes(const lbm_config_t& c, int path) {
     size_t n = (size_t)c.nx * c.ny * c.nz, n2 = (size_t)c.nx * c.ny;
     size_t R = real_size(c);
     size_t bytes = 2 * 19 * n * store_size(c);  // ping-pong populations
     bytes += 5 * n * R + n;                      // rho, u, omega, cell flags
     if (path == LBM_PATH_STAGED) bytes += 2 * n * R + 2 * n * 4;  // nu_t, smag, obstacle masks
-    bytes += 2 * n2 * (78 * R + 8) + (c.nx + c.nz + (size_t)c.nz * c.ny) * R;
+    bytes += (size_t)bed_ring(c) * n2 * (78 * R + 8) + (c.nx + c.nz + (size_t)c.nz * c.ny) * R;
     if (c.fs_enabled) bytes += n * (11 * R + 7 * 4) + 6 * n * R + 3 * n2 * R;
     if (c.phi_fraction) bytes += n * R;
     return bytes + (1 << 20);
@@ -207,7 +215,7 @@ static void allocate(lbm_handle* h) {
         b.waveq = (int*)dalloc(h, wq * sizeof(int));
         CHECK(hipMemsetAsync(b.waveq, 0, wq * sizeof(int), h->stream));
     }
-    for (int s = 0; s < 2; s++) {
+    for (int s = 0; s < b.ring; s++) {
         b.h[s] = dalloc(h, n2 * R);
         b.uw[s] = dalloc(h, n2 * R);
         b.vw[s] = dalloc(h, n2 * R);
@@ -273,8 +281,7 @@ static void allocate(lbm_handle* h) {
     CHECK(hipMemsetAsync(b.uz, 0, n * R, h->stream));
     double om0 = R == 8 ? 1.0 / (double)c.tau0 : (double)(1.0f / c.tau0);
     fill_real(h, b.omega, n, om0);
-    fill_real(h, b.h[0], n2, (double)c.offset_m);
-    fill_real(h, b.h[1], n2, (double)c.offset_m);
+    for (int sl = 0; sl < b.ring; sl++) fill_real(h, b.h[sl], n2, (double)c.offset_m);
     fill_real(h, b.piecewise, c.nx, (double)c.offset_m);
     CHECK(hipStreamSynchronize(h->stream));
 }
@@ -283,7 +290,8 @@ static void ensure_ftmp(lbm_handle* h) {
     if (!h->b.ftmp) h->b.ftmp = dalloc(h, 19 * h->b.n * real_size(h->cfg));
 }
 
-static int slot_of(int64_t step) { return (int)(step & 1); }
+// bed time level (slot of the ring) of a step
+static int slot_of(const lbm_handle* h, int64_t step) { return (int)(step % h->b.ring); }
 
 // the reference's post-stream populations f(step) as an arithmetic-type
 // device buffer (either the state itself or ftmp)
@@ -295,7 +303,7 @@ static const void* post_stream_populations(lbm_handle* h) {
         return h->b.ftmp;
     }
     ensure_ftmp(h);
-    launch_materialize(h->cfg, h->b, h->cur, slot_of(h->step), h->b.ftmp, h->stream);
+    launch_materialize(h->cfg, h->b, h->cur, slot_of(h, h->step), h->b.ftmp, h->stream);
     return h->b.ftmp;
 }
 
@@ -351,6 +359,7 @@ int lbm_create(const lbm_config_t* cfg, lbm_handle_t** out) {
     h->omega_band_top = band_top(*cfg);
     h->piecewise_max = (double)cfg->offset_m;
     h->wall_top = wall_band_top(*cfg, h->piecewise_max);
+    h->b.ring = bed_ring(*cfg);
     try {
         int ndev = 0;
         if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) throw std::runtime_error("no HIP device visible");
@@ -385,8 +394,7 @@ const char* lbm_last_error(lbm_handle_t* h) { return h ? h->err.c_str() : g_crea
 int lbm_geometry(lbm_handle_t* h, float t) {
     API_BEGIN(h)
     launch_bed2d(h->cfg, h->b, t, 0, h->stream);
-    h->slot_valid[0] = 1;
-    h->slot_valid[1] = 0;
+    for (int sl = 0; sl < BED_RING; sl++) h->slot_valid[sl] = sl == 0;
     launch_obstacle(h->cfg, h->b, 0, h->b.obst, h->stream);
     h->geometry_done = 1;
     h->step = 0;
@@ -420,11 +428,20 @@ int lbm_step(lbm_handle_t* h, int32_t nsteps) {
     }
     if (h->timing) CHECK(hipEventRecord(h->ev[0], h->stream));
     const bool fixed = c.bed_profile == LBM_BED_FLAT || c.bed_profile == LBM_BED_PIECEWISE;
+    const bool batched = h->b.ring == BED_RING;  // a moving bed: its levels come BED_BATCH steps at a time
+    const int64_t last_step = h->step + nsteps;
+    int64_t bed_ready = h->step;  // the level of the current step is current
     for (int s = 0; s < nsteps; s++) {
         int64_t step = h->step + 1;
         float t = (float)((double)(step + 1) * c.dt_phys);  // CODEX:3874
-        int slot = slot_of(step);
-        if (!(fixed && h->slot_valid[slot])) {  // a static bed is the same at every time level
+        int slot = slot_of(h, step);
+        if (batched) {
+            if (step > bed_ready) {
+                const int nb = (int)std::min<int64_t>(BED_BATCH, last_step - step + 1);
+                launch_bed2d_batch(c, h->b, step, nb, h->stream);
+                bed_ready = step + nb - 1;
+            }
+        } else if (!(fixed && h->slot_valid[slot])) {  // a static bed is the same at every time level
             launch_bed2d(c, h->b, t, slot, h->stream);
             h->slot_valid[slot] = 1;
         }
@@ -441,7 +458,7 @@ int lbm_step(lbm_handle_t* h, int32_t nsteps) {
                 h->sched.epoch = 0;
             }
             h->sched.epoch++;
-            launch_fused_step(c, h->b, h->cur, h->post_stream ? 0 : 1, slot_of(step - 1), slot, step, last, h->omega_band_top,
+            launch_fused_step(c, h->b, h->cur, h->post_stream ? 0 : 1, slot_of(h, step - 1), slot, step, last, h->omega_band_top,
                               h->wall_top, h->block_x, h->sched, h->stream);
             h->post_stream = 0;
         }
@@ -468,7 +485,8 @@ int lbm_step(lbm_handle_t* h, int32_t nsteps) {
 int lbm_upload_field(lbm_handle_t* h, int32_t field, const void* src, size_t bytes) {
     API_BEGIN(h)
     if (!src) return fail(h, LBM_E_ARG, "src is NULL");
-    if (field == LBM_FIELD_PIECEWISE_BED_Z) h->slot_valid[0] = h->slot_valid[1] = 0;
+    if (field == LBM_FIELD_PIECEWISE_BED_Z)
+        for (int sl = 0; sl < BED_RING; sl++) h->slot_valid[sl] = 0;
     const lbm_config_t& c = h->cfg;
     DevBufs& b = h->b;
     size_t n = b.n, n2 = b.n2, R = real_size(c);
@@ -517,8 +535,8 @@ int lbm_upload_field(lbm_handle_t* h, int32_t field, const void* src, size_t byt
             // init_constants seeds the bed height with the profile (CODEX:1050-1054)
             std::vector<char> plane(n2 * R);
             for (size_t j = 0; j < (size_t)c.ny; j++) memcpy(plane.data() + j * c.nx * R, src, c.nx * R);
-            CHECK(hipMemcpyAsync(b.h[0], plane.data(), n2 * R, hipMemcpyHostToDevice, h->stream));
-            CHECK(hipMemcpyAsync(b.h[1], plane.data(), n2 * R, hipMemcpyHostToDevice, h->stream));
+            for (int sl = 0; sl < b.ring; sl++)
+                CHECK(hipMemcpyAsync(b.h[sl], plane.data(), n2 * R, hipMemcpyHostToDevice, h->stream));
         }
     }
     CHECK(hipStreamSynchronize(h->stream));
@@ -531,7 +549,7 @@ int lbm_download_field(lbm_handle_t* h, int32_t field, void* dst, size_t bytes) 
     const lbm_config_t& c = h->cfg;
     DevBufs& b = h->b;
     size_t n = b.n, n2 = b.n2, R = real_size(c);
-    int slot = slot_of(h->step);
+    int slot = slot_of(h, h->step);
     const void* src = nullptr;
     size_t need = 0;
     std::vector<void*> tmp;
@@ -654,7 +672,7 @@ int lbm_download_field(lbm_handle_t* h, int32_t field, void* dst, size_t bytes) 
 int lbm_get_counters(lbm_handle_t* h, lbm_counters_t* out) {
     API_BEGIN(h)
     if (!out) return fail(h, LBM_E_ARG, "out is NULL");
-    if (h->path == LBM_PATH_FUSED && h->step > 0) launch_cut_link_count(h->cfg, h->b, slot_of(h->step), h->stream);
+    if (h->path == LBM_PATH_FUSED && h->step > 0) launch_cut_link_count(h->cfg, h->b, slot_of(h, h->step), h->stream);
     int ic[IC_COUNT];
     double dc[DC_COUNT];
     CHECK(hipMemcpyAsync(ic, h->b.icnt, sizeof(ic), hipMemcpyDeviceToHost, h->stream));
@@ -693,8 +711,8 @@ static int reduce_checks(lbm_handle* h, int32_t what, const double* out) {
 
 // the reduction into the device slot dst; part: partial sums (max_parts)
 static void reduce_into(lbm_handle* h, int32_t what, double* part, size_t max_parts, double* dst, double* out) {
-    launch_cellflags(h->cfg, h->b, slot_of(h->step), h->stream);
-    launch_reduce(h->cfg, h->b, h->cur, h->post_stream, slot_of(h->step), what, part, max_parts, dst, h->stream);
+    launch_cellflags(h->cfg, h->b, slot_of(h, h->step), h->stream);
+    launch_reduce(h->cfg, h->b, h->cur, h->post_stream, slot_of(h, h->step), what, part, max_parts, dst, h->stream);
     double v = 0.0;
     CHECK(hipMemcpyAsync(&v, dst, sizeof(double), hipMemcpyDeviceToHost, h->stream));
     CHECK(hipStreamSynchronize(h->stream));

@@ -43,17 +43,24 @@ enum {
     DC_COUNT = 12
 };
 
+constexpr int BED_BATCH = 8;              // moving-bed time levels computed per pair of launches
+constexpr int BED_RING = BED_BATCH + 1;   // the batch never overwrites the level of the pending stream
+
 struct DevBufs {
     int storage;
     size_t n, n2;
     void* f[2];  // populations, storage type, 19*n each; f[cur] is the state
     void* ftmp;  // arithmetic-type scratch of 19*n for materialised downloads
     void *rho, *ux, *uy, *uz, *omega, *nu_t, *smag;
-    void *h[2], *uw[2], *vw[2], *ww[2], *eta9[2], *denom[2], *uw9[2];
-    int* kband[2];
-    int* ksolid[2];
-    void* fsol[2];
-    void* ibb[2];
+    // bed time levels: a ring of `ring` slots, slot of step s = s % ring
+    // (BED_RING for moving beds, whose levels are computed BED_BATCH steps
+    // at a time; 2 for static beds)
+    int ring;
+    void *h[BED_RING], *uw[BED_RING], *vw[BED_RING], *ww[BED_RING], *eta9[BED_RING], *denom[BED_RING], *uw9[BED_RING];
+    int* kband[BED_RING];
+    int* ksolid[BED_RING];
+    void* fsol[BED_RING];
+    void* ibb[BED_RING];
     void* phi;  // phi of the latest bed time level (phi boundary-geometry modes only)
     void *piecewise, *u_inlet, *inflow;
     uint8_t* cellflags;  // bit0 lattice_open[0], bit1 near_obstacle (staged path, current slot)
@@ -92,7 +99,12 @@ struct FusedSched {
 };
 
 // host-side launchers (lbm_kernels.hip)
+// bed arrays of one time level at time t into `slot` (the previous level,
+// for the bed velocity, is slot - 1 of the ring)
 void launch_bed2d(const lbm_config_t& c, DevBufs& b, float t, int slot, hipStream_t s);
+// moving bed: the levels of steps step0 .. step0 + nb - 1 (nb <= BED_BATCH)
+// in two launches; the level of step0 - 1 must be current in its slot
+void launch_bed2d_batch(const lbm_config_t& c, DevBufs& b, int64_t step0, int nb, hipStream_t s);
 void launch_geometry_fields(const lbm_config_t& c, DevBufs& b, int slot, int32_t* lo, void* lf, int32_t* wt, int32_t* near,
                             void* phi, hipStream_t s);
 void launch_cellflags(const lbm_config_t& c, DevBufs& b, int slot, hipStream_t s);

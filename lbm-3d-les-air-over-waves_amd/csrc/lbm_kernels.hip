@@ -188,58 +188,110 @@ __device__ __forceinline__ double block_tree(double v) {
 // 2-D bed kernels
 // ===========================================================================
 
-// update_wave_bed_and_velocities (CODEX:1082-1162), one thread per column;
-// moving_sine1 is the single-harmonic bed of the new configuration
+// update_wave_bed_and_velocities (CODEX:1082-1162), one column; moving_sine1
+// is the single-harmonic bed of the new configuration
+template <class T>
+struct BedEta {
+    T eta1, eta2, eta, phase, phase2, amp_now;
+};
+template <class T>
+__device__ __forceinline__ BedEta<T> bed_eta(const Consts<T>& c, T t, int i) {
+    BedEta<T> e{T(0), T(0), T(0), T(0), T(0), T(0)};
+    const T dx = c.dx;
+    const T kw = T(2.0) * T(3.141592653589793) / c.bed_wavelength_m;
+    const T cs = c.wave_speed;
+    const T ramp = T(2.0) * c.wave_period;
+    if (c.bed_profile == LBM_BED_MOVING_SINE)
+        e.amp_now = t < ramp ? c.bed_amp_m * smoothstep01(t / ramp) : c.bed_amp_m;
+    const T x0 = (T)i * dx;
+    if (c.bed_profile == LBM_BED_MOVING_SINE) {
+        T x_phase = x0 - cs * t;
+        T x_phase2 = x0 - T(0.5) * cs * t;
+        e.phase = T(0.5) * kw * x_phase;
+        e.phase2 = kw * x_phase2;
+        e.eta1 = e.amp_now * (T)det_sincos((double)e.phase, 0);
+        e.eta2 = e.amp_now * (T)det_sincos((double)e.phase2, 0);
+        e.eta = e.eta1 + e.eta2;
+    } else if (c.bed_profile == LBM_BED_MOVING_SINE1) {
+        e.phase = kw * (x0 - c.phase_speed * t);
+        e.eta1 = c.bed_amp_m * (T)det_sincos((double)e.phase, 0);
+        e.eta = e.eta1;
+    }
+    return e;
+}
+template <class T>
+__device__ __forceinline__ T bed_height(const Consts<T>& c, const BedEta<T>& e, int i, const T* piecewise) {
+    T bed_h = c.offset_m + e.eta;
+    if (c.bed_profile == LBM_BED_PIECEWISE) bed_h = piecewise[i];
+    return bed_h;
+}
+// bed height and wall velocity of column (i, .) at time t; hl: the height
+// of the previous time level
+template <class T>
+__device__ __forceinline__ void bed_column(const Consts<T>& c, T t, int i, T hl, const T* piecewise, T& h_out, T& u_out, T& v_out,
+                                           T& w_out) {
+    const BedEta<T> e = bed_eta(c, t, i);
+    const T kw = T(2.0) * T(3.141592653589793) / c.bed_wavelength_m;
+    const T cs = c.wave_speed;
+    const T bed_h = bed_height(c, e, i, piecewise);
+    h_out = bed_h;
+    T u_s = T(0), v_s = T(0), w_s = T(0);
+    T h0 = c.depth_swl_m;
+    if (c.bed_profile == LBM_BED_MOVING_SINE) {
+        T eta_x1 = e.amp_now * T(0.5) * kw * (T)det_sincos((double)e.phase, 1);
+        T eta_x2 = e.amp_now * kw * (T)det_sincos((double)e.phase2, 1);
+        T eta_x = eta_x1 + eta_x2;
+        T eta_y = T(0);
+        T eta_t = (bed_h - hl) / c.dt_phys;
+        u_s = cs * e.eta1 / (h0 + e.eta1) + T(0.5) * cs * e.eta2 / (h0 + e.eta2);
+        w_s = eta_t + u_s * eta_x + v_s * eta_y;
+    } else if (c.bed_profile == LBM_BED_MOVING_SINE1) {
+        T eta_x = c.bed_amp_m * kw * (T)det_sincos((double)e.phase, 1);
+        T eta_t = (bed_h - hl) / c.dt_phys;
+        u_s = c.phase_speed * e.eta1 / (h0 + e.eta1);
+        w_s = eta_t + u_s * eta_x;
+    }
+    u_out = u_s / c.vel_scale;
+    v_out = v_s / c.vel_scale;
+    w_out = w_s / c.vel_scale;
+}
+
 template <class T>
 __global__ __launch_bounds__(NT) void k_bed2d_a(Consts<T> c, T t, T* h, T* uw, T* vw, T* ww, const T* h_last, const T* piecewise) {
     size_t id = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (id >= (size_t)c.nx * c.ny) return;
-    int i = (int)(id % (size_t)c.nx);
-    T dx = c.dx;
-    T kw = T(2.0) * T(3.141592653589793) / c.bed_wavelength_m;
-    T cs = c.wave_speed;
-    T ramp = T(2.0) * c.wave_period;
-    T amp_now = T(0);
-    if (c.bed_profile == LBM_BED_MOVING_SINE)
-        amp_now = t < ramp ? c.bed_amp_m * smoothstep01(t / ramp) : c.bed_amp_m;
-    T x0 = (T)i * dx;
-    T hl = h_last[id];
-    T eta1 = T(0), eta2 = T(0), eta = T(0), phase = T(0), phase2 = T(0);
-    if (c.bed_profile == LBM_BED_MOVING_SINE) {
-        T x_phase = x0 - cs * t;
-        T x_phase2 = x0 - T(0.5) * cs * t;
-        phase = T(0.5) * kw * x_phase;
-        phase2 = kw * x_phase2;
-        eta1 = amp_now * (T)det_sincos((double)phase, 0);
-        eta2 = amp_now * (T)det_sincos((double)phase2, 0);
-        eta = eta1 + eta2;
-    } else if (c.bed_profile == LBM_BED_MOVING_SINE1) {
-        phase = kw * (x0 - c.phase_speed * t);
-        eta1 = c.bed_amp_m * (T)det_sincos((double)phase, 0);
-        eta = eta1;
-    }
-    T bed_h = c.offset_m + eta;
-    if (c.bed_profile == LBM_BED_PIECEWISE) bed_h = piecewise[i];
-    h[id] = bed_h;
-    T u_s = T(0), v_s = T(0), w_s = T(0);
-    T h0 = c.depth_swl_m;
-    if (c.bed_profile == LBM_BED_MOVING_SINE) {
-        T eta_x1 = amp_now * T(0.5) * kw * (T)det_sincos((double)phase, 1);
-        T eta_x2 = amp_now * kw * (T)det_sincos((double)phase2, 1);
-        T eta_x = eta_x1 + eta_x2;
-        T eta_y = T(0);
-        T eta_t = (bed_h - hl) / c.dt_phys;
-        u_s = cs * eta1 / (h0 + eta1) + T(0.5) * cs * eta2 / (h0 + eta2);
-        w_s = eta_t + u_s * eta_x + v_s * eta_y;
-    } else if (c.bed_profile == LBM_BED_MOVING_SINE1) {
-        T eta_x = c.bed_amp_m * kw * (T)det_sincos((double)phase, 1);
-        T eta_t = (bed_h - hl) / c.dt_phys;
-        u_s = c.phase_speed * eta1 / (h0 + eta1);
-        w_s = eta_t + u_s * eta_x;
-    }
-    uw[id] = u_s / c.vel_scale;
-    vw[id] = v_s / c.vel_scale;
-    ww[id] = w_s / c.vel_scale;
+    const int i = (int)(id % (size_t)c.nx);
+    bed_column(c, t, i, h_last[id], piecewise, h[id], uw[id], vw[id], ww[id]);
+}
+
+// BED_BATCH time levels of a moving bed in one launch (blockIdx.y = level).
+// The previous height of level 0 is the stored one; level b > 0 recomputes
+// the height of level b - 1 from its time with the same operations, so it
+// equals the value that level b - 1 stores.
+template <class T>
+struct BedBatch {
+    T t[BED_BATCH], tprev[BED_BATCH];
+    T* h[BED_BATCH];
+    T* uw[BED_BATCH];
+    T* vw[BED_BATCH];
+    T* ww[BED_BATCH];
+    T* eta9[BED_BATCH];
+    T* denom[BED_BATCH];
+    int* kband[BED_BATCH];
+    T* uw9[BED_BATCH];
+    int* ksolid[BED_BATCH];
+    T* fsol[BED_BATCH];
+    T* ibb[BED_BATCH];
+    const T* h_last0;
+};
+template <class T>
+__global__ __launch_bounds__(NT) void k_bed2d_a_batch(Consts<T> c, BedBatch<T> a, const T* piecewise) {
+    const size_t id = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= (size_t)c.nx * c.ny) return;
+    const int b = (int)blockIdx.y;
+    const int i = (int)(id % (size_t)c.nx);
+    const T hl = b == 0 ? a.h_last0[id] : bed_height(c, bed_eta(c, a.tprev[b], i), i, piecewise);
+    bed_column(c, a.t[b], i, hl, piecewise, a.h[b][id], a.uw[b][id], a.vw[b][id], a.ww[b][id]);
 }
 
 // per-column link-endpoint bed heights (CODEX:2536-2569 at the 9 (cx,cy)
@@ -253,13 +305,13 @@ constexpr int BED_COLS = 64;
 // one wave per endpoint: the per-link loop reads e from the first lane
 static_assert(BED_COLS == 64, "k_bed2d_b needs one 64-lane wave per endpoint");
 template <class T>
-__global__ __launch_bounds__(9 * BED_COLS) void k_bed2d_b(Consts<T> c, const T* h, T* eta9, T* denom, int* kband, const T* uw,
-                                                          const T* vw, const T* ww, T* uw9, int* ksolid, T* fsol, T* ibb) {
+__device__ __forceinline__ void bed2d_b_block(const Consts<T>& c, const T* h, T* eta9, T* denom, int* kband, const T* uw,
+                                              const T* vw, const T* ww, T* uw9, int* ksolid, T* fsol, T* ibb, unsigned blk) {
     __shared__ T se[9][BED_COLS];
     __shared__ T sw[27][BED_COLS];
     const int lane = (int)(threadIdx.x % BED_COLS), e = (int)(threadIdx.x / BED_COLS);
     const size_t plane = (size_t)c.nx * c.ny;
-    const size_t id = (size_t)blockIdx.x * BED_COLS + lane;
+    const size_t id = (size_t)blk * BED_COLS + lane;
     const bool live = id < plane;
     const int i = live ? (int)(id % (size_t)c.nx) : 0, j = live ? (int)(id / (size_t)c.nx) : 0;
     const T dx = c.dx;
@@ -380,6 +432,18 @@ __global__ __launch_bounds__(9 * BED_COLS) void k_bed2d_b(Consts<T> c, const T* 
         }
     }
     ksolid[id] = ks;
+}
+
+template <class T>
+__global__ __launch_bounds__(9 * BED_COLS) void k_bed2d_b(Consts<T> c, const T* h, T* eta9, T* denom, int* kband, const T* uw,
+                                                          const T* vw, const T* ww, T* uw9, int* ksolid, T* fsol, T* ibb) {
+    bed2d_b_block(c, h, eta9, denom, kband, uw, vw, ww, uw9, ksolid, fsol, ibb, blockIdx.x);
+}
+template <class T>
+__global__ __launch_bounds__(9 * BED_COLS) void k_bed2d_b_batch(Consts<T> c, BedBatch<T> a) {
+    const int b = (int)blockIdx.y;
+    bed2d_b_block(c, (const T*)a.h[b], a.eta9[b], a.denom[b], a.kband[b], (const T*)a.uw[b], (const T*)a.vw[b],
+                  (const T*)a.ww[b], a.uw9[b], a.ksolid[b], a.fsol[b], a.ibb[b], blockIdx.x);
 }
 
 // ===========================================================================
@@ -2907,7 +2971,7 @@ template <class T>
 static void bed2d_t(const lbm_config_t& cfg, DevBufs& b, float t, int slot, hipStream_t s) {
     Consts<T> c = make_consts<T>(cfg, -1);
     size_t n2 = b.n2;
-    int prev = slot ^ 1;
+    const int prev = (slot + b.ring - 1) % b.ring;
     k_bed2d_a<T><<<nblocks(n2, 64), 64, 0, s>>>(c, (T)t, (T*)b.h[slot], (T*)b.uw[slot], (T*)b.vw[slot], (T*)b.ww[slot],
                                              (const T*)b.h[prev], (const T*)b.piecewise);
     k_bed2d_b<T><<<nblocks(n2, BED_COLS), 9 * BED_COLS, 0, s>>>(c, (const T*)b.h[slot], (T*)b.eta9[slot], (T*)b.denom[slot], b.kband[slot],
@@ -2921,6 +2985,44 @@ void launch_bed2d(const lbm_config_t& c, DevBufs& b, float t, int slot, hipStrea
         bed2d_t<double>(c, b, t, slot, s);
     else
         bed2d_t<float>(c, b, t, slot, s);
+}
+
+// the time of step `step` as the step loop forms it (CODEX:3874)
+static float step_time(const lbm_config_t& c, int64_t step) { return (float)((double)(step + 1) * c.dt_phys); }
+
+template <class T>
+static void bed2d_batch_t(const lbm_config_t& cfg, DevBufs& b, int64_t step0, int nb, hipStream_t s) {
+    if (nb < 1 || nb > BED_BATCH || b.ring != BED_RING) throw std::runtime_error("bed batch: bad size or ring");
+    if (cfg.phi_fraction) throw std::runtime_error("bed batch: the phi modes keep one phi field per step");
+    Consts<T> c = make_consts<T>(cfg, -1);
+    BedBatch<T> a{};
+    for (int k = 0; k < nb; k++) {
+        const int64_t st = step0 + k;
+        const int sl = (int)(st % b.ring);
+        a.t[k] = (T)step_time(cfg, st);
+        a.tprev[k] = (T)step_time(cfg, st - 1);
+        a.h[k] = (T*)b.h[sl];
+        a.uw[k] = (T*)b.uw[sl];
+        a.vw[k] = (T*)b.vw[sl];
+        a.ww[k] = (T*)b.ww[sl];
+        a.eta9[k] = (T*)b.eta9[sl];
+        a.denom[k] = (T*)b.denom[sl];
+        a.kband[k] = b.kband[sl];
+        a.uw9[k] = (T*)b.uw9[sl];
+        a.ksolid[k] = b.ksolid[sl];
+        a.fsol[k] = (T*)b.fsol[sl];
+        a.ibb[k] = (T*)b.ibb[sl];
+    }
+    a.h_last0 = (const T*)b.h[(int)((step0 - 1 + b.ring) % b.ring)];
+    k_bed2d_a_batch<T><<<dim3(nblocks(b.n2, 64), nb), 64, 0, s>>>(c, a, (const T*)b.piecewise);
+    k_bed2d_b_batch<T><<<dim3(nblocks(b.n2, BED_COLS), nb), 9 * BED_COLS, 0, s>>>(c, a);
+    HIP_OK(hipGetLastError());
+}
+void launch_bed2d_batch(const lbm_config_t& c, DevBufs& b, int64_t step0, int nb, hipStream_t s) {
+    if (b.storage == LBM_STORE_FP64)
+        bed2d_batch_t<double>(c, b, step0, nb, s);
+    else
+        bed2d_batch_t<float>(c, b, step0, nb, s);
 }
 
 template <class T>

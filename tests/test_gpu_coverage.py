@@ -271,3 +271,25 @@ def test_c4_full_size_with_a_collapsing_pocket():
     assert c.reduce(abi.REDUCE_VOF_MASS) == pytest.approx(o.reduce(abi.REDUCE_VOF_MASS), rel=1e-9)
     o.close()
     c.close()
+
+
+# ---- moving-bed time levels computed several steps at a time ---------------------
+@pytest.mark.parametrize("profile", ["moving_sine", "moving_sine1"])
+@pytest.mark.parametrize("path", ["fused", "staged"])
+def test_bed_levels_across_batches(profile, path):
+    """The moving bed's levels are computed up to 8 steps per pair of
+    launches into a ring of 9 slots: calls of 1, 8, 9 and 5 steps cross the
+    batch and ring boundaries; the bed, its velocities and the populations
+    stay equal to the oracle's after each call."""
+    print("case bed batches", profile, path, flush=True)
+    env = env_of("waves_256x128x128", LBM_LX_M=64, LBM_LY_M=16, LBM_LZ_M=24, LBM_BED_AMP_M=2, LBM_BED_WAVELENGTH_M=32,
+                 LBM_BED_PROFILE=profile, LBM_KERNEL_PATH=path)
+    cfg, o, c = pair(env)
+    for n in (1, 8, 9, 5):
+        o.step(n)
+        c.step(n)
+        for fid in (abi.FIELD_WATER_H, abi.FIELD_U_WAVE, abi.FIELD_W_WAVE, abi.FIELD_LATTICE_OPEN_FRAC):
+            assert np.array_equal(c.download(fid), o.download(fid)), (n, fid)
+        assert relerr(c.download(abi.FIELD_F), o.download(abi.FIELD_F)) <= 1e-5, n
+    o.close()
+    c.close()

@@ -215,6 +215,34 @@ typedef struct lbm_counters {
 #define LBM_REDUCE_MAX_SPEED 3   /* max |u| over active cells (CODEX:3930-3940) */
 #define LBM_REDUCE_NONFINITE 4   /* count of non-finite populations          */
 
+/* ---- parameters of the host half of the init sequence ------------------
+ * (CODEX:3777-3792): the options of the initial state that the step itself
+ * does not need.  Filled by the Python config layer (lbm_init.init_params)
+ * or by a C caller. */
+#define LBM_INIT_PARAMS_VERSION 1
+#define LBM_INIT_REFERENCE 0     /* init_fields: inlet profile x (1 + TI (2U - 1)) (CODEX:2714-2749) */
+#define LBM_INIT_UNIFORM_NOISE 1 /* new: u0 + U(-a, a) per component */
+#define LBM_INIT_LOGLAW_NOISE 2  /* new: log-law mean x (1 + n U(-1, 1)), also the inflow plane */
+#define LBM_FSI_FLAT 0           /* LBM_FREE_SURFACE_INITIAL (CODEX:1197-1207) */
+#define LBM_FSI_GAUSSIAN 1
+#define LBM_FSI_SOLITARY 2
+#define LBM_SOLVEL_NONE 0        /* LBM_SOLITARY_INITIAL_VELOCITY (CODEX:2756-2800) */
+#define LBM_SOLVEL_SHALLOW 1
+#define LBM_SOLVEL_SHALLOW_DIVFREE 2
+typedef struct lbm_init_params {
+    int32_t version; /* = LBM_INIT_PARAMS_VERSION */
+    int32_t initial_state, fs_initial, vof_block, linear_wave_pressure, solitary_velocity;
+    int32_t hydrostatic_balanced, component_balanced; /* use_*_balanced_pressure (CODEX:79-86) */
+    int32_t pad0;
+    uint64_t seed; /* numpy.random.default_rng(seed) draws */
+    double dx, offset_m, free_surface_base_m, free_surface_depth_m, rho0, g_lb, u_current_lb, u_top_lb, tau0, kappa;
+    double vel_scale, turbulence_intensity;
+    double init_u_lu, init_noise_lu, inflow_u_lu, inflow_z0_cells, inflow_noise;
+    double gaussian_amp_m, gaussian_sigma_m, gaussian_center_x_m;
+    double solitary_amp_m, solitary_x0_m, solitary_direction, solitary_celerity_phys, solitary_kappa_m_inv;
+    double vof_block_size_m[3], vof_block_center_m[3];
+} lbm_init_params_t;
+
 typedef struct lbm_handle lbm_handle_t;
 
 /* Module import + field allocation (CODEX:35-620): validate the config and
@@ -236,6 +264,23 @@ int lbm_geometry(lbm_handle_t *h, float t);
    initialize_vof_mass_from_fill + enforce_vof_interface_layer +
    update_free_surface_height_from_vof (CODEX:1290-1303, 1627-1656, 2146-2161). */
 int lbm_init_populations(lbm_handle_t *h);
+
+/* The whole init sequence of main() (CODEX:3777-3792) in the library, for
+   callers without Python: geometry at t = 0, the host initial fields
+   (lbm_initial_fields) from the t = 0 solid mask and bed, their upload, and
+   lbm_init_populations.  With a piecewise-linear bed the caller uploads
+   LBM_FIELD_PIECEWISE_BED_Z first, as the reference reads it at import. */
+int lbm_initialize(lbm_handle_t *h, const lbm_init_params_t *p);
+
+/* Host half of the init sequence (init_fields CODEX:2714-2800, the free
+   surface CODEX:1197-1287, prepare_initial_density_field CODEX:743-818)
+   into caller buffers, without a device: obstacle0 (nz,ny,nx) the t = 0
+   solid mask, bed (ny,nx) the t = 0 bed height (R) or NULL for the flat
+   level; outputs R (float, or double with FP64 storage): rho, ux, uy, uz
+   (nz,ny,nx), u_inlet (nz), inflow (nz,ny); water runs also fill (R), type
+   (int32) and fs_h (ny,nx, R), else they may be NULL. */
+int lbm_initial_fields(const lbm_config_t *cfg, const lbm_init_params_t *p, const int32_t *obstacle0, const void *bed, void *rho,
+                       void *ux, void *uy, void *uz, void *u_inlet, void *inflow, void *fill, int32_t *type, void *fs_h);
 
 /* nsteps iterations of the step loop body (CODEX:3873-3925); the first step
    advanced is number step_index+1 of the run. */

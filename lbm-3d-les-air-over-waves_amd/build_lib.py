@@ -21,7 +21,7 @@ PKG_DIR = Path(__file__).resolve().parent
 CSRC = PKG_DIR / "csrc"
 BUILD = PKG_DIR / "_build"
 LIB = PKG_DIR / "liblbm.so"
-SOURCES = ["lbm_kernels.hip", "lbm_capi.hip", "lbm_server.cpp"]
+SOURCES = ["lbm_kernels.hip", "lbm_capi.hip", "lbm_server.cpp", "lbm_init.cpp"]
 ARCH = os.environ.get("LBM_OFFLOAD_ARCH", "gfx950")
 
 FLAGS = [

@@ -414,6 +414,41 @@ int lbm_init_populations(lbm_handle_t* h) {
     API_END(h)
 }
 
+int lbm_initialize(lbm_handle_t* h, const lbm_init_params_t* p) {
+    if (!h) return LBM_E_ARG;
+    if (!p) return fail(h, LBM_E_ARG, "params is NULL");
+    if (p->version != LBM_INIT_PARAMS_VERSION) return fail(h, LBM_E_CONFIG, "init params version mismatch");
+    const lbm_config_t& c = h->cfg;
+    const size_t n = h->b.n, n2 = h->b.n2, R = real_size(c);
+    int rc = lbm_geometry(h, 0.0f);
+    if (rc) return rc;
+    std::vector<int32_t> obst(n), type(c.fs_enabled ? n : 0);
+    std::vector<char> bed(n2 * R), rho(n * R), ux(n * R), uy(n * R), uz(n * R), uin(c.nz * R), infl((size_t)c.nz * c.ny * R);
+    std::vector<char> fill(c.fs_enabled ? n * R : 0), fsh(c.fs_enabled ? n2 * R : 0);
+    if ((rc = lbm_download_field(h, LBM_FIELD_OBSTACLE, obst.data(), n * 4))) return rc;
+    if ((rc = lbm_download_field(h, LBM_FIELD_WATER_H, bed.data(), n2 * R))) return rc;
+    rc = lbm_initial_fields(&c, p, obst.data(), bed.data(), rho.data(), ux.data(), uy.data(), uz.data(), uin.data(), infl.data(),
+                            c.fs_enabled ? fill.data() : nullptr, c.fs_enabled ? type.data() : nullptr,
+                            c.fs_enabled ? fsh.data() : nullptr);
+    if (rc) return fail(h, rc, "lbm_initial_fields failed");
+    struct Up {
+        int32_t f;
+        const void* d;
+        size_t b;
+    } ups[] = {{LBM_FIELD_U_INLET_PROFILE, uin.data(), uin.size()}, {LBM_FIELD_INFLOW_UX, infl.data(), infl.size()},
+               {LBM_FIELD_RHO, rho.data(), rho.size()},          {LBM_FIELD_UX, ux.data(), ux.size()},
+               {LBM_FIELD_UY, uy.data(), uy.size()},             {LBM_FIELD_UZ, uz.data(), uz.size()}};
+    for (const Up& u : ups)
+        if ((rc = lbm_upload_field(h, u.f, u.d, u.b))) return rc;
+    if (c.fs_enabled) {
+        if ((rc = lbm_upload_field(h, LBM_FIELD_FS_FILL, fill.data(), fill.size()))) return rc;
+        if ((rc = lbm_upload_field(h, LBM_FIELD_FS_TYPE, type.data(), n * 4))) return rc;
+        if ((rc = lbm_upload_field(h, LBM_FIELD_FS_TYPE_PREV, type.data(), n * 4))) return rc;
+        if ((rc = lbm_upload_field(h, LBM_FIELD_FS_H, fsh.data(), fsh.size()))) return rc;
+    }
+    return lbm_init_populations(h);
+}
+
 int lbm_step(lbm_handle_t* h, int32_t nsteps) {
     API_BEGIN(h)
     if (!h->initialized) return fail(h, LBM_E_STATE, "lbm_init_populations (or an upload of F) must run first");

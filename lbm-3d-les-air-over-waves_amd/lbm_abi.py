@@ -196,6 +196,39 @@ class LbmConfig(ctypes.Structure):
         return out
 
 
+LBM_INIT_PARAMS_VERSION = 1
+INIT_REFERENCE, INIT_UNIFORM_NOISE, INIT_LOGLAW_NOISE = 0, 1, 2
+FSI_FLAT, FSI_GAUSSIAN, FSI_SOLITARY = 0, 1, 2
+SOLVEL_NONE, SOLVEL_SHALLOW, SOLVEL_SHALLOW_DIVFREE = 0, 1, 2
+
+_D = ctypes.c_double
+
+
+class LbmInitParams(ctypes.Structure):
+    """lbm_init_params_t: the options of the host half of the init
+    sequence (CODEX:3777-3792)."""
+    _fields_ = [
+        ("version", _I),
+        ("initial_state", _I),
+        ("fs_initial", _I),
+        ("vof_block", _I),
+        ("linear_wave_pressure", _I),
+        ("solitary_velocity", _I),
+        ("hydrostatic_balanced", _I),
+        ("component_balanced", _I),
+        ("pad0", _I),
+        ("seed", ctypes.c_uint64),
+        *[(n, _D) for n in (
+            "dx", "offset_m", "free_surface_base_m", "free_surface_depth_m", "rho0", "g_lb", "u_current_lb",
+            "u_top_lb", "tau0", "kappa", "vel_scale", "turbulence_intensity", "init_u_lu", "init_noise_lu",
+            "inflow_u_lu", "inflow_z0_cells", "inflow_noise", "gaussian_amp_m", "gaussian_sigma_m",
+            "gaussian_center_x_m", "solitary_amp_m", "solitary_x0_m", "solitary_direction",
+            "solitary_celerity_phys", "solitary_kappa_m_inv")],
+        ("vof_block_size_m", _D * 3),
+        ("vof_block_center_m", _D * 3),
+    ]
+
+
 class LbmCounters(ctypes.Structure):
     _fields_ = [
         ("boundary_cut_link_count", _I),

@@ -74,6 +74,38 @@ class SolverConfig:
         except KeyError as exc:
             raise AttributeError(name) from exc
 
+    def init_params(self) -> abi.LbmInitParams:
+        """lbm_init_params_t of this configuration (the host initial fields,
+        CODEX:2714-2800, 1197-1287, 743-818)."""
+        v = self.values
+        p = abi.LbmInitParams()
+        p.version = abi.LBM_INIT_PARAMS_VERSION
+        p.initial_state = {"reference": abi.INIT_REFERENCE, "uniform_noise": abi.INIT_UNIFORM_NOISE,
+                           "loglaw_noise": abi.INIT_LOGLAW_NOISE}[v["initial_state"]]
+        p.fs_initial = {"gaussian": abi.FSI_GAUSSIAN, "solitary": abi.FSI_SOLITARY}.get(
+            v["free_surface_initial_condition"], abi.FSI_FLAT)
+        p.vof_block = int(v["vof_initial_shape"] == "block")
+        p.linear_wave_pressure = int(v["initial_pressure_mode"] == "linear_wave")
+        p.solitary_velocity = {"none": abi.SOLVEL_NONE, "shallow_water": abi.SOLVEL_SHALLOW,
+                               "shallow_water_divfree": abi.SOLVEL_SHALLOW_DIVFREE}[v["solitary_initial_velocity_mode"]]
+        p.hydrostatic_balanced = int(v["use_hydrostatic_balanced_pressure"])
+        p.component_balanced = int(v["use_component_balanced_pressure"])
+        seed = int(v["seed"])
+        if not 0 <= seed < 1 << 64:
+            raise ValueError(f"LBM_SEED={seed}: the host draws take a seed in [0, 2**64)")
+        p.seed = seed
+        for name in ("dx", "offset_m", "free_surface_base_m", "free_surface_depth_m", "rho0", "g_lb", "u_current_lb",
+                     "u_top_lb", "tau0", "kappa", "vel_scale", "init_u_lu", "init_noise_lu", "inflow_u_lu",
+                     "inflow_z0_cells", "inflow_noise", "gaussian_amp_m", "gaussian_sigma_m", "gaussian_center_x_m",
+                     "solitary_amp_m", "solitary_x0_m", "solitary_direction", "solitary_celerity_phys",
+                     "solitary_kappa_m_inv"):
+            setattr(p, name, float(v[name]))
+        p.turbulence_intensity = float(v["TI"])
+        for a, ax in enumerate("xyz"):
+            p.vof_block_size_m[a] = float(v[f"vof_block_size_{ax}_m"])
+            p.vof_block_center_m[a] = float(v[f"vof_block_center_{ax}_m"])
+        return p
+
     def to_struct(self) -> abi.LbmConfig:
         v = self.values
         s = abi.LbmConfig()

@@ -17,10 +17,17 @@ perturbations, also used as the inflow plane).  All draws use
 numpy.random.default_rng(seed), arrays in C order (nz, ny, nx), components
 ux -> uy -> uz.
 
+Transcendentals are evaluated in double by the C library (math.exp/log/cos/
+sin) and rounded once where the value is float32, and sums run in index
+order, so the library's own restatement (lbm_initial_fields in
+csrc/lbm_init.cpp) reproduces every array bit for bit.
+
 The device finishes the sequence (solid-cell wall velocities, f = feq,
 VOF mass, interface layer) in lbm_init_populations.
 """
 from __future__ import annotations
+
+import math
 
 import numpy as np
 
@@ -37,7 +44,7 @@ def inlet_profile(cfg: SolverConfig) -> np.ndarray:
     k = np.arange(cfg.Nz, dtype=np.float32)
     y_plus = k * u_tau / nu_lbm
     y_plus = np.where(y_plus <= f32(1.0), f32(1.0), y_plus).astype(np.float32)
-    tmp = (f32(1.0) / f32(cfg.kappa)) * np.log(y_plus).astype(np.float32) + B
+    tmp = (f32(1.0) / f32(cfg.kappa)) * _f32_map(math.log, y_plus) + B
     top = tmp[-1] if tmp[-1] != 0.0 else f32(1.0)
     return (tmp / top * f32(cfg.u_top_lb)).astype(np.float32)
 
@@ -50,10 +57,25 @@ def loglaw_profile(cfg: SolverConfig) -> np.ndarray:
     z = (np.arange(cfg.Nz, dtype=np.float64) + 0.5) * dx
     z_rel = z - cfg.offset_m
     z0 = cfg.inflow_z0_cells * dx
-    prof = np.log(np.maximum(z_rel, z0) / z0)
+    prof = np.array([math.log(v) for v in np.maximum(z_rel, z0) / z0], dtype=np.float64)
     above = z_rel > 0.0
-    mean = prof[above].mean() if np.any(above) else 1.0
+    mean = _seq_sum(prof[above]) / int(above.sum()) if np.any(above) else 1.0
     return (cfg.inflow_u_lu * prof / mean).astype(np.float64)
+
+
+def _f32_map(fn, x: np.ndarray) -> np.ndarray:
+    """fn (a math-module function) of each float32 value, evaluated in
+    double and rounded once to float32."""
+    flat = np.asarray(x, dtype=np.float32).ravel()
+    return np.array([fn(float(v)) for v in flat], dtype=np.float64).astype(np.float32).reshape(np.shape(x))
+
+
+def _seq_sum(x: np.ndarray) -> float:
+    """Sum in index order (numpy's pairwise sum rounds differently)."""
+    s = 0.0
+    for v in np.asarray(x, dtype=np.float64).ravel():
+        s += float(v)
+    return s
 
 
 def initial_fields(cfg: SolverConfig, obstacle0: np.ndarray, bed: np.ndarray = None) -> dict:
@@ -134,8 +156,8 @@ def solitary_profile(cfg: SolverConfig, t: float = 0.0):
     center = f32(cfg.solitary_x0_m) + f32(cfg.solitary_direction) * f32(cfg.solitary_celerity_phys) * f32(t)
     xp = ((np.arange(cfg.Nx, dtype=np.float32) + f32(0.5)) * dx).astype(np.float32)
     arg = (kappa * _periodic_offset(cfg, (xp - center).astype(np.float32))).astype(np.float32)
-    ep = np.exp(arg).astype(np.float32)
-    em = np.exp(-arg).astype(np.float32)
+    ep = _f32_map(math.exp, arg)
+    em = _f32_map(math.exp, -arg)
     den = (ep + em).astype(np.float32)
     return (f32(2.0) / den).astype(np.float32), ((ep - em) / den).astype(np.float32)
 
@@ -160,7 +182,7 @@ def surface_height(cfg: SolverConfig) -> np.ndarray:
         xrel = np.where(xrel > f32(0.5) * lx, xrel - lx, xrel).astype(np.float32)
         xrel = np.where(xrel < f32(-0.5) * lx, xrel + lx, xrel).astype(np.float32)
     arg = (xrel / f32(cfg.gaussian_sigma_m)).astype(np.float32)
-    h = (f32(cfg.free_surface_base_m) + f32(cfg.gaussian_amp_m) * np.exp(f32(-0.5) * arg * arg)).astype(np.float32)
+    h = (f32(cfg.free_surface_base_m) + f32(cfg.gaussian_amp_m) * _f32_map(math.exp, f32(-0.5) * arg * arg)).astype(np.float32)
     return np.broadcast_to(h[None, :], (ny, nx)).astype(np.float32)
 
 
@@ -239,8 +261,10 @@ def linear_wave_head(cfg: SolverConfig, eta: np.ndarray, bed: float) -> np.ndarr
     horizontal mode k of eta decays with depth as cosh(k z') / cosh(k h)
     (z' = height above the bed, capped at h = max(base - bed, dx)).  Modes:
     cos(n pi x / Lx) for solid x walls (discrete cosine series), else the
-    periodic Fourier modes.  Returns head[k_plane, i] in float64."""
+    periodic Fourier modes through a direct real DFT.  Sums run in index
+    order.  Returns head[k_plane, i] in float64."""
     nx, nz, dx = cfg.Nx, cfg.Nz, cfg.dx
+    eta = np.asarray(eta, dtype=np.float64)
     depth = max(cfg.free_surface_base_m - bed, dx)
     x = (np.arange(nx, dtype=np.float64) + 0.5) * dx
     z = (np.arange(nz, dtype=np.float64) + 0.5) * dx
@@ -248,27 +272,50 @@ def linear_wave_head(cfg: SolverConfig, eta: np.ndarray, bed: float) -> np.ndarr
     def decay(k, zr):
         # cosh(k zr) / cosh(k depth) without overflow: (e^{k(zr-h)} + e^{-k(zr+h)}) / (1 + e^{-2kh})
         out = np.ones_like(k)
-        m = k > 0.0
-        kh = k[m] * depth
-        kz = k[m] * zr
-        out[m] = (np.exp(kz - kh) + np.exp(-kz - kh)) / (1.0 + np.exp(-2.0 * kh))
+        for m, km in enumerate(k):
+            if km > 0.0:
+                kh, kz = km * depth, km * zr
+                out[m] = (math.exp(kz - kh) + math.exp(-kz - kh)) / (1.0 + math.exp(-2.0 * kh))
         return out
 
     head = np.empty((nz, nx), dtype=np.float64)
     if cfg.x_boundary_mode == "solid":
-        k = np.arange(nx, dtype=np.float64) * np.pi / (nx * dx)
-        basis = np.cos(np.outer(k, x))             # basis[n, i] = cos(k_n x_i)
-        amp = (2.0 / nx) * (basis @ eta)
-        amp[0] = np.mean(eta)
+        k = np.arange(nx, dtype=np.float64) * math.pi / (nx * dx)
+        basis = np.array([[math.cos(km * xi) for xi in x] for km in k])   # basis[n, i] = cos(k_n x_i)
+        amp = np.zeros(nx)
+        for i in range(nx):
+            amp += basis[:, i] * eta[i]
+        amp = (2.0 / nx) * amp
+        amp[0] = _seq_sum(eta) / nx
         for kk, zc in enumerate(z):
             zr = min(max(zc - bed, 0.0), depth)
-            head[kk] = (amp * decay(k, zr)) @ basis
-    else:
-        k = 2.0 * np.pi * np.fft.rfftfreq(nx, d=dx)
-        spec = np.fft.rfft(eta)
-        for kk, zc in enumerate(z):
-            zr = min(max(zc - bed, 0.0), depth)
-            head[kk] = np.fft.irfft(spec * decay(k, zr), n=nx)
+            w = amp * decay(k, zr)
+            s = np.zeros(nx)
+            for m in range(nx):
+                s += w[m] * basis[m]
+            head[kk] = s
+        return head
+    nm = nx // 2 + 1
+    ang = [[2.0 * math.pi * ((m * i) % nx) / nx for i in range(nx)] for m in range(nm)]
+    cs = np.array([[math.cos(a) for a in row] for row in ang])
+    sn = np.array([[math.sin(a) for a in row] for row in ang])
+    re = np.zeros(nm)
+    im = np.zeros(nm)
+    for i in range(nx):
+        re += eta[i] * cs[:, i]
+        im -= eta[i] * sn[:, i]
+    k = 2.0 * math.pi * np.arange(nm, dtype=np.float64) / (nx * dx)
+    nyq = nx // 2 if nx % 2 == 0 else -1
+    for kk, zc in enumerate(z):
+        zr = min(max(zc - bed, 0.0), depth)
+        d = decay(k, zr)
+        s = np.full(nx, re[0] * d[0])
+        for m in range(1, nm):
+            if m == nyq:
+                s += d[m] * re[m] * cs[m]
+            else:
+                s += 2.0 * (d[m] * (re[m] * cs[m] - im[m] * sn[m]))
+        head[kk] = s / nx
     return head
 
 
@@ -282,7 +329,11 @@ def initial_density(cfg: SolverConfig, fs_h: np.ndarray, bed: np.ndarray = None)
     bed64 = bed.astype(np.float64)
     flat = np.allclose(bed64, bed64[0, 0], atol=1.0e-12, rtol=0.0)
     if cfg.initial_pressure_mode == "linear_wave" and cfg.free_surface_initial_condition == "gaussian" and flat:
-        eta = fs_h.astype(np.float64).mean(axis=0) - cfg.free_surface_base_m
+        rows = fs_h.astype(np.float64)
+        eta = np.zeros(nx)
+        for j in range(ny):
+            eta += rows[j]
+        eta = eta / ny - cfg.free_surface_base_m
         head = linear_wave_head(cfg, eta, float(bed64[0, 0]))
         if cfg.use_hydrostatic_balanced_pressure != 1:
             z = (np.arange(nz, dtype=np.float64) + 0.5) * cfg.dx

@@ -24,6 +24,8 @@ EXPORTS = [
     "lbm_destroy",
     "lbm_geometry",
     "lbm_init_populations",
+    "lbm_initialize",
+    "lbm_initial_fields",
     "lbm_step",
     "lbm_upload_field",
     "lbm_download_field",
@@ -62,6 +64,10 @@ def load_library(path: Optional[os.PathLike] = None) -> ctypes.CDLL:
     lib.lbm_destroy.restype = None
     lib.lbm_geometry.argtypes = [H, ctypes.c_float]
     lib.lbm_init_populations.argtypes = [H]
+    lib.lbm_initialize.argtypes = [H, ctypes.POINTER(abi.LbmInitParams)]
+    P = ctypes.c_void_p
+    lib.lbm_initial_fields.argtypes = [ctypes.POINTER(abi.LbmConfig), ctypes.POINTER(abi.LbmInitParams), P, P, P, P, P, P,
+                                       P, P, P, P, P]
     lib.lbm_step.argtypes = [H, ctypes.c_int32]
     lib.lbm_upload_field.argtypes = [H, ctypes.c_int32, ctypes.c_void_p, ctypes.c_size_t]
     lib.lbm_download_field.argtypes = [H, ctypes.c_int32, ctypes.c_void_p, ctypes.c_size_t]
@@ -103,6 +109,39 @@ def device_bytes(cfg: abi.LbmConfig) -> int:
     if rc != 0:
         raise LbmError(f"lbm_device_bytes: config rejected ({rc})")
     return int(out.value)
+
+
+def initial_fields_native(cfg: abi.LbmConfig, params: abi.LbmInitParams, obstacle0: np.ndarray,
+                          bed: Optional[np.ndarray] = None) -> dict:
+    """lbm_initial_fields: the host initial fields, computed by the library
+    without a device, keyed by field id like lbm_init.initial_fields."""
+    lib = load_library()
+    nx, ny, nz = cfg.nx, cfg.ny, cfg.nz
+    real = np.float64 if cfg.storage == abi.STORE_FP64 else np.float32
+    obst = np.ascontiguousarray(obstacle0, dtype=np.int32).reshape(nz, ny, nx)
+    bed_a = None if bed is None else np.ascontiguousarray(bed, dtype=real).reshape(ny, nx)
+    out = {f: np.empty((nz, ny, nx), real) for f in (abi.FIELD_RHO, abi.FIELD_UX, abi.FIELD_UY, abi.FIELD_UZ)}
+    out[abi.FIELD_U_INLET_PROFILE] = np.empty(nz, real)
+    out[abi.FIELD_INFLOW_UX] = np.empty((nz, ny), real)
+    water = cfg.fs_enabled != 0
+    if water:
+        out[abi.FIELD_FS_FILL] = np.empty((nz, ny, nx), real)
+        out[abi.FIELD_FS_TYPE] = np.empty((nz, ny, nx), np.int32)
+        out[abi.FIELD_FS_H] = np.empty((ny, nx), real)
+
+    def ptr(f):
+        return out[f].ctypes.data if f in out else None
+
+    rc = lib.lbm_initial_fields(ctypes.byref(cfg), ctypes.byref(params), obst.ctypes.data,
+                                None if bed_a is None else bed_a.ctypes.data, ptr(abi.FIELD_RHO), ptr(abi.FIELD_UX),
+                                ptr(abi.FIELD_UY), ptr(abi.FIELD_UZ), ptr(abi.FIELD_U_INLET_PROFILE),
+                                ptr(abi.FIELD_INFLOW_UX), ptr(abi.FIELD_FS_FILL), ptr(abi.FIELD_FS_TYPE),
+                                ptr(abi.FIELD_FS_H))
+    if rc != 0:
+        raise LbmError(f"lbm_initial_fields failed ({rc})")
+    if water:
+        out[abi.FIELD_FS_TYPE_PREV] = out[abi.FIELD_FS_TYPE].copy()
+    return out
 
 
 class Case:
@@ -147,6 +186,10 @@ class Case:
 
     def init_populations(self):
         self._check(self.lib.lbm_init_populations(self.h), "lbm_init_populations")
+
+    def initialize(self, params: abi.LbmInitParams):
+        """lbm_initialize: the whole init sequence inside the library."""
+        self._check(self.lib.lbm_initialize(self.h, ctypes.byref(params)), "lbm_initialize")
 
     def step(self, n: int = 1):
         self._check(self.lib.lbm_step(self.h, int(n)), "lbm_step")

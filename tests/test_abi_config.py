@@ -41,12 +41,15 @@ def test_library_exports_every_declared_symbol():
 def test_struct_layout_matches_c():
     fields = [n for n, _ in abi.LbmConfig._fields_]
     cnt = [n for n, _ in abi.LbmCounters._fields_]
+    ini = [n for n, _ in abi.LbmInitParams._fields_]
     src = ['#include <stdio.h>', '#include <stddef.h>', f'#include "{HEADER}"', "int main(void){"]
-    src.append('printf("%zu %zu\\n", sizeof(lbm_config_t), sizeof(lbm_counters_t));')
+    src.append('printf("%zu %zu %zu\\n", sizeof(lbm_config_t), sizeof(lbm_counters_t), sizeof(lbm_init_params_t));')
     for n in fields:
         src.append(f'printf("c {n} %zu\\n", offsetof(lbm_config_t, {n}));')
     for n in cnt:
         src.append(f'printf("k {n} %zu\\n", offsetof(lbm_counters_t, {n}));')
+    for n in ini:
+        src.append(f'printf("i {n} %zu\\n", offsetof(lbm_init_params_t, {n}));')
     src.append("return 0;}")
     with tempfile.TemporaryDirectory() as d:
         c = Path(d) / "l.c"
@@ -54,14 +57,15 @@ def test_struct_layout_matches_c():
         exe = Path(d) / "l"
         subprocess.run(["gcc", str(c), "-o", str(exe)], check=True)
         lines = subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split("\n")
-    a, b = map(int, lines[0].split())
+    a, b, i = map(int, lines[0].split())
     assert a == ctypes.sizeof(abi.LbmConfig)
     assert b == ctypes.sizeof(abi.LbmCounters)
+    assert i == ctypes.sizeof(abi.LbmInitParams)
     for line in lines[1:]:
         if not line:
             continue
         kind, name, off = line.split()
-        struct = abi.LbmConfig if kind == "c" else abi.LbmCounters
+        struct = {"c": abi.LbmConfig, "k": abi.LbmCounters, "i": abi.LbmInitParams}[kind]
         assert getattr(struct, name).offset == int(off), name
 
 

@@ -364,6 +364,10 @@ int lbm_create(const lbm_config_t* cfg, lbm_handle_t** out) {
         int ndev = 0;
         if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) throw std::runtime_error("no HIP device visible");
         CHECK(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+        int dev = 0, cus = 0;
+        CHECK(hipGetDevice(&dev));
+        CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+        h->sched.cus = cus > 0 ? cus : 256;
         allocate(h);
         // the static scratch of lbm_reduce_noalloc: its address on this
         // device, looked up here so that the entry point itself does nothing
@@ -827,8 +831,12 @@ int lbm_set_option(lbm_handle_t* h, const char* key, int64_t value) {
         return LBM_OK;
     }
     if (!strcmp(key, "wave_lag")) {
-        if (value < 1 || value > 4096) return fail(h, LBM_E_ARG, "wave_lag must be in [1, 4096]");
+        if (value < 0 || value > 4096) return fail(h, LBM_E_ARG, "wave_lag must be in [0, 4096] (0: from the tiles a device holds)");
         h->sched.lag = (int)value;
+        return LBM_OK;
+    }
+    if (!strcmp(key, "xcd_strips")) {
+        h->sched.strips = value ? 1 : 0;
         return LBM_OK;
     }
     if (!strcmp(key, "block_x")) {

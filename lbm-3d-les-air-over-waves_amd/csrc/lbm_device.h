@@ -98,9 +98,12 @@ struct Store<T, T> {  // full-precision storage: the population itself
     static __device__ __forceinline__ T load(const T* p, I i, int, const Codec&) {
         return p[i];
     }
+    // populations are stored once per step and read back only by the next
+    // step: a non-temporal store keeps them from displacing the lines the
+    // running step still reads from the L2
     template <class I>
     static __device__ __forceinline__ void store(T* p, I i, int, T v, const Codec&, uint32_t) {
-        p[i] = v;
+        __builtin_nontemporal_store(v, p + i);
     }
 };
 template <>
@@ -129,7 +132,7 @@ struct Store<__half, float> {  // (f_q - s_q) * 2^k as IEEE half
     }
     template <class I>
     static __device__ __forceinline__ void store(__half* p, I i, int q, float v, const Codec& k, uint32_t base) {
-        p[i] = __ushort_as_half(encode(v, q, k, base));
+        __builtin_nontemporal_store(encode(v, q, k, base), reinterpret_cast<uint16_t*>(p) + i);  // as above
     }
 };
 __device__ __forceinline__ float bf16_to_float(bf16_t b) { return __uint_as_float(((uint32_t)b.v) << 16); }
@@ -173,7 +176,7 @@ struct Store<bf16_t, float> {  // (f_q - s_q) * 2^k as bfloat16
     }
     template <class I>
     static __device__ __forceinline__ void store(bf16_t* p, I i, int q, float v, const Codec& k, uint32_t base) {
-        p[i].v = encode(v, q, k, base);
+        __builtin_nontemporal_store(encode(v, q, k, base), reinterpret_cast<uint16_t*>(p) + i);  // as above
     }
 };
 

@@ -90,11 +90,14 @@ inline size_t wave_flag_ints(int nx, int ny, int nz) {
 }
 
 // schedule of the fused step: wavefront (both passes in one launch, tiles
-// taken in ticket order) or three launches; lag = planes of moments ahead
-// of the collision
+// in block-index order) or three launches; lag = planes of moments ahead of
+// the collision, 0 for about as many tiles as the device holds at once
+// (cus x 6 blocks: 12 planes at 256x128, 3 at 512x256, the measured best)
 struct FusedSched {
     int wavefront = 1;  // FP32 storage (the other storage types keep three launches)
-    int lag = 8;
+    int lag = 0;
+    int cus = 256;      // compute units of the device
+    int strips = 1;     // XCD-aware tile order of the wavefront schedule (rows of tiles a multiple of 8)
     int epoch = 0;  // launches so far (the tile flags of launch e hold e)
 };
 

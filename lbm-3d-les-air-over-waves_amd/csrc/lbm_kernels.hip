@@ -2044,6 +2044,7 @@ struct FusedArgs {
     int write_macro;
     int plane0;  // first plane of this launch
     int band_planes;  // k_collide_all: planes 0 .. band_planes - 1 take the band rules
+    int strips;       // tiles of a plane dealt to the XCDs in strips of rows (tile_of)
     uint32_t step;
 };
 
@@ -2548,10 +2549,25 @@ __device__ __forceinline__ void fused_moments(const Consts<T>& c, int k, const T
 }
 
 // cell of this thread; false outside the grid
+// XCD-aware order of the tiles of a plane.  The blocks of a launch are dealt
+// to the 8 XCDs in turn (block L to XCD L mod 8) and each XCD has an L2 of
+// its own.  With rows of tiles a multiple of 8 (so a plane is too), the b-th
+// tile of a plane in dispatch order is taken from strip b mod 8, a band of
+// gy / 8 consecutive rows of tiles: the velocity rows j +- 1 that a
+// collision tile reads were written through the same L2, except at the 16
+// strip edges.  Planes k +- 1 keep the same XCD either way.
+__device__ __forceinline__ void tile_of(int b, int gx, int gy, int& bxi, int& byi) {
+    const int s = b >> 3;
+    byi = (b & 7) * (gy >> 3) + s / gx;
+    bxi = s % gx;
+}
+
 template <class S, class T>
 __device__ __forceinline__ bool fused_cell(const FusedArgs<S, T>& a, int& i, int& j, int& k) {
-    i = blockIdx.x * (int)blockDim.x + (int)threadIdx.x;
-    j = blockIdx.y * (int)blockDim.y + (int)threadIdx.y;
+    int bxi = (int)blockIdx.x, byi = (int)blockIdx.y;
+    if (a.strips) tile_of(bxi + (int)gridDim.x * byi, (int)gridDim.x, (int)gridDim.y, bxi, byi);
+    i = bxi * (int)blockDim.x + (int)threadIdx.x;
+    j = byi * (int)blockDim.y + (int)threadIdx.y;
     k = blockIdx.z + a.plane0;
     return i < a.c.nx && j < a.c.ny;
 }
@@ -2803,10 +2819,17 @@ __global__ __launch_bounds__(FB_THREADS, 6) void k_step_wave(FusedArgs<S, T> a, 
     const int tid = (int)threadIdx.x;
     const int nz = a.c.nz;
     const int t = (int)blockIdx.x;
-    const int u = t / w.bpp, b = t - u * w.bpp;
+    const int u = t / w.bpp, slot = t - u * w.bpp;
     int pass, k;
     wave_unit(u, w.lag, nz, pass, k);
-    const int byi = b / w.gx, bxi = b - byi * w.gx;
+    int bxi, byi;
+    if (a.strips) {
+        tile_of(slot, w.gx, w.gy, bxi, byi);
+    } else {
+        byi = slot / w.gx;
+        bxi = slot - byi * w.gx;
+    }
+    const int b = byi * w.gx + bxi;  // the tile's own index (its flag)
     const int i = bxi * w.bx + tid % w.bx, j = byi * (FB_THREADS / w.bx) + tid / w.bx;
     const bool in = i < a.c.nx && j < a.c.ny;
     const bool band = k < a.band_planes;
@@ -3337,6 +3360,9 @@ static void fused_t(const lbm_config_t& cfg, DevBufs& b, int& cur, int mode, int
                    : (!cfg.enable_top_sponge && !(cfg.c_backscatter > 0.0f)) ? CM_KBC
                                                                                : CM_ANY;
     a.band_planes = nb;
+    // the XCD strips pay on the wavefront schedule (FP32: 6 % at 512x256x192,
+    // its traffic 20 % lower); the VALU-bound 16-bit passes ran 1.5 % slower
+    a.strips = fs.strips && gy % 8 == 0 && fs.wavefront && std::is_same<S, float>::value;
     // FP32 storage only: with FP64 the merged kernel spills to scratch, and
     // the VALU-bound 16-bit forms ran 20-30 % slower merged (SGPR spills)
     if (fs.wavefront && std::is_same<S, float>::value) {
@@ -3344,11 +3370,12 @@ static void fused_t(const lbm_config_t& cfg, DevBufs& b, int& cur, int mode, int
         w.flag = b.waveq + WAVE_FLAG0;
         w.err = b.waveq;
         w.epoch = fs.epoch;
-        w.lag = std::max(1, std::min(fs.lag, cfg.nz));
+        w.bpp = (int)(gx * gy);
+        const int lag = fs.lag > 0 ? fs.lag : (6 * fs.cus + w.bpp - 1) / w.bpp;
+        w.lag = std::max(1, std::min(lag, cfg.nz));
         w.bx = bxd;
         w.gx = (int)gx;
         w.gy = (int)gy;
-        w.bpp = (int)(gx * gy);
         auto kern = cm == CM_REG ? k_step_wave<S, T, CM_REG> : (cm == CM_KBC ? k_step_wave<S, T, CM_KBC> : k_step_wave<S, T, CM_ANY>);
         a.plane0 = 0;
         kern<<<2 * cfg.nz * w.bpp, FB_THREADS, 0, s>>>(a, w);

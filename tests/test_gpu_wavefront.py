@@ -79,3 +79,19 @@ def test_wavefront_counters_reset_between_calls():
     f = c.download(abi.FIELD_F)
     c.close()
     assert np.array_equal(f, ref[abi.FIELD_F])
+
+
+@pytest.mark.parametrize("block_x,lx", [(64, 64), (128, 128), (256, 256)])
+def test_wavefront_xcd_strips(block_x, lx):
+    """Rows of tiles a multiple of 8 (32 rows: 8, 16 and 32 rows of tiles),
+    so the tiles of a plane are dealt to the XCDs in strips: the strip order
+    and the plain order give the three-launch state bit for bit, at a lag
+    below and above the tiles of a plane."""
+    env = env_of("waves_256x128x128", LBM_LX_M=lx, LBM_LY_M=32, LBM_LZ_M=20, LBM_BED_AMP_M=2, LBM_BED_WAVELENGTH_M=32)
+    print("case strips", block_x, lx, flush=True)
+    ref = run(env, 20, wavefront=0, block_x=block_x)
+    for strips in (0, 1):
+        for lag in (1, 0, 5):
+            got = run(env, 20, wavefront=1, wave_lag=lag, block_x=block_x, xcd_strips=strips)
+            for fid, a in ref.items():
+                assert np.array_equal(got[fid], a), (block_x, strips, lag, fid, int(np.sum(got[fid] != a)))

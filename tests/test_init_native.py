@@ -21,9 +21,10 @@ pytestmark = pytest.mark.skipif(not LIB_PATH.exists(), reason="liblbm.so not bui
 
 
 def water(**kw):
-    env = {"LBM_PHYSICS_MODE": "water", "LBM_FREE_SURFACE_MODE": "height_kinematic", "LBM_FREE_SURFACE_TRACKING": "vof",
-           "LBM_DX_M": "1", "LBM_LX_M": "48", "LBM_LY_M": "4", "LBM_LZ_M": "24", "LBM_BED_PROFILE": "flat",
-           "LBM_BED_LEVEL_M": "2", "LBM_FREE_SURFACE_DEPTH_M": "12", "LBM_G_LB": "1e-5"}
+    env = dict(preset_env("vof_192x96x160"), LBM_LX_M="48", LBM_LY_M="4", LBM_LZ_M="24", LBM_BED_LEVEL_M="2",
+               LBM_FREE_SURFACE_DEPTH_M="12", LBM_VOF_INITIAL_SHAPE="none", LBM_PRESSURE_FORMULATION="total_pressure",
+               LBM_VOF_BLOCK_SIZE_M="6", LBM_VOF_BLOCK_BOTTOM_M="16", LBM_VOF_COLLAPSE_INTERVAL="25",
+               LBM_VOF_COLLAPSE_MAX_VOLUME_M3="64")
     env.update({k: str(v) for k, v in kw.items()})
     return env
 

@@ -47,6 +47,7 @@ struct lbm_handle {
     double piecewise_max = 0.0;
     int block_x = 64;  // x extent of a fused-kernel block (64, 128 or 256 cells; set from nx at creation)
     FusedSched sched;  // fused path: wavefront (one launch per step) or three launches
+    StagedState staged;  // staged path: what a step may reuse from the previous one
     int initialized = 0;
     int geometry_done = 0;
     int timing = 0;
@@ -149,7 +150,7 @@ static size_t device_bytes(const lbm_config_t& c, int path) {
     bytes += 5 * n * R + n;                      // rho, u, omega, cell flags
     if (path == LBM_PATH_STAGED) bytes += 2 * n * R + 2 * n * 4;  // nu_t, smag, obstacle masks
     bytes += (size_t)bed_ring(c) * n2 * (78 * R + 8) + (c.nx + c.nz + (size_t)c.nz * c.ny) * R;
-    if (c.fs_enabled) bytes += n * (11 * R + 7 * 4) + 6 * n * R + 3 * n2 * R;
+    if (c.fs_enabled) bytes += n * (12 * R + 7 * 4 + 1) + 6 * n * R + 3 * n2 * R;
     if (c.phi_fraction) bytes += n * R;
     return bytes + (1 << 20);
 }
@@ -248,6 +249,7 @@ static void allocate(lbm_handle* h) {
         b.mass2 = dalloc(h, n * R);
         b.mdelta = dalloc(h, n * R);
         b.excess = dalloc(h, n * R);
+        b.excess2 = dalloc(h, n * R);
         b.sa = dalloc(h, n * R);
         b.sb = dalloc(h, n * R);
         b.sc = dalloc(h, n * R);
@@ -262,6 +264,8 @@ static void allocate(lbm_handle* h) {
         b.excess_dir = (int*)dalloc(h, n * 4);
         b.ext_gas = (int*)dalloc(h, n * 4);
         b.si = (int*)dalloc(h, n * 4);
+        b.mdirty = (uint8_t*)dalloc(h, n);
+        CHECK(hipMemsetAsync(b.mdirty, 0, n, h->stream));
         CHECK(hipMemsetAsync(b.mdelta, 0, n * R, h->stream));
         CHECK(hipMemsetAsync(b.excess, 0, n * R, h->stream));
         CHECK(hipMemsetAsync(b.excess_dir, 0, n * 4, h->stream));
@@ -399,6 +403,7 @@ int lbm_geometry(lbm_handle_t* h, float t) {
     API_BEGIN(h)
     launch_bed2d(h->cfg, h->b, t, 0, h->stream);
     for (int sl = 0; sl < BED_RING; sl++) h->slot_valid[sl] = sl == 0;
+    h->staged = StagedState{};
     launch_obstacle(h->cfg, h->b, 0, h->b.obst, h->stream);
     h->geometry_done = 1;
     h->step = 0;
@@ -410,6 +415,7 @@ int lbm_init_populations(lbm_handle_t* h) {
     API_BEGIN(h)
     if (!h->geometry_done) return fail(h, LBM_E_STATE, "lbm_geometry must run before lbm_init_populations");
     h->cur = 0;
+    h->staged = StagedState{};
     launch_init_populations(h->cfg, h->b, 0, h->stream);
     h->post_stream = 1;
     h->step = 0;
@@ -487,7 +493,7 @@ int lbm_step(lbm_handle_t* h, int32_t nsteps) {
         if (h->timing) CHECK(hipEventRecord(h->ev[2 + 2 * s], h->stream));
         if (h->path == LBM_PATH_STAGED) {
             if (!h->post_stream) throw std::runtime_error("internal: staged step on a post-collision state");
-            launch_staged_step(c, h->b, h->cur, slot, step, h->omega_band_top, s == nsteps - 1, h->stream);
+            launch_staged_step(c, h->b, h->cur, slot, step, h->omega_band_top, s == nsteps - 1, h->staged, h->stream);
         } else {
             int last = s == nsteps - 1;
             if (h->sched.epoch >= INT32_MAX - 1) {
@@ -524,6 +530,7 @@ int lbm_step(lbm_handle_t* h, int32_t nsteps) {
 int lbm_upload_field(lbm_handle_t* h, int32_t field, const void* src, size_t bytes) {
     API_BEGIN(h)
     if (!src) return fail(h, LBM_E_ARG, "src is NULL");
+    h->staged = StagedState{};  // any field set from outside: recompute what the staged step reuses
     if (field == LBM_FIELD_PIECEWISE_BED_Z)
         for (int sl = 0; sl < BED_RING; sl++) h->slot_valid[sl] = 0;
     const lbm_config_t& c = h->cfg;

@@ -27,7 +27,8 @@ enum {
     IC_VOF_COMPRESS,
     IC_FLOOD_CHANGED,      // set by a flood-fill sweep that marked a cell
     IC_FLOOD_UNCONVERGED,  // sweeps exhausted before the fixed point
-    IC_REDIST_DONORS,      // nonzero when a redistribution round has a donor
+    IC_REDIST_DONORS,      // nonzero when a redistribution round has a donor (two slots,
+    IC_REDIST_DONORS2,     // used by alternate rounds)
     IC_COUNT
 };
 // FP64 counters (indices into DevBufs::dcnt)
@@ -65,9 +66,10 @@ struct DevBufs {
     void *piecewise, *u_inlet, *inflow;
     uint8_t* cellflags;  // bit0 lattice_open[0], bit1 near_obstacle (staged path, current slot)
     // water / VOF
-    void *fill, *mass, *mass2, *mdelta, *excess, *sa, *sb, *sc, *sd, *se, *smv, *fs_h;
+    void *fill, *mass, *mass2, *mdelta, *excess, *excess2, *sa, *sb, *sc, *sd, *se, *smv, *fs_h;
     void *fs_qx, *fs_qy;  // column volume fluxes (height-function tracking)
     int *type, *type_prev, *excess_dir, *ext_gas, *obst, *obst_prev, *si;
+    uint8_t* mdirty;  // VOF runs: cells whose populations changed after the stream's macro step
     int* icnt;      // [IC_COUNT]
     double* dcnt;   // [DC_COUNT]
     int* waveq;     // fused path, wavefront step: [0] error flag, [WAVE_FLAG0 + t] epoch of moments tile t
@@ -101,6 +103,13 @@ struct FusedSched {
     int epoch = 0;  // launches so far (the tile flags of launch e hold e)
 };
 
+// what the staged step may reuse from the step before (cleared by the C ABI
+// whenever the geometry or any field is set from outside)
+struct StagedState {
+    int geo_ready = 0;    // cell flags and obstacle masks hold the (static) geometry
+    int macro_ready = 0;  // rho and u hold the last stream's macro step (VOF runs, no conditioning)
+};
+
 // host-side launchers (lbm_kernels.hip)
 // bed arrays of one time level at time t into `slot` (the previous level,
 // for the bed velocity, is slot - 1 of the ring)
@@ -115,7 +124,7 @@ void launch_init_populations(const lbm_config_t& c, DevBufs& b, int slot, hipStr
 // one staged step: bed arrays for slot must be current; advances b.f[cur]
 // count_links: count the stream's link counters (the last step of a call)
 void launch_staged_step(const lbm_config_t& c, DevBufs& b, int& cur, int slot, int64_t step, int omega_band_top, int count_links,
-                        hipStream_t s);
+                        StagedState& st, hipStream_t s);
 // fused step: mode 0 reads post-stream populations, mode 1 post-collision
 // ones; slot_prev is the time level of the pending stream, slot the new one
 // band_top: highest k at which any cell can have a cut or closed link

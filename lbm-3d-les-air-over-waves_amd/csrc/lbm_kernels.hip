@@ -620,18 +620,41 @@ __global__ __launch_bounds__(NT) void k_macro(Consts<T> c, const T* f, T* rho, T
     uz[id] = w;
 }
 
-// compute_LES (CODEX:3011-3141)
+// the macro step of a water run whose rho and u already hold the macro step
+// of the populations as the last stream wrote them (VOF tracking, no
+// conditioning): macro_cell depends on the populations, the open flag, the
+// plane and whether the cell is gas, so only the cells that changed between
+// gas and wet since that stream (type_prev holds the types it saw) and the
+// cells whose populations were rewritten afterwards (marked in dirty) are
+// recomputed; every other cell already holds what k_macro would write
 template <class T>
-__global__ __launch_bounds__(NT) void k_les(Consts<T> c, Slot<T> s, const T* ux, const T* uy, const T* uz, T* omega, T* nu_t, T* smag,
-                      const int* type, const uint8_t* flags) {
+__global__ __launch_bounds__(NT) void k_macro_fix(Consts<T> c, const T* f, T* rho, T* ux, T* uy, T* uz, const int* type,
+                                                  const int* type_prev, const uint8_t* flags, uint8_t* dirty) {
     size_t id = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     size_t n = (size_t)c.nx * c.ny * c.nz;
     if (id >= n) return;
+    const int ty = type[id];
+    const bool d = dirty[id] != 0;
+    if ((ty == LBM_FS_GAS) == (type_prev[id] == LBM_FS_GAS) && !d) return;
+    if (d) dirty[id] = 0;
     int i, j, k;
     cell_ijk(c, id, i, j, k);
-    bool active = flags[id] & 1u;
-    if (c.fs_water && type[id] == LBM_FS_GAS) active = false;
-    if (!active) return;
+    T fl[19];
+#pragma unroll
+    for (int q = 0; q < 19; q++) fl[q] = f[(size_t)q * n + id];
+    T r, u, v, w;
+    macro_cell(c, k, fl, ty, (flags[id] & 1u) != 0, r, u, v, w);
+    rho[id] = r;
+    ux[id] = u;
+    uy[id] = v;
+    uz[id] = w;
+}
+
+// compute_LES (CODEX:3011-3141) of one active cell: omega, nu_t and the
+// Smagorinsky term from the velocities of its six neighbours
+template <class T>
+__device__ __forceinline__ T les_cell(const Consts<T>& c, const Slot<T>& s, const T* ux, const T* uy, const T* uz, const uint8_t* flags,
+                                      size_t id, int i, int j, int k, T& nt, T& sm) {
     int nx = c.nx, ny = c.ny, nz = c.nz;
     int ip = wrap1(i + 1, nx), im = wrap1(i - 1, nx), jp = wrap1(j + 1, ny), jm = wrap1(j - 1, ny);
     int kp = k + 1 >= nz ? nz - 1 : k + 1, km = k - 1 < 0 ? 0 : k - 1;
@@ -649,18 +672,35 @@ __global__ __launch_bounds__(NT) void k_les(Consts<T> c, Slot<T> s, const T* ux,
             uwy = s.vw[(size_t)j * nx + i];
         }
     }
+    return les_omega(c, ux[xp], ux[xm], ux[yp], ux[ym], ux[zp], ux[zm], uy[xp], uy[xm], uy[yp], uy[ym], uy[zp], uy[zm], uz[xp], uz[xm],
+                     uz[yp], uz[ym], uz[zp], uz[zm], near_wall, phi, ux[id], uy[id], uwx, uwy, &nt, &sm);
+}
+
+// compute_LES (CODEX:3011-3141)
+template <class T>
+__global__ __launch_bounds__(NT) void k_les(Consts<T> c, Slot<T> s, const T* ux, const T* uy, const T* uz, T* omega, T* nu_t, T* smag,
+                      const int* type, const uint8_t* flags) {
+    size_t id = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    size_t n = (size_t)c.nx * c.ny * c.nz;
+    if (id >= n) return;
+    int i, j, k;
+    cell_ijk(c, id, i, j, k);
+    bool active = flags[id] & 1u;
+    if (c.fs_water && type[id] == LBM_FS_GAS) active = false;
+    if (!active) return;
     T nt, sm;
-    T om = les_omega(c, ux[xp], ux[xm], ux[yp], ux[ym], ux[zp], ux[zm], uy[xp], uy[xm], uy[yp], uy[ym], uy[zp], uy[zm], uz[xp],
-                     uz[xm], uz[yp], uz[ym], uz[zp], uz[zm], near_wall, phi, ux[id], uy[id], uwx, uwy, &nt, &sm);
+    T om = les_cell(c, s, ux, uy, uz, flags, id, i, j, k, nt, sm);
     omega[id] = om;
     nu_t[id] = nt;
     smag[id] = sm;
 }
 
 // collide_regularized / collide_KBC (CODEX:3143-3416)
-template <class T>
-__global__ __launch_bounds__(NT) void k_collide(Consts<T> c, T* f, const T* rho, const T* ux, const T* uy, const T* uz, const T* omega,
-                          const int* type, const uint8_t* flags, uint32_t step) {
+// LES: compute_LES of the cell first (as k_les: active cells only; the
+// others keep their omega), then the collision with that omega
+template <class T, bool LES>
+__global__ __launch_bounds__(NT) void k_collide(Consts<T> c, Slot<T> s, T* f, const T* rho, const T* ux, const T* uy, const T* uz,
+                                                T* omega, T* nu_t, T* smag, const int* type, const uint8_t* flags, uint32_t step) {
     size_t id = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     size_t n = (size_t)c.nx * c.ny * c.nz;
     if (id >= n) return;
@@ -681,7 +721,16 @@ __global__ __launch_bounds__(NT) void k_collide(Consts<T> c, T* f, const T* rho,
     T fl[19];
 #pragma unroll
     for (int q = 0; q < 19; q++) fl[q] = f[(size_t)q * n + id];
-    T om = omega[id];
+    T om;
+    if (LES && open0 && !gas) {
+        T nt, sm;
+        om = les_cell(c, s, ux, uy, uz, flags, id, i, j, k, nt, sm);
+        omega[id] = om;
+        nu_t[id] = nt;
+        smag[id] = sm;
+    } else {
+        om = omega[id];
+    }
     if (c.collision_reg) {
         if (gas) {
 #pragma unroll
@@ -1039,10 +1088,11 @@ struct VofArgs {
     Consts<T> c;
     const T* f;
     T *rho, *ux, *uy, *uz;
-    T *fill, *mass, *mass2, *mdelta, *excess;
+    T *fill, *mass, *mass2, *mdelta, *excess, *excess2;
     T *sa, *sb, *sc, *sd, *se, *smv;
     int *type, *type_prev, *excess_dir, *ext_gas, *si;
     const uint8_t* flags;
+    uint8_t* mdirty;  // cells whose populations the repair rewrote (for k_macro_fix)
     int* icnt;
     double* dcnt;
     const T* fs_h;  // column surface height
@@ -1136,7 +1186,10 @@ __global__ __launch_bounds__(NT) void k_vof_exchange(VofArgs<T> a) {
 template <class T>
 __global__ __launch_bounds__(NT) void k_vof_preclassify(VofArgs<T> a, T empty_thr) {
     VOF_PROLOGUE
-    if (id == 0) a.icnt[IC_REDIST_DONORS] = 0;  // the next redistribution round starts without donors
+    if (id == 0) {  // the next redistribution round starts without donors
+        a.icnt[IC_REDIST_DONORS] = 0;
+        a.icnt[IC_REDIST_DONORS + 1] = 0;
+    }
     int old = a.type[id];
     a.type_prev[id] = old;
     a.excess[id] = T(0);
@@ -1202,7 +1255,7 @@ __device__ __forceinline__ T vof_room(const VofArgs<T>& a, size_t nid, T ex) {
 
 // redistribute_vof_mass_excess (CODEX:1417-1574) phase A: donor sums
 template <class T>
-__global__ __launch_bounds__(NT) void k_vof_redist_a(VofArgs<T> a) {
+__global__ __launch_bounds__(NT) void k_vof_redist_a(VofArgs<T> a, int slot) {
     VOF_PROLOGUE
     a.si[id] = 0;
     T ex = a.excess[id];
@@ -1259,17 +1312,13 @@ __global__ __launch_bounds__(NT) void k_vof_redist_a(VofArgs<T> a) {
     a.sc[id] = nzv;
     a.sd[id] = ws;
     a.se[id] = cs;
-    a.icnt[IC_REDIST_DONORS] = 1;  // every donor stores the same value
+    a.icnt[IC_REDIST_DONORS + slot] = 1;  // every donor stores the same value
 }
 
-// phase B: every receiver adds its shares (own retained excess, then q order)
+// phase B: the mass of cell id after the round: its own mass, its own
+// retained excess, then the shares of its donors in q order
 template <class T>
-__global__ __launch_bounds__(NT) void k_vof_redist_b(VofArgs<T> a) {
-    VOF_PROLOGUE
-    if (a.icnt[IC_REDIST_DONORS] == 0) {  // no donor this round: the mass is unchanged
-        a.mass2[id] = a.mass[id];
-        return;
-    }
+__device__ __forceinline__ T redist_gather(const VofArgs<T>& a, int i, int j, int k, size_t id) {
     T acc = a.mass[id];
     if (a.si[id] != 0 && !(a.se[id] > T(1.0e-12))) acc += a.excess[id];
     if (a.flags[id] & 1u) {
@@ -1296,17 +1345,16 @@ __global__ __launch_bounds__(NT) void k_vof_redist_b(VofArgs<T> a) {
             acc += share;
         }
     }
-    a.mass2[id] = acc;
+    return acc;
 }
 
-// finalize_vof_classification_from_mass (CODEX:1577-1624)
+// finalize_vof_classification_from_mass (CODEX:1577-1624) of cell id whose
+// mass is m: mass, fill, type, excess (into ex_out) and its direction
 template <class T>
-__global__ __launch_bounds__(NT) void k_vof_finalize(VofArgs<T> a, T empty_thr) {
-    VOF_PROLOGUE
-    if (id == 0) a.icnt[IC_REDIST_DONORS] = 0;  // the next redistribution round starts without donors
-    a.excess[id] = T(0);
-    a.excess_dir[id] = 0;
+__device__ __forceinline__ void finalize_cell(const VofArgs<T>& a, size_t id, T m, T empty_thr, T* ex_out) {
     if (!(a.flags[id] & 1u)) {
+        ex_out[id] = T(0);
+        a.excess_dir[id] = 0;
         a.mass[id] = T(0);
         a.fill[id] = T(0);
         a.type[id] = LBM_FS_GAS;
@@ -1314,7 +1362,6 @@ __global__ __launch_bounds__(NT) void k_vof_finalize(VofArgs<T> a, T empty_thr) 
     }
     T rr = a.rho[id];
     if (rr < T(1.0e-8)) rr = a.c.rho0;
-    T m = a.mass[id];
     T ex = T(0);
     int ed = 0, cnt = 0;
     if (m < T(0)) {
@@ -1346,11 +1393,39 @@ __global__ __launch_bounds__(NT) void k_vof_finalize(VofArgs<T> a, T empty_thr) 
         ct = LBM_FS_LIQUID;
     }
     a.mass[id] = m;
-    a.excess[id] = ex;
+    ex_out[id] = ex;
     a.excess_dir[id] = ed;
     a.fill[id] = fl;
     a.type[id] = ct;
     if (cnt) atomicAdd(&a.icnt[IC_VOF_EXCESS], cnt);
+}
+
+// one redistribution round's phase B followed by the classification
+// (redistribute_vof_mass_excess + finalize_vof_classification_from_mass).
+// A cell's gather reads its own mass, type and excess and its donors'
+// excess and weights (phase A's arrays), so the mass is updated in place and
+// the excess is written to the other buffer (excess2), which the host then
+// swaps in.  The donor flag of a round is slot `slot` of two; this kernel
+// clears the other one, which the next round's phase A sets.
+template <class T>
+__global__ __launch_bounds__(NT) void k_vof_redist_fin(VofArgs<T> a, T empty_thr, int slot) {
+    VOF_PROLOGUE
+    if (id == 0) a.icnt[IC_REDIST_DONORS + (slot ^ 1)] = 0;
+    // no donor this round: the mass is unchanged
+    const T m = a.icnt[IC_REDIST_DONORS + slot] != 0 ? redist_gather(a, i, j, k, id) : a.mass[id];
+    finalize_cell(a, id, m, empty_thr, a.excess2);
+}
+
+// finalize_vof_classification_from_mass (CODEX:1577-1624) alone: the cell's
+// own fields only, in place
+template <class T>
+__global__ __launch_bounds__(NT) void k_vof_finalize(VofArgs<T> a, T empty_thr) {
+    VOF_PROLOGUE
+    if (id == 0) {  // the next redistribution round starts without donors
+        a.icnt[IC_REDIST_DONORS] = 0;
+        a.icnt[IC_REDIST_DONORS + 1] = 0;
+    }
+    finalize_cell(a, id, a.mass[id], empty_thr, a.excess);
 }
 
 // enforce_vof_interface_layer (CODEX:1627-1656): only LIQUID cells change
@@ -1522,11 +1597,13 @@ __global__ __launch_bounds__(NT) void k_vof_detached_a(VofArgs<T> a, int mode, T
         }
     }
     T moved = T(0);
-    for (int d = 0; d < 6; d++) {
-        a.smv[6 * id + d] = mv[d];
-        moved += mv[d];
-    }
-    if (moved > T(1.0e-12)) {
+    for (int d = 0; d < 6; d++) moved += mv[d];
+    // every nonzero share exceeds 1e-12, so a cell that moved no more than
+    // that moved nothing: si marks the movers, and only they store shares
+    const bool mover = moved > T(1.0e-12);
+    a.si[id] = mover ? 1 : 0;
+    if (mover) {
+        for (int d = 0; d < 6; d++) a.smv[6 * id + d] = mv[d];
         atomicAdd(&a.icnt[mode == 0 ? IC_VOF_ADVECT : IC_VOF_COMPRESS], 1);
         atomicAdd(&a.dcnt[mode == 0 ? DC_ADVECT_MASS : DC_COMPRESS_MASS], (double)moved);
     }
@@ -1538,13 +1615,16 @@ __global__ __launch_bounds__(NT) void k_vof_detached_b(VofArgs<T> a, int mode) {
     VOF_PROLOGUE
     // phase A moved nothing (no cell counted): every share is zero
     if (a.icnt[mode == 0 ? IC_VOF_ADVECT : IC_VOF_COMPRESS] == 0) return;
+    // the shares of a cell that moved nothing are all zero (not stored)
     T out = T(0);
-    for (int d = 0; d < 6; d++) out -= a.smv[6 * id + d];
+    if (a.si[id])
+        for (int d = 0; d < 6; d++) out -= a.smv[6 * id + d];
     T in = T(0);
     for (int d = 0; d < 6; d++) {
         int si_, sj, sk;
         if (!NBR(-FD6[d][0], -FD6[d][1], -FD6[d][2], si_, sj, sk)) continue;
-        in += a.smv[6 * CID(sk, sj, si_) + d];
+        const size_t sid = CID(sk, sj, si_);
+        if (a.si[sid]) in += a.smv[6 * sid + d];
     }
     T dm = out + in;
     if (fabs(dm) > T(1.0e-12)) a.mass[id] = a.mass[id] + dm;
@@ -1948,6 +2028,9 @@ __global__ __launch_bounds__(NT) void k_vof_refill(VofArgs<T> a, T* f) {
 template <class T>
 __global__ __launch_bounds__(NT) void k_vof_repair(VofArgs<T> a, T* f) {
     VOF_PROLOGUE
+    // a fresh source is a cell the refill just filled (the refill's own
+    // condition), so a step that refilled nothing repairs nothing
+    if (a.icnt[IC_FS_REFILL] == 0) return;
     if (!((a.flags[id] & 1u) && a.type[id] != LBM_FS_GAS && a.type_prev[id] != LBM_FS_GAS)) return;
     bool rep = false;
     int cnt = 0;
@@ -1983,6 +2066,7 @@ __global__ __launch_bounds__(NT) void k_vof_repair(VofArgs<T> a, T* f) {
             a.uy[id] = my / r;
             a.uz[id] = mz / r;
         }
+        a.mdirty[id] = 1;
         atomicAdd(&a.icnt[IC_FS_REPAIR], cnt);
     }
 }
@@ -3090,6 +3174,7 @@ static VofArgs<T> make_vof(const lbm_config_t& cfg, DevBufs& b, const T* f) {
     a.mass2 = (T*)b.mass2;
     a.mdelta = (T*)b.mdelta;
     a.excess = (T*)b.excess;
+    a.excess2 = (T*)b.excess2;
     a.sa = (T*)b.sa;
     a.sb = (T*)b.sb;
     a.sc = (T*)b.sc;
@@ -3101,6 +3186,7 @@ static VofArgs<T> make_vof(const lbm_config_t& cfg, DevBufs& b, const T* f) {
     a.excess_dir = b.excess_dir;
     a.ext_gas = b.ext_gas;
     a.si = b.si;
+    a.mdirty = b.mdirty;
     a.flags = b.cellflags;
     a.icnt = b.icnt;
     a.fs_h = (const T*)b.fs_h;
@@ -3210,16 +3296,18 @@ static void vof_pipeline(const lbm_config_t& cfg, DevBufs& b, T* f, int slot, in
     k_vof_counters_reset<<<1, 64, 0, s>>>(b.icnt, b.dcnt);
     k_vof_exchange<T><<<nb, NT, 0, s>>>(A());
     k_vof_preclassify<T><<<nb, NT, 0, s>>>(A(), empty_thr);
-    auto redistribute = [&]() {
-        k_vof_redist_a<T><<<nb, NT, 0, s>>>(A());
-        k_vof_redist_b<T><<<nb, NT, 0, s>>>(A());
-        swap_mass();
+    // a redistribution round and the classification that follows it; the
+    // donor flags alternate between two slots (the preclassification and
+    // every lone classification clear both)
+    int rslot = 0;
+    auto redistribute_finalize = [&]() {
+        k_vof_redist_a<T><<<nb, NT, 0, s>>>(A(), rslot);
+        k_vof_redist_fin<T><<<nb, NT, 0, s>>>(A(), empty_thr, rslot);
+        std::swap(b.excess, b.excess2);
+        rslot ^= 1;
     };
     auto finalize = [&]() { k_vof_finalize<T><<<nb, NT, 0, s>>>(A(), empty_thr); };
-    for (int r = 0; r < 3; r++) {
-        redistribute();
-        finalize();
-    }
+    for (int r = 0; r < 3; r++) redistribute_finalize();
     k_vof_enforce<T><<<nb, NT, 0, s>>>(A());
     T othr = (T)cfg.vof_orphan_empty_threshold;
     k_vof_orphan_a<T><<<nb, NT, 0, s>>>(A(), empty_thr, othr, cfg.vof_orphan_search_radius, cfg.vof_orphan_max_weak_neighbors);
@@ -3236,8 +3324,7 @@ static void vof_pipeline(const lbm_config_t& cfg, DevBufs& b, T* f, int slot, in
         }
         finalize();
     }
-    redistribute();
-    finalize();
+    redistribute_finalize();
     k_vof_enforce<T><<<nb, NT, 0, s>>>(A());
     // collapse_trapped_gas_if_due (CODEX:2129-2143)
     if (cfg.vof_collapse_trapped_gas && step % cfg.vof_collapse_interval == 0) {
@@ -3271,22 +3358,36 @@ static void vof_pipeline(const lbm_config_t& cfg, DevBufs& b, T* f, int slot, in
 
 template <class T>
 static void staged_t(const lbm_config_t& cfg, DevBufs& b, int& cur, int slot, int64_t step, int omega_band_top, int count_links,
-                     hipStream_t s) {
+                     StagedState& st, hipStream_t s) {
     Consts<T> c = make_consts<T>(cfg, omega_band_top);
     Slot<T> sl = make_slot<T>(b, slot);
     const unsigned nb = nblocks(b.n);
     T* f = (T*)b.f[cur];
     T* fn = (T*)b.f[cur ^ 1];
-    k_cellflags<T><<<nb, NT, 0, s>>>(c, sl, b.cellflags);
+    // a flat or piecewise bed (and the box) never moves: its cell flags and
+    // obstacle masks are the same at every step once computed
+    const bool static_geo = !c.bed_moving;
+    const bool geo_cached = static_geo && st.geo_ready;
+    // the conditioning rewrites populations in air runs and over a moving bed
+    const bool cond_writes_f = c.air_walls || c.bed_moving;
+    if (!geo_cached) k_cellflags<T><<<nb, NT, 0, s>>>(c, sl, b.cellflags);
     const bool height = cfg.fs_enabled && cfg.fs_tracking != LBM_FST_VOF;
     if (height) k_fs_classify<T><<<nb, NT, 0, s>>>(c, (const T*)b.fs_h, b.cellflags, (T*)b.fill, b.type, b.type_prev);
-    k_cond<T><<<nb, NT, 0, s>>>(c, sl, f, b.obst, b.obst_prev);
-    k_macro<T><<<nb, NT, 0, s>>>(c, f, (T*)b.rho, (T*)b.ux, (T*)b.uy, (T*)b.uz, b.type, b.cellflags);
-    if (cfg.use_les)
-        k_les<T><<<nb, NT, 0, s>>>(c, sl, (const T*)b.ux, (const T*)b.uy, (const T*)b.uz, (T*)b.omega, (T*)b.nu_t, (T*)b.smag,
-                                   b.type, b.cellflags);
-    k_collide<T><<<nb, NT, 0, s>>>(c, f, (const T*)b.rho, (const T*)b.ux, (const T*)b.uy, (const T*)b.uz, (const T*)b.omega,
-                                   b.type, b.cellflags, (uint32_t)step);
+    // over a static geometry the obstacle masks settle after one pass
+    // (obst_prev = obst = the closed mask), so without population writes
+    // the conditioning has nothing left to do
+    if (!geo_cached || cond_writes_f) k_cond<T><<<nb, NT, 0, s>>>(c, sl, f, b.obst, b.obst_prev);
+    st.geo_ready = static_geo;
+    // a VOF run without conditioning: the stream of the last step left rho
+    // and u as the macro step of its populations (see k_macro_fix)
+    const bool vof_macro = cfg.fs_enabled && cfg.fs_tracking == LBM_FST_VOF && !cond_writes_f;
+    if (vof_macro && st.macro_ready)
+        k_macro_fix<T><<<nb, NT, 0, s>>>(c, f, (T*)b.rho, (T*)b.ux, (T*)b.uy, (T*)b.uz, b.type, b.type_prev, b.cellflags, b.mdirty);
+    else
+        k_macro<T><<<nb, NT, 0, s>>>(c, f, (T*)b.rho, (T*)b.ux, (T*)b.uy, (T*)b.uz, b.type, b.cellflags);
+    auto collide = cfg.use_les ? k_collide<T, true> : k_collide<T, false>;
+    collide<<<nb, NT, 0, s>>>(c, sl, f, (const T*)b.rho, (const T*)b.ux, (const T*)b.uy, (const T*)b.uz, (T*)b.omega, (T*)b.nu_t,
+                              (T*)b.smag, b.type, b.cellflags, (uint32_t)step);
     // the link counters of stream describe one step and are overwritten by
     // the next: they are counted on the last step of a call, the one a
     // caller can observe
@@ -3304,6 +3405,7 @@ static void staged_t(const lbm_config_t& cfg, DevBufs& b, int& cur, int slot, in
         T* f2 = (T*)b.f[cur];
         if (cfg.fs_tracking == LBM_FST_VOF) {
             vof_pipeline<T>(cfg, b, f2, slot, step, s);
+            st.macro_ready = vof_macro;
         } else {
             // height-function tracking (CODEX:3917-3921)
             const unsigned n2b = nblocks(b.n2);
@@ -3319,11 +3421,11 @@ static void staged_t(const lbm_config_t& cfg, DevBufs& b, int& cur, int slot, in
     }
 }
 void launch_staged_step(const lbm_config_t& c, DevBufs& b, int& cur, int slot, int64_t step, int omega_band_top, int count_links,
-                        hipStream_t s) {
+                        StagedState& st, hipStream_t s) {
     if (b.storage == LBM_STORE_FP64)
-        staged_t<double>(c, b, cur, slot, step, omega_band_top, count_links, s);
+        staged_t<double>(c, b, cur, slot, step, omega_band_top, count_links, st, s);
     else if (b.storage == LBM_STORE_FP32)
-        staged_t<float>(c, b, cur, slot, step, omega_band_top, count_links, s);
+        staged_t<float>(c, b, cur, slot, step, omega_band_top, count_links, st, s);
     else
         throw std::runtime_error("the staged path needs fp32 or fp64 storage");
 }

@@ -1093,6 +1093,7 @@ struct VofArgs {
     int *type, *type_prev, *excess_dir, *ext_gas, *si;
     const uint8_t* flags;
     uint8_t* mdirty;  // cells whose populations the repair rewrote (for k_macro_fix)
+    int* rowdon;      // [2][nz * ny]: rows holding a donor of the redistribution round of each slot
     int* icnt;
     double* dcnt;
     const T* fs_h;  // column surface height
@@ -1159,9 +1160,8 @@ __device__ __forceinline__ void wet_resolved(const VofArgs<T>& a, int i, int j, 
 template <class T>
 __global__ __launch_bounds__(NT) void k_vof_exchange(VofArgs<T> a) {
     VOF_PROLOGUE
-    a.mdelta[id] = T(0);
-    a.excess[id] = T(0);
-    a.excess_dir[id] = 0;
+    // the delta is read only for these cells (the preclassification, which
+    // also sets every cell's excess and its direction)
     if (!((a.flags[id] & 1u) && a.type[id] == LBM_FS_INTERFACE)) return;
     T dm = T(0);
     const T fill_c = a.fill[id];
@@ -1190,6 +1190,8 @@ __global__ __launch_bounds__(NT) void k_vof_preclassify(VofArgs<T> a, T empty_th
         a.icnt[IC_REDIST_DONORS] = 0;
         a.icnt[IC_REDIST_DONORS + 1] = 0;
     }
+    clear_donor_rows(a, 0, i, j, k);
+    clear_donor_rows(a, 1, i, j, k);
     int old = a.type[id];
     a.type_prev[id] = old;
     a.excess[id] = T(0);
@@ -1313,6 +1315,7 @@ __global__ __launch_bounds__(NT) void k_vof_redist_a(VofArgs<T> a, int slot) {
     a.sd[id] = ws;
     a.se[id] = cs;
     a.icnt[IC_REDIST_DONORS + slot] = 1;  // every donor stores the same value
+    a.rowdon[(size_t)slot * a.c.nz * a.c.ny + (size_t)k * a.c.ny + j] = 1;
 }
 
 // phase B: the mass of cell id after the round: its own mass, its own
@@ -1407,12 +1410,44 @@ __device__ __forceinline__ void finalize_cell(const VofArgs<T>& a, size_t id, T 
 // the excess is written to the other buffer (excess2), which the host then
 // swaps in.  The donor flag of a round is slot `slot` of two; this kernel
 // clears the other one, which the next round's phase A sets.
+// Rows of donors: a cell with no donor in its own row or the 8 rows around
+// it (the rows of its 18 neighbours, y wrapped as the neighbours are) has
+// no share to gather and keeps its mass.
+template <class T>
+__device__ __forceinline__ bool donor_rows_near(const VofArgs<T>& a, const int* rows, int j, int k) {
+    const int ny = a.c.ny, nz = a.c.nz;
+    const bool yper = a.c.y_boundary == LBM_YB_PERIODIC;
+    bool any = false;
+#pragma unroll
+    for (int dk = -1; dk <= 1; dk++) {
+        const int kk = k + dk;
+        if (kk < 0 || kk >= nz) continue;
+#pragma unroll
+        for (int dj = -1; dj <= 1; dj++) {
+            int jj = j + dj;
+            if (yper)
+                jj = wrap1(jj, ny);
+            else if (jj < 0 || jj >= ny)
+                continue;
+            any |= rows[(size_t)kk * ny + jj] != 0;
+        }
+    }
+    return any;
+}
+template <class T>
+__device__ __forceinline__ void clear_donor_rows(const VofArgs<T>& a, int slot, int i, int j, int k) {
+    if (i == 0) a.rowdon[(size_t)slot * a.c.nz * a.c.ny + (size_t)k * a.c.ny + j] = 0;
+}
+
 template <class T>
 __global__ __launch_bounds__(NT) void k_vof_redist_fin(VofArgs<T> a, T empty_thr, int slot) {
     VOF_PROLOGUE
     if (id == 0) a.icnt[IC_REDIST_DONORS + (slot ^ 1)] = 0;
-    // no donor this round: the mass is unchanged
-    const T m = a.icnt[IC_REDIST_DONORS + slot] != 0 ? redist_gather(a, i, j, k, id) : a.mass[id];
+    clear_donor_rows(a, slot ^ 1, i, j, k);
+    // no donor this round, or none within reach: the mass is unchanged
+    const int* rows = a.rowdon + (size_t)slot * a.c.nz * a.c.ny;
+    const bool gather = a.icnt[IC_REDIST_DONORS + slot] != 0 && donor_rows_near(a, rows, j, k);
+    const T m = gather ? redist_gather(a, i, j, k, id) : a.mass[id];
     finalize_cell(a, id, m, empty_thr, a.excess2);
 }
 
@@ -1425,6 +1460,8 @@ __global__ __launch_bounds__(NT) void k_vof_finalize(VofArgs<T> a, T empty_thr) 
         a.icnt[IC_REDIST_DONORS] = 0;
         a.icnt[IC_REDIST_DONORS + 1] = 0;
     }
+    clear_donor_rows(a, 0, i, j, k);
+    clear_donor_rows(a, 1, i, j, k);
     finalize_cell(a, id, a.mass[id], empty_thr, a.excess);
 }
 
@@ -3187,6 +3224,7 @@ static VofArgs<T> make_vof(const lbm_config_t& cfg, DevBufs& b, const T* f) {
     a.ext_gas = b.ext_gas;
     a.si = b.si;
     a.mdirty = b.mdirty;
+    a.rowdon = b.rowdon;
     a.flags = b.cellflags;
     a.icnt = b.icnt;
     a.fs_h = (const T*)b.fs_h;

@@ -85,17 +85,11 @@ static const char* validate(const lbm_config_t* c) {
     if (c->storage_rounding != LBM_ROUND_NEAREST && c->storage_rounding != LBM_ROUND_STOCHASTIC) return "unknown storage_rounding";
     for (int a = 0; a < 3; a++)
         if (!(fabsf(c->storage_u_ref[a]) <= 0.3f)) return "storage_u_ref components must lie in [-0.3, 0.3]";
-    if ((c->storage == LBM_STORE_FP16 || c->storage == LBM_STORE_BF16) && c->kernel_path == LBM_PATH_STAGED)
-        return "fp16/bf16 storage runs on the fused path only";
-    if (c->physics_mode == LBM_PHYS_WATER || c->fs_enabled) {
-        if (c->storage == LBM_STORE_FP16 || c->storage == LBM_STORE_BF16) return "water and free-surface runs need fp32 or fp64 storage";
-        if (c->kernel_path == LBM_PATH_FUSED) return "the fused path covers air mode without a free surface only";
-    }
+    if ((c->physics_mode == LBM_PHYS_WATER || c->fs_enabled) && c->kernel_path == LBM_PATH_FUSED)
+        return "the fused path covers air mode without a free surface only";
     if (!(c->tau0 > 0.5f)) return "tau0 must exceed 0.5";
     if (!(c->dx > 0.0f)) return "dx must be positive";
     if (c->vof_collapse_interval < 1) return "vof_collapse_interval must be positive";
-    if ((c->storage == LBM_STORE_FP16 || c->storage == LBM_STORE_BF16) && !fused_supported(*c))
-        return "fp16/bf16 storage runs on the fused path: air mode without a free surface or the box obstacle";
     if (c->kernel_path == LBM_PATH_FUSED && !fused_supported(*c)) return "the fused path covers air mode without a free surface or the box obstacle";
     return nullptr;
 }
@@ -103,7 +97,6 @@ static const char* validate(const lbm_config_t* c) {
 static int resolve_path(const lbm_config_t& c) {
     if (c.kernel_path == LBM_PATH_STAGED) return LBM_PATH_STAGED;
     if (c.kernel_path == LBM_PATH_FUSED) return LBM_PATH_FUSED;
-    if (c.storage == LBM_STORE_FP16 || c.storage == LBM_STORE_BF16) return LBM_PATH_FUSED;
     return fused_supported(c) ? LBM_PATH_FUSED : LBM_PATH_STAGED;
 }
 

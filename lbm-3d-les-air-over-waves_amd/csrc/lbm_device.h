@@ -89,6 +89,13 @@ __host__ __device__ inline uint32_t dither_base(uint32_t step, uint32_t cell) {
     return fmix32_hd(cell * 0x9E3779B1u ^ step * 0x85EBCA77u);
 }
 __host__ __device__ constexpr uint32_t dither_q(uint32_t base, int q) { return base + (uint32_t)q * 0x9E3779B9u; }
+// the store points of a staged step draw their dither from the step number
+// with a salt of their own (the collision's store, the only one of the fused
+// step, keeps the plain step number)
+constexpr uint32_t SALT_STREAM = 0x40000000u;  // post-stream populations
+constexpr uint32_t SALT_REFILL = 0x20000000u;  // refilled free-surface cells
+constexpr uint32_t SALT_COND = 0x10000000u;    // bed conditioning of closed links
+constexpr uint32_t SALT_INIT = 0x08000000u;    // free-surface projection at initialisation
 
 template <class S, class T>
 struct Store;
@@ -104,6 +111,12 @@ struct Store<T, T> {  // full-precision storage: the population itself
     template <class I>
     static __device__ __forceinline__ void store(T* p, I i, int, T v, const Codec&, uint32_t) {
         __builtin_nontemporal_store(v, p + i);
+    }
+    // store and return the value a later load reads back
+    template <class I>
+    static __device__ __forceinline__ T store_rt(T* p, I i, int, T v, const Codec&, uint32_t) {
+        p[i] = v;
+        return v;
     }
 };
 template <>
@@ -133,6 +146,12 @@ struct Store<__half, float> {  // (f_q - s_q) * 2^k as IEEE half
     template <class I>
     static __device__ __forceinline__ void store(__half* p, I i, int q, float v, const Codec& k, uint32_t base) {
         __builtin_nontemporal_store(encode(v, q, k, base), reinterpret_cast<uint16_t*>(p) + i);  // as above
+    }
+    template <class I>
+    static __device__ __forceinline__ float store_rt(__half* p, I i, int q, float v, const Codec& k, uint32_t base) {
+        const uint16_t h = encode(v, q, k, base);
+        reinterpret_cast<uint16_t*>(p)[i] = h;
+        return decode(h, q, k);
     }
 };
 __device__ __forceinline__ float bf16_to_float(bf16_t b) { return __uint_as_float(((uint32_t)b.v) << 16); }
@@ -177,6 +196,12 @@ struct Store<bf16_t, float> {  // (f_q - s_q) * 2^k as bfloat16
     template <class I>
     static __device__ __forceinline__ void store(bf16_t* p, I i, int q, float v, const Codec& k, uint32_t base) {
         __builtin_nontemporal_store(encode(v, q, k, base), reinterpret_cast<uint16_t*>(p) + i);  // as above
+    }
+    template <class I>
+    static __device__ __forceinline__ float store_rt(bf16_t* p, I i, int q, float v, const Codec& k, uint32_t base) {
+        const uint16_t h = encode(v, q, k, base);
+        reinterpret_cast<uint16_t*>(p)[i] = h;
+        return decode(h, q, k);
     }
 };
 

@@ -543,8 +543,10 @@ __global__ __launch_bounds__(NT) void k_init_populations(Consts<T> c, Slot<T> s,
 // ===========================================================================
 
 // update_bed_populations_and_reinit (CODEX:2878-2940)
-template <class T>
-__global__ __launch_bounds__(NT) void k_cond(Consts<T> c, Slot<T> s, T* f, int* obst, int* obst_prev) {
+// S != T (16-bit storage): a population the conditioning changed is stored
+// (rounded, dither of step ^ SALT_COND); the others keep their stored bits
+template <class S, class T>
+__global__ __launch_bounds__(NT) void k_cond(Consts<T> c, Slot<T> s, S* f, int* obst, int* obst_prev, uint32_t step) {
     size_t id = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     size_t n = (size_t)c.nx * c.ny * c.nz;
     if (id >= n) return;
@@ -556,12 +558,19 @@ __global__ __launch_bounds__(NT) void k_cond(Consts<T> c, Slot<T> s, T* f, int* 
     obst[id] = (g.open_mask & 1u) ? 0 : 1;
     if ((g.open_mask & 0x7ffffu) == 0x7ffffu) return;  // no closed link
     if (!c.air_walls && !c.bed_moving) return;          // water: only a moving bed conditions its links
-    T fl[19];
+    T fl[19], f0[19];
 #pragma unroll
-    for (int q = 0; q < 19; q++) fl[q] = f[(size_t)q * n + id];
+    for (int q = 0; q < 19; q++) f0[q] = fl[q] = Store<S, T>::load(f, (size_t)q * n + id, q, c.store);
     condition_cell(c, s, g, i, j, fl);
+    if constexpr (std::is_same<S, T>::value) {
 #pragma unroll
-    for (int q = 0; q < 19; q++) f[(size_t)q * n + id] = fl[q];
+        for (int q = 0; q < 19; q++) f[(size_t)q * n + id] = fl[q];
+    } else {
+        const uint32_t base = dither_base(step ^ SALT_COND, (uint32_t)id);
+#pragma unroll
+        for (int q = 0; q < 19; q++)
+            if (__float_as_uint(fl[q]) != __float_as_uint(f0[q])) Store<S, T>::store(f, (size_t)q * n + id, q, fl[q], c.store, base);
+    }
 }
 
 // macro_step (CODEX:2943-3008)
@@ -602,8 +611,8 @@ __device__ __forceinline__ void macro_cell(const Consts<T>& c, int k, const T fl
     }
 }
 
-template <class T>
-__global__ __launch_bounds__(NT) void k_macro(Consts<T> c, const T* f, T* rho, T* ux, T* uy, T* uz, const int* type, const uint8_t* flags) {
+template <class S, class T>
+__global__ __launch_bounds__(NT) void k_macro(Consts<T> c, const S* f, T* rho, T* ux, T* uy, T* uz, const int* type, const uint8_t* flags) {
     size_t id = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     size_t n = (size_t)c.nx * c.ny * c.nz;
     if (id >= n) return;
@@ -611,7 +620,7 @@ __global__ __launch_bounds__(NT) void k_macro(Consts<T> c, const T* f, T* rho, T
     cell_ijk(c, id, i, j, k);
     T fl[19];
 #pragma unroll
-    for (int q = 0; q < 19; q++) fl[q] = f[(size_t)q * n + id];
+    for (int q = 0; q < 19; q++) fl[q] = Store<S, T>::load(f, (size_t)q * n + id, q, c.store);
     T r, u, v, w;
     macro_cell(c, k, fl, c.fs_water ? type[id] : LBM_FS_LIQUID, (flags[id] & 1u) != 0, r, u, v, w);
     rho[id] = r;
@@ -627,8 +636,8 @@ __global__ __launch_bounds__(NT) void k_macro(Consts<T> c, const T* f, T* rho, T
 // gas and wet since that stream (type_prev holds the types it saw) and the
 // cells whose populations were rewritten afterwards (marked in dirty) are
 // recomputed; every other cell already holds what k_macro would write
-template <class T>
-__global__ __launch_bounds__(NT) void k_macro_fix(Consts<T> c, const T* f, T* rho, T* ux, T* uy, T* uz, const int* type,
+template <class S, class T>
+__global__ __launch_bounds__(NT) void k_macro_fix(Consts<T> c, const S* f, T* rho, T* ux, T* uy, T* uz, const int* type,
                                                   const int* type_prev, const uint8_t* flags, uint8_t* dirty) {
     size_t id = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     size_t n = (size_t)c.nx * c.ny * c.nz;
@@ -641,7 +650,7 @@ __global__ __launch_bounds__(NT) void k_macro_fix(Consts<T> c, const T* f, T* rh
     cell_ijk(c, id, i, j, k);
     T fl[19];
 #pragma unroll
-    for (int q = 0; q < 19; q++) fl[q] = f[(size_t)q * n + id];
+    for (int q = 0; q < 19; q++) fl[q] = Store<S, T>::load(f, (size_t)q * n + id, q, c.store);
     T r, u, v, w;
     macro_cell(c, k, fl, ty, (flags[id] & 1u) != 0, r, u, v, w);
     rho[id] = r;
@@ -698,8 +707,8 @@ __global__ __launch_bounds__(NT) void k_les(Consts<T> c, Slot<T> s, const T* ux,
 // collide_regularized / collide_KBC (CODEX:3143-3416)
 // LES: compute_LES of the cell first (as k_les: active cells only; the
 // others keep their omega), then the collision with that omega
-template <class T, bool LES>
-__global__ __launch_bounds__(NT) void k_collide(Consts<T> c, Slot<T> s, T* f, const T* rho, const T* ux, const T* uy, const T* uz,
+template <class S, class T, bool LES>
+__global__ __launch_bounds__(NT) void k_collide(Consts<T> c, Slot<T> s, S* f, const T* rho, const T* ux, const T* uy, const T* uz,
                                                 T* omega, T* nu_t, T* smag, const int* type, const uint8_t* flags, uint32_t step) {
     size_t id = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     size_t n = (size_t)c.nx * c.ny * c.nz;
@@ -720,7 +729,7 @@ __global__ __launch_bounds__(NT) void k_collide(Consts<T> c, Slot<T> s, T* f, co
     }
     T fl[19];
 #pragma unroll
-    for (int q = 0; q < 19; q++) fl[q] = f[(size_t)q * n + id];
+    for (int q = 0; q < 19; q++) fl[q] = Store<S, T>::load(f, (size_t)q * n + id, q, c.store);
     T om;
     if (LES && open0 && !gas) {
         T nt, sm;
@@ -755,8 +764,14 @@ __global__ __launch_bounds__(NT) void k_collide(Consts<T> c, Slot<T> s, T* f, co
             }
         }
     }
+    if constexpr (std::is_same<S, T>::value) {
 #pragma unroll
-    for (int q = 0; q < 19; q++) f[(size_t)q * n + id] = fl[q];
+        for (int q = 0; q < 19; q++) f[(size_t)q * n + id] = fl[q];
+    } else {  // the store point of the fused step too (same dither)
+        const uint32_t base = dither_base(step, (uint32_t)id);
+#pragma unroll
+        for (int q = 0; q < 19; q++) Store<S, T>::store(f, (size_t)q * n + id, q, fl[q], c.store, base);
+    }
 }
 
 // free_surface_hydrostatic_density_at_cell (CODEX:821-847): one pressure
@@ -1027,18 +1042,27 @@ __device__ __forceinline__ void stream_bc_cell(const Consts<T>& c, const Slot<T>
     }
 }
 
-template <class S, class T, bool COUNT, bool MACRO>
-__global__ __launch_bounds__(NT) void k_stream_bc(Consts<T> c, Slot<T> s, const S* fin, T* fout, const T* ux, const T* uy, const T* uz,
+// D: the destination's type: the arithmetic type (materialised downloads)
+// or the storage type (the staged step, whose 16-bit stores round with the
+// dither of step ^ SALT_STREAM; the macro step then sees the stored values)
+template <class S, class D, class T, bool COUNT, bool MACRO>
+__global__ __launch_bounds__(NT) void k_stream_bc(Consts<T> c, Slot<T> s, const S* fin, D* fout, const T* ux, const T* uy, const T* uz,
                             const int* type, const T* fill, const T* u_inlet, const T* inflow, const T* fs_h,
-                            const uint8_t* flags, int* icnt, T* mrho, T* mux, T* muy, T* muz) {
+                            const uint8_t* flags, int* icnt, T* mrho, T* mux, T* muy, T* muz, uint32_t step) {
     size_t id = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     size_t n = (size_t)c.nx * c.ny * c.nz;
     if (id >= n) return;
     T fo[19];
     int ty, k, n_cut, n_fs, n_bridge;
     stream_bc_cell<S, T>(c, s, fin, ux, uy, uz, type, fill, u_inlet, inflow, fs_h, flags, id, fo, ty, k, n_cut, n_fs, n_bridge);
+    if constexpr (std::is_same<D, T>::value) {
 #pragma unroll
-    for (int q = 0; q < 19; q++) fout[(size_t)q * n + id] = fo[q];
+        for (int q = 0; q < 19; q++) fout[(size_t)q * n + id] = fo[q];
+    } else {
+        const uint32_t base = dither_base(step ^ SALT_STREAM, (uint32_t)id);
+#pragma unroll
+        for (int q = 0; q < 19; q++) fo[q] = Store<D, T>::store_rt(fout, (size_t)q * n + id, q, fo[q], c.store, base);
+    }
     if (MACRO) {
         T r, u, v, w;
         macro_cell(c, k, fo, ty, (flags[id] & 1u) != 0, r, u, v, w);
@@ -1086,7 +1110,6 @@ __global__ __launch_bounds__(NT) void k_cut_links(Consts<T> c, Slot<T> s, int* i
 template <class T>
 struct VofArgs {
     Consts<T> c;
-    const T* f;
     T *rho, *ux, *uy, *uz;
     T *fill, *mass, *mass2, *mdelta, *excess, *excess2;
     T *sa, *sb, *sc, *sd, *se, *smv;
@@ -1157,8 +1180,8 @@ __device__ __forceinline__ void wet_resolved(const VofArgs<T>& a, int i, int j, 
     cell_ijk(a.c, id, i, j, k);
 
 // compute_vof_mass_exchange (CODEX:1305-1350)
-template <class T>
-__global__ __launch_bounds__(NT) void k_vof_exchange(VofArgs<T> a) {
+template <class S, class T>
+__global__ __launch_bounds__(NT) void k_vof_exchange(VofArgs<T> a, const S* f) {
     VOF_PROLOGUE
     // the delta is read only for these cells (the preclassification, which
     // also sets every cell's excess and its direction)
@@ -1172,7 +1195,8 @@ __global__ __launch_bounds__(NT) void k_vof_exchange(VofArgs<T> a) {
         const bool open = (a.flags[nid] & 1u) != 0;
         const int ty = a.type[nid];
         const T fn = a.fill[nid];
-        const T fa = a.f[(size_t)oppq(q) * n + id], fb = a.f[(size_t)q * n + nid];
+        const T fa = Store<S, T>::load(f, (size_t)oppq(q) * n + id, oppq(q), a.c.store);
+        const T fb = Store<S, T>::load(f, (size_t)q * n + nid, q, a.c.store);
         if (v & open & (ty != LBM_FS_GAS)) {
             T lw = T(1.0);
             if (ty == LBM_FS_INTERFACE) lw = T(0.5) * (fill_c + fn);
@@ -1953,8 +1977,10 @@ __global__ __launch_bounds__(NT) void k_fs_advect(Consts<T> c, const T* qx, cons
 }
 
 // refill_new_free_surface_cells (CODEX:2163-2316)
-template <class T>
-__global__ __launch_bounds__(NT) void k_vof_refill(VofArgs<T> a, T* f) {
+// 16-bit storage: the refilled cell's populations are stored rounded with
+// the dither of step ^ SALT_REFILL
+template <class S, class T>
+__global__ __launch_bounds__(NT) void k_vof_refill(VofArgs<T> a, S* f, uint32_t step) {
     VOF_PROLOGUE
     if (!(a.type_prev[id] == LBM_FS_GAS && a.type[id] != LBM_FS_GAS && (a.flags[id] & 1u))) return;
     const Consts<T>& c = a.c;
@@ -1982,7 +2008,7 @@ __global__ __launch_bounds__(NT) void k_vof_refill(VofArgs<T> a, T* f) {
         za += lw * uzn;
         if (c.refill_mode == LBM_REFILL_NONEQ)
             for (int qq = 0; qq < 19; qq++) {
-                T fneq = f[(size_t)qq * n + nid] - feq(rn, uxn, uyn, uzn, qq);
+                T fneq = Store<S, T>::load(f, (size_t)qq * n + nid, qq, c.store) - feq(rn, uxn, uyn, uzn, qq);
                 fna[qq] += lw * fneq;
                 fnf[qq] += wn * fneq;
             }
@@ -2022,19 +2048,27 @@ __global__ __launch_bounds__(NT) void k_vof_refill(VofArgs<T> a, T* f) {
             int si_, sj, sk;
             if (NBR(-cxq(q), -cyq(q), -czq(q), si_, sj, sk)) {
                 size_t sid = CID(sk, sj, si_);
-                if ((a.flags[sid] & 1u) && a.type_prev[sid] != LBM_FS_GAS && a.type[sid] != LBM_FS_GAS) fr = f[(size_t)q * n + sid];
+                if ((a.flags[sid] & 1u) && a.type_prev[sid] != LBM_FS_GAS && a.type[sid] != LBM_FS_GAS)
+                    fr = Store<S, T>::load(f, (size_t)q * n + sid, q, c.store);
             }
         }
         fv[q] = fr;
         rp += fr;
     }
+    const uint32_t base = dither_base(step ^ SALT_REFILL, (uint32_t)id);
+    auto put = [&](int q, T v) {
+        if constexpr (std::is_same<S, T>::value)
+            f[(size_t)q * n + id] = v;
+        else
+            Store<S, T>::store(f, (size_t)q * n + id, q, v, c.store, base);
+    };
     if (c.refill_mode == LBM_REFILL_DIRECTIONAL) {
         T scale = T(1.0);
         if (rp > T(1.0e-8)) scale = rr / rp;
         T r2 = 0, mx = 0, my = 0, mz = 0;
         for (int q = 0; q < 19; q++) {
             T fvv = fv[q] * scale;
-            f[(size_t)q * n + id] = fvv;
+            put(q, fvv);
             r2 += fvv;
             mx += fvv * (T)cxq(q);
             my += fvv * (T)cyq(q);
@@ -2055,15 +2089,17 @@ __global__ __launch_bounds__(NT) void k_vof_refill(VofArgs<T> a, T* f) {
         a.ux[id] = urx;
         a.uy[id] = ury;
         a.uz[id] = urz;
-        for (int q = 0; q < 19; q++) f[(size_t)q * n + id] = fv[q];
+        for (int q = 0; q < 19; q++) put(q, fv[q]);
     }
     atomicAdd(&a.icnt[IC_FS_REFILL], 1);
 }
 
 // repair_vof_coalescence_links (CODEX:2319-2373): donors (refilled cells,
 // type_prev gas) and repaired cells (type_prev wet) are disjoint
-template <class T>
-__global__ __launch_bounds__(NT) void k_vof_repair(VofArgs<T> a, T* f) {
+// the copies move stored values unchanged (the shift of a population
+// depends on q only); the moments are taken from the values as loaded
+template <class S, class T>
+__global__ __launch_bounds__(NT) void k_vof_repair(VofArgs<T> a, S* f) {
     VOF_PROLOGUE
     // a fresh source is a cell the refill just filled (the refill's own
     // condition), so a step that refilled nothing repairs nothing
@@ -2091,7 +2127,7 @@ __global__ __launch_bounds__(NT) void k_vof_repair(VofArgs<T> a, T* f) {
     if (rep) {
         T r = 0, mx = 0, my = 0, mz = 0;
         for (int q = 0; q < 19; q++) {
-            T fq = f[(size_t)q * n + id];
+            T fq = Store<S, T>::load(f, (size_t)q * n + id, q, a.c.store);
             r += fq;
             mx += fq * (T)cxq(q);
             my += fq * (T)cyq(q);
@@ -3198,10 +3234,9 @@ void launch_cellflags(const lbm_config_t& c, DevBufs& b, int slot, hipStream_t s
 }
 
 template <class T>
-static VofArgs<T> make_vof(const lbm_config_t& cfg, DevBufs& b, const T* f) {
+static VofArgs<T> make_vof(const lbm_config_t& cfg, DevBufs& b) {
     VofArgs<T> a;
     a.c = make_consts<T>(cfg, -1);
-    a.f = f;
     a.rho = (T*)b.rho;
     a.ux = (T*)b.ux;
     a.uy = (T*)b.uy;
@@ -3237,8 +3272,10 @@ static VofArgs<T> make_vof(const lbm_config_t& cfg, DevBufs& b, const T* f) {
 // open gas cell (or from above the lid), which take the gas-pressure closure
 // of the pull stream feq_q(rho_fs, u) + feq_qbar(rho_fs, u) - f_qbar; sources
 // outside solid x (or y) walls are not gas
-template <class T>
-__global__ __launch_bounds__(NT) void k_fs_project(Consts<T> c, const T* fin, T* fout, const T* ux, const T* uy, const T* uz,
+// 16-bit storage: the projected links are stored rounded (dither of
+// SALT_INIT), every other population keeps its stored bits
+template <class S, class T>
+__global__ __launch_bounds__(NT) void k_fs_project(Consts<T> c, const S* fin, S* fout, const T* ux, const T* uy, const T* uz,
                                                    const int* type, const int* obst, const T* fs_h) {
     size_t id = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     size_t n = (size_t)c.nx * c.ny * c.nz;
@@ -3273,7 +3310,11 @@ __global__ __launch_bounds__(NT) void k_fs_project(Consts<T> c, const T* fin, T*
         T rho_fs = c.rho0;
         if (c.pf != LBM_PF_TOTAL || c.fs_boundary_hydro) rho_fs = fs_density_link(c, k, q, true, type[id], fs_h, col);
         const T u = ux[id], v = uy[id], w = uz[id];
-        fout[(size_t)q * n + id] = feq(rho_fs, u, v, w, q) + feq(rho_fs, u, v, w, qb) - fin[(size_t)qb * n + id];
+        const T fp = feq(rho_fs, u, v, w, q) + feq(rho_fs, u, v, w, qb) - Store<S, T>::load(fin, (size_t)qb * n + id, qb, c.store);
+        if constexpr (std::is_same<S, T>::value)
+            fout[(size_t)q * n + id] = fp;
+        else
+            Store<S, T>::store(fout, (size_t)q * n + id, q, fp, c.store, dither_base(SALT_INIT, (uint32_t)id));
     }
 }
 
@@ -3285,13 +3326,12 @@ static void init_t(const lbm_config_t& cfg, DevBufs& b, int slot, hipStream_t s)
     k_init_populations<S, T><<<nblocks(b.n), NT, 0, s>>>(c, sl, (S*)b.f[0], (T*)b.rho, (T*)b.ux, (T*)b.uy, (T*)b.uz, b.obst,
                                                          b.type);
     if (cfg.fs_enabled && cfg.init_fs_populations) {
-        if (sizeof(S) != sizeof(T)) throw std::runtime_error("the free-surface population projection needs fp32 or fp64 storage");
-        k_fs_project<T><<<nblocks(b.n), NT, 0, s>>>(c, (const T*)b.f[0], (T*)b.f[1], (const T*)b.ux, (const T*)b.uy,
-                                                     (const T*)b.uz, b.type, b.obst, (const T*)b.fs_h);
-        HIP_OK(hipMemcpyAsync(b.f[0], b.f[1], 19 * b.n * sizeof(T), hipMemcpyDeviceToDevice, s));
+        k_fs_project<S, T><<<nblocks(b.n), NT, 0, s>>>(c, (const S*)b.f[0], (S*)b.f[1], (const T*)b.ux, (const T*)b.uy,
+                                                        (const T*)b.uz, b.type, b.obst, (const T*)b.fs_h);
+        HIP_OK(hipMemcpyAsync(b.f[0], b.f[1], 19 * b.n * sizeof(S), hipMemcpyDeviceToDevice, s));
     }
     if (cfg.fs_enabled && cfg.fs_tracking == LBM_FST_VOF) {
-        VofArgs<T> a = make_vof<T>(cfg, b, (const T*)b.f[0]);
+        VofArgs<T> a = make_vof<T>(cfg, b);
         k_vof_init_mass<T><<<nblocks(b.n), NT, 0, s>>>(a);
         k_vof_enforce<T><<<nblocks(b.n), NT, 0, s>>>(a);
         k_vof_height<T><<<nblocks(b.n2, 64), 64, 0, s>>>(c, sl, (const T*)b.fill, b.cellflags, (T*)b.fs_h);
@@ -3324,15 +3364,15 @@ __global__ void k_vof_counters_reset(int* icnt, double* dcnt) {
     if (t < 5) dcnt[DC_CAND_MASS + t] = 0.0;    // the collapse sums
 }
 
-template <class T>
-static void vof_pipeline(const lbm_config_t& cfg, DevBufs& b, T* f, int slot, int64_t step, hipStream_t s) {
+template <class S, class T>
+static void vof_pipeline(const lbm_config_t& cfg, DevBufs& b, S* f, int slot, int64_t step, hipStream_t s) {
     const size_t n = b.n;
     const unsigned nb = nblocks(n);
-    auto A = [&]() { return make_vof<T>(cfg, b, f); };
+    auto A = [&]() { return make_vof<T>(cfg, b); };
     auto swap_mass = [&]() { std::swap(b.mass, b.mass2); };
     T empty_thr = (T)cfg.vof_empty_fill_threshold;
     k_vof_counters_reset<<<1, 64, 0, s>>>(b.icnt, b.dcnt);
-    k_vof_exchange<T><<<nb, NT, 0, s>>>(A());
+    k_vof_exchange<S, T><<<nb, NT, 0, s>>>(A(), f);
     k_vof_preclassify<T><<<nb, NT, 0, s>>>(A(), empty_thr);
     // a redistribution round and the classification that follows it; the
     // donor flags alternate between two slots (the preclassification and
@@ -3389,19 +3429,21 @@ static void vof_pipeline(const lbm_config_t& cfg, DevBufs& b, T* f, int slot, in
         Consts<T> c = make_consts<T>(cfg, -1);
         k_vof_height<T><<<nblocks(b.n2, 64), 64, 0, s>>>(c, make_slot<T>(b, slot), (const T*)b.fill, b.cellflags, (T*)b.fs_h);
     }
-    k_vof_refill<T><<<nb, NT, 0, s>>>(A(), f);
-    k_vof_repair<T><<<nb, NT, 0, s>>>(A(), f);
+    k_vof_refill<S, T><<<nb, NT, 0, s>>>(A(), f, (uint32_t)step);
+    k_vof_repair<S, T><<<nb, NT, 0, s>>>(A(), f);
     HIP_OK(hipGetLastError());
 }
 
-template <class T>
+// S: the storage type (16-bit storage rounds at each of the step's stores:
+// collision, stream, and the conditioning and refill where they write)
+template <class S, class T>
 static void staged_t(const lbm_config_t& cfg, DevBufs& b, int& cur, int slot, int64_t step, int omega_band_top, int count_links,
                      StagedState& st, hipStream_t s) {
     Consts<T> c = make_consts<T>(cfg, omega_band_top);
     Slot<T> sl = make_slot<T>(b, slot);
     const unsigned nb = nblocks(b.n);
-    T* f = (T*)b.f[cur];
-    T* fn = (T*)b.f[cur ^ 1];
+    S* f = (S*)b.f[cur];
+    S* fn = (S*)b.f[cur ^ 1];
     // a flat or piecewise bed (and the box) never moves: its cell flags and
     // obstacle masks are the same at every step once computed
     const bool static_geo = !c.bed_moving;
@@ -3414,16 +3456,16 @@ static void staged_t(const lbm_config_t& cfg, DevBufs& b, int& cur, int slot, in
     // over a static geometry the obstacle masks settle after one pass
     // (obst_prev = obst = the closed mask), so without population writes
     // the conditioning has nothing left to do
-    if (!geo_cached || cond_writes_f) k_cond<T><<<nb, NT, 0, s>>>(c, sl, f, b.obst, b.obst_prev);
+    if (!geo_cached || cond_writes_f) k_cond<S, T><<<nb, NT, 0, s>>>(c, sl, f, b.obst, b.obst_prev, (uint32_t)step);
     st.geo_ready = static_geo;
     // a VOF run without conditioning: the stream of the last step left rho
     // and u as the macro step of its populations (see k_macro_fix)
     const bool vof_macro = cfg.fs_enabled && cfg.fs_tracking == LBM_FST_VOF && !cond_writes_f;
     if (vof_macro && st.macro_ready)
-        k_macro_fix<T><<<nb, NT, 0, s>>>(c, f, (T*)b.rho, (T*)b.ux, (T*)b.uy, (T*)b.uz, b.type, b.type_prev, b.cellflags, b.mdirty);
+        k_macro_fix<S, T><<<nb, NT, 0, s>>>(c, f, (T*)b.rho, (T*)b.ux, (T*)b.uy, (T*)b.uz, b.type, b.type_prev, b.cellflags, b.mdirty);
     else
-        k_macro<T><<<nb, NT, 0, s>>>(c, f, (T*)b.rho, (T*)b.ux, (T*)b.uy, (T*)b.uz, b.type, b.cellflags);
-    auto collide = cfg.use_les ? k_collide<T, true> : k_collide<T, false>;
+        k_macro<S, T><<<nb, NT, 0, s>>>(c, f, (T*)b.rho, (T*)b.ux, (T*)b.uy, (T*)b.uz, b.type, b.cellflags);
+    auto collide = cfg.use_les ? k_collide<S, T, true> : k_collide<S, T, false>;
     collide<<<nb, NT, 0, s>>>(c, sl, f, (const T*)b.rho, (const T*)b.ux, (const T*)b.uy, (const T*)b.uz, (T*)b.omega, (T*)b.nu_t,
                               (T*)b.smag, b.type, b.cellflags, (uint32_t)step);
     // the link counters of stream describe one step and are overwritten by
@@ -3432,17 +3474,17 @@ static void staged_t(const lbm_config_t& cfg, DevBufs& b, int& cur, int slot, in
     if (count_links) HIP_OK(hipMemsetAsync(b.icnt + IC_BOUNDARY_CUT, 0, 3 * sizeof(int), s));
     // a free-surface run takes the macro step of the streamed populations
     // inside the stream kernel
-    auto stream = cfg.fs_enabled ? (count_links ? k_stream_bc<T, T, true, true> : k_stream_bc<T, T, false, true>)
-                                 : (count_links ? k_stream_bc<T, T, true, false> : k_stream_bc<T, T, false, false>);
+    auto stream = cfg.fs_enabled ? (count_links ? k_stream_bc<S, S, T, true, true> : k_stream_bc<S, S, T, false, true>)
+                                 : (count_links ? k_stream_bc<S, S, T, true, false> : k_stream_bc<S, S, T, false, false>);
     stream<<<nb, NT, 0, s>>>(c, sl, f, fn, (const T*)b.ux, (const T*)b.uy, (const T*)b.uz, b.type, (const T*)b.fill,
                              (const T*)b.u_inlet, (const T*)b.inflow, (const T*)b.fs_h, b.cellflags, b.icnt, (T*)b.rho, (T*)b.ux,
-                             (T*)b.uy, (T*)b.uz);
+                             (T*)b.uy, (T*)b.uz, (uint32_t)step);
     cur ^= 1;
     HIP_OK(hipGetLastError());
     if (cfg.fs_enabled) {
-        T* f2 = (T*)b.f[cur];
+        S* f2 = (S*)b.f[cur];
         if (cfg.fs_tracking == LBM_FST_VOF) {
-            vof_pipeline<T>(cfg, b, f2, slot, step, s);
+            vof_pipeline<S, T>(cfg, b, f2, slot, step, s);
             st.macro_ready = vof_macro;
         } else {
             // height-function tracking (CODEX:3917-3921)
@@ -3453,19 +3495,19 @@ static void staged_t(const lbm_config_t& cfg, DevBufs& b, int& cur, int slot, in
             k_fs_advect<T><<<n2b, NT, 0, s>>>(c, (const T*)b.fs_qx, (const T*)b.fs_qy, (const T*)b.h[slot], (T*)b.fs_h, t);
             k_fs_classify<T><<<nb, NT, 0, s>>>(c, (const T*)b.fs_h, b.cellflags, (T*)b.fill, b.type, b.type_prev);
             HIP_OK(hipMemsetAsync(b.icnt + IC_FS_REFILL, 0, sizeof(int), s));
-            k_vof_refill<T><<<nb, NT, 0, s>>>(make_vof<T>(cfg, b, f2), f2);
+            k_vof_refill<S, T><<<nb, NT, 0, s>>>(make_vof<T>(cfg, b), f2, (uint32_t)step);
             HIP_OK(hipGetLastError());
         }
     }
 }
 void launch_staged_step(const lbm_config_t& c, DevBufs& b, int& cur, int slot, int64_t step, int omega_band_top, int count_links,
                         StagedState& st, hipStream_t s) {
-    if (b.storage == LBM_STORE_FP64)
-        staged_t<double>(c, b, cur, slot, step, omega_band_top, count_links, st, s);
-    else if (b.storage == LBM_STORE_FP32)
-        staged_t<float>(c, b, cur, slot, step, omega_band_top, count_links, st, s);
-    else
-        throw std::runtime_error("the staged path needs fp32 or fp64 storage");
+    switch (b.storage) {
+        case LBM_STORE_FP64: staged_t<double, double>(c, b, cur, slot, step, omega_band_top, count_links, st, s); break;
+        case LBM_STORE_FP16: staged_t<__half, float>(c, b, cur, slot, step, omega_band_top, count_links, st, s); break;
+        case LBM_STORE_BF16: staged_t<bf16_t, float>(c, b, cur, slot, step, omega_band_top, count_links, st, s); break;
+        default: staged_t<float, float>(c, b, cur, slot, step, omega_band_top, count_links, st, s); break;
+    }
 }
 
 template <class S, class T>
@@ -3587,10 +3629,10 @@ static void materialize_t(const lbm_config_t& cfg, DevBufs& b, int cur, int slot
     Consts<T> c = make_consts<T>(cfg, -1);
     Slot<T> sl = make_slot<T>(b, slot);
     k_cellflags<T><<<nblocks(b.n), NT, 0, s>>>(c, sl, b.cellflags);
-    k_stream_bc<S, T, false, false><<<nblocks(b.n), NT, 0, s>>>(c, sl, (const S*)b.f[cur], (T*)dst, (const T*)b.ux, (const T*)b.uy,
-                                                                (const T*)b.uz, b.type, (const T*)b.fill, (const T*)b.u_inlet,
-                                                                (const T*)b.inflow, (const T*)b.fs_h, b.cellflags, b.icnt,
-                                                                nullptr, nullptr, nullptr, nullptr);
+    k_stream_bc<S, T, T, false, false><<<nblocks(b.n), NT, 0, s>>>(c, sl, (const S*)b.f[cur], (T*)dst, (const T*)b.ux, (const T*)b.uy,
+                                                                   (const T*)b.uz, b.type, (const T*)b.fill, (const T*)b.u_inlet,
+                                                                   (const T*)b.inflow, (const T*)b.fs_h, b.cellflags, b.icnt,
+                                                                   nullptr, nullptr, nullptr, nullptr, 0u);
     HIP_OK(hipGetLastError());
 }
 void launch_materialize(const lbm_config_t& c, DevBufs& b, int cur, int slot, void* dst, hipStream_t s) {

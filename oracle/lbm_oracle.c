@@ -665,11 +665,28 @@ static REAL liquid_fraction(const O *o, size_t id) {
 
 /* ---- step kernels ------------------------------------------------------ */
 
+/* the 16-bit codec of the run: shift s_q, scale 2^k and rounding (on = 0
+   for the FP64 build and for 32/64-bit storage, which round nothing) */
+typedef struct {
+    int on, sr, half;
+    float sh[19], scale, inv;
+} Codec16;
+static void codec16(const O *o, Codec16 *k);
+static REAL round16(const Codec16 *k, REAL v, int q, uint32_t base);
+static int staged_rounds(const O *o);
+static uint32_t bits_of(REAL v);
+static uint32_t dither_base(uint32_t step, uint32_t cell);
+enum { SALT_STREAM = 0x40000000u, SALT_REFILL = 0x20000000u, SALT_COND = 0x10000000u, SALT_INIT = 0x08000000u };
+
 /* update_bed_populations_and_reinit (CODEX:2878-2940) */
 static void bed_populations(O *o) {
     const lbm_config_t *c = &o->c;
     int air = c->physics_mode == LBM_PHYS_AIR;
     int moving = bed_is_moving(o);
+    Codec16 cd;
+    codec16(o, &cd);
+    const int rnd = staged_rounds(o);
+    const uint32_t salt_step = (uint32_t)o->step ^ SALT_COND;
     for (size_t id = 0; id < o->n; id++) {
         o->obst_prev[id] = o->obst[id];
         o->obst[id] = LO(0, id) == 0 ? 1 : 0;
@@ -692,7 +709,9 @@ static void bed_populations(O *o) {
                         continue;
                     }
                     REAL fe = feq(o, R(c->rho0), uw[0], uw[1], uw[2], q);
-                    F(q, id) = frac * F(q, id) + (R(1.0) - frac) * fe;
+                    REAL nv = frac * F(q, id) + (R(1.0) - frac) * fe;
+                    if (rnd && bits_of(nv) != bits_of(F(q, id))) nv = round16(&cd, nv, q, dither_base(salt_step, (uint32_t)id));
+                    F(q, id) = nv;
                 }
             }
 }
@@ -833,8 +852,10 @@ static float bs_uniform(uint32_t seed, uint32_t step, uint32_t cell) {
  * The device stores h_q = round((f_q - s_q) 2^k) as binary16 or bfloat16
  * and reads f_q = s_q + h_q 2^-k, s_q = feq_q(1, u_ref).  The FP32 oracle
  * applies the same round trip wherever the device stores populations
- * (initialisation, upload of F, after every collision), so the FP16/BF16
- * device run is checked against it like the FP32 one.  The roundings are
+ * (initialisation, upload of F, after every collision; on the staged path
+ * also after the stream and where the conditioning, the refill and the
+ * initial free-surface projection write), so the FP16/BF16 device run is
+ * checked against it like the FP32 one.  The roundings are
  * restated from the IEEE 754 binary16 and bfloat16 formats:
  *  - nearest: round to nearest, ties to even (half: 10 fraction bits above
  *    2^-14, quantum 2^-24 below; overflow to infinity);
@@ -907,31 +928,60 @@ static float bf16_sr(float x, uint32_t r) {
 }
 static uint32_t dither_base(uint32_t step, uint32_t cell) { return fmix32(cell * 0x9E3779B1u ^ step * 0x85EBCA77u); }
 
-static void store_round(O *o, uint32_t step) {
+static void codec16(const O *o, Codec16 *k) {
     const lbm_config_t *c = &o->c;
-    if (sizeof(REAL) != sizeof(float)) return;
-    if (c->storage != LBM_STORE_FP16 && c->storage != LBM_STORE_BF16) return;
-    float sh[19];
+    k->on = sizeof(REAL) == sizeof(float) && (c->storage == LBM_STORE_FP16 || c->storage == LBM_STORE_BF16);
     const float ux = c->storage_u_ref[0], uy = c->storage_u_ref[1], uz = c->storage_u_ref[2];
     const float u2 = ux * ux + uy * uy + uz * uz;
     for (int q = 0; q < 19; q++) {
         float cu = 3.0f * ((float)CX[q] * ux + (float)CY[q] * uy + (float)CZ[q] * uz);
-        sh[q] = (float)WD[q] * (1.0f + cu + 0.5f * cu * cu - 1.5f * u2);
+        k->sh[q] = (float)WD[q] * (1.0f + cu + 0.5f * cu * cu - 1.5f * u2);
     }
-    float scale = 1.0f;
-    for (int i = 0; i < c->storage_shift; i++) scale *= 2.0f;
-    const float inv = 1.0f / scale;
-    const int sr = c->storage_rounding == LBM_ROUND_STOCHASTIC;
-    const int half = c->storage == LBM_STORE_FP16;
+    k->scale = 1.0f;
+    for (int i = 0; i < c->storage_shift; i++) k->scale *= 2.0f;
+    k->inv = 1.0f / k->scale;
+    k->sr = c->storage_rounding == LBM_ROUND_STOCHASTIC;
+    k->half = c->storage == LBM_STORE_FP16;
+}
+/* the stored value of population q with value v: round((v - s_q) 2^k) read
+   back as s_q + h 2^-k, with the dither word of (base, q) */
+static REAL round16(const Codec16 *k, REAL v, int q, uint32_t base) {
+    float d = ((float)v - k->sh[q]) * k->scale;
+    uint32_t r = base + (uint32_t)q * 0x9E3779B9u;
+    float h = k->half ? (k->sr ? half_sr(d, r) : half_rne(d)) : (k->sr ? bf16_sr(d, r) : bf16_rne(d));
+    return (REAL)(k->sh[q] + h * k->inv);
+}
+static void store_round(O *o, uint32_t step) {
+    Codec16 k;
+    codec16(o, &k);
+    if (!k.on) return;
     for (size_t id = 0; id < o->n; id++) {
         uint32_t base = dither_base(step, (uint32_t)id);
-        for (int q = 0; q < 19; q++) {
-            float d = ((float)F(q, id) - sh[q]) * scale;
-            uint32_t r = base + (uint32_t)q * 0x9E3779B9u;
-            float h = half ? (sr ? half_sr(d, r) : half_rne(d)) : (sr ? bf16_sr(d, r) : bf16_rne(d));
-            F(q, id) = (REAL)(sh[q] + h * inv);
-        }
+        for (int q = 0; q < 19; q++) F(q, id) = round16(&k, F(q, id), q, base);
     }
+}
+/* The device's fused step (air without a free surface, the box or the phi
+   modes) stores populations once per step, after the collision.  Its staged
+   step also stores the post-stream populations and, where they write, the
+   conditioning (the values it changed), the refill (the refilled cells) and
+   the initial free-surface projection (the projected links); each of those
+   store points draws its dither from the step number with a salt of its
+   own.  A 16-bit run on the staged path rounds at all of them. */
+static int fused_path(const O *o) {
+    const lbm_config_t *c = &o->c;
+    if (c->kernel_path == LBM_PATH_STAGED) return 0;
+    return c->physics_mode == LBM_PHYS_AIR && !c->fs_enabled && !c->include_cube && !c->phi_fraction && o->nz >= 3;
+}
+static int staged_rounds(const O *o) {
+    Codec16 k;
+    codec16(o, &k);
+    return k.on && !fused_path(o);
+}
+static uint32_t bits_of(REAL v) {
+    float f = (float)v;
+    uint32_t u;
+    memcpy(&u, &f, sizeof u);
+    return u;
 }
 
 /* collide_KBC (CODEX:3143-3321) */
@@ -2054,6 +2104,9 @@ static void fs_advect_height(O *o, REAL t) {
 /* refill_new_free_surface_cells (CODEX:2163-2316) */
 static void vof_refill(O *o) {
     const lbm_config_t *c = &o->c;
+    Codec16 cd;
+    codec16(o, &cd);
+    const int rnd = staged_rounds(o);  /* the refilled cells are stored (rounded) */
     o->cnt.free_surface_refill_count = 0;
     for (int k = 0; k < o->nz; k++)
         for (int j = 0; j < o->ny; j++)
@@ -2156,6 +2209,10 @@ static void vof_refill(O *o) {
                     o->uy[id] = ury;
                     o->uz[id] = urz;
                     for (int q = 0; q < 19; q++) F(q, id) = fv[q];
+                }
+                if (rnd) {
+                    const uint32_t base = dither_base((uint32_t)o->step ^ SALT_REFILL, (uint32_t)id);
+                    for (int q = 0; q < 19; q++) F(q, id) = round16(&cd, F(q, id), q, base);
                 }
                 o->cnt.free_surface_refill_count++;
             }
@@ -2397,6 +2454,9 @@ void API(orc_geometry)(void *p, float t) {
    x (or y) walls are not gas */
 static void fs_init_projection(O *o) {
     const lbm_config_t *c = &o->c;
+    Codec16 cd;
+    codec16(o, &cd);
+    const int rnd = staged_rounds(o);
     memcpy(o->fn, o->f, 19 * o->n * sizeof(REAL));
     for (int k = 0; k < o->nz; k++)
         for (int j = 0; j < o->ny; j++)
@@ -2428,7 +2488,9 @@ static void fs_init_projection(O *o) {
                     REAL rho_fs = R(c->rho0);
                     if (use_hbp(o) || c->fs_boundary_mode == LBM_FSB_HYDROSTATIC) rho_fs = fs_density_link(o, k, j, i, q);
                     REAL ux = o->ux[id], uy = o->uy[id], uz = o->uz[id];
-                    o->fn[(size_t)q * o->n + id] = feq(o, rho_fs, ux, uy, uz, q) + feq(o, rho_fs, ux, uy, uz, qb) - F(qb, id);
+                    REAL fp = feq(o, rho_fs, ux, uy, uz, q) + feq(o, rho_fs, ux, uy, uz, qb) - F(qb, id);
+                    if (rnd) fp = round16(&cd, fp, q, dither_base(SALT_INIT, (uint32_t)id));  /* a projected link is stored */
+                    o->fn[(size_t)q * o->n + id] = fp;
                 }
             }
     memcpy(o->f, o->fn, 19 * o->n * sizeof(REAL));
@@ -2496,6 +2558,7 @@ static void one_step(O *o) {
     x_wall_bounceback(o);
     open_bc(o);
     swap_f(o);
+    if (staged_rounds(o)) store_round(o, (uint32_t)o->step ^ SALT_STREAM);
     if (is_water(o)) {
         macro(o);
         if (is_vof(o)) {

@@ -178,3 +178,38 @@ def test_stochastic_rounding_matches_hardware_vectors():
     naive = (((u & 0x7FFFFFFF) + (r >> 19)) & ~np.uint32(0x1FFF)) | (u & 0x80000000)
     naive16 = naive.view(np.float32).astype(np.float16).view(np.uint16)
     assert (naive16[sub] != v["f16"][sub]).any()
+
+
+def _on_fp16_grid(f, sh, k):
+    """True where a population value v is a stored value: v = fl(s_q + h 2^-k)
+    for the binary16 h nearest to (v - s_q) 2^k (float32 arithmetic)."""
+    scale = np.float32(2.0**k)
+    d = (f - sh[:, None, None, None]) * scale
+    h = d.astype(np.float16).astype(np.float32)
+    return (sh[:, None, None, None] + h * np.float32(2.0**-k)).astype(np.float32) == f
+
+
+def test_staged_fp16_stores_the_post_stream_populations():
+    """On the staged path (here: a VOF run) a 16-bit run stores the
+    post-stream populations, so every population the oracle holds after a
+    step lies on the storage grid.  The fused path stores only the
+    post-collision set: streaming moves stored values unchanged, but the
+    populations it blends (bounce-back at cut links, the inflow and outflow
+    layers) leave the grid."""
+    from test_oracle_vof import vof_env
+
+    cfg = parse_env(vof_env(LBM_STORAGE="fp16"))
+    o = OracleCase(cfg.to_struct())
+    setup_case(o, cfg)
+    o.step(3)
+    f = o.download(abi.FIELD_F)
+    assert np.all(_on_fp16_grid(f, shifts(cfg.storage_u_ref), cfg.storage_shift))
+    o.close()
+    cfg = parse_env(env_of("waves_256x128x128", LBM_LX_M=32, LBM_LY_M=8, LBM_LZ_M=16, LBM_BED_AMP_M=2,
+                           LBM_BED_WAVELENGTH_M=16, LBM_STORAGE="fp16"))
+    o = OracleCase(cfg.to_struct())
+    setup_case(o, cfg)
+    o.step(3)
+    f = o.download(abi.FIELD_F)
+    assert not np.all(_on_fp16_grid(f, shifts(cfg.storage_u_ref), cfg.storage_shift))
+    o.close()

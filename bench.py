@@ -20,8 +20,10 @@ Prints ONE JSON line:
                            on the same workload
   workloads                the plain channel at 64x32x32 and 512x256x192
                            (FP32 / FP16 / BF16) and the VOF box 192x96x160
-  accuracy                 FP32 / FP16 / BF16 against FP64 on the plain
-                           channel at 256x128x128 after 2000 steps
+                           (FP32 / FP16)
+  accuracy                 FP32 / FP16 / BF16 against FP64: the plain
+                           channel and the waves case at 256x128x128 after
+                           2000 steps, the VOF box after 1500 steps
   serving                  the ensemble server against sequential runs
 (all reported beside the headline, which is the default workload)
 """
@@ -215,7 +217,7 @@ def serving(n_cases: int, storage: str, seed: int = 7):
     progress(f"serving {storage}: Poisson stream done")
     return {
         "cases": n_cases,
-        "storage_air": storage,
+        "storage": storage,
         "lattice_updates": lups,
         "sequential_mlups": round(lups / seq_wall / 1e6, 1),
         "ensemble_mlups": round(burst["aggregate_mlups"], 1),
@@ -236,6 +238,7 @@ WORKLOADS = (
     ("c3_plain_512x256x192_fp16", "plain_512x256x192", {"LBM_STORAGE": "fp16"}, 76),
     ("c3_plain_512x256x192_bf16", "plain_512x256x192", {"LBM_STORAGE": "bf16"}, 76),
     ("c4_vof_192x96x160_fp32", "vof_192x96x160", {}, 170),
+    ("c4_vof_192x96x160_fp16", "vof_192x96x160", {"LBM_STORAGE": "fp16"}, 94),
 )
 
 
@@ -261,46 +264,94 @@ def workloads(steps: int):
         # whole-step MLUPS ratio (the metric), and the step-kernel time ratio beside it
         out[f"c3_{st}_speedup_vs_fp32"] = round(r["mlups"] / ref["mlups"], 3)
         out[f"c3_{st}_kernel_speedup_vs_fp32"] = round(ref["kernel_ms"] / r["kernel_ms"], 3)
+    c4, c4h = out["c4_vof_192x96x160_fp32"], out["c4_vof_192x96x160_fp16"]
+    out["c4_fp16_speedup_vs_fp32"] = round(c4h["mlups"] / c4["mlups"], 3)
     return out
 
 
-def accuracy(steps: int = 2000):
-    """Reduced storage against the FP64 run of the same case: the plain
-    channel physics at 256x128x128, seed 12345, after `steps` steps: max
-    |u - u64| / max |u64|, max |dp| with dp = (rho - rho64)/3, and the
-    relative drift of the total mass."""
+def accuracy():
+    """Reduced storage against the FP64 run of the same case, three cases:
+      * plain channel physics at 256x128x128 and the air-over-waves case
+        (256x128x128), seed 12345, after 2000 steps: max |u - u64| /
+        max |u64|, max |dp| with dp = (rho - rho64)/3, and the relative drift
+        of the total mass;
+      * the VOF box 192x96x160 after 1500 steps: water volume error, the
+        fraction of cells of another type, and over the cells liquid in both
+        runs the velocity error (max and rms, relative to the FP64 field's)
+        and the pressure perturbation error in units of the pool's
+        hydrostatic head rho0 g H."""
     import numpy as np
 
     import lbm_abi as abi
     from lbm_config import parse_env, preset_env
     from lbm_solver import Solver
 
-    base = dict(preset_env("plain_64x32x32"), LBM_LX_M="256", LBM_LY_M="128", LBM_LZ_M="128")
+    def air_case(env, steps):
+        def run(storage):
+            solver = Solver(config=parse_env(dict(env, LBM_STORAGE=storage)))
+            solver.initialize()
+            case = solver.case
+            m0 = case.reduce(abi.REDUCE_MASS)
+            case.step(steps)
+            u = np.stack([case.download(x).astype(np.float64) for x in (abi.FIELD_UX, abi.FIELD_UY, abi.FIELD_UZ)])
+            rho = case.download(abi.FIELD_RHO).astype(np.float64)
+            m1 = case.reduce(abi.REDUCE_MASS)
+            case.close()
+            return u, rho, m0, m1
 
-    def run(storage):
-        solver = Solver(config=parse_env(dict(base, LBM_STORAGE=storage)))
-        solver.initialize()
-        case = solver.case
-        m0 = case.reduce(abi.REDUCE_MASS)
-        case.step(steps)
-        u = np.stack([case.download(x).astype(np.float64) for x in (abi.FIELD_UX, abi.FIELD_UY, abi.FIELD_UZ)])
-        rho = case.download(abi.FIELD_RHO).astype(np.float64)
-        m1 = case.reduce(abi.REDUCE_MASS)
-        case.close()
-        return u, rho, m0, m1
+        u64, r64, m0, m1 = run("fp64")
+        out = {"steps": steps, "fp64_mass_drift": abs(m1 - m0) / m0}
+        umax = float(np.max(np.linalg.norm(u64, axis=0)))
+        for st in ("fp32", "fp16", "bf16"):
+            u, r, a0, a1 = run(st)
+            out[st] = {
+                "velocity_rel_error": float(np.max(np.linalg.norm(u - u64, axis=0)) / umax),
+                "pressure_abs_error": float(np.max(np.abs(r - r64)) / 3.0),
+                "mass_drift": abs(a1 - a0) / a0,
+            }
+        return out
 
-    u64, r64, m0, m1 = run("fp64")
-    progress("accuracy: fp64 reference run done")
-    out = {"case": "plain channel physics 256x128x128, seed 12345", "steps": steps,
-           "fp64_mass_drift": abs(m1 - m0) / m0}
-    umax = float(np.max(np.linalg.norm(u64, axis=0)))
-    for st in ("fp32", "fp16", "bf16"):
-        u, r, a0, a1 = run(st)
-        out[st] = {
-            "velocity_rel_error": float(np.max(np.linalg.norm(u - u64, axis=0)) / umax),
-            "pressure_abs_error": float(np.max(np.abs(r - r64)) / 3.0),
-            "mass_drift": abs(a1 - a0) / a0,
-        }
+    def water_case(env, steps):
+        cfg = parse_env(env)
+        p_scale = float(cfg.g_lb) * float(cfg.free_surface_base_m) / float(cfg.dx)
+
+        def run(storage):
+            solver = Solver(config=parse_env(dict(env, LBM_STORAGE=storage)))
+            solver.initialize()
+            case = solver.case
+            case.step(steps)
+            r = {"type": case.download(abi.FIELD_FS_TYPE), "rho": case.download(abi.FIELD_RHO).astype(np.float64),
+                 "u": np.stack([case.download(x).astype(np.float64) for x in (abi.FIELD_UX, abi.FIELD_UY, abi.FIELD_UZ)]),
+                 "volume": case.reduce(abi.REDUCE_VOF_VOLUME)}
+            case.close()
+            return r
+
+        ref = run("fp64")
+        rms = lambda x: float(np.sqrt(np.mean(np.square(x))))
+        out = {"steps": steps, "pressure_scale": p_scale}
+        for st in ("fp32", "fp16", "bf16"):
+            r = run(st)
+            liq = (ref["type"] == abi.FS_LIQUID) & (r["type"] == abi.FS_LIQUID)
+            un = np.linalg.norm(ref["u"], axis=0)[liq]
+            dun = np.linalg.norm(r["u"] - ref["u"], axis=0)[liq]
+            dp = ((r["rho"] - ref["rho"]) / 3.0)[liq]
+            out[st] = {
+                "volume_rel_error": abs(r["volume"] - ref["volume"]) / ref["volume"],
+                "type_mismatch_frac": float(np.mean(r["type"] != ref["type"])),
+                "velocity_rel_error_max": float(np.max(dun) / np.max(un)),
+                "velocity_rel_error_rms": rms(dun) / rms(un),
+                "pressure_error_max_over_head": float(np.max(np.abs(dp))) / p_scale,
+            }
+        return out
+
+    out = {}
+    out["plain_256x128x128"] = air_case(dict(preset_env("plain_64x32x32"), LBM_LX_M="256", LBM_LY_M="128",
+                                             LBM_LZ_M="128"), 2000)
+    progress("accuracy: plain channel done")
+    out["waves_256x128x128"] = air_case(preset_env("waves_256x128x128"), 2000)
+    progress("accuracy: waves done")
+    out["vof_192x96x160"] = water_case(preset_env("vof_192x96x160"), 1500)
+    progress("accuracy: vof done")
     return out
 
 

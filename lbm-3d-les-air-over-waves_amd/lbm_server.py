@@ -237,20 +237,19 @@ def ensemble_case_env(kind: str, size, rng: np.random.Generator, storage: str = 
         "LBM_BED_PROFILE": "flat", "LBM_BED_LEVEL_M": "0", "LBM_FREE_SURFACE_DEPTH_M": str(nz // 2),
         "LBM_GAUSSIAN_AMPLITUDE_M": f"{amp:.6f}", "LBM_GAUSSIAN_SIGMA_M": str(nx / 8.0),
         "LBM_GAUSSIAN_CENTER_X_M": str(nx / 2.0), "LBM_X_BOUNDARY": "solid", "LBM_Y_BOUNDARY": "solid",
-        "LBM_VOF_INITIAL_SHAPE": "none", "LBM_G_LB": "1e-5", "LBM_SEED": "7",
+        "LBM_VOF_INITIAL_SHAPE": "none", "LBM_G_LB": "1e-5", "LBM_SEED": "7", "LBM_STORAGE": storage,
     }
 
 
 def ensemble_stream(n_cases: int, seed: int = 7, storage: str = "fp32", steps_range=(500, 2000)):
     """(env, steps) of n_cases requests: half air, half water, sizes uniform
-    over SIZES, steps ~ U[steps_range]; water cases stay FP32 (the VOF path
-    stores FP32)."""
+    over SIZES, steps ~ U[steps_range], every case stored in `storage`."""
     rng = np.random.default_rng(seed)
     out = []
     for c in range(n_cases):
         kind = "air" if c % 2 == 0 else "water"
         size = SIZES[int(rng.integers(0, len(SIZES)))]
         steps = int(rng.integers(steps_range[0], steps_range[1] + 1))
-        env = ensemble_case_env(kind, size, rng, storage if kind == "air" else "fp32")
+        env = ensemble_case_env(kind, size, rng, storage)
         out.append((env, steps))
     return out

@@ -19,6 +19,7 @@ import pytest
 
 import lbm_abi as abi
 from harness import env_of
+from lbm_config import parse_env
 from test_gpu_cube import CASES as BOX_CASES
 from test_gpu_height import CASES as HEIGHT_CASES
 from test_gpu_parity import F32_TOL, device_case, oracle_case
@@ -89,11 +90,12 @@ def test_staged_air_16bit_parity(name, storage):
     o.close()
 
 
-def water_errors(ref, c):
+def water_errors(ref, c, p_scale):
     """Errors of run c against the FP64 run ref: water volume
-    |V - V64| / V64; the fraction of cells whose type differs; velocity and
-    pressure perturbation p' = (rho - rho0) / 3 over the cells liquid in both
-    runs, max |d| / max |x64| and rms |d| / rms |x64|."""
+    |V - V64| / V64; the fraction of cells whose type differs; over the cells
+    liquid in both runs, the velocity error max |du| / max |u64| and
+    rms |du| / rms |u64|, and the error of the pressure perturbation
+    p' = (rho - rho0) / 3 in units of p_scale (max and rms)."""
     t64, t = ref.download(abi.FIELD_FS_TYPE), c.download(abi.FIELD_FS_TYPE)
     liq = (t64 == abi.FS_LIQUID) & (t == abi.FS_LIQUID)
     u = np.stack([ref.download(x) for x in (abi.FIELD_UX, abi.FIELD_UY, abi.FIELD_UZ)])
@@ -106,31 +108,35 @@ def water_errors(ref, c):
     rms = lambda x: float(np.sqrt(np.mean(np.square(x))))
     return {"volume": abs(c.reduce(abi.REDUCE_VOF_VOLUME) - v64) / v64, "type_mismatch": float(np.mean(t != t64)),
             "velocity_max": float(np.max(dun) / np.max(un)), "velocity_rms": rms(dun) / rms(un),
-            "pressure_max": float(np.max(np.abs(dpl)) / np.max(np.abs(pl))), "pressure_rms": rms(dpl) / rms(pl)}
+            "pressure_max": float(np.max(np.abs(dpl))) / p_scale, "pressure_rms": rms(dpl) / p_scale}
 
 
 def test_vof_fp16_error_against_fp64():
     """FP16 storage of the falling-block case against its FP64 run after 150
     steps (the FP32 run beside it for scale).  Bounds: water volume <= 1e-5;
-    over the cells liquid in both runs, velocity and pressure perturbation
-    rms <= 2e-2 and max <= 1e-1 of the FP64 field's; cells of a different
-    type <= 1e-3.  Interface cells are left out of the field norms: a cell
-    that is gas in one run and wet in the other differs by its whole
-    velocity."""
+    cells of a different type <= 1e-3; over the cells liquid in both runs,
+    velocity rms <= 2e-3 and max <= 1e-2 of the FP64 field's, pressure
+    perturbation max <= 1e-3 of the pool's hydrostatic head rho0 g H (the
+    component-balanced p' is a small difference from rho0, so its own
+    magnitude is no scale: FP32 rounding of rho alone is 10 % of it).
+    Interface cells are left out of the field norms: a cell that is gas in
+    one run and wet in the other differs by its whole velocity."""
     env = vof_env()
+    cfg = parse_env(env)
+    p_scale = float(cfg.g_lb) * float(cfg.free_surface_base_m) / float(cfg.dx)
     _, ref = device_case(dict(env, LBM_STORAGE="fp64"))
     runs = {s: device_case(dict(env, LBM_STORAGE=s))[1] for s in ("fp32", "fp16")}
     ref.step(150)
     for c in runs.values():
         c.step(150)
-    err = {s: water_errors(ref, c) for s, c in runs.items()}
+    err = {s: water_errors(ref, c, p_scale) for s, c in runs.items()}
     for s, e in err.items():
         print(s, e)
     e = err["fp16"]
     assert e["volume"] <= 1e-5
     assert e["type_mismatch"] <= 1e-3
-    assert e["velocity_rms"] <= 2e-2 and e["velocity_max"] <= 1e-1
-    assert e["pressure_rms"] <= 2e-2 and e["pressure_max"] <= 1e-1
+    assert e["velocity_rms"] <= 2e-3 and e["velocity_max"] <= 1e-2
+    assert e["pressure_max"] <= 1e-3
     ref.close()
     for c in runs.values():
         c.close()

@@ -48,7 +48,8 @@ def progress(msg: str):
     """A progress line on stderr (stdout carries only the JSON result)."""
     print(f"bench: {msg}", file=sys.stderr, flush=True)
 BYTES_PER_LU = {"fp32": 152, "fp16": 76, "bf16": 76, "fp64": 304}
-VOF_BYTES_PER_LU = 170
+# VOF: the populations once each way plus mass, fill and type (18 B)
+VOF_BYTES_PER_LU = {k: v + 18 for k, v in BYTES_PER_LU.items()}
 PROFILES = ROOT / "profiles"
 
 
@@ -400,7 +401,7 @@ def main():
     progress(f"main run {args.workload}: {main_run['mlups']:.1f} MLUPS")
     bpl = BYTES_PER_LU[cfg.storage]
     if cfg.water_free_surface_enabled and cfg.free_surface_tracking_mode == "vof":
-        bpl = VOF_BYTES_PER_LU  # populations + mass, fill and type
+        bpl = VOF_BYTES_PER_LU[cfg.storage]  # populations + mass, fill and type
     achieved = main_run["cells"] * bpl / (main_run["ms_kernel"] * 1e-3) / 1e9
     achieved_wall = main_run["cells"] * bpl / (main_run["ms_per_step"] * 1e-3) / 1e9
     traffic, traffic_src = measured_traffic(args.workload, cfg.storage)

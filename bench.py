@@ -24,7 +24,10 @@ Prints ONE JSON line:
   accuracy                 FP32 / FP16 / BF16 against FP64: the plain
                            channel and the waves case at 256x128x128 after
                            2000 steps, the VOF box after 1500 steps
+  graph_launch             one small case, calls of 16 steps: direct
+                           launches against HIP-graph launches
   serving                  the ensemble server against sequential runs
+                           (streams, and streams + graph-launched chunks)
 (all reported beside the headline, which is the default workload)
 """
 from __future__ import annotations
@@ -191,14 +194,21 @@ def serving(n_cases: int, storage: str, seed: int = 7):
             progress(f"serving {storage}: {c + 1}/{n_cases} sequential cases done")
     seq_wall = time.perf_counter() - t0
     budget = 200 << 30
-    srv = EnsembleServer(budget, max_active=64, chunk_steps=16, n_workers=4)
-    t0 = time.perf_counter()
-    for env, steps in stream:
-        srv.submit(env, steps)
-    srv.run_until_idle()
-    burst = srv.stats(t0)
-    srv.close()
+
+    def run_burst(graphs):
+        srv = EnsembleServer(budget, max_active=64, chunk_steps=16, n_workers=4, graphs=graphs)
+        t0 = time.perf_counter()
+        for env, steps in stream:
+            srv.submit(env, steps)
+        srv.run_until_idle()
+        st = srv.stats(t0)
+        srv.close()
+        return st
+
+    burst = run_burst(False)
     progress(f"serving {storage}: burst of {n_cases} done in {burst['wall_s']:.1f} s")
+    burst_graphs = run_burst(True)
+    progress(f"serving {storage}: burst with graphs done in {burst_graphs['wall_s']:.1f} s")
     capacity = n_cases / burst["wall_s"]
     rate = 0.8 * capacity
     rng = np.random.default_rng(seed + 1)
@@ -222,13 +232,14 @@ def serving(n_cases: int, storage: str, seed: int = 7):
         "lattice_updates": lups,
         "sequential_mlups": round(lups / seq_wall / 1e6, 1),
         "ensemble_mlups": round(burst["aggregate_mlups"], 1),
+        "ensemble_mlups_graphs": round(burst_graphs["aggregate_mlups"], 1),
         "speedup_vs_sequential": round(burst["aggregate_mlups"] / (lups / seq_wall / 1e6), 3),
         "poisson_rate_per_s": round(rate, 3),
         "poisson_mlups": round(poisson["aggregate_mlups"], 1),
         "latency_p50_s": round(poisson["latency_p50_s"], 4),
         "latency_p99_s": round(poisson["latency_p99_s"], 4),
         "peak_active": max(burst["peak_active"], poisson["peak_active"]),
-        "failed": burst["failed"] + poisson["failed"],
+        "failed": burst["failed"] + poisson["failed"] + burst_graphs["failed"],
     }
 
 
@@ -267,6 +278,44 @@ def workloads(steps: int):
         out[f"c3_{st}_kernel_speedup_vs_fp32"] = round(ref["kernel_ms"] / r["kernel_ms"], 3)
     c4, c4h = out["c4_vof_192x96x160_fp32"], out["c4_vof_192x96x160_fp16"]
     out["c4_fp16_speedup_vs_fp32"] = round(c4h["mlups"] / c4["mlups"], 3)
+    return out
+
+
+def graph_launch(chunks: int = 100, chunk: int = 16):
+    """One small case alone, `chunks` calls of `chunk` steps: direct launches
+    against each call captured into a HIP graph (MLUPS from the wall clock,
+    the same case and steps)."""
+    from lbm_config import parse_env, preset_env
+    from lbm_solver import Solver
+
+    cases = {
+        "plain_64x32x32": preset_env("plain_64x32x32"),
+        "waves_32x32x32": dict(preset_env("waves_256x128x128"), LBM_LX_M="32", LBM_LY_M="32", LBM_LZ_M="32",
+                               LBM_BED_AMP_M="2", LBM_BED_WAVELENGTH_M="8"),
+        "waves_128x64x64": dict(preset_env("waves_256x128x128"), LBM_LX_M="128", LBM_LY_M="64", LBM_LZ_M="64",
+                                LBM_BED_AMP_M="3", LBM_BED_WAVELENGTH_M="32"),
+    }
+    out = {"chunk_steps": chunk, "chunks": chunks}
+    for label, env in cases.items():
+        cfg = parse_env(env)
+        r = {}
+        for mode in (0, 1):
+            solver = Solver(config=cfg)
+            solver.initialize()
+            case = solver.case
+            case.set_option("graph", mode)
+            case.step(2 * chunk)
+            case.sync()
+            t0 = time.perf_counter()
+            for _ in range(chunks):
+                case.step(chunk)
+            case.sync()
+            wall = time.perf_counter() - t0
+            r["graph_mlups" if mode else "direct_mlups"] = round(cfg.Nx * cfg.Ny * cfg.Nz * chunks * chunk / wall / 1e6, 1)
+            solver.close()
+        r["graph_speedup"] = round(r["graph_mlups"] / r["direct_mlups"], 3)
+        out[label] = r
+        progress(f"graph launch {label}: {r}")
     return out
 
 
@@ -460,6 +509,7 @@ def main():
         result["variants"] = variants
     if not args.no_variants:
         result["workloads"] = workloads(max(20, args.steps // 4))
+        result["graph_launch"] = graph_launch()
         result["accuracy"] = accuracy()
     if args.serving_cases > 0 and not args.no_variants:
         result["serving"] = {st: serving(args.serving_cases, st) for st in ("fp32", "fp16")}

@@ -334,7 +334,11 @@ int lbm_device_bytes(const lbm_config_t *cfg, size_t *out);
 /* Tuning and instrumentation knobs (no counterpart in the reference):
    "timing" (0/1: record HIP events around every step kernel),
    "block_x" (64, 128 or 256: x extent of a block of the fused passes;
-   default from nx). */
+   default from nx),
+   "graph" (0/1: on the fused path, an lbm_step call of several steps is
+   captured into a HIP graph and launched as one graph; the executable
+   graph is kept and updated by the next call; ignored with "timing"),
+   "wavefront", "wave_lag", "xcd_strips" (the fused step's schedule). */
 int lbm_set_option(lbm_handle_t *h, const char *key, int64_t value);
 
 /* Resolved execution path of the handle: LBM_PATH_STAGED or LBM_PATH_FUSED. */

@@ -48,6 +48,12 @@ struct lbm_handle {
     int block_x = 64;  // x extent of a fused-kernel block (64, 128 or 256 cells; set from nx at creation)
     FusedSched sched;  // fused path: wavefront (one launch per step) or three launches
     StagedState staged;  // staged path: what a step may reuse from the previous one
+    // fused path, option "graph": the launches of an lbm_step call are
+    // captured into a HIP graph and launched as one; the executable graph is
+    // kept and updated with the next call's launches (same kernels, new
+    // arguments), and instantiated again when the launch sequence differs
+    int graph = 0;
+    hipGraphExec_t gexec = nullptr;
     int initialized = 0;
     int geometry_done = 0;
     int timing = 0;
@@ -386,6 +392,7 @@ int lbm_create(const lbm_config_t* cfg, lbm_handle_t** out) {
 void lbm_destroy(lbm_handle_t* h) {
     if (!h) return;
     if (h->stream) (void)hipStreamSynchronize(h->stream);
+    if (h->gexec) (void)hipGraphExecDestroy(h->gexec);
     for (hipEvent_t e : h->ev) (void)hipEventDestroy(e);
     free_all(h);
     if (h->stream) (void)hipStreamDestroy(h->stream);
@@ -454,18 +461,9 @@ int lbm_initialize(lbm_handle_t* h, const lbm_init_params_t* p) {
     return lbm_init_populations(h);
 }
 
-int lbm_step(lbm_handle_t* h, int32_t nsteps) {
-    API_BEGIN(h)
-    if (!h->initialized) return fail(h, LBM_E_STATE, "lbm_init_populations (or an upload of F) must run first");
-    if (nsteps < 0) return fail(h, LBM_E_ARG, "nsteps must be >= 0");
-    if (nsteps == 0) return LBM_OK;
+// enqueue the launches of nsteps steps on the handle's stream
+static void enqueue_steps(lbm_handle* h, int32_t nsteps) {
     const lbm_config_t& c = h->cfg;
-    size_t need = h->timing ? (size_t)2 * nsteps + 2 : 0;
-    while (h->ev.size() < need) {
-        hipEvent_t e;
-        CHECK(hipEventCreate(&e));
-        h->ev.push_back(e);
-    }
     if (h->timing) CHECK(hipEventRecord(h->ev[0], h->stream));
     const bool fixed = c.bed_profile == LBM_BED_FLAT || c.bed_profile == LBM_BED_PIECEWISE;
     const bool batched = h->b.ring == BED_RING;  // a moving bed: its levels come BED_BATCH steps at a time
@@ -505,6 +503,70 @@ int lbm_step(lbm_handle_t* h, int32_t nsteps) {
         if (h->timing) CHECK(hipEventRecord(h->ev[3 + 2 * s], h->stream));
         h->step = step;
     }
+}
+
+// the same launches captured into a graph (fused path only: its steps are
+// kernel launches alone) and launched as one
+static void graph_steps(lbm_handle* h, int32_t nsteps) {
+    if (h->sched.epoch >= INT32_MAX - 1 - nsteps) {  // the flag reset (a fill) stays outside the graph
+        const size_t wq = wave_flag_ints(h->cfg.nx, h->cfg.ny, h->cfg.nz);
+        CHECK(hipMemsetAsync(h->b.waveq + WAVE_FLAG0, 0, (wq - WAVE_FLAG0) * sizeof(int), h->stream));
+        h->sched.epoch = 0;
+    }
+    // the handle's state advances while the launches are recorded: keep it
+    // to restore if the capture fails
+    const int64_t step0 = h->step;
+    const int cur0 = h->cur, post0 = h->post_stream, epoch0 = h->sched.epoch;
+    int valid0[BED_RING];
+    memcpy(valid0, h->slot_valid, sizeof valid0);
+    CHECK(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+    hipGraph_t g = nullptr;
+    try {
+        enqueue_steps(h, nsteps);
+    } catch (...) {
+        (void)hipStreamEndCapture(h->stream, &g);
+        if (g) (void)hipGraphDestroy(g);
+        h->step = step0;
+        h->cur = cur0;
+        h->post_stream = post0;
+        h->sched.epoch = epoch0;
+        memcpy(h->slot_valid, valid0, sizeof valid0);
+        throw;
+    }
+    CHECK(hipStreamEndCapture(h->stream, &g));
+    bool updated = false;
+    if (h->gexec) {
+        hipGraphNode_t err_node = nullptr;
+        hipGraphExecUpdateResult res;
+        updated = hipGraphExecUpdate(h->gexec, g, &err_node, &res) == hipSuccess && res == hipGraphExecUpdateSuccess;
+        if (!updated) {
+            (void)hipGetLastError();
+            (void)hipGraphExecDestroy(h->gexec);
+            h->gexec = nullptr;
+        }
+    }
+    hipError_t e = hipSuccess;
+    if (!updated) e = hipGraphInstantiate(&h->gexec, g, nullptr, nullptr, 0);
+    if (e == hipSuccess) e = hipGraphLaunch(h->gexec, h->stream);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) throw std::runtime_error(std::string("graph launch: ") + hipGetErrorString(e));
+}
+
+int lbm_step(lbm_handle_t* h, int32_t nsteps) {
+    API_BEGIN(h)
+    if (!h->initialized) return fail(h, LBM_E_STATE, "lbm_init_populations (or an upload of F) must run first");
+    if (nsteps < 0) return fail(h, LBM_E_ARG, "nsteps must be >= 0");
+    if (nsteps == 0) return LBM_OK;
+    size_t need = h->timing ? (size_t)2 * nsteps + 2 : 0;
+    while (h->ev.size() < need) {
+        hipEvent_t e;
+        CHECK(hipEventCreate(&e));
+        h->ev.push_back(e);
+    }
+    if (h->graph && h->path == LBM_PATH_FUSED && !h->timing && nsteps > 1)
+        graph_steps(h, nsteps);
+    else
+        enqueue_steps(h, nsteps);
     if (h->timing) {
         CHECK(hipEventRecord(h->ev[1], h->stream));
         CHECK(hipEventSynchronize(h->ev[1]));
@@ -826,6 +888,10 @@ int lbm_set_option(lbm_handle_t* h, const char* key, int64_t value) {
     if (!h || !key) return LBM_E_ARG;
     if (!strcmp(key, "timing")) {
         h->timing = value ? 1 : 0;
+        return LBM_OK;
+    }
+    if (!strcmp(key, "graph")) {
+        h->graph = value ? 1 : 0;
         return LBM_OK;
     }
     if (!strcmp(key, "wavefront")) {

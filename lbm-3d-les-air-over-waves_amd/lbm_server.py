@@ -72,8 +72,12 @@ def footprint_bytes(config: SolverConfig) -> int:
 class EnsembleServer:
     """Continuous-batching scheduler over the native stepping runtime."""
 
-    def __init__(self, hbm_budget_bytes: int, max_active: int = 64, chunk_steps: int = 16, n_workers: int = 4):
+    def __init__(self, hbm_budget_bytes: int, max_active: int = 64, chunk_steps: int = 16, n_workers: int = 4,
+                 graphs: bool = False):
+        """graphs: the chunks of the cases on the fused path are captured
+        into HIP graphs and launched as one (option "graph" of the C ABI)."""
         self.lib = load_library()
+        self.graphs = bool(graphs)
         self.budget = int(hbm_budget_bytes)
         self.max_active = int(max_active)
         self.srv = ctypes.c_void_p()
@@ -139,6 +143,8 @@ class EnsembleServer:
                 solver.initialize()
                 if req.prepare is not None:
                     req.prepare(solver)
+                if self.graphs and solver.case.path() == abi.PATH_FUSED:
+                    solver.case.set_option("graph", 1)
                 self.used += req.footprint
                 self.active[req.ticket] = (req, solver)
                 rc = self.lib.lbm_server_add(self.srv, solver.case.h, req.steps, req.ticket)

@@ -286,6 +286,16 @@ int lbm_initial_fields(const lbm_config_t *cfg, const lbm_init_params_t *p, cons
    advanced is number step_index+1 of the run. */
 int lbm_step(lbm_handle_t *h, int32_t nsteps);
 
+/* Several independent cases advanced together (no counterpart in the
+   reference: the ensemble server's merged launch grid).  Handle hs[i]
+   advances nsteps[i] steps, exactly as lbm_step(hs[i], nsteps[i]) would
+   (bit for bit); each step of all handles that still have steps is one
+   launch whose grid holds the tiles of every case.  The handles must be
+   distinct, on the fused path, and share the storage type and the
+   collision model (LBM_E_ARG otherwise).  The work runs on hs[0]'s stream,
+   ordered after every handle's earlier work and before its later work. */
+int lbm_step_batch(lbm_handle_t *const *hs, const int32_t *nsteps, int32_t n);
+
 /* Host <-> device copies of one field: `.from_numpy()` / `.to_numpy()`. */
 int lbm_upload_field(lbm_handle_t *h, int32_t field, const void *src, size_t bytes);
 int lbm_download_field(lbm_handle_t *h, int32_t field, void *dst, size_t bytes);
@@ -361,6 +371,13 @@ typedef struct lbm_server lbm_server_t;
 
 /* Start n_workers (1..64) stepping threads. */
 int lbm_server_create(int32_t chunk_steps, int32_t n_workers, lbm_server_t **out);
+
+/* Batching (before the first lbm_server_add): the fused-path cases go to one
+   more thread that advances every group of them sharing a storage type and
+   collision model with lbm_step_batch, one launch per step for the whole
+   group (a merged launch grid); the other cases stay with the workers.
+   on = 0 after batching started is LBM_E_STATE. */
+int lbm_server_batch(lbm_server_t *s, int32_t on);
 
 /* Hand over an initialised case to advance by nsteps; ticket is the caller's
    identifier, reported back by lbm_server_poll.  LBM_E_STATE if the handle

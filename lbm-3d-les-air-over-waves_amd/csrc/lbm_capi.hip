@@ -54,6 +54,13 @@ struct lbm_handle {
     // arguments), and instantiated again when the launch sequence differs
     int graph = 0;
     hipGraphExec_t gexec = nullptr;
+    // lbm_step_batch with this handle first: the table of every step's
+    // arguments (device copy and pinned staging), and the ordering events
+    void* batch_dev = nullptr;
+    void* batch_host = nullptr;
+    size_t batch_cap = 0;
+    hipEvent_t batch_copied = nullptr;  // the last table copy has read the staging buffer
+    hipEvent_t ev_order = nullptr;      // this handle's stream has reached a batch boundary
     int initialized = 0;
     int geometry_done = 0;
     int timing = 0;
@@ -344,6 +351,15 @@ static void check_wave_flag(lbm_handle* h) {
     }
 }
 
+namespace lbm {
+// cases with the same key can share a batch (fused path, one storage type
+// and collision model); -1 for a case that cannot be batched
+int batch_key(lbm_handle_t* h) {
+    if (!h || h->path != LBM_PATH_FUSED || !h->initialized) return -1;
+    return h->cfg.storage * 4 + wave_case_model(h->cfg);
+}
+}  // namespace lbm
+
 extern "C" {
 
 int lbm_create(const lbm_config_t* cfg, lbm_handle_t** out) {
@@ -393,6 +409,11 @@ void lbm_destroy(lbm_handle_t* h) {
     if (!h) return;
     if (h->stream) (void)hipStreamSynchronize(h->stream);
     if (h->gexec) (void)hipGraphExecDestroy(h->gexec);
+    if (h->batch_copied) (void)hipEventSynchronize(h->batch_copied);
+    if (h->batch_host) (void)hipHostFree(h->batch_host);
+    if (h->batch_dev) (void)hipFree(h->batch_dev);
+    if (h->batch_copied) (void)hipEventDestroy(h->batch_copied);
+    if (h->ev_order) (void)hipEventDestroy(h->ev_order);
     for (hipEvent_t e : h->ev) (void)hipEventDestroy(e);
     free_all(h);
     if (h->stream) (void)hipStreamDestroy(h->stream);
@@ -582,6 +603,134 @@ int lbm_step(lbm_handle_t* h, int32_t nsteps) {
         h->last_ms_kernel = k / nsteps;
     }
     API_END(h)
+}
+
+// several fused-path handles advanced together: one launch per step runs
+// the wavefront step of every handle that still has steps (its tiles in
+// one grid, each case's arguments from a table); the bed levels of each
+// case are launched as in lbm_step.  Runs on the first handle's stream,
+// ordered after every handle's earlier work and before its later work.
+static int step_batch(lbm_handle* const* hs, const int32_t* nsteps, int n) {
+    lbm_handle* h0 = hs[0];
+    const hipStream_t bs = h0->stream;
+    const int storage = hs[0]->cfg.storage;
+    const int cm = wave_case_model(hs[0]->cfg);
+    int32_t smax = 0;
+    for (int i = 0; i < n; i++) {
+        lbm_handle* h = hs[i];
+        if (!h) return fail(h0, LBM_E_ARG, "a handle is NULL");
+        if (!h->initialized) return fail(h0, LBM_E_STATE, "every handle must be initialised");
+        if (h->path != LBM_PATH_FUSED) return fail(h0, LBM_E_ARG, "lbm_step_batch runs fused-path cases only");
+        if (h->cfg.storage != storage || wave_case_model(h->cfg) != cm)
+            return fail(h0, LBM_E_ARG, "the cases of one batch share the storage type and the collision model");
+        if (nsteps[i] < 0) return fail(h0, LBM_E_ARG, "nsteps must be >= 0");
+        for (int k = 0; k < i; k++)
+            if (hs[k] == h) return fail(h0, LBM_E_ARG, "a handle appears twice in the batch");
+        smax = std::max(smax, nsteps[i]);
+    }
+    if (smax == 0) return LBM_OK;
+    const size_t E = wave_case_bytes(storage);
+    size_t entries = 0;
+    for (int i = 0; i < n; i++) entries += (size_t)nsteps[i];
+    const size_t need = entries * E;
+    if (!h0->batch_copied) CHECK(hipEventCreateWithFlags(&h0->batch_copied, hipEventDisableTiming));
+    CHECK(hipEventSynchronize(h0->batch_copied));  // the staging buffer is free again
+    if (need > h0->batch_cap) {
+        if (h0->batch_host) CHECK(hipHostFree(h0->batch_host));
+        if (h0->batch_dev) CHECK(hipFree(h0->batch_dev));
+        h0->batch_host = h0->batch_dev = nullptr;
+        h0->batch_cap = 0;
+        CHECK(hipHostMalloc(&h0->batch_host, need, hipHostMallocDefault));
+        CHECK(hipMalloc(&h0->batch_dev, need));
+        h0->batch_cap = need;
+    }
+    // every handle's earlier work comes first
+    for (int i = 0; i < n; i++) {
+        lbm_handle* h = hs[i];
+        if (!h->ev_order) CHECK(hipEventCreateWithFlags(&h->ev_order, hipEventDisableTiming));
+        if (h != h0) {
+            CHECK(hipEventRecord(h->ev_order, h->stream));
+            CHECK(hipStreamWaitEvent(bs, h->ev_order, 0));
+        }
+        if (h->sched.epoch >= INT32_MAX - 1 - nsteps[i]) {  // the tile flags hold launch numbers: start them over
+            const size_t wq = wave_flag_ints(h->cfg.nx, h->cfg.ny, h->cfg.nz);
+            CHECK(hipMemsetAsync(h->b.waveq + WAVE_FLAG0, 0, (wq - WAVE_FLAG0) * sizeof(int), bs));
+            h->sched.epoch = 0;
+        }
+    }
+    // the table: for step s, the entries of the cases that still step
+    std::vector<size_t> off(smax);
+    std::vector<int> cnt(smax), blocks(smax);
+    {
+        size_t e = 0;
+        for (int32_t s = 0; s < smax; s++) {
+            off[s] = e;
+            int block0 = 0, c = 0;
+            for (int i = 0; i < n; i++) {
+                lbm_handle* h = hs[i];
+                if (s >= nsteps[i]) continue;
+                const int64_t step = h->step + 1 + s;
+                const int cur = (h->cur + s) & 1;
+                const int mode = (s == 0 && h->post_stream) ? 0 : 1;
+                FusedSched fs = h->sched;
+                fs.epoch = h->sched.epoch + 1 + s;
+                block0 += fill_wave_case(h->cfg, h->b, cur, mode, slot_of(h, step - 1), slot_of(h, step), step, s == nsteps[i] - 1,
+                                         h->omega_band_top, h->wall_top, h->block_x, fs, block0, (char*)h0->batch_host + e * E);
+                e++;
+                c++;
+            }
+            cnt[s] = c;
+            blocks[s] = block0;
+        }
+    }
+    CHECK(hipMemcpyAsync(h0->batch_dev, h0->batch_host, need, hipMemcpyHostToDevice, bs));
+    CHECK(hipEventRecord(h0->batch_copied, bs));
+    // the launches: each case's bed levels as lbm_step has them, then the
+    // step of every case
+    std::vector<int64_t> bed_ready(n);
+    for (int i = 0; i < n; i++) bed_ready[i] = hs[i]->step;
+    for (int32_t s = 0; s < smax; s++) {
+        for (int i = 0; i < n; i++) {
+            lbm_handle* h = hs[i];
+            if (s >= nsteps[i]) continue;
+            const lbm_config_t& c = h->cfg;
+            const int64_t step = h->step + 1;
+            const int slot = slot_of(h, step);
+            const bool fixed = c.bed_profile == LBM_BED_FLAT || c.bed_profile == LBM_BED_PIECEWISE;
+            if (h->b.ring == BED_RING) {
+                if (step > bed_ready[i]) {
+                    const int nb = (int)std::min<int64_t>(BED_BATCH, (int64_t)nsteps[i] - s);
+                    launch_bed2d_batch(c, h->b, step, nb, bs);
+                    bed_ready[i] = step + nb - 1;
+                }
+            } else if (!(fixed && h->slot_valid[slot])) {
+                launch_bed2d(c, h->b, (float)((double)(step + 1) * c.dt_phys), slot, bs);  // CODEX:3874
+                h->slot_valid[slot] = 1;
+            }
+        }
+        launch_wave_batch(storage, cm, (const char*)h0->batch_dev + off[s] * E, cnt[s], blocks[s], bs);
+        for (int i = 0; i < n; i++) {
+            lbm_handle* h = hs[i];
+            if (s >= nsteps[i]) continue;
+            h->sched.epoch++;
+            h->cur ^= 1;
+            h->post_stream = 0;
+            h->step++;
+        }
+    }
+    // every handle's later work comes after
+    CHECK(hipEventRecord(h0->ev_order, bs));
+    for (int i = 1; i < n; i++) CHECK(hipStreamWaitEvent(hs[i]->stream, h0->ev_order, 0));
+    return LBM_OK;
+}
+
+int lbm_step_batch(lbm_handle_t* const* hs, const int32_t* nsteps, int32_t n) {
+    if (!hs || !nsteps || n < 1 || !hs[0]) return LBM_E_ARG;
+    lbm_handle* h0 = hs[0];
+    API_BEGIN(h0)
+    int rc = step_batch(hs, nsteps, n);
+    if (rc) return rc;
+    API_END(h0)
 }
 
 int lbm_upload_field(lbm_handle_t* h, int32_t field, const void* src, size_t bytes) {

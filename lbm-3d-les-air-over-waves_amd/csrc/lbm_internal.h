@@ -131,6 +131,18 @@ void launch_staged_step(const lbm_config_t& c, DevBufs& b, int& cur, int slot, i
 // band_top: highest k at which any cell can have a cut or closed link
 void launch_fused_step(const lbm_config_t& c, DevBufs& b, int& cur, int mode, int slot_prev, int slot, int64_t step,
                        int write_macro, int omega_band_top, int band_top, int block_x, const FusedSched& fs, hipStream_t s);
+// the wavefront step of several fused-path cases in one launch: the
+// size of one case's table entry, its collision model (cases of one launch
+// share the storage type and the model), the entry of one case's step
+// (returns the case's block count; the entry is written to `out`, a host
+// buffer of wave_case_bytes), and the launch over a device copy of the table
+size_t wave_case_bytes(int storage);
+// cases with equal keys can be advanced by one lbm_step_batch (-1: none)
+int batch_key(lbm_handle_t* h);
+int wave_case_model(const lbm_config_t& c);
+int fill_wave_case(const lbm_config_t& c, DevBufs& b, int cur, int mode, int slot_prev, int slot, int64_t step, int write_macro,
+                   int omega_band_top, int band_top, int block_x, const FusedSched& fs, int block0, void* out);
+void launch_wave_batch(int storage, int cm, const void* dev_table, int ncases, int nblocks, hipStream_t s);
 // post-collision state -> post-stream populations (arithmetic type) in dst
 void launch_materialize(const lbm_config_t& c, DevBufs& b, int cur, int slot, void* dst, hipStream_t s);
 // convert arithmetic-type populations in src into the storage buffer f[cur]

@@ -17,6 +17,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstring>
 #include <mutex>
 #include <stdexcept>
 #include <string>
@@ -2971,11 +2972,12 @@ __device__ __forceinline__ int wave_dep(const WaveQ& w, int nz, int k, int bxi, 
     return kk * w.bpp + ty * w.gx + tx;
 }
 
+// tile t of a case's wavefront step (the body of k_step_wave and of the
+// batched form)
 template <class S, class T, int CM>
-__global__ __launch_bounds__(FB_THREADS, 6) void k_step_wave(FusedArgs<S, T> a, WaveQ w) {
+__device__ __forceinline__ void wave_tile(const FusedArgs<S, T>& a, const WaveQ& w, int t) {
     const int tid = (int)threadIdx.x;
     const int nz = a.c.nz;
-    const int t = (int)blockIdx.x;
     const int u = t / w.bpp, slot = t - u * w.bpp;
     int pass, k;
     wave_unit(u, w.lag, nz, pass, k);
@@ -3026,6 +3028,39 @@ __global__ __launch_bounds__(FB_THREADS, 6) void k_step_wave(FusedArgs<S, T> a, 
                 collide_cell<S, T, false, CM, true>(a, i, j, k);
         }
     }
+}
+
+template <class S, class T, int CM>
+__global__ __launch_bounds__(FB_THREADS, 6) void k_step_wave(FusedArgs<S, T> a, WaveQ w) {
+    wave_tile<S, T, CM>(a, w, (int)blockIdx.x);
+}
+
+// Several cases advanced by one launch (the ensemble server's batches): the
+// tiles of case c are blocks block0[c] .. block0[c + 1] - 1 of the grid, in
+// the order of its own wavefront step, so every tile a collision tile waits
+// on still has a lower block index.  The table holds each case's arguments
+// of this step.
+template <class S, class T>
+struct WaveCase {
+    FusedArgs<S, T> a;
+    WaveQ w;
+    int block0;  // first block of the case
+    int pad;
+};
+template <class S, class T, int CM>
+__global__ __launch_bounds__(FB_THREADS, 6) void k_step_wave_batch(const WaveCase<S, T>* __restrict__ tab, int ncases) {
+    const int t = (int)blockIdx.x;
+    int lo = 0, hi = ncases - 1;  // the last case whose block0 <= t
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (tab[mid].block0 <= t)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    const FusedArgs<S, T> a = tab[lo].a;
+    const WaveQ w = tab[lo].w;
+    wave_tile<S, T, CM>(a, w, t - tab[lo].block0);
 }
 
 // ===========================================================================
@@ -3511,8 +3546,8 @@ void launch_staged_step(const lbm_config_t& c, DevBufs& b, int& cur, int slot, i
 }
 
 template <class S, class T>
-static void fused_t(const lbm_config_t& cfg, DevBufs& b, int& cur, int mode, int slot_prev, int slot, int64_t step, int write_macro,
-                    int omega_band_top, int band_top, int block_x, const FusedSched& fs, hipStream_t s) {
+static FusedArgs<S, T> make_fused_args(const lbm_config_t& cfg, DevBufs& b, int cur, int mode, int slot_prev, int slot,
+                                       int64_t step, int write_macro, int omega_band_top) {
     FusedArgs<S, T> a;
     a.c = make_consts<T>(cfg, omega_band_top);
     a.s0 = make_slot<T>(b, slot_prev);
@@ -3530,17 +3565,44 @@ static void fused_t(const lbm_config_t& cfg, DevBufs& b, int& cur, int mode, int
     a.mode = mode;
     a.write_macro = write_macro;
     a.step = (uint32_t)step;
+    a.plane0 = 0;
+    a.strips = 0;
+    return a;
+}
+
+// the collision model is fixed at compile time unless the top sponge or
+// backscatter can select another one per cell
+static int collision_model(const lbm_config_t& cfg) {
+    return cfg.collision == LBM_COLL_REGULARIZED ? CM_REG : (!cfg.enable_top_sponge && !(cfg.c_backscatter > 0.0f)) ? CM_KBC : CM_ANY;
+}
+
+// the tile flags of a case's wavefront step (gx x gy tiles per plane of
+// bx x 256/bx cells)
+static WaveQ make_waveq(const lbm_config_t& cfg, DevBufs& b, const FusedSched& fs, int bx, int gx, int gy) {
+    WaveQ w;
+    w.flag = b.waveq + WAVE_FLAG0;
+    w.err = b.waveq;
+    w.epoch = fs.epoch;
+    w.bpp = gx * gy;
+    const int lag = fs.lag > 0 ? fs.lag : (6 * fs.cus + w.bpp - 1) / w.bpp;
+    w.lag = std::max(1, std::min(lag, cfg.nz));
+    w.bx = bx;
+    w.gx = gx;
+    w.gy = gy;
+    return w;
+}
+
+template <class S, class T>
+static void fused_t(const lbm_config_t& cfg, DevBufs& b, int& cur, int mode, int slot_prev, int slot, int64_t step, int write_macro,
+                    int omega_band_top, int band_top, int block_x, const FusedSched& fs, hipStream_t s) {
+    FusedArgs<S, T> a = make_fused_args<S, T>(cfg, b, cur, mode, slot_prev, slot, step, write_macro, omega_band_top);
     // planes 0 .. band_top can meet a cut or closed link (general kernels);
     // the planes above cannot (lean kernels)
     const int nb = band_top < 0 ? 0 : std::min(cfg.nz, band_top + 1);
     const int bxd = block_x > 0 ? block_x : 64;
     const dim3 blk(bxd, FB_THREADS / bxd);
     const unsigned gx = (cfg.nx + blk.x - 1) / blk.x, gy = (cfg.ny + blk.y - 1) / blk.y;
-    // the collision model is fixed at compile time unless the top sponge or
-    // backscatter can select another one per cell
-    const int cm = cfg.collision == LBM_COLL_REGULARIZED                         ? CM_REG
-                   : (!cfg.enable_top_sponge && !(cfg.c_backscatter > 0.0f)) ? CM_KBC
-                                                                               : CM_ANY;
+    const int cm = collision_model(cfg);
     a.band_planes = nb;
     // the XCD strips pay on the wavefront schedule (FP32: 6 % at 512x256x192,
     // its traffic 20 % lower); the VALU-bound 16-bit passes ran 1.5 % slower
@@ -3548,16 +3610,7 @@ static void fused_t(const lbm_config_t& cfg, DevBufs& b, int& cur, int mode, int
     // FP32 storage only: with FP64 the merged kernel spills to scratch, and
     // the VALU-bound 16-bit forms ran 20-30 % slower merged (SGPR spills)
     if (fs.wavefront && std::is_same<S, float>::value) {
-        WaveQ w;
-        w.flag = b.waveq + WAVE_FLAG0;
-        w.err = b.waveq;
-        w.epoch = fs.epoch;
-        w.bpp = (int)(gx * gy);
-        const int lag = fs.lag > 0 ? fs.lag : (6 * fs.cus + w.bpp - 1) / w.bpp;
-        w.lag = std::max(1, std::min(lag, cfg.nz));
-        w.bx = bxd;
-        w.gx = (int)gx;
-        w.gy = (int)gy;
+        const WaveQ w = make_waveq(cfg, b, fs, bxd, (int)gx, (int)gy);
         auto kern = cm == CM_REG ? k_step_wave<S, T, CM_REG> : (cm == CM_KBC ? k_step_wave<S, T, CM_KBC> : k_step_wave<S, T, CM_ANY>);
         a.plane0 = 0;
         kern<<<2 * cfg.nz * w.bpp, FB_THREADS, 0, s>>>(a, w);
@@ -3611,6 +3664,54 @@ static void fused_t(const lbm_config_t& cfg, DevBufs& b, int& cur, int mode, int
     HIP_OK(hipGetLastError());
     cur ^= 1;
 }
+template <class S, class T>
+static int fill_wave_case_t(const lbm_config_t& cfg, DevBufs& b, int cur, int mode, int slot_prev, int slot, int64_t step,
+                            int write_macro, int omega_band_top, int band_top, int block_x, const FusedSched& fs, int block0, void* out) {
+    WaveCase<S, T> e;
+    memset(&e, 0, sizeof e);
+    e.a = make_fused_args<S, T>(cfg, b, cur, mode, slot_prev, slot, step, write_macro, omega_band_top);
+    e.a.band_planes = band_top < 0 ? 0 : std::min(cfg.nz, band_top + 1);
+    const int bxd = block_x > 0 ? block_x : 64;
+    const int gx = (cfg.nx + bxd - 1) / bxd, gy = (cfg.ny + FB_THREADS / bxd - 1) / (FB_THREADS / bxd);
+    e.w = make_waveq(cfg, b, fs, bxd, gx, gy);
+    e.block0 = block0;
+    memcpy(out, &e, sizeof e);
+    return 2 * cfg.nz * e.w.bpp;
+}
+size_t wave_case_bytes(int storage) {
+    switch (storage) {
+        case LBM_STORE_FP64: return sizeof(WaveCase<double, double>);
+        case LBM_STORE_FP16: return sizeof(WaveCase<__half, float>);
+        case LBM_STORE_BF16: return sizeof(WaveCase<bf16_t, float>);
+        default: return sizeof(WaveCase<float, float>);
+    }
+}
+int wave_case_model(const lbm_config_t& c) { return collision_model(c); }
+int fill_wave_case(const lbm_config_t& c, DevBufs& b, int cur, int mode, int slot_prev, int slot, int64_t step, int write_macro,
+                   int omega_band_top, int band_top, int block_x, const FusedSched& fs, int block0, void* out) {
+    if (!fused_supported(c)) throw std::runtime_error("fused path: needs air mode without a free surface");
+    switch (b.storage) {
+        case LBM_STORE_FP64: return fill_wave_case_t<double, double>(c, b, cur, mode, slot_prev, slot, step, write_macro, omega_band_top, band_top, block_x, fs, block0, out);
+        case LBM_STORE_FP16: return fill_wave_case_t<__half, float>(c, b, cur, mode, slot_prev, slot, step, write_macro, omega_band_top, band_top, block_x, fs, block0, out);
+        case LBM_STORE_BF16: return fill_wave_case_t<bf16_t, float>(c, b, cur, mode, slot_prev, slot, step, write_macro, omega_band_top, band_top, block_x, fs, block0, out);
+        default: return fill_wave_case_t<float, float>(c, b, cur, mode, slot_prev, slot, step, write_macro, omega_band_top, band_top, block_x, fs, block0, out);
+    }
+}
+template <class S, class T>
+static void wave_batch_t(int cm, const void* tab, int ncases, int nblocks, hipStream_t s) {
+    auto kern = cm == CM_REG ? k_step_wave_batch<S, T, CM_REG> : (cm == CM_KBC ? k_step_wave_batch<S, T, CM_KBC> : k_step_wave_batch<S, T, CM_ANY>);
+    kern<<<nblocks, FB_THREADS, 0, s>>>((const WaveCase<S, T>*)tab, ncases);
+    HIP_OK(hipGetLastError());
+}
+void launch_wave_batch(int storage, int cm, const void* dev_table, int ncases, int nblocks, hipStream_t s) {
+    switch (storage) {
+        case LBM_STORE_FP64: wave_batch_t<double, double>(cm, dev_table, ncases, nblocks, s); break;
+        case LBM_STORE_FP16: wave_batch_t<__half, float>(cm, dev_table, ncases, nblocks, s); break;
+        case LBM_STORE_BF16: wave_batch_t<bf16_t, float>(cm, dev_table, ncases, nblocks, s); break;
+        default: wave_batch_t<float, float>(cm, dev_table, ncases, nblocks, s); break;
+    }
+}
+
 void launch_fused_step(const lbm_config_t& c, DevBufs& b, int& cur, int mode, int slot_prev, int slot, int64_t step,
                        int write_macro, int omega_band_top, int band_top, int block_x, const FusedSched& fs, hipStream_t s) {
     if (!fused_supported(c)) throw std::runtime_error("fused path: needs air mode without a free surface");

@@ -11,6 +11,13 @@
 // (fewest active first), so host launch work is spread over several threads.
 // The server never allocates device memory; admission by footprint,
 // initialisation and downloads are the caller's.
+//
+// With batching on (lbm_server_batch), the fused-path cases go to one batch
+// thread instead: every chunk it advances all of its cases that share a
+// storage type and collision model with one lbm_step_batch, so each step of
+// all of them is a single launch whose grid holds every case's tiles (a
+// merged launch grid); a case leaves the batch when its steps are done and
+// is reported once its stream has drained.
 #include <algorithm>
 #include <atomic>
 #include <chrono>
@@ -23,6 +30,7 @@
 #include <vector>
 
 #include "../../include/lbm.h"
+#include "lbm_internal.h"
 
 namespace {
 
@@ -47,6 +55,7 @@ struct Worker {
 struct lbm_server {
     int32_t chunk = 16;
     std::vector<Worker*> workers;
+    Worker* batcher = nullptr;  // the batch thread (batching on)
     std::atomic<bool> stop{false};
     std::mutex fin_m;
     std::deque<std::pair<int64_t, int32_t>> finished;
@@ -56,6 +65,91 @@ struct lbm_server {
         std::lock_guard<std::mutex> g(fin_m);
         inflight.erase(h);
         finished.emplace_back(ticket, status);
+    }
+
+    // the batch thread: one lbm_step_batch per group of cases that can share
+    // a launch, every round; cases whose steps are all enqueued wait on
+    // their streams before they are reported
+    void run_batch(Worker* w) {
+        std::vector<Job> draining;
+        while (true) {
+            {
+                std::unique_lock<std::mutex> lk(w->m);
+                if (w->active.empty() && w->incoming.empty() && draining.empty()) {
+                    if (stop.load()) return;
+                    w->cv.wait_for(lk, std::chrono::milliseconds(2));
+                    continue;
+                }
+                for (auto& j : w->incoming) w->active.push_back(j);
+                w->incoming.clear();
+            }
+            bool progressed = false;
+            if (!stop.load() && !w->active.empty()) {
+                // groups by batch key, in order of arrival
+                std::vector<int> keys;
+                for (auto& j : w->active) {
+                    const int k = lbm::batch_key(j.h);
+                    if (std::find(keys.begin(), keys.end(), k) == keys.end()) keys.push_back(k);
+                }
+                for (int key : keys) {
+                    std::vector<lbm_handle_t*> hs;
+                    std::vector<int32_t> ns;
+                    std::vector<size_t> idx;
+                    for (size_t a = 0; a < w->active.size(); a++) {
+                        Job& j = w->active[a];
+                        if (lbm::batch_key(j.h) != key) continue;
+                        hs.push_back(j.h);
+                        ns.push_back(std::min(chunk, j.nsteps - j.done));
+                        idx.push_back(a);
+                    }
+                    const int rc = key < 0 ? LBM_E_ARG : lbm_step_batch(hs.data(), ns.data(), (int32_t)hs.size());
+                    for (size_t m = 0; m < idx.size(); m++) {
+                        Job& j = w->active[idx[m]];
+                        if (rc == LBM_OK) j.done += ns[m];
+                        else j.done = -1 - rc;  // failed: reported with the error code below
+                    }
+                    progressed = true;
+                }
+                // cases with every step enqueued (or failed) leave the batch
+                for (size_t a = 0; a < w->active.size();) {
+                    Job& j = w->active[a];
+                    if (j.done < 0 || j.done >= j.nsteps) {
+                        draining.push_back(j);
+                        w->active.erase(w->active.begin() + (long)a);
+                    } else {
+                        a++;
+                    }
+                }
+            }
+            for (size_t d = 0; d < draining.size();) {
+                Job& j = draining[d];
+                int32_t idle = 0;
+                // the device must be done with the case before it is
+                // reported, failed or not (its stream may hold its chunks)
+                int rc = lbm_query(j.h, &idle);
+                if (rc == LBM_OK && !idle) {
+                    d++;
+                    continue;
+                }
+                if (rc == LBM_OK && j.done < 0) rc = -(j.done + 1);
+                finish(j.h, j.ticket, rc);
+                draining.erase(draining.begin() + (long)d);
+                w->n_active.fetch_sub(1);
+                progressed = true;
+            }
+            if (stop.load() && w->active.empty() && draining.empty()) {
+                std::lock_guard<std::mutex> g(w->m);
+                if (w->incoming.empty()) return;
+            }
+            if (stop.load() && !w->active.empty()) {  // stopping: abandon the cases not yet done
+                for (auto& j : w->active) {
+                    j.done = -1 - LBM_E_STATE;
+                    draining.push_back(j);
+                }
+                w->active.clear();
+            }
+            if (!progressed) std::this_thread::sleep_for(std::chrono::microseconds(20));
+        }
     }
 
     void run(Worker* w) {
@@ -151,6 +245,7 @@ int lbm_server_add(lbm_server_t* s, lbm_handle_t* h, int32_t nsteps, int64_t tic
     Worker* best = s->workers[0];
     for (Worker* w : s->workers)
         if (w->n_active.load() < best->n_active.load()) best = w;
+    if (s->batcher && lbm::batch_key(h) >= 0) best = s->batcher;
     {
         std::lock_guard<std::mutex> g(best->m);
         best->incoming.push_back(Job{h, nsteps, 0, ticket});
@@ -178,7 +273,26 @@ int lbm_server_active(lbm_server_t* s, int32_t* n_active) {
     if (!s || !n_active) return LBM_E_ARG;
     int32_t n = 0;
     for (Worker* w : s->workers) n += w->n_active.load();
+    if (s->batcher) n += s->batcher->n_active.load();
     *n_active = n;
+    return LBM_OK;
+}
+
+int lbm_server_batch(lbm_server_t* s, int32_t on) {
+    if (!s) return LBM_E_ARG;
+    if (!on) return s->batcher ? LBM_E_STATE : LBM_OK;  // a running batch thread stays
+    if (s->batcher) return LBM_OK;
+    {
+        std::lock_guard<std::mutex> g(s->fin_m);
+        if (!s->inflight.empty()) return LBM_E_STATE;  // before the first case only
+    }
+    try {
+        Worker* w = new Worker();
+        w->thread = std::thread([s, w]() { s->run_batch(w); });
+        s->batcher = w;
+    } catch (const std::exception&) {
+        return LBM_E_NOMEM;
+    }
     return LBM_OK;
 }
 
@@ -189,6 +303,11 @@ int lbm_server_destroy(lbm_server_t* s) {
         w->cv.notify_all();
         if (w->thread.joinable()) w->thread.join();
         delete w;
+    }
+    if (s->batcher) {
+        s->batcher->cv.notify_all();
+        if (s->batcher->thread.joinable()) s->batcher->thread.join();
+        delete s->batcher;
     }
     delete s;
     return LBM_OK;

@@ -27,6 +27,7 @@ EXPORTS = [
     "lbm_initialize",
     "lbm_initial_fields",
     "lbm_step",
+    "lbm_step_batch",
     "lbm_upload_field",
     "lbm_download_field",
     "lbm_get_counters",
@@ -41,6 +42,7 @@ EXPORTS = [
     "lbm_get_path",
     "lbm_query",
     "lbm_server_create",
+    "lbm_server_batch",
     "lbm_server_add",
     "lbm_server_poll",
     "lbm_server_active",
@@ -69,6 +71,7 @@ def load_library(path: Optional[os.PathLike] = None) -> ctypes.CDLL:
     lib.lbm_initial_fields.argtypes = [ctypes.POINTER(abi.LbmConfig), ctypes.POINTER(abi.LbmInitParams), P, P, P, P, P, P,
                                        P, P, P, P, P]
     lib.lbm_step.argtypes = [H, ctypes.c_int32]
+    lib.lbm_step_batch.argtypes = [ctypes.POINTER(H), ctypes.POINTER(ctypes.c_int32), ctypes.c_int32]
     lib.lbm_upload_field.argtypes = [H, ctypes.c_int32, ctypes.c_void_p, ctypes.c_size_t]
     lib.lbm_download_field.argtypes = [H, ctypes.c_int32, ctypes.c_void_p, ctypes.c_size_t]
     lib.lbm_get_counters.argtypes = [H, ctypes.POINTER(abi.LbmCounters)]
@@ -85,6 +88,7 @@ def load_library(path: Optional[os.PathLike] = None) -> ctypes.CDLL:
     lib.lbm_query.argtypes = [H, ctypes.POINTER(ctypes.c_int32)]
     S = ctypes.c_void_p
     lib.lbm_server_create.argtypes = [ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(S)]
+    lib.lbm_server_batch.argtypes = [S, ctypes.c_int32]
     lib.lbm_server_add.argtypes = [S, H, ctypes.c_int32, ctypes.c_int64]
     lib.lbm_server_poll.argtypes = [S, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int32), ctypes.c_int32,
                                     ctypes.POINTER(ctypes.c_int32)]
@@ -100,6 +104,18 @@ def load_library(path: Optional[os.PathLike] = None) -> ctypes.CDLL:
 
 class LbmError(RuntimeError):
     pass
+
+
+def step_batch(cases, nsteps):
+    """lbm_step_batch: advance case i by nsteps[i] steps, every step of all
+    of them one launch (fused path; one storage type and collision model)."""
+    lib = load_library()
+    n = len(cases)
+    hs = (ctypes.c_void_p * n)(*[c.h for c in cases])
+    ns = (ctypes.c_int32 * n)(*[int(x) for x in nsteps])
+    rc = lib.lbm_step_batch(hs, ns, n)
+    if rc != 0:
+        raise LbmError(f"lbm_step_batch failed ({rc}): {lib.lbm_last_error(cases[0].h).decode()}")
 
 
 def device_bytes(cfg: abi.LbmConfig) -> int:

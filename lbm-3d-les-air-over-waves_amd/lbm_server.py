@@ -73,9 +73,12 @@ class EnsembleServer:
     """Continuous-batching scheduler over the native stepping runtime."""
 
     def __init__(self, hbm_budget_bytes: int, max_active: int = 64, chunk_steps: int = 16, n_workers: int = 4,
-                 graphs: bool = False):
+                 graphs: bool = False, batch: bool = False):
         """graphs: the chunks of the cases on the fused path are captured
-        into HIP graphs and launched as one (option "graph" of the C ABI)."""
+        into HIP graphs and launched as one (option "graph" of the C ABI).
+        batch: the fused-path cases that share a storage type and collision
+        model advance together, one launch per step for all of them
+        (lbm_server_batch); the others keep their own streams."""
         self.lib = load_library()
         self.graphs = bool(graphs)
         self.budget = int(hbm_budget_bytes)
@@ -84,6 +87,11 @@ class EnsembleServer:
         rc = self.lib.lbm_server_create(int(chunk_steps), int(n_workers), ctypes.byref(self.srv))
         if rc != 0:
             raise LbmError(f"lbm_server_create failed ({rc})")
+        if batch:
+            rc = self.lib.lbm_server_batch(self.srv, 1)
+            if rc != 0:
+                self.lib.lbm_server_destroy(self.srv)
+                raise LbmError(f"lbm_server_batch failed ({rc})")
         self.queue: deque = deque()
         self.active: Dict[int, tuple] = {}
         self.finished: List[Request] = []

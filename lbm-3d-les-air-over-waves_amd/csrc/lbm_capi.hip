@@ -55,12 +55,9 @@ struct lbm_handle {
     int graph = 0;
     hipGraphExec_t gexec = nullptr;
     // lbm_step_batch with this handle first: the table of every step's
-    // arguments (device copy and pinned staging), and the ordering events
-    void* batch_dev = nullptr;
-    void* batch_host = nullptr;
-    size_t batch_cap = 0;
-    hipEvent_t batch_copied = nullptr;  // the last table copy has read the staging buffer
-    hipEvent_t ev_order = nullptr;      // this handle's stream has reached a batch boundary
+    // arguments, and the ordering event
+    BatchBuf batch;
+    hipEvent_t ev_order = nullptr;  // this handle's stream has reached a batch boundary
     int initialized = 0;
     int geometry_done = 0;
     int timing = 0;
@@ -409,10 +406,7 @@ void lbm_destroy(lbm_handle_t* h) {
     if (!h) return;
     if (h->stream) (void)hipStreamSynchronize(h->stream);
     if (h->gexec) (void)hipGraphExecDestroy(h->gexec);
-    if (h->batch_copied) (void)hipEventSynchronize(h->batch_copied);
-    if (h->batch_host) (void)hipHostFree(h->batch_host);
-    if (h->batch_dev) (void)hipFree(h->batch_dev);
-    if (h->batch_copied) (void)hipEventDestroy(h->batch_copied);
+    batch_buf_free(h->batch);
     if (h->ev_order) (void)hipEventDestroy(h->ev_order);
     for (hipEvent_t e : h->ev) (void)hipEventDestroy(e);
     free_all(h);
@@ -605,14 +599,18 @@ int lbm_step(lbm_handle_t* h, int32_t nsteps) {
     API_END(h)
 }
 
-// several fused-path handles advanced together: one launch per step runs
-// the wavefront step of every handle that still has steps (its tiles in
-// one grid, each case's arguments from a table); the bed levels of each
-// case are launched as in lbm_step.  Runs on the first handle's stream,
-// ordered after every handle's earlier work and before its later work.
-static int step_batch(lbm_handle* const* hs, const int32_t* nsteps, int n) {
+}  // extern "C"
+
+namespace lbm {
+// several fused-path handles advanced together on stream bs: one launch
+// per step runs the wavefront step of every handle that still has steps
+// (its tiles in one grid, each case's arguments from a table in buf); the
+// bed levels of each case are launched as in lbm_step.  order: wait for
+// every handle's earlier work on its own stream first, and make its later
+// work wait for the batch (otherwise the caller keeps the handles' streams
+// free of work while they are batched).  Errors are reported on hs[0].
+int step_batch(lbm_handle* const* hs, const int32_t* nsteps, int n, hipStream_t bs, BatchBuf& buf, bool order) {
     lbm_handle* h0 = hs[0];
-    const hipStream_t bs = h0->stream;
     const int storage = hs[0]->cfg.storage;
     const int cm = wave_case_model(hs[0]->cfg);
     int32_t smax = 0;
@@ -629,26 +627,31 @@ static int step_batch(lbm_handle* const* hs, const int32_t* nsteps, int n) {
         smax = std::max(smax, nsteps[i]);
     }
     if (smax == 0) return LBM_OK;
-    const size_t E = wave_case_bytes(storage);
-    size_t entries = 0;
-    for (int i = 0; i < n; i++) entries += (size_t)nsteps[i];
-    const size_t need = entries * E;
-    if (!h0->batch_copied) CHECK(hipEventCreateWithFlags(&h0->batch_copied, hipEventDisableTiming));
-    CHECK(hipEventSynchronize(h0->batch_copied));  // the staging buffer is free again
-    if (need > h0->batch_cap) {
-        if (h0->batch_host) CHECK(hipHostFree(h0->batch_host));
-        if (h0->batch_dev) CHECK(hipFree(h0->batch_dev));
-        h0->batch_host = h0->batch_dev = nullptr;
-        h0->batch_cap = 0;
-        CHECK(hipHostMalloc(&h0->batch_host, need, hipHostMallocDefault));
-        CHECK(hipMalloc(&h0->batch_dev, need));
-        h0->batch_cap = need;
+    const size_t E = wave_case_bytes(storage), EB = bed_case_bytes(storage);
+    size_t entries = 0, bed_entries = 0;
+    for (int i = 0; i < n; i++) {
+        entries += (size_t)nsteps[i];
+        if (hs[i]->b.ring == BED_RING) bed_entries += ((size_t)nsteps[i] + BED_BATCH - 1) / BED_BATCH;
+    }
+    const size_t bed_base = (entries * E + 255) / 256 * 256;  // the bed tables after the step tables
+    const size_t need = bed_base + bed_entries * EB;
+    if (!buf.copied) CHECK(hipEventCreateWithFlags(&buf.copied, hipEventDisableTiming));
+    CHECK(hipEventSynchronize(buf.copied));  // the staging buffer is free again
+    if (need > buf.cap) {
+        CHECK(hipStreamSynchronize(bs));  // the old table may still be read
+        if (buf.host) CHECK(hipHostFree(buf.host));
+        if (buf.dev) CHECK(hipFree(buf.dev));
+        buf.host = buf.dev = nullptr;
+        buf.cap = 0;
+        CHECK(hipHostMalloc(&buf.host, need, hipHostMallocDefault));
+        CHECK(hipMalloc(&buf.dev, need));
+        buf.cap = need;
     }
     // every handle's earlier work comes first
     for (int i = 0; i < n; i++) {
         lbm_handle* h = hs[i];
-        if (!h->ev_order) CHECK(hipEventCreateWithFlags(&h->ev_order, hipEventDisableTiming));
-        if (h != h0) {
+        if (order && h->stream != bs) {
+            if (!h->ev_order) CHECK(hipEventCreateWithFlags(&h->ev_order, hipEventDisableTiming));
             CHECK(hipEventRecord(h->ev_order, h->stream));
             CHECK(hipStreamWaitEvent(bs, h->ev_order, 0));
         }
@@ -658,10 +661,45 @@ static int step_batch(lbm_handle* const* hs, const int32_t* nsteps, int n) {
             h->sched.epoch = 0;
         }
     }
-    // the table: for step s, the entries of the cases that still step
-    std::vector<size_t> off(smax);
-    std::vector<int> cnt(smax), blocks(smax);
+    // the table: for step s, the entries of the cases that still step; each
+    // case's entry is built once (its constants) and moved to every step
+    std::vector<char> tmpl((size_t)n * E);
+    std::vector<int> half_of(n);
+    for (int i = 0; i < n; i++) {
+        lbm_handle* h = hs[i];
+        half_of[i] = fill_wave_case(h->cfg, h->b, h->cur, 1, slot_of(h, h->step), slot_of(h, h->step + 1), h->step + 1, 0,
+                                    h->omega_band_top, h->wall_top, h->block_x, h->sched, 0, tmpl.data() + (size_t)i * E);
+    }
+    std::vector<size_t> off(smax), bed_off(smax);
+    std::vector<int> cnt(smax), blocks(smax), bed_cnt(smax), bed_na(smax), bed_nb(smax);
     {
+        // moving beds: the levels of BED_BATCH steps at a time, as lbm_step
+        // computes them, for every case that needs them at step s in one
+        // pair of launches before the step
+        std::vector<int64_t> ready(n);
+        for (int i = 0; i < n; i++) ready[i] = hs[i]->step;
+        size_t eb = 0;
+        for (int32_t s = 0; s < smax; s++) {
+            bed_off[s] = eb;
+            int na = 0, nbb = 0, c = 0;
+            for (int i = 0; i < n; i++) {
+                lbm_handle* h = hs[i];
+                if (s >= nsteps[i] || h->b.ring != BED_RING) continue;
+                const int64_t step = h->step + 1 + s;
+                if (step <= ready[i]) continue;
+                const int nb = (int)std::min<int64_t>(BED_BATCH, (int64_t)nsteps[i] - s);
+                int ca = 0, cb = 0;
+                fill_bed_case(h->cfg, h->b, step, nb, na, nbb, (char*)buf.host + bed_base + eb * EB, &ca, &cb);
+                na += ca;
+                nbb += cb;
+                ready[i] = step + nb - 1;
+                eb++;
+                c++;
+            }
+            bed_cnt[s] = c;
+            bed_na[s] = na;
+            bed_nb[s] = nbb;
+        }
         size_t e = 0;
         for (int32_t s = 0; s < smax; s++) {
             off[s] = e;
@@ -672,10 +710,11 @@ static int step_batch(lbm_handle* const* hs, const int32_t* nsteps, int n) {
                 const int64_t step = h->step + 1 + s;
                 const int cur = (h->cur + s) & 1;
                 const int mode = (s == 0 && h->post_stream) ? 0 : 1;
-                FusedSched fs = h->sched;
-                fs.epoch = h->sched.epoch + 1 + s;
-                block0 += fill_wave_case(h->cfg, h->b, cur, mode, slot_of(h, step - 1), slot_of(h, step), step, s == nsteps[i] - 1,
-                                         h->omega_band_top, h->wall_top, h->block_x, fs, block0, (char*)h0->batch_host + e * E);
+                char* ent = (char*)buf.host + e * E;
+                memcpy(ent, tmpl.data() + (size_t)i * E, E);
+                patch_wave_case(storage, ent, h->b, cur, mode, slot_of(h, step - 1), slot_of(h, step), step, s == nsteps[i] - 1,
+                                h->sched.epoch + 1 + s, block0);
+                block0 += half_of[i];
                 e++;
                 c++;
             }
@@ -683,32 +722,26 @@ static int step_batch(lbm_handle* const* hs, const int32_t* nsteps, int n) {
             blocks[s] = block0;
         }
     }
-    CHECK(hipMemcpyAsync(h0->batch_dev, h0->batch_host, need, hipMemcpyHostToDevice, bs));
-    CHECK(hipEventRecord(h0->batch_copied, bs));
-    // the launches: each case's bed levels as lbm_step has them, then the
-    // step of every case
-    std::vector<int64_t> bed_ready(n);
-    for (int i = 0; i < n; i++) bed_ready[i] = hs[i]->step;
+    CHECK(hipMemcpyAsync(buf.dev, buf.host, need, hipMemcpyHostToDevice, bs));
+    CHECK(hipEventRecord(buf.copied, bs));
+    // the launches: the bed levels (moving beds: the batches of this step
+    // in one pair of launches; static beds: once per slot, as lbm_step), then
+    // the step of every case
     for (int32_t s = 0; s < smax; s++) {
+        if (bed_cnt[s]) launch_bed_multi(storage, (const char*)buf.dev + bed_base + bed_off[s] * EB, bed_cnt[s], bed_na[s], bed_nb[s], bs);
         for (int i = 0; i < n; i++) {
             lbm_handle* h = hs[i];
-            if (s >= nsteps[i]) continue;
+            if (s >= nsteps[i] || h->b.ring == BED_RING) continue;
             const lbm_config_t& c = h->cfg;
             const int64_t step = h->step + 1;
             const int slot = slot_of(h, step);
             const bool fixed = c.bed_profile == LBM_BED_FLAT || c.bed_profile == LBM_BED_PIECEWISE;
-            if (h->b.ring == BED_RING) {
-                if (step > bed_ready[i]) {
-                    const int nb = (int)std::min<int64_t>(BED_BATCH, (int64_t)nsteps[i] - s);
-                    launch_bed2d_batch(c, h->b, step, nb, bs);
-                    bed_ready[i] = step + nb - 1;
-                }
-            } else if (!(fixed && h->slot_valid[slot])) {
+            if (!(fixed && h->slot_valid[slot])) {
                 launch_bed2d(c, h->b, (float)((double)(step + 1) * c.dt_phys), slot, bs);  // CODEX:3874
                 h->slot_valid[slot] = 1;
             }
         }
-        launch_wave_batch(storage, cm, (const char*)h0->batch_dev + off[s] * E, cnt[s], blocks[s], bs);
+        launch_wave_batch(storage, cm, (const char*)buf.dev + off[s] * E, cnt[s], blocks[s], bs);
         for (int i = 0; i < n; i++) {
             lbm_handle* h = hs[i];
             if (s >= nsteps[i]) continue;
@@ -719,16 +752,24 @@ static int step_batch(lbm_handle* const* hs, const int32_t* nsteps, int n) {
         }
     }
     // every handle's later work comes after
-    CHECK(hipEventRecord(h0->ev_order, bs));
-    for (int i = 1; i < n; i++) CHECK(hipStreamWaitEvent(hs[i]->stream, h0->ev_order, 0));
+    if (order) {
+        if (!h0->ev_order) CHECK(hipEventCreateWithFlags(&h0->ev_order, hipEventDisableTiming));
+        CHECK(hipEventRecord(h0->ev_order, bs));
+        for (int i = 0; i < n; i++)
+            if (hs[i]->stream != bs) CHECK(hipStreamWaitEvent(hs[i]->stream, h0->ev_order, 0));
+    }
     return LBM_OK;
 }
+
+}  // namespace lbm
+
+extern "C" {
 
 int lbm_step_batch(lbm_handle_t* const* hs, const int32_t* nsteps, int32_t n) {
     if (!hs || !nsteps || n < 1 || !hs[0]) return LBM_E_ARG;
     lbm_handle* h0 = hs[0];
     API_BEGIN(h0)
-    int rc = step_batch(hs, nsteps, n);
+    int rc = step_batch(hs, nsteps, n, h0->stream, h0->batch, true);
     if (rc) return rc;
     API_END(h0)
 }

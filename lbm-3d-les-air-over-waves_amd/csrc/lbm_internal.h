@@ -134,15 +134,46 @@ void launch_fused_step(const lbm_config_t& c, DevBufs& b, int& cur, int mode, in
 // the wavefront step of several fused-path cases in one launch: the
 // size of one case's table entry, its collision model (cases of one launch
 // share the storage type and the model), the entry of one case's step
-// (returns the case's block count; the entry is written to `out`, a host
-// buffer of wave_case_bytes), and the launch over a device copy of the table
+// (returns the case's blocks in each half of the grid, the moments half and
+// the collision half; the entry is written to `out`, a host buffer of
+// wave_case_bytes), and the launch over a device copy of the table (half:
+// the blocks of one half)
 size_t wave_case_bytes(int storage);
+// the bed batches (launch_bed2d_batch) of several cases in one pair of
+// launches: entry size, the entry of one case (its block counts in the two
+// launches to na and nbb) and the launch over a device copy of the table
+size_t bed_case_bytes(int storage);
+void fill_bed_case(const lbm_config_t& c, DevBufs& b, int64_t step0, int nb, int a0, int b0, void* out, int* na, int* nbb);
+void launch_bed_multi(int storage, const void* dev_table, int ncases, int na, int nbb, hipStream_t s);
 // cases with equal keys can be advanced by one lbm_step_batch (-1: none)
 int batch_key(lbm_handle_t* h);
+// the argument table of a batch (device copy and pinned staging) and the
+// event after its last copy (the staging buffer is free again)
+struct BatchBuf {
+    void* dev = nullptr;
+    void* host = nullptr;
+    size_t cap = 0;
+    hipEvent_t copied = nullptr;
+};
+inline void batch_buf_free(BatchBuf& b) {
+    if (b.copied) (void)hipEventSynchronize(b.copied);
+    if (b.host) (void)hipHostFree(b.host);
+    if (b.dev) (void)hipFree(b.dev);
+    if (b.copied) (void)hipEventDestroy(b.copied);
+    b = BatchBuf{};
+}
+// lbm_step_batch on stream bs with the table in buf; order = false: the
+// caller keeps the handles' own streams free of work meanwhile
+int step_batch(lbm_handle_t* const* hs, const int32_t* nsteps, int n, hipStream_t bs, BatchBuf& buf, bool order);
 int wave_case_model(const lbm_config_t& c);
 int fill_wave_case(const lbm_config_t& c, DevBufs& b, int cur, int mode, int slot_prev, int slot, int64_t step, int write_macro,
                    int omega_band_top, int band_top, int block_x, const FusedSched& fs, int block0, void* out);
-void launch_wave_batch(int storage, int cm, const void* dev_table, int ncases, int nblocks, hipStream_t s);
+void launch_wave_batch(int storage, int cm, const void* dev_table, int ncases, int half, hipStream_t s);
+// an entry of fill_wave_case moved to another step: the fields that change
+// from step to step (time levels, population buffers, mode, step, epoch,
+// first block)
+void patch_wave_case(int storage, void* entry, DevBufs& b, int cur, int mode, int slot_prev, int slot, int64_t step, int write_macro,
+                     int epoch, int block0);
 // post-collision state -> post-stream populations (arithmetic type) in dst
 void launch_materialize(const lbm_config_t& c, DevBufs& b, int cur, int slot, void* dst, hipStream_t s);
 // convert arithmetic-type populations in src into the storage buffer f[cur]

@@ -447,6 +447,57 @@ __global__ __launch_bounds__(9 * BED_COLS) void k_bed2d_b_batch(Consts<T> c, Bed
                   (const T*)a.ww[b], a.uw9[b], a.ksolid[b], a.fsol[b], a.ibb[b], blockIdx.x);
 }
 
+// the bed batches of several cases in one launch each (the ensemble
+// server's batches): case k's blocks start at a0 (k_bed2d_a_multi) and b0
+// (k_bed2d_b_multi); within a case, block x of level l is block
+// l * (blocks per level) + x, as blockIdx.(x, y) of the one-case launch
+template <class T>
+struct BedCase {
+    Consts<T> c;
+    BedBatch<T> a;
+    const T* piecewise;
+    int nb;      // levels
+    int a0, b0;  // first block of the case in each launch
+    int pad;
+};
+template <class T>
+__device__ __forceinline__ int bed_case_of(const BedCase<T>* tab, int ncases, int t, bool second) {
+    int lo = 0, hi = ncases - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if ((second ? tab[mid].b0 : tab[mid].a0) <= t)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return lo;
+}
+template <class T>
+__global__ __launch_bounds__(64) void k_bed2d_a_multi(const BedCase<T>* __restrict__ tab, int ncases) {
+    const int k = bed_case_of(tab, ncases, (int)blockIdx.x, false);
+    const Consts<T> c = tab[k].c;
+    const int per = (int)(((size_t)c.nx * c.ny + 63) / 64);
+    const int local = (int)blockIdx.x - tab[k].a0;
+    const int b = local / per;
+    const size_t id = (size_t)(local - b * per) * 64 + threadIdx.x;
+    if (id >= (size_t)c.nx * c.ny) return;
+    const BedBatch<T>& a = tab[k].a;
+    const int i = (int)(id % (size_t)c.nx);
+    const T hl = b == 0 ? a.h_last0[id] : bed_height(c, bed_eta(c, a.tprev[b], i), i, tab[k].piecewise);
+    bed_column(c, a.t[b], i, hl, tab[k].piecewise, a.h[b][id], a.uw[b][id], a.vw[b][id], a.ww[b][id]);
+}
+template <class T>
+__global__ __launch_bounds__(9 * BED_COLS) void k_bed2d_b_multi(const BedCase<T>* __restrict__ tab, int ncases) {
+    const int k = bed_case_of(tab, ncases, (int)blockIdx.x, true);
+    const Consts<T> c = tab[k].c;
+    const int per = (int)(((size_t)c.nx * c.ny + BED_COLS - 1) / BED_COLS);
+    const int local = (int)blockIdx.x - tab[k].b0;
+    const int b = local / per;
+    const BedBatch<T>& a = tab[k].a;
+    bed2d_b_block(c, (const T*)a.h[b], a.eta9[b], a.denom[b], a.kband[b], (const T*)a.uw[b], (const T*)a.vw[b], (const T*)a.ww[b],
+                  a.uw9[b], a.ksolid[b], a.fsol[b], a.ibb[b], (unsigned)(local - b * per));
+}
+
 // ===========================================================================
 // geometry materialisation (downloads) and per-cell flags
 // ===========================================================================
@@ -3035,21 +3086,26 @@ __global__ __launch_bounds__(FB_THREADS, 6) void k_step_wave(FusedArgs<S, T> a, 
     wave_tile<S, T, CM>(a, w, (int)blockIdx.x);
 }
 
-// Several cases advanced by one launch (the ensemble server's batches): the
-// tiles of case c are blocks block0[c] .. block0[c + 1] - 1 of the grid, in
-// the order of its own wavefront step, so every tile a collision tile waits
-// on still has a lower block index.  The table holds each case's arguments
-// of this step.
+// Several cases advanced by one launch (the ensemble server's batches).  The
+// first half of the grid holds the moments tiles of every case (case c:
+// blocks block0[c] .. block0[c] + nz bpp - 1, plane by plane), the second
+// half their collision tiles in the same order, so every tile a collision
+// tile waits on has a lower block index and has usually finished long
+// before (the cases' tiles run side by side, with no lag to tune).  Each
+// case's flags use a lag of nz planes, which is this order.  The table holds
+// each case's arguments of this step.
 template <class S, class T>
 struct WaveCase {
     FusedArgs<S, T> a;
-    WaveQ w;
-    int block0;  // first block of the case
+    WaveQ w;     // lag = nz
+    int block0;  // first moments block of the case (the half of the grid: + half)
     int pad;
 };
 template <class S, class T, int CM>
-__global__ __launch_bounds__(FB_THREADS, 6) void k_step_wave_batch(const WaveCase<S, T>* __restrict__ tab, int ncases) {
-    const int t = (int)blockIdx.x;
+__global__ __launch_bounds__(FB_THREADS, 6) void k_step_wave_batch(const WaveCase<S, T>* __restrict__ tab, int ncases, int half) {
+    int t = (int)blockIdx.x;
+    const bool coll = t >= half;
+    if (coll) t -= half;
     int lo = 0, hi = ncases - 1;  // the last case whose block0 <= t
     while (lo < hi) {
         const int mid = (lo + hi + 1) >> 1;
@@ -3060,7 +3116,8 @@ __global__ __launch_bounds__(FB_THREADS, 6) void k_step_wave_batch(const WaveCas
     }
     const FusedArgs<S, T> a = tab[lo].a;
     const WaveQ w = tab[lo].w;
-    wave_tile<S, T, CM>(a, w, t - tab[lo].block0);
+    const int local = t - tab[lo].block0;
+    wave_tile<S, T, CM>(a, w, coll ? a.c.nz * w.bpp + local : local);
 }
 
 // ===========================================================================
@@ -3206,10 +3263,9 @@ void launch_bed2d(const lbm_config_t& c, DevBufs& b, float t, int slot, hipStrea
 static float step_time(const lbm_config_t& c, int64_t step) { return (float)((double)(step + 1) * c.dt_phys); }
 
 template <class T>
-static void bed2d_batch_t(const lbm_config_t& cfg, DevBufs& b, int64_t step0, int nb, hipStream_t s) {
+static BedBatch<T> make_bed_batch(const lbm_config_t& cfg, DevBufs& b, int64_t step0, int nb) {
     if (nb < 1 || nb > BED_BATCH || b.ring != BED_RING) throw std::runtime_error("bed batch: bad size or ring");
     if (cfg.phi_fraction) throw std::runtime_error("bed batch: the phi modes keep one phi field per step");
-    Consts<T> c = make_consts<T>(cfg, -1);
     BedBatch<T> a{};
     for (int k = 0; k < nb; k++) {
         const int64_t st = step0 + k;
@@ -3229,10 +3285,48 @@ static void bed2d_batch_t(const lbm_config_t& cfg, DevBufs& b, int64_t step0, in
         a.ibb[k] = (T*)b.ibb[sl];
     }
     a.h_last0 = (const T*)b.h[(int)((step0 - 1 + b.ring) % b.ring)];
+    return a;
+}
+template <class T>
+static void bed2d_batch_t(const lbm_config_t& cfg, DevBufs& b, int64_t step0, int nb, hipStream_t s) {
+    Consts<T> c = make_consts<T>(cfg, -1);
+    BedBatch<T> a = make_bed_batch<T>(cfg, b, step0, nb);
     k_bed2d_a_batch<T><<<dim3(nblocks(b.n2, 64), nb), 64, 0, s>>>(c, a, (const T*)b.piecewise);
     k_bed2d_b_batch<T><<<dim3(nblocks(b.n2, BED_COLS), nb), 9 * BED_COLS, 0, s>>>(c, a);
     HIP_OK(hipGetLastError());
 }
+template <class T>
+static void fill_bed_case_t(const lbm_config_t& cfg, DevBufs& b, int64_t step0, int nb, int a0, int b0, void* out, int* na, int* nbb) {
+    BedCase<T> e;
+    memset(&e, 0, sizeof e);
+    e.c = make_consts<T>(cfg, -1);
+    e.a = make_bed_batch<T>(cfg, b, step0, nb);
+    e.piecewise = (const T*)b.piecewise;
+    e.nb = nb;
+    e.a0 = a0;
+    e.b0 = b0;
+    memcpy(out, &e, sizeof e);
+    *na = (int)nblocks(b.n2, 64) * nb;
+    *nbb = (int)nblocks(b.n2, BED_COLS) * nb;
+}
+size_t bed_case_bytes(int storage) { return storage == LBM_STORE_FP64 ? sizeof(BedCase<double>) : sizeof(BedCase<float>); }
+void fill_bed_case(const lbm_config_t& c, DevBufs& b, int64_t step0, int nb, int a0, int b0, void* out, int* na, int* nbb) {
+    if (b.storage == LBM_STORE_FP64)
+        fill_bed_case_t<double>(c, b, step0, nb, a0, b0, out, na, nbb);
+    else
+        fill_bed_case_t<float>(c, b, step0, nb, a0, b0, out, na, nbb);
+}
+void launch_bed_multi(int storage, const void* dev_table, int ncases, int na, int nbb, hipStream_t s) {
+    if (storage == LBM_STORE_FP64) {
+        k_bed2d_a_multi<double><<<na, 64, 0, s>>>((const BedCase<double>*)dev_table, ncases);
+        k_bed2d_b_multi<double><<<nbb, 9 * BED_COLS, 0, s>>>((const BedCase<double>*)dev_table, ncases);
+    } else {
+        k_bed2d_a_multi<float><<<na, 64, 0, s>>>((const BedCase<float>*)dev_table, ncases);
+        k_bed2d_b_multi<float><<<nbb, 9 * BED_COLS, 0, s>>>((const BedCase<float>*)dev_table, ncases);
+    }
+    HIP_OK(hipGetLastError());
+}
+
 void launch_bed2d_batch(const lbm_config_t& c, DevBufs& b, int64_t step0, int nb, hipStream_t s) {
     if (b.storage == LBM_STORE_FP64)
         bed2d_batch_t<double>(c, b, step0, nb, s);
@@ -3674,10 +3768,35 @@ static int fill_wave_case_t(const lbm_config_t& cfg, DevBufs& b, int cur, int mo
     const int bxd = block_x > 0 ? block_x : 64;
     const int gx = (cfg.nx + bxd - 1) / bxd, gy = (cfg.ny + FB_THREADS / bxd - 1) / (FB_THREADS / bxd);
     e.w = make_waveq(cfg, b, fs, bxd, gx, gy);
+    e.w.lag = cfg.nz;  // the batch's order: every moments tile before every collision tile
     e.block0 = block0;
     memcpy(out, &e, sizeof e);
-    return 2 * cfg.nz * e.w.bpp;
+    return cfg.nz * e.w.bpp;  // blocks in each half of the grid
 }
+template <class S, class T>
+static void patch_wave_case_t(void* entry, DevBufs& b, int cur, int mode, int slot_prev, int slot, int64_t step, int write_macro,
+                              int epoch, int block0) {
+    WaveCase<S, T>* e = (WaveCase<S, T>*)entry;
+    e->a.s0 = make_slot<T>(b, slot_prev);
+    e->a.s1 = make_slot<T>(b, slot);
+    e->a.fin = (const S*)b.f[cur];
+    e->a.fout = (S*)b.f[cur ^ 1];
+    e->a.mode = mode;
+    e->a.write_macro = write_macro;
+    e->a.step = (uint32_t)step;
+    e->w.epoch = epoch;
+    e->block0 = block0;
+}
+void patch_wave_case(int storage, void* entry, DevBufs& b, int cur, int mode, int slot_prev, int slot, int64_t step, int write_macro,
+                     int epoch, int block0) {
+    switch (storage) {
+        case LBM_STORE_FP64: patch_wave_case_t<double, double>(entry, b, cur, mode, slot_prev, slot, step, write_macro, epoch, block0); break;
+        case LBM_STORE_FP16: patch_wave_case_t<__half, float>(entry, b, cur, mode, slot_prev, slot, step, write_macro, epoch, block0); break;
+        case LBM_STORE_BF16: patch_wave_case_t<bf16_t, float>(entry, b, cur, mode, slot_prev, slot, step, write_macro, epoch, block0); break;
+        default: patch_wave_case_t<float, float>(entry, b, cur, mode, slot_prev, slot, step, write_macro, epoch, block0); break;
+    }
+}
+
 size_t wave_case_bytes(int storage) {
     switch (storage) {
         case LBM_STORE_FP64: return sizeof(WaveCase<double, double>);
@@ -3698,17 +3817,17 @@ int fill_wave_case(const lbm_config_t& c, DevBufs& b, int cur, int mode, int slo
     }
 }
 template <class S, class T>
-static void wave_batch_t(int cm, const void* tab, int ncases, int nblocks, hipStream_t s) {
+static void wave_batch_t(int cm, const void* tab, int ncases, int half, hipStream_t s) {
     auto kern = cm == CM_REG ? k_step_wave_batch<S, T, CM_REG> : (cm == CM_KBC ? k_step_wave_batch<S, T, CM_KBC> : k_step_wave_batch<S, T, CM_ANY>);
-    kern<<<nblocks, FB_THREADS, 0, s>>>((const WaveCase<S, T>*)tab, ncases);
+    kern<<<2 * half, FB_THREADS, 0, s>>>((const WaveCase<S, T>*)tab, ncases, half);
     HIP_OK(hipGetLastError());
 }
-void launch_wave_batch(int storage, int cm, const void* dev_table, int ncases, int nblocks, hipStream_t s) {
+void launch_wave_batch(int storage, int cm, const void* dev_table, int ncases, int half, hipStream_t s) {
     switch (storage) {
-        case LBM_STORE_FP64: wave_batch_t<double, double>(cm, dev_table, ncases, nblocks, s); break;
-        case LBM_STORE_FP16: wave_batch_t<__half, float>(cm, dev_table, ncases, nblocks, s); break;
-        case LBM_STORE_BF16: wave_batch_t<bf16_t, float>(cm, dev_table, ncases, nblocks, s); break;
-        default: wave_batch_t<float, float>(cm, dev_table, ncases, nblocks, s); break;
+        case LBM_STORE_FP64: wave_batch_t<double, double>(cm, dev_table, ncases, half, s); break;
+        case LBM_STORE_FP16: wave_batch_t<__half, float>(cm, dev_table, ncases, half, s); break;
+        case LBM_STORE_BF16: wave_batch_t<bf16_t, float>(cm, dev_table, ncases, half, s); break;
+        default: wave_batch_t<float, float>(cm, dev_table, ncases, half, s); break;
     }
 }
 

@@ -29,6 +29,8 @@
 #include <thread>
 #include <vector>
 
+#include <hip/hip_runtime.h>
+
 #include "../../include/lbm.h"
 #include "lbm_internal.h"
 
@@ -39,6 +41,7 @@ struct Job {
     int32_t nsteps;
     int32_t done;
     int64_t ticket;
+    hipEvent_t last = nullptr;  // batch thread: recorded after the job's last chunk
 };
 
 struct Worker {
@@ -71,7 +74,22 @@ struct lbm_server {
     // a launch, every round; cases whose steps are all enqueued wait on
     // their streams before they are reported
     void run_batch(Worker* w) {
+        // the batches run on a stream of their own, one chunk after another;
+        // a batched handle's own stream stays free of work, and the thread
+        // reports a case once the event after its last chunk has completed
+        hipStream_t bs = nullptr;
+        lbm::BatchBuf buf;
+        if (hipStreamCreateWithFlags(&bs, hipStreamNonBlocking) != hipSuccess) bs = nullptr;
         std::vector<Job> draining;
+        struct Cleanup {
+            hipStream_t& s;
+            lbm::BatchBuf& b;
+            ~Cleanup() {
+                if (s) (void)hipStreamSynchronize(s);
+                lbm::batch_buf_free(b);
+                if (s) (void)hipStreamDestroy(s);
+            }
+        } cleanup{bs, buf};
         while (true) {
             {
                 std::unique_lock<std::mutex> lk(w->m);
@@ -102,7 +120,7 @@ struct lbm_server {
                         ns.push_back(std::min(chunk, j.nsteps - j.done));
                         idx.push_back(a);
                     }
-                    const int rc = key < 0 ? LBM_E_ARG : lbm_step_batch(hs.data(), ns.data(), (int32_t)hs.size());
+                    const int rc = (key < 0 || !bs) ? LBM_E_ARG : lbm::step_batch(hs.data(), ns.data(), (int)hs.size(), bs, buf, false);
                     for (size_t m = 0; m < idx.size(); m++) {
                         Job& j = w->active[idx[m]];
                         if (rc == LBM_OK) j.done += ns[m];
@@ -114,6 +132,12 @@ struct lbm_server {
                 for (size_t a = 0; a < w->active.size();) {
                     Job& j = w->active[a];
                     if (j.done < 0 || j.done >= j.nsteps) {
+                        if (bs && hipEventCreateWithFlags(&j.last, hipEventDisableTiming) == hipSuccess &&
+                            hipEventRecord(j.last, bs) != hipSuccess) {
+                            (void)hipEventDestroy(j.last);
+                            j.last = nullptr;
+                        }
+                        if (!j.last && bs) (void)hipStreamSynchronize(bs);
                         draining.push_back(j);
                         w->active.erase(w->active.begin() + (long)a);
                     } else {
@@ -125,8 +149,19 @@ struct lbm_server {
                 Job& j = draining[d];
                 int32_t idle = 0;
                 // the device must be done with the case before it is
-                // reported, failed or not (its stream may hold its chunks)
-                int rc = lbm_query(j.h, &idle);
+                // reported, failed or not
+                int rc = LBM_OK;
+                if (j.last) {
+                    const hipError_t e = hipEventQuery(j.last);
+                    if (e == hipErrorNotReady) {
+                        d++;
+                        continue;
+                    }
+                    (void)hipEventDestroy(j.last);
+                    j.last = nullptr;
+                    if (e != hipSuccess) rc = LBM_E_HIP;
+                }
+                if (rc == LBM_OK) rc = lbm_query(j.h, &idle);
                 if (rc == LBM_OK && !idle) {
                     d++;
                     continue;
@@ -142,6 +177,7 @@ struct lbm_server {
                 if (w->incoming.empty()) return;
             }
             if (stop.load() && !w->active.empty()) {  // stopping: abandon the cases not yet done
+                if (bs) (void)hipStreamSynchronize(bs);
                 for (auto& j : w->active) {
                     j.done = -1 - LBM_E_STATE;
                     draining.push_back(j);

@@ -1088,6 +1088,10 @@ int lbm_set_option(lbm_handle_t* h, const char* key, int64_t value) {
         h->sched.wavefront = value ? 1 : 0;
         return LBM_OK;
     }
+    if (!strcmp(key, "resident")) {
+        h->sched.resident = value ? 1 : 0;
+        return LBM_OK;
+    }
     if (!strcmp(key, "wave_lag")) {
         if (value < 0 || value > 4096) return fail(h, LBM_E_ARG, "wave_lag must be in [0, 4096] (0: from the tiles a device holds)");
         h->sched.lag = (int)value;

@@ -76,19 +76,22 @@ struct DevBufs {
     int* waveq;     // fused path, wavefront step: [0] error flag, [WAVE_FLAG0 + t] epoch of moments tile t
     double* rpart;  // FP64 partial sums of the reductions: 3 * reduce_chunks(n)
 };
-constexpr int WAVE_FLAG0 = 64;  // first tile flag (the error flag keeps a cache line of its own)
-// tiles per plane of the fused kernels (block of 256 cells, block_x cells in x)
-inline size_t fused_tiles_per_plane(int nx, int ny, int block_x) {
-    const int by = 256 / block_x;
-    return (size_t)((nx + block_x - 1) / block_x) * (size_t)((ny + by - 1) / by);
+constexpr int WAVE_FLAG0 = 64;   // first tile flag (the error flag keeps a cache line of its own)
+// tiles per plane of the fused kernels (tiles of `cells` cells, block_x in x)
+inline size_t fused_tiles_per_plane(int nx, int ny, int block_x, int cells = 256) {
+    const int bx = block_x < cells ? block_x : cells;
+    const int by = cells / bx;
+    return (size_t)((nx + bx - 1) / bx) * (size_t)((ny + by - 1) / by);
 }
-// ints of the wavefront flag buffer: room for the tiles of any block_x
+// ints of the wavefront flag buffer: room for the tiles of any block_x, of
+// 256 cells (wavefront) and of 128 cells (tile-resident form)
 inline size_t wave_flag_ints(int nx, int ny, int nz) {
     size_t m = 0;
-    for (int bx = 64; bx <= 256; bx *= 2) {
-        const size_t t = fused_tiles_per_plane(nx, ny, bx);
-        m = t > m ? t : m;
-    }
+    for (int cells = 128; cells <= 256; cells *= 2)
+        for (int bx = 64; bx <= 256; bx *= 2) {
+            const size_t t = fused_tiles_per_plane(nx, ny, bx, cells);
+            m = t > m ? t : m;
+        }
     return WAVE_FLAG0 + (size_t)nz * m;
 }
 
@@ -97,6 +100,9 @@ inline size_t wave_flag_ints(int nx, int ny, int nz) {
 // the collision, 0 for about as many tiles as the device holds at once
 // (cus x 6 blocks: 12 planes at 256x128, 3 at 512x256, the measured best)
 struct FusedSched {
+    // FP32 storage, option "resident": the tile-resident form (measured
+    // slower than the wavefront at 256x128x128: off by default)
+    int resident = 0;
     int wavefront = 1;  // FP32 storage (the other storage types keep three launches)
     int lag = 0;
     int cus = 256;      // compute units of the device
@@ -130,7 +136,7 @@ void launch_staged_step(const lbm_config_t& c, DevBufs& b, int& cur, int slot, i
 // ones; slot_prev is the time level of the pending stream, slot the new one
 // band_top: highest k at which any cell can have a cut or closed link
 void launch_fused_step(const lbm_config_t& c, DevBufs& b, int& cur, int mode, int slot_prev, int slot, int64_t step,
-                       int write_macro, int omega_band_top, int band_top, int block_x, const FusedSched& fs, hipStream_t s);
+                       int write_macro, int omega_band_top, int band_top, int block_x, FusedSched& fs, hipStream_t s);
 // the wavefront step of several fused-path cases in one launch: the
 // size of one case's table entry, its collision model (cases of one launch
 // share the storage type and the model), the entry of one case's step

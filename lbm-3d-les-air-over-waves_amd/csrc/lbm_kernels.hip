@@ -3086,6 +3086,139 @@ __global__ __launch_bounds__(FB_THREADS, 6) void k_step_wave(FusedArgs<S, T> a, 
     wave_tile<S, T, CM>(a, w, (int)blockIdx.x);
 }
 
+// ---------------------------------------------------------------------------
+// Tile-resident form of the fused step: one persistent, co-resident grid
+// ---------------------------------------------------------------------------
+//
+// Block b of a grid of R blocks runs the tiles b, b + R, b + 2R, ... in
+// plane-major order, one iteration ahead: in iteration i it runs the
+// moments pass of its next tile (pull, conditioning, u; the pulled
+// populations go to one of two LDS buffers), publishes that tile's flag,
+// waits for the flags of the tiles whose velocities the collision of its
+// current tile needs (the 3 x 3 tiles of its plane, the tiles above and
+// below), and runs that collision with the populations it pulled one
+// iteration earlier, read back from LDS: the collision's second pull of the
+// wavefront form (76 B per update in FP32) is gone.  The waits point
+// forward (the tile above is bpp tiles later), so the grid is launched
+// cooperatively (all blocks resident) and only when R exceeds the longest
+// forward wait D (bpp + gx + 1 tiles in plane order; 2 bpp with the XCD
+// strip order of the tiles of a plane).  Then the lowest unfinished
+// collision, of tile t0, waits only on the moments of tiles t0 < t <= t0 + D;
+// the block of such a tile runs its moments at the start of an iteration
+// whose earlier collisions lie R or more tiles before t (below t0, so
+// finished), and moments never wait, so t0 completes (the host checks the
+// size; the waits are bounded as in k_step_wave all the same).
+// tiles of RES_THREADS cells, RES_OCC waves per SIMD (the launch bound:
+// 128 VGPRs; 2 x 19.5 KB of LDS per block, 4 blocks per CU).  Tiles of 128
+// cells (8 blocks per CU, 134 VGPRs) ran 20 % slower.
+constexpr int RES_THREADS = 256;
+constexpr int RES_OCC = 4;
+struct ResTile {
+    int k, bxi, byi, b, i, j;
+    bool in, band;
+    uint32_t id;
+};
+template <class S, class T, int CM>
+__global__ __launch_bounds__(RES_THREADS, RES_OCC) void k_step_resident(FusedArgs<S, T> a, WaveQ w, int total) {
+    __shared__ T sf[2][19][RES_THREADS];
+    __shared__ uint8_t sopen[2][RES_THREADS];
+    const int tid = (int)threadIdx.x;
+    const int nx = a.c.nx, ny = a.c.ny, nz = a.c.nz;
+    const uint32_t plane = (uint32_t)nx * ny;
+    auto tile = [&](int t) {
+        ResTile x;
+        x.k = t / w.bpp;
+        const int slot = t - x.k * w.bpp;
+        if (a.strips) {
+            tile_of(slot, w.gx, w.gy, x.bxi, x.byi);
+        } else {
+            x.byi = slot / w.gx;
+            x.bxi = slot - x.byi * w.gx;
+        }
+        x.b = x.byi * w.gx + x.bxi;  // the tile's own index (its flag)
+        x.i = x.bxi * w.bx + tid % w.bx;
+        x.j = x.byi * (RES_THREADS / w.bx) + tid / w.bx;
+        x.in = x.i < nx && x.j < ny;
+        x.band = x.k < a.band_planes;
+        x.id = (uint32_t)x.k * plane + (uint32_t)x.j * nx + x.i;
+        return x;
+    };
+    // the moments pass of tile x into LDS buffer buf, then its flag
+    auto moments = [&](const ResTile& x, int buf) {
+        if (x.in) {
+            T f[19];
+            bool open_cur = true;
+            if (x.band)
+                band_gather(a, x.i, x.j, x.k, f, open_cur);
+            else
+                lean_gather(a, x.i, x.j, x.k, f);
+            T r, u, v, ww;
+            fused_moments(a.c, x.k, f, r, u, v, ww);
+            xput<true>(a.ux + x.id, u);
+            xput<true>(a.uy + x.id, v);
+            xput<true>(a.uz + x.id, ww);
+            if (x.band) xput<true>(a.open + x.id, (uint8_t)(open_cur ? 1 : 0));
+#pragma unroll
+            for (int q = 0; q < 19; q++) sf[buf][q][tid] = f[q];
+            sopen[buf][tid] = open_cur ? 1 : 0;
+        }
+        __builtin_amdgcn_s_waitcnt(0);
+        __syncthreads();
+        if (tid == 0) __hip_atomic_store(&w.flag[x.k * w.bpp + x.b], w.epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    };
+    int t = (int)blockIdx.x;
+    if (t >= total) return;
+    ResTile cur = tile(t);
+    moments(cur, 0);
+    for (int it = 0;; it++) {
+        const int tn = t + (int)gridDim.x;
+        const bool more = tn < total;
+        ResTile nxt = cur;
+        if (more) {
+            nxt = tile(tn);
+            moments(nxt, (it + 1) & 1);
+        }
+        if (tid < 64) {
+            const int d = wave_dep(w, nz, cur.k, cur.bxi, cur.byi, tid);
+            bool wait = d >= 0;
+            int spins = 0;
+            while (__builtin_amdgcn_ballot_w64(wait) != 0) {
+                if (wait) wait = __hip_atomic_load(&w.flag[d], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != w.epoch;
+                if (__builtin_amdgcn_ballot_w64(wait) == 0) break;
+                if (++spins > WAVE_SPIN_LIMIT || __hip_atomic_load(w.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
+                    if (tid == 0) __hip_atomic_store(w.err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(2);
+            }
+        }
+        __syncthreads();  // the velocity and flag loads below are device-coherent
+        if (cur.in) {
+            const int i = cur.i, j = cur.j, k = cur.k;
+            const uint32_t id = cur.id;
+            T f[19];
+#pragma unroll
+            for (int q = 0; q < 19; q++) f[q] = sf[it & 1][q][tid];
+            const bool open_cur = sopen[it & 1][tid] != 0;
+            T rcur, ucur, vcur, wcur;
+            fused_moments(a.c, k, f, rcur, ucur, vcur, wcur);  // as in the moments pass, bit for bit
+            const bool open_below = cur.band && k > 0 ? xget<true>(a.open + (id - plane)) != 0 : true;
+            const int ip = i == nx - 1 ? 0 : i + 1, im = i == 0 ? nx - 1 : i - 1;
+            const int jp = j == ny - 1 ? 0 : j + 1, jm = j == 0 ? ny - 1 : j - 1;
+            const uint32_t row = (uint32_t)k * plane + (uint32_t)j * nx;
+            const uint32_t nbi[6] = {row + ip, row + im, (uint32_t)k * plane + (uint32_t)jp * nx + i,
+                                     (uint32_t)k * plane + (uint32_t)jm * nx + i, (k + 1 < nz ? id + plane : id), (k > 0 ? id - plane : id)};
+            const T* u3[3] = {a.ux, a.uy, a.uz};
+            auto nbu = [&](int comp, int dir) { return xget<true>(u3[comp] + nbi[dir]); };
+            les_collide_store<CM>(a, i, j, k, id, f, open_cur, open_below, rcur, ucur, vcur, wcur, nbu);
+        }
+        if (!more) break;
+        __syncthreads();  // buffer it & 1 is the next moments pass's but one
+        t = tn;
+        cur = nxt;
+    }
+}
+
 // Several cases advanced by one launch (the ensemble server's batches).  The
 // first half of the grid holds the moments tiles of every case (case c:
 // blocks block0[c] .. block0[c] + nz bpp - 1, plane by plane), the second
@@ -3688,7 +3821,7 @@ static WaveQ make_waveq(const lbm_config_t& cfg, DevBufs& b, const FusedSched& f
 
 template <class S, class T>
 static void fused_t(const lbm_config_t& cfg, DevBufs& b, int& cur, int mode, int slot_prev, int slot, int64_t step, int write_macro,
-                    int omega_band_top, int band_top, int block_x, const FusedSched& fs, hipStream_t s) {
+                    int omega_band_top, int band_top, int block_x, FusedSched& fs, hipStream_t s) {
     FusedArgs<S, T> a = make_fused_args<S, T>(cfg, b, cur, mode, slot_prev, slot, step, write_macro, omega_band_top);
     // planes 0 .. band_top can meet a cut or closed link (general kernels);
     // the planes above cannot (lean kernels)
@@ -3703,6 +3836,43 @@ static void fused_t(const lbm_config_t& cfg, DevBufs& b, int& cur, int mode, int
     a.strips = fs.strips && gy % 8 == 0 && fs.wavefront && std::is_same<S, float>::value;
     // FP32 storage only: with FP64 the merged kernel spills to scratch, and
     // the VALU-bound 16-bit forms ran 20-30 % slower merged (SGPR spills)
+    if constexpr (std::is_same<S, float>::value) {
+    if (fs.wavefront && fs.resident) {
+        // the tile-resident form: a co-resident grid of persistent blocks,
+        // when it holds more blocks than a plane of tiles plus a row (or one
+        // block per tile), and not inside a graph capture
+        hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+        HIP_OK(hipStreamIsCapturing(s, &cap));
+        if (cap == hipStreamCaptureStatusNone) {
+            // tiles of RES_THREADS cells: bx x (RES_THREADS / bx)
+            const int rbx = std::min(bxd, RES_THREADS);
+            const int rgx = (cfg.nx + rbx - 1) / rbx, rgy = (cfg.ny + RES_THREADS / rbx - 1) / (RES_THREADS / rbx);
+            WaveQ w = make_waveq(cfg, b, fs, rbx, rgx, rgy);
+            a.strips = fs.strips && rgy % 8 == 0;
+            auto kern = cm == CM_REG ? k_step_resident<S, T, CM_REG>
+                                     : (cm == CM_KBC ? k_step_resident<S, T, CM_KBC> : k_step_resident<S, T, CM_ANY>);
+            int per_cu = 0;
+            HIP_OK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(kern), RES_THREADS, 0));
+            int total = cfg.nz * w.bpp;
+            const long long resident = (long long)per_cu * fs.cus;
+            const int grid = (int)std::min<long long>(total, resident);
+            // the longest forward wait, in tiles (a grid of one block per
+            // tile never waits on a block that is not running)
+            int reach = a.strips ? 2 * w.bpp : w.bpp + w.gx + 1;
+            if (a.strips && !(grid == total || grid > reach)) {  // the plain order waits less far
+                a.strips = 0;
+                reach = w.bpp + w.gx + 1;
+            }
+            if (grid == total || grid > reach) {
+                a.plane0 = 0;
+                void* args[] = {&a, &w, &total};
+                HIP_OK(hipLaunchCooperativeKernel(reinterpret_cast<const void*>(kern), dim3(grid), dim3(RES_THREADS), args, 0, s));
+                cur ^= 1;
+                return;
+            }
+        }
+    }
+    }
     if (fs.wavefront && std::is_same<S, float>::value) {
         const WaveQ w = make_waveq(cfg, b, fs, bxd, (int)gx, (int)gy);
         auto kern = cm == CM_REG ? k_step_wave<S, T, CM_REG> : (cm == CM_KBC ? k_step_wave<S, T, CM_KBC> : k_step_wave<S, T, CM_ANY>);
@@ -3832,7 +4002,7 @@ void launch_wave_batch(int storage, int cm, const void* dev_table, int ncases, i
 }
 
 void launch_fused_step(const lbm_config_t& c, DevBufs& b, int& cur, int mode, int slot_prev, int slot, int64_t step,
-                       int write_macro, int omega_band_top, int band_top, int block_x, const FusedSched& fs, hipStream_t s) {
+                       int write_macro, int omega_band_top, int band_top, int block_x, FusedSched& fs, hipStream_t s) {
     if (!fused_supported(c)) throw std::runtime_error("fused path: needs air mode without a free surface");
     switch (b.storage) {
         case LBM_STORE_FP64: fused_t<double, double>(c, b, cur, mode, slot_prev, slot, step, write_macro, omega_band_top, band_top, block_x, fs, s); break;

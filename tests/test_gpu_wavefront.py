@@ -1,9 +1,11 @@
 """The wavefront schedule of the fused step (both passes in one launch, the
-collision of a tile waiting on the moments tiles around it) against the
+collision of a tile waiting on the moments tiles around it) and the
+tile-resident form (persistent blocks that run a tile's moments and
+collision with the populations kept in registers) against the
 three-launch schedule: the same arithmetic in another order of tiles, so
 the states agree bit for bit, on every collision model and on ragged grids.
 FP32 storage only: the 16-bit storage forms keep the three launches (the
-option is accepted and has no effect there, which the fp16 / bf16 cases
+options are accepted and have no effect there, which the fp16 / bf16 cases
 check)."""
 import numpy as np
 import pytest
@@ -46,7 +48,7 @@ def test_wavefront_equals_three_launches(name):
     env = CASES[name]
     ref = run(env, 30, wavefront=0)
     for lag in (1, 8, 64):
-        got = run(env, 30, wavefront=1, wave_lag=lag)
+        got = run(env, 30, wavefront=1, wave_lag=lag, resident=0)
         for fid, a in ref.items():
             assert np.array_equal(got[fid], a), (name, lag, fid, int(np.sum(got[fid] != a)))
 
@@ -58,7 +60,7 @@ def test_wavefront_tile_shapes(block_x):
     edges) are what the collision waits on."""
     env = CASES["ragged"]
     ref = run(env, 20, wavefront=0, block_x=block_x)
-    got = run(env, 20, wavefront=1, wave_lag=2, block_x=block_x)
+    got = run(env, 20, wavefront=1, wave_lag=2, block_x=block_x, resident=0)
     for fid, a in ref.items():
         assert np.array_equal(got[fid], a), (block_x, fid, int(np.sum(got[fid] != a)))
 
@@ -92,6 +94,37 @@ def test_wavefront_xcd_strips(block_x, lx):
     ref = run(env, 20, wavefront=0, block_x=block_x)
     for strips in (0, 1):
         for lag in (1, 0, 5):
-            got = run(env, 20, wavefront=1, wave_lag=lag, block_x=block_x, xcd_strips=strips)
+            got = run(env, 20, wavefront=1, wave_lag=lag, block_x=block_x, xcd_strips=strips, resident=0)
             for fid, a in ref.items():
                 assert np.array_equal(got[fid], a), (block_x, strips, lag, fid, int(np.sum(got[fid] != a)))
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_resident_equals_three_launches(name):
+    env = CASES[name]
+    ref = run(env, 30, wavefront=0)
+    got = run(env, 30, wavefront=1, resident=1)
+    for fid, a in ref.items():
+        assert np.array_equal(got[fid], a), (name, fid, int(np.sum(got[fid] != a)))
+
+
+@pytest.mark.parametrize("block_x", [64, 128, 256])
+def test_resident_tile_shapes_and_calls(block_x):
+    """Tiles of 64 x 4, 128 x 2 and 256 x 1 cells, and several calls in a
+    row (the ticket counter carries on from launch to launch): the state
+    of three launches bit for bit."""
+    env = env_of("waves_256x128x128", LBM_LX_M=256, LBM_LY_M=64, LBM_LZ_M=24, LBM_BED_AMP_M=2, LBM_BED_WAVELENGTH_M=64)
+    ref = run(env, 1 + 2 + 7, wavefront=0, block_x=block_x)
+    from lbm_native import Case
+
+    cfg = parse_env(dict(env, LBM_KERNEL_PATH="fused"))
+    c = Case(cfg.to_struct())
+    c.set_option("block_x", block_x)
+    c.set_option("resident", 1)
+    setup_case(c, cfg)
+    for n in (1, 2, 7):
+        c.step(n)
+    for fid, a in ref.items():
+        got = c.download(fid)
+        assert np.array_equal(got, a), (block_x, fid, int(np.sum(got != a)))
+    c.close()

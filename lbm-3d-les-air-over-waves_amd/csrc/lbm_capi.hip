@@ -153,7 +153,7 @@ static size_t device_bytes(const lbm_config_t& c, int path) {
     bytes += 5 * n * R + n;                      // rho, u, omega, cell flags
     if (path == LBM_PATH_STAGED) bytes += 2 * n * R + 2 * n * 4;  // nu_t, smag, obstacle masks
     bytes += (size_t)bed_ring(c) * n2 * (78 * R + 8) + (c.nx + c.nz + (size_t)c.nz * c.ny) * R;
-    if (c.fs_enabled) bytes += n * (12 * R + 7 * 4 + 1) + 6 * n * R + 3 * n2 * R + 8 * (size_t)c.nz * c.ny;
+    if (c.fs_enabled) bytes += n * (12 * R + 7 * 4 + 2) + 6 * n * R + 3 * n2 * R + 8 * (size_t)c.nz * c.ny;
     if (c.phi_fraction) bytes += n * R;
     return bytes + (1 << 20);
 }
@@ -267,6 +267,7 @@ static void allocate(lbm_handle* h) {
         b.excess_dir = (int*)dalloc(h, n * 4);
         b.ext_gas = (int*)dalloc(h, n * 4);
         b.si = (int*)dalloc(h, n * 4);
+        b.tstate = (uint8_t*)dalloc(h, n);
         b.mdirty = (uint8_t*)dalloc(h, n);
         CHECK(hipMemsetAsync(b.mdirty, 0, n, h->stream));
         b.rowdon = (int*)dalloc(h, 2 * (size_t)c.nz * c.ny * 4);

@@ -71,6 +71,7 @@ struct DevBufs {
     int *type, *type_prev, *excess_dir, *ext_gas, *obst, *obst_prev, *si;
     uint8_t* mdirty;  // VOF runs: cells whose populations changed after the stream's macro step
     int* rowdon;      // VOF runs: [2][nz * ny] rows holding a donor of a redistribution round
+    uint8_t* tstate;  // free-surface runs: 2 type + open flag of every cell, for the stream (written by the macro step)
     int* icnt;      // [IC_COUNT]
     double* dcnt;   // [DC_COUNT]
     int* waveq;     // fused path, wavefront step: [0] error flag, [WAVE_FLAG0 + t] epoch of moments tile t

@@ -663,11 +663,15 @@ __device__ __forceinline__ void macro_cell(const Consts<T>& c, int k, const T fl
     }
 }
 
+// water runs also pack each cell's type and open flag into one byte for the
+// stream's per-link source states (tstate; NULL otherwise)
 template <class S, class T>
-__global__ __launch_bounds__(NT) void k_macro(Consts<T> c, const S* f, T* rho, T* ux, T* uy, T* uz, const int* type, const uint8_t* flags) {
+__global__ __launch_bounds__(NT) void k_macro(Consts<T> c, const S* f, T* rho, T* ux, T* uy, T* uz, const int* type, const uint8_t* flags,
+                                              uint8_t* tstate) {
     size_t id = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     size_t n = (size_t)c.nx * c.ny * c.nz;
     if (id >= n) return;
+    if (tstate) tstate[id] = (uint8_t)(type[id] * 2 + (int)(flags[id] & 1u));
     int i, j, k;
     cell_ijk(c, id, i, j, k);
     T fl[19];
@@ -690,11 +694,13 @@ __global__ __launch_bounds__(NT) void k_macro(Consts<T> c, const S* f, T* rho, T
 // recomputed; every other cell already holds what k_macro would write
 template <class S, class T>
 __global__ __launch_bounds__(NT) void k_macro_fix(Consts<T> c, const S* f, T* rho, T* ux, T* uy, T* uz, const int* type,
-                                                  const int* type_prev, const uint8_t* flags, uint8_t* dirty) {
+                                                  const int* type_prev, const uint8_t* flags, uint8_t* dirty, uint8_t* tstate) {
     size_t id = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     size_t n = (size_t)c.nx * c.ny * c.nz;
     if (id >= n) return;
     const int ty = type[id];
+    const uint8_t fl0 = flags[id];
+    tstate[id] = (uint8_t)(ty * 2 + (int)(fl0 & 1u));  // the stream's source state of this cell
     const bool d = dirty[id] != 0;
     if ((ty == LBM_FS_GAS) == (type_prev[id] == LBM_FS_GAS) && !d) return;
     if (d) dirty[id] = 0;
@@ -704,7 +710,7 @@ __global__ __launch_bounds__(NT) void k_macro_fix(Consts<T> c, const S* f, T* rh
 #pragma unroll
     for (int q = 0; q < 19; q++) fl[q] = Store<S, T>::load(f, (size_t)q * n + id, q, c.store);
     T r, u, v, w;
-    macro_cell(c, k, fl, ty, (flags[id] & 1u) != 0, r, u, v, w);
+    macro_cell(c, k, fl, ty, (fl0 & 1u) != 0, r, u, v, w);
     rho[id] = r;
     ux[id] = u;
     uy[id] = v;
@@ -889,6 +895,7 @@ __device__ __forceinline__ T src_value(const Consts<T>& c, const S* f, size_t n,
 // cell's free-surface type and plane, and the link counts of the step
 template <class S, class T>
 __device__ __forceinline__ void stream_bc_cell(const Consts<T>& c, const Slot<T>& s, const S* fin, const T* ux, const T* uy, const T* uz,
+                                               const uint8_t* tstate,
                                                const int* type, const T* fill, const T* u_inlet, const T* inflow, const T* fs_h,
                                                const uint8_t* flags, size_t id, T fo[19], int& ty_out, int& k_out, int& n_cut,
                                                int& n_fs, int& n_bridge) {
@@ -962,7 +969,7 @@ __device__ __forceinline__ void stream_bc_cell(const Consts<T>& c, const Slot<T>
             const int sic = si < 0 ? 0 : (si >= nx ? nx - 1 : si);  // outflow: copy of the last layer
             const size_t sidx = src_ok ? ((size_t)sk * ny + sj) * nx + sic : id;
             const size_t tid = (src_ok && inx) ? sidx : id;
-            st_bits |= (uint64_t)(type[tid] * 2 + (int)(flags[tid] & 1u)) << (3 * q);
+            st_bits |= (uint64_t)tstate[tid] << (3 * q);  // 2 type + open of the source
             sv_a[q] = Store<S, T>::load(fin, (size_t)q * n + sidx, q, inv);
         }
         // pass 2: the rules of stream (CODEX:3438-3551) for a liquid or
@@ -1099,14 +1106,14 @@ __device__ __forceinline__ void stream_bc_cell(const Consts<T>& c, const Slot<T>
 // dither of step ^ SALT_STREAM; the macro step then sees the stored values)
 template <class S, class D, class T, bool COUNT, bool MACRO>
 __global__ __launch_bounds__(NT) void k_stream_bc(Consts<T> c, Slot<T> s, const S* fin, D* fout, const T* ux, const T* uy, const T* uz,
-                            const int* type, const T* fill, const T* u_inlet, const T* inflow, const T* fs_h,
+                            const uint8_t* tstate, const int* type, const T* fill, const T* u_inlet, const T* inflow, const T* fs_h,
                             const uint8_t* flags, int* icnt, T* mrho, T* mux, T* muy, T* muz, uint32_t step) {
     size_t id = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     size_t n = (size_t)c.nx * c.ny * c.nz;
     if (id >= n) return;
     T fo[19];
     int ty, k, n_cut, n_fs, n_bridge;
-    stream_bc_cell<S, T>(c, s, fin, ux, uy, uz, type, fill, u_inlet, inflow, fs_h, flags, id, fo, ty, k, n_cut, n_fs, n_bridge);
+    stream_bc_cell<S, T>(c, s, fin, ux, uy, uz, tstate, type, fill, u_inlet, inflow, fs_h, flags, id, fo, ty, k, n_cut, n_fs, n_bridge);
     if constexpr (std::is_same<D, T>::value) {
 #pragma unroll
         for (int q = 0; q < 19; q++) fout[(size_t)q * n + id] = fo[q];
@@ -3287,7 +3294,8 @@ __device__ __forceinline__ double reduce_cell(const RedArgs<S, T>& a, int what, 
     T fo[19];
     if (SRC == 1) {
         int ty, k, n1, n2, n3;
-        stream_bc_cell<S, T>(c, a.s, a.f, a.ux, a.uy, a.uz, a.type, a.fill, a.u_inlet, a.inflow, a.fs_h, a.flags, id, fo, ty, k, n1, n2, n3);
+        stream_bc_cell<S, T>(c, a.s, a.f, a.ux, a.uy, a.uz, nullptr, a.type, a.fill, a.u_inlet, a.inflow, a.fs_h, a.flags, id, fo, ty, k, n1,
+                             n2, n3);
     } else {
 #pragma unroll
         for (int q = 0; q < 19; q++) fo[q] = Store<S, T>::load(a.f, (size_t)q * n + id, q, c.store);
@@ -3724,9 +3732,10 @@ static void staged_t(const lbm_config_t& cfg, DevBufs& b, int& cur, int slot, in
     // and u as the macro step of its populations (see k_macro_fix)
     const bool vof_macro = cfg.fs_enabled && cfg.fs_tracking == LBM_FST_VOF && !cond_writes_f;
     if (vof_macro && st.macro_ready)
-        k_macro_fix<S, T><<<nb, NT, 0, s>>>(c, f, (T*)b.rho, (T*)b.ux, (T*)b.uy, (T*)b.uz, b.type, b.type_prev, b.cellflags, b.mdirty);
+        k_macro_fix<S, T><<<nb, NT, 0, s>>>(c, f, (T*)b.rho, (T*)b.ux, (T*)b.uy, (T*)b.uz, b.type, b.type_prev, b.cellflags, b.mdirty,
+                                            b.tstate);
     else
-        k_macro<S, T><<<nb, NT, 0, s>>>(c, f, (T*)b.rho, (T*)b.ux, (T*)b.uy, (T*)b.uz, b.type, b.cellflags);
+        k_macro<S, T><<<nb, NT, 0, s>>>(c, f, (T*)b.rho, (T*)b.ux, (T*)b.uy, (T*)b.uz, b.type, b.cellflags, c.fs_water ? b.tstate : nullptr);
     auto collide = cfg.use_les ? k_collide<S, T, true> : k_collide<S, T, false>;
     collide<<<nb, NT, 0, s>>>(c, sl, f, (const T*)b.rho, (const T*)b.ux, (const T*)b.uy, (const T*)b.uz, (T*)b.omega, (T*)b.nu_t,
                               (T*)b.smag, b.type, b.cellflags, (uint32_t)step);
@@ -3738,7 +3747,7 @@ static void staged_t(const lbm_config_t& cfg, DevBufs& b, int& cur, int slot, in
     // inside the stream kernel
     auto stream = cfg.fs_enabled ? (count_links ? k_stream_bc<S, S, T, true, true> : k_stream_bc<S, S, T, false, true>)
                                  : (count_links ? k_stream_bc<S, S, T, true, false> : k_stream_bc<S, S, T, false, false>);
-    stream<<<nb, NT, 0, s>>>(c, sl, f, fn, (const T*)b.ux, (const T*)b.uy, (const T*)b.uz, b.type, (const T*)b.fill,
+    stream<<<nb, NT, 0, s>>>(c, sl, f, fn, (const T*)b.ux, (const T*)b.uy, (const T*)b.uz, b.tstate, b.type, (const T*)b.fill,
                              (const T*)b.u_inlet, (const T*)b.inflow, (const T*)b.fs_h, b.cellflags, b.icnt, (T*)b.rho, (T*)b.ux,
                              (T*)b.uy, (T*)b.uz, (uint32_t)step);
     cur ^= 1;
@@ -3836,7 +3845,7 @@ static void fused_t(const lbm_config_t& cfg, DevBufs& b, int& cur, int mode, int
     a.strips = fs.strips && gy % 8 == 0 && fs.wavefront && std::is_same<S, float>::value;
     // FP32 storage only: with FP64 the merged kernel spills to scratch, and
     // the VALU-bound 16-bit forms ran 20-30 % slower merged (SGPR spills)
-    if constexpr (std::is_same<S, float>::value) {
+    if constexpr (std::is_same<T, float>::value) {
     if (fs.wavefront && fs.resident) {
         // the tile-resident form: a co-resident grid of persistent blocks,
         // when it holds more blocks than a plane of tiles plus a row (or one
@@ -4020,7 +4029,7 @@ static void materialize_t(const lbm_config_t& cfg, DevBufs& b, int cur, int slot
     Slot<T> sl = make_slot<T>(b, slot);
     k_cellflags<T><<<nblocks(b.n), NT, 0, s>>>(c, sl, b.cellflags);
     k_stream_bc<S, T, T, false, false><<<nblocks(b.n), NT, 0, s>>>(c, sl, (const S*)b.f[cur], (T*)dst, (const T*)b.ux, (const T*)b.uy,
-                                                                   (const T*)b.uz, b.type, (const T*)b.fill, (const T*)b.u_inlet,
+                                                                   (const T*)b.uz, b.tstate, b.type, (const T*)b.fill, (const T*)b.u_inlet,
                                                                    (const T*)b.inflow, (const T*)b.fs_h, b.cellflags, b.icnt,
                                                                    nullptr, nullptr, nullptr, nullptr, 0u);
     HIP_OK(hipGetLastError());

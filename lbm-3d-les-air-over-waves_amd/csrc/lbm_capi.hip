@@ -48,6 +48,7 @@ struct lbm_handle {
     int block_x = 64;  // x extent of a fused-kernel block (64, 128 or 256 cells; set from nx at creation)
     FusedSched sched;  // fused path: wavefront (one launch per step) or three launches
     StagedState staged;  // staged path: what a step may reuse from the previous one
+    int vof_band = 1;    // option vof_band: VOF passes skip the rows settled far from the interface
     // fused path, option "graph": the launches of an lbm_step call are
     // captured into a HIP graph and launched as one; the executable graph is
     // kept and updated with the next call's launches (same kernels, new
@@ -153,7 +154,7 @@ static size_t device_bytes(const lbm_config_t& c, int path) {
     bytes += 5 * n * R + n;                      // rho, u, omega, cell flags
     if (path == LBM_PATH_STAGED) bytes += 2 * n * R + 2 * n * 4;  // nu_t, smag, obstacle masks
     bytes += (size_t)bed_ring(c) * n2 * (78 * R + 8) + (c.nx + c.nz + (size_t)c.nz * c.ny) * R;
-    if (c.fs_enabled) bytes += n * (12 * R + 7 * 4 + 2) + 6 * n * R + 3 * n2 * R + 8 * (size_t)c.nz * c.ny;
+    if (c.fs_enabled) bytes += n * (12 * R + 7 * 4 + 2) + 6 * n * R + 3 * n2 * R + 20 * (size_t)c.nz * c.ny;
     if (c.phi_fraction) bytes += n * R;
     return bytes + (1 << 20);
 }
@@ -272,6 +273,8 @@ static void allocate(lbm_handle* h) {
         CHECK(hipMemsetAsync(b.mdirty, 0, n, h->stream));
         b.rowdon = (int*)dalloc(h, 2 * (size_t)c.nz * c.ny * 4);
         CHECK(hipMemsetAsync(b.rowdon, 0, 2 * (size_t)c.nz * c.ny * 4, h->stream));
+        b.rowact = (int*)dalloc(h, 3 * (size_t)c.nz * c.ny * 4);
+        CHECK(hipMemsetAsync(b.rowact, 0, 3 * (size_t)c.nz * c.ny * 4, h->stream));
         CHECK(hipMemsetAsync(b.mdelta, 0, n * R, h->stream));
         CHECK(hipMemsetAsync(b.excess, 0, n * R, h->stream));
         CHECK(hipMemsetAsync(b.excess_dir, 0, n * 4, h->stream));
@@ -502,6 +505,7 @@ static void enqueue_steps(lbm_handle* h, int32_t nsteps) {
         if (h->timing) CHECK(hipEventRecord(h->ev[2 + 2 * s], h->stream));
         if (h->path == LBM_PATH_STAGED) {
             if (!h->post_stream) throw std::runtime_error("internal: staged step on a post-collision state");
+            h->staged.band_allowed = h->vof_band;
             launch_staged_step(c, h->b, h->cur, slot, step, h->omega_band_top, s == nsteps - 1, h->staged, h->stream);
         } else {
             int last = s == nsteps - 1;
@@ -1096,6 +1100,10 @@ int lbm_set_option(lbm_handle_t* h, const char* key, int64_t value) {
     if (!strcmp(key, "wave_lag")) {
         if (value < 0 || value > 4096) return fail(h, LBM_E_ARG, "wave_lag must be in [0, 4096] (0: from the tiles a device holds)");
         h->sched.lag = (int)value;
+        return LBM_OK;
+    }
+    if (!strcmp(key, "vof_band")) {
+        h->vof_band = value ? 1 : 0;
         return LBM_OK;
     }
     if (!strcmp(key, "xcd_strips")) {

@@ -71,6 +71,7 @@ struct DevBufs {
     int *type, *type_prev, *excess_dir, *ext_gas, *obst, *obst_prev, *si;
     uint8_t* mdirty;  // VOF runs: cells whose populations changed after the stream's macro step
     int* rowdon;      // VOF runs: [2][nz * ny] rows holding a donor of a redistribution round
+    int* rowact;      // VOF runs: [3][nz * ny] rows marked by the passes of a step (this step's and the last), rows with an isolated open cell
     uint8_t* tstate;  // free-surface runs: 2 type + open flag of every cell, for the stream (written by the macro step)
     int* icnt;      // [IC_COUNT]
     double* dcnt;   // [DC_COUNT]
@@ -116,6 +117,9 @@ struct FusedSched {
 struct StagedState {
     int geo_ready = 0;    // cell flags and obstacle masks hold the (static) geometry
     int macro_ready = 0;  // rho and u hold the last stream's macro step (VOF runs, no conditioning)
+    int band_ok = 0;      // the VOF row marks of the last step cover its end state (narrow band usable)
+    int act_slot = 0;     // row-mark slot of the next step
+    int band_allowed = 1;  // option vof_band (set by the C ABI before each step)
 };
 
 // host-side launchers (lbm_kernels.hip)

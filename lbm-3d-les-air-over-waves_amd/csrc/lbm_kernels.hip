@@ -1176,6 +1176,10 @@ struct VofArgs {
     const uint8_t* flags;
     uint8_t* mdirty;  // cells whose populations the repair rewrote (for k_macro_fix)
     int* rowdon;      // [2][nz * ny]: rows holding a donor of the redistribution round of each slot
+    int* act;         // [nz * ny]: rows marked active by this step's passes (the narrow band below)
+    const int* act_prev;  // the marks of the previous step's passes
+    int* iso;         // [nz * ny]: rows holding an open cell with no open neighbour (set on steps with band 0)
+    int band;         // 1: passes skip the rows the band shows settled; 0: every row is processed
     int* icnt;
     double* dcnt;
     const T* fs_h;  // column surface height
@@ -1238,10 +1242,85 @@ __device__ __forceinline__ void wet_resolved(const VofArgs<T>& a, int i, int j, 
     int i, j, k;                                                          \
     cell_ijk(a.c, id, i, j, k);
 
+// Narrow band.  A cell is settled when it is LIQUID or GAS, holds what
+// the classification of its own mass gives (mass, fill, type), with no
+// excess and no excess direction, and no pass of this step has changed it
+// otherwise.  Every pass that leaves a cell unsettled, changes its type, or
+// changes its mass, fill or type outside a classification marks the cell's
+// row (k, j) in act; the preclassification, which classifies every cell,
+// marks from scratch each step.  For a settled cell the passes below are
+// no-ops, except for the stores of values it already holds:
+//   the classification rewrites the same mass, fill and type, excess 0;
+//   the redistribution has no donor there (a donor has excess);
+//   the interface layer converts a LIQUID cell next to a GAS cell, and the
+//   orphan and detached tests act on a cell whose wet neighbours are few
+//   or unresolved: each needs an unsettled cell or a LIQUID-GAS contact
+//   among the 18 neighbours, i.e. an active row within one row.
+// LIQUID-GAS contacts: the previous step ended with the interface layer
+// enforced, which leaves none, and the types of settled cells did not
+// change since, so every contact has a marked cell.  What this does not
+// cover runs with band = 0 (every row processed): the first step after the
+// state is set from outside (its marks are not known) and a moving geometry
+// (cells open and close).  A LIQUID cell with no open
+// neighbour has no resolved one either: the orphan test always processes
+// the rows holding such a cell (iso, found on the steps with band = 0).
+// The scratch si of a cell is nonzero only where a pass marked it (donors,
+// orphans, movers), and the preclassification clears it everywhere.
+template <class T>
+__device__ __forceinline__ void band_mark(const VofArgs<T>& a, int j, int k) {
+    a.act[(size_t)k * a.c.ny + j] = 1;
+}
+// rows (k, j) and the 8 around it (y wrapped as the neighbours are; a row
+// beyond a z end or a solid y wall is replaced by (k, j) itself): the nine
+// loads are issued together, with no branch between them
+template <class T>
+__device__ __forceinline__ bool rows_near(const VofArgs<T>& a, const int* rows, int j, int k) {
+    const int ny = a.c.ny, nz = a.c.nz;
+    const int km = k > 0 ? k - 1 : k, kp = k + 1 < nz ? k + 1 : k;
+    int jm, jp;
+    if (a.c.y_boundary == LBM_YB_PERIODIC) {
+        jm = wrap1(j - 1, ny);
+        jp = wrap1(j + 1, ny);
+    } else {
+        jm = j > 0 ? j - 1 : j;
+        jp = j + 1 < ny ? j + 1 : j;
+    }
+    const int* r0 = rows + (size_t)km * ny;
+    const int* r1 = rows + (size_t)k * ny;
+    const int* r2 = rows + (size_t)kp * ny;
+    return (r0[jm] | r0[j] | r0[jp] | r1[jm] | r1[j] | r1[jp] | r2[jm] | r2[j] | r2[jp]) != 0;
+}
+// row (k, j) may hold a cell that a pass changes: itself active, or (near)
+// one of its 8 neighbouring rows active
+template <class T>
+__device__ __forceinline__ bool band_row(const VofArgs<T>& a, int j, int k, bool near) {
+    if (!a.band) return true;
+    if (!near) return a.act[(size_t)k * a.c.ny + j] != 0;
+    return rows_near(a, a.act, j, k);
+}
+
 // compute_vof_mass_exchange (CODEX:1305-1350)
 template <class S, class T>
 __global__ __launch_bounds__(NT) void k_vof_exchange(VofArgs<T> a, const S* f) {
     VOF_PROLOGUE
+    if (id == 0) {  // the per-step counters of the pipeline (each is only added to after this pass)
+        a.icnt[IC_VOF_EXCESS] = 0;
+        a.icnt[IC_VOF_ORPHAN] = 0;
+        a.icnt[IC_VOF_ADVECT] = 0;
+        a.icnt[IC_VOF_COMPRESS] = 0;
+        a.icnt[IC_VOF_COLLAPSE_CELLS] = 0;
+        a.icnt[IC_FS_REFILL] = 0;
+        a.icnt[IC_FS_REPAIR] = 0;
+        for (int t = 0; t < 2; t++) a.dcnt[DC_ADVECT_MASS + t] = 0.0;  // advected and compressed mass
+        for (int t = 0; t < 5; t++) a.dcnt[DC_CAND_MASS + t] = 0.0;    // the collapse sums
+    }
+    if (i == 0) {
+        a.act[(size_t)k * a.c.ny + j] = 0;  // this step's marks start here
+        if (!a.band) a.iso[(size_t)k * a.c.ny + j] = 0;
+    }
+    // every INTERFACE cell was marked where it became one (the band's marks
+    // of the previous step cover its end state)
+    if (a.band && a.act_prev[(size_t)k * a.c.ny + j] == 0) return;
     // the delta is read only for these cells (the preclassification, which
     // also sets every cell's excess and its direction)
     if (!((a.flags[id] & 1u) && a.type[id] == LBM_FS_INTERFACE)) return;
@@ -1279,11 +1358,25 @@ __global__ __launch_bounds__(NT) void k_vof_preclassify(VofArgs<T> a, T empty_th
     a.type_prev[id] = old;
     a.excess[id] = T(0);
     a.excess_dir[id] = 0;
+    // si is nonzero only at cells the last step marked (its flood-fill flags
+    // are cleared by the collapse)
+    if (!a.band || a.act_prev[(size_t)k * a.c.ny + j] != 0) a.si[id] = 0;
     if (!(a.flags[id] & 1u)) {
         a.mass[id] = T(0);
         a.fill[id] = T(0);
         a.type[id] = LBM_FS_GAS;
+        if (old != LBM_FS_GAS) band_mark(a, j, k);
         return;
+    }
+    if (!a.band) {  // (the geometry is the one of every later step with band 1)
+        bool any = false;
+#pragma unroll
+        for (int q = 1; q < 19; q++) {
+            bool v;
+            const size_t nid = NBR_ID(cxq(q), cyq(q), czq(q), v);
+            any |= v & ((a.flags[nid] & 1u) != 0);
+        }
+        if (!any) a.iso[(size_t)k * a.c.ny + j] = 1;
     }
     T rr = a.rho[id];
     if (rr < T(1.0e-8)) rr = a.c.rho0;
@@ -1326,6 +1419,7 @@ __global__ __launch_bounds__(NT) void k_vof_preclassify(VofArgs<T> a, T empty_th
     if (fabs(ex) > T(1.0e-7)) atomicAdd(&a.icnt[IC_VOF_EXCESS], 1);
     a.fill[id] = fl;
     a.type[id] = ct;
+    if (ct == LBM_FS_INTERFACE || ct != old || ex != T(0)) band_mark(a, j, k);
 }
 
 template <class T>
@@ -1342,6 +1436,8 @@ __device__ __forceinline__ T vof_room(const VofArgs<T>& a, size_t nid, T ex) {
 template <class T>
 __global__ __launch_bounds__(NT) void k_vof_redist_a(VofArgs<T> a, int slot) {
     VOF_PROLOGUE
+    // a donor has excess (marked); si is already zero in a settled row
+    if (!band_row(a, j, k, false)) return;
     a.si[id] = 0;
     T ex = a.excess[id];
     if (!(fabs(ex) > T(1.0e-12))) return;
@@ -1374,17 +1470,29 @@ __global__ __launch_bounds__(NT) void k_vof_redist_a(VofArgs<T> a, int slot) {
         nyv *= invn;
         nzv *= invn;
     }
+    // every neighbour's flags, type, density and mass loaded before the sums
+    // (the cell itself stands in for a neighbour beyond a wall)
+    bool nv[19];
+    uint8_t nfl[19];
+    int nty[19];
+    T nrho[19], nmass[19];
+#pragma unroll
+    for (int q = 1; q < 19; q++) {
+        const size_t nid = NBR_ID(cxq(q), cyq(q), czq(q), nv[q]);
+        nfl[q] = a.flags[nid];
+        nty[q] = a.type[nid];
+        nrho[q] = a.rho[nid];
+        nmass[q] = a.mass[nid];
+    }
     T ws = T(0), cs = T(0);
 #pragma unroll
     for (int q = 1; q < 19; q++) {
-        int ni, nj, nk;
-        if (!NBR(cxq(q), cyq(q), czq(q), ni, nj, nk)) continue;
-        size_t nid = CID(nk, nj, ni);
-        if (!(a.flags[nid] & 1u)) continue;
+        if (!nv[q] || !(nfl[q] & 1u)) continue;
         T nd = (T)cxq(q) * nxv + (T)cyq(q) * nyv + (T)czq(q) * nzv;
-        bool ok = ed > 0 ? a.type[nid] != LBM_FS_LIQUID : a.type[nid] != LBM_FS_GAS;
+        bool ok = ed > 0 ? nty[q] != LBM_FS_LIQUID : nty[q] != LBM_FS_GAS;
         if (!ok) continue;
-        T room = vof_room(a, nid, ex);
+        T room = nmass[q];  // vof_room
+        if (ex > T(0)) room = (nrho[q] < T(1.0e-8) ? a.c.rho0 : nrho[q]) - nmass[q];
         if (room > T(1.0e-12)) {
             cs += room;
             T wgt = ed > 0 ? nd : -nd;
@@ -1437,7 +1545,7 @@ __device__ __forceinline__ T redist_gather(const VofArgs<T>& a, int i, int j, in
 // finalize_vof_classification_from_mass (CODEX:1577-1624) of cell id whose
 // mass is m: mass, fill, type, excess (into ex_out) and its direction
 template <class T>
-__device__ __forceinline__ void finalize_cell(const VofArgs<T>& a, size_t id, T m, T empty_thr, T* ex_out) {
+__device__ __forceinline__ void finalize_cell(const VofArgs<T>& a, size_t id, int j, int k, T m, T empty_thr, T* ex_out) {
     if (!(a.flags[id] & 1u)) {
         ex_out[id] = T(0);
         a.excess_dir[id] = 0;
@@ -1478,12 +1586,14 @@ __device__ __forceinline__ void finalize_cell(const VofArgs<T>& a, size_t id, T 
         fl = T(1.0);
         ct = LBM_FS_LIQUID;
     }
+    const int old = a.type[id];
     a.mass[id] = m;
     ex_out[id] = ex;
     a.excess_dir[id] = ed;
     a.fill[id] = fl;
     a.type[id] = ct;
     if (cnt) atomicAdd(&a.icnt[IC_VOF_EXCESS], cnt);
+    if (ct == LBM_FS_INTERFACE || ct != old || cnt) band_mark(a, j, k);
 }
 
 // one redistribution round's phase B followed by the classification
@@ -1497,27 +1607,6 @@ __device__ __forceinline__ void finalize_cell(const VofArgs<T>& a, size_t id, T 
 // it (the rows of its 18 neighbours, y wrapped as the neighbours are) has
 // no share to gather and keeps its mass.
 template <class T>
-__device__ __forceinline__ bool donor_rows_near(const VofArgs<T>& a, const int* rows, int j, int k) {
-    const int ny = a.c.ny, nz = a.c.nz;
-    const bool yper = a.c.y_boundary == LBM_YB_PERIODIC;
-    bool any = false;
-#pragma unroll
-    for (int dk = -1; dk <= 1; dk++) {
-        const int kk = k + dk;
-        if (kk < 0 || kk >= nz) continue;
-#pragma unroll
-        for (int dj = -1; dj <= 1; dj++) {
-            int jj = j + dj;
-            if (yper)
-                jj = wrap1(jj, ny);
-            else if (jj < 0 || jj >= ny)
-                continue;
-            any |= rows[(size_t)kk * ny + jj] != 0;
-        }
-    }
-    return any;
-}
-template <class T>
 __device__ __forceinline__ void clear_donor_rows(const VofArgs<T>& a, int slot, int i, int j, int k) {
     if (i == 0) a.rowdon[(size_t)slot * a.c.nz * a.c.ny + (size_t)k * a.c.ny + j] = 0;
 }
@@ -1529,23 +1618,15 @@ __global__ __launch_bounds__(NT) void k_vof_redist_fin(VofArgs<T> a, T empty_thr
     clear_donor_rows(a, slot ^ 1, i, j, k);
     // no donor this round, or none within reach: the mass is unchanged
     const int* rows = a.rowdon + (size_t)slot * a.c.nz * a.c.ny;
-    const bool gather = a.icnt[IC_REDIST_DONORS + slot] != 0 && donor_rows_near(a, rows, j, k);
-    const T m = gather ? redist_gather(a, i, j, k, id) : a.mass[id];
-    finalize_cell(a, id, m, empty_thr, a.excess2);
-}
-
-// finalize_vof_classification_from_mass (CODEX:1577-1624) alone: the cell's
-// own fields only, in place
-template <class T>
-__global__ __launch_bounds__(NT) void k_vof_finalize(VofArgs<T> a, T empty_thr) {
-    VOF_PROLOGUE
-    if (id == 0) {  // the next redistribution round starts without donors
-        a.icnt[IC_REDIST_DONORS] = 0;
-        a.icnt[IC_REDIST_DONORS + 1] = 0;
+    const bool gather = a.icnt[IC_REDIST_DONORS + slot] != 0 && rows_near(a, rows, j, k);
+    // a settled cell with no share: the classification gives excess 0 and
+    // rewrites what the cell holds
+    if (!gather && !band_row(a, j, k, false)) {
+        a.excess2[id] = T(0);
+        return;
     }
-    clear_donor_rows(a, 0, i, j, k);
-    clear_donor_rows(a, 1, i, j, k);
-    finalize_cell(a, id, a.mass[id], empty_thr, a.excess);
+    const T m = gather ? redist_gather(a, i, j, k, id) : a.mass[id];
+    finalize_cell(a, id, j, k, m, empty_thr, a.excess2);
 }
 
 // enforce_vof_interface_layer (CODEX:1627-1656): only LIQUID cells change
@@ -1553,6 +1634,7 @@ __global__ __launch_bounds__(NT) void k_vof_finalize(VofArgs<T> a, T empty_thr) 
 template <class T>
 __global__ __launch_bounds__(NT) void k_vof_enforce(VofArgs<T> a) {
     VOF_PROLOGUE
+    if (!band_row(a, j, k, true)) return;  // a LIQUID-GAS contact has a marked cell
     if (!((a.flags[id] & 1u) && a.type[id] == LBM_FS_LIQUID)) return;
     bool touches = false;
 #pragma unroll
@@ -1565,6 +1647,7 @@ __global__ __launch_bounds__(NT) void k_vof_enforce(VofArgs<T> a) {
         a.type[id] = LBM_FS_INTERFACE;
         a.fill[id] = T(1.0);
         a.mass[id] = a.rho[id];
+        band_mark(a, j, k);
     }
 }
 
@@ -1572,6 +1655,9 @@ __global__ __launch_bounds__(NT) void k_vof_enforce(VofArgs<T> a) {
 template <class T>
 __global__ __launch_bounds__(NT) void k_vof_orphan_a(VofArgs<T> a, T empty_thr, T thr, int rmax, int max_weak) {
     VOF_PROLOGUE
+    // a settled LIQUID cell with no unsettled or GAS neighbour has a
+    // resolved one, unless it has no open neighbour at all (iso)
+    if (!band_row(a, j, k, true) && a.iso[(size_t)k * a.c.ny + j] == 0) return;
     a.si[id] = 0;
     if (!((a.flags[id] & 1u) && a.type[id] != LBM_FS_GAS && a.fill[id] > empty_thr)) return;
     int wet = 0, res = 0;
@@ -1601,6 +1687,7 @@ __global__ __launch_bounds__(NT) void k_vof_orphan_a(VofArgs<T> a, T empty_thr, 
         a.sb[id] = cw;
         a.sc[id] = tw;
         atomicAdd(&a.icnt[IC_VOF_ORPHAN], 1);
+        band_mark(a, j, k);
     }
 }
 
@@ -1608,10 +1695,7 @@ __global__ __launch_bounds__(NT) void k_vof_orphan_a(VofArgs<T> a, T empty_thr, 
 template <class T>
 __global__ __launch_bounds__(NT) void k_vof_orphan_b(VofArgs<T> a, T thr, int rmax) {
     VOF_PROLOGUE
-    if (a.icnt[IC_VOF_ORPHAN] == 0) {  // phase A found no orphan (the common case): nothing moves
-        a.mass2[id] = a.mass[id];
-        return;
-    }
+    if (a.icnt[IC_VOF_ORPHAN] == 0) return;  // phase A found no orphan (the common case): nothing moves
     bool orphan = a.si[id] != 0;
     T acc = orphan ? T(0) : a.mass[id];
     if ((a.flags[id] & 1u) && a.type[id] != LBM_FS_GAS && a.fill[id] >= thr) {
@@ -1639,14 +1723,19 @@ __global__ __launch_bounds__(NT) void k_vof_orphan_b(VofArgs<T> a, T thr, int rm
     a.mass2[id] = acc;
 }
 
-// phase C: the orphans become gas
+// phase C: the gathered masses replace the masses and the orphans become gas
 template <class T>
 __global__ __launch_bounds__(NT) void k_vof_orphan_c(VofArgs<T> a) {
     VOF_PROLOGUE
+    if (a.icnt[IC_VOF_ORPHAN] == 0) return;
+    const T m = a.mass2[id];
+    const bool moved = m != a.mass[id];
+    a.mass[id] = m;
     if (a.si[id]) {
         a.fill[id] = T(0);
         a.type[id] = LBM_FS_GAS;
     }
+    if (moved || a.si[id]) band_mark(a, j, k);
 }
 
 // vof_unresolved_detached_cell (CODEX:1779-1810)
@@ -1668,6 +1757,8 @@ __device__ __constant__ int FD6[6][3] = {{1, 0, 0}, {-1, 0, 0}, {0, 1, 0}, {0, -
 template <class T>
 __global__ __launch_bounds__(NT) void k_vof_detached_a(VofArgs<T> a, int mode, T orphan_thr, int max_wet, int max_res, T resid_thr) {
     VOF_PROLOGUE
+    // a mover has a GAS face neighbour: a settled mover would be a contact
+    if (!band_row(a, j, k, true)) return;
     T mv[6] = {T(0), T(0), T(0), T(0), T(0), T(0)};
     if (vof_detached(a, i, j, k, orphan_thr, max_wet, max_res)) {
         T m = a.mass[id];
@@ -1723,31 +1814,52 @@ __global__ __launch_bounds__(NT) void k_vof_detached_a(VofArgs<T> a, int mode, T
     const bool mover = moved > T(1.0e-12);
     a.si[id] = mover ? 1 : 0;
     if (mover) {
+        band_mark(a, j, k);
         for (int d = 0; d < 6; d++) a.smv[6 * id + d] = mv[d];
         atomicAdd(&a.icnt[mode == 0 ? IC_VOF_ADVECT : IC_VOF_COMPRESS], 1);
         atomicAdd(&a.dcnt[mode == 0 ? DC_ADVECT_MASS : DC_COMPRESS_MASS], (double)moved);
     }
 }
 
-// phase B: apply outgoing then incoming moves (fixed direction order)
+// finalize_vof_classification_from_mass (CODEX:1577-1624) after a detached
+// pass (mode 0 advection, 1 compaction; -1: none ran): its phase B, the
+// moves out of the cell then into it in fixed direction order, then the
+// classification of the cell's own fields, in place (phase B reads only the
+// movers' shares, which this kernel does not write)
 template <class T>
-__global__ __launch_bounds__(NT) void k_vof_detached_b(VofArgs<T> a, int mode) {
+__global__ __launch_bounds__(NT) void k_vof_detached_fin(VofArgs<T> a, T empty_thr, int mode) {
     VOF_PROLOGUE
-    // phase A moved nothing (no cell counted): every share is zero
-    if (a.icnt[mode == 0 ? IC_VOF_ADVECT : IC_VOF_COMPRESS] == 0) return;
-    // the shares of a cell that moved nothing are all zero (not stored)
-    T out = T(0);
-    if (a.si[id])
-        for (int d = 0; d < 6; d++) out -= a.smv[6 * id + d];
-    T in = T(0);
-    for (int d = 0; d < 6; d++) {
-        int si_, sj, sk;
-        if (!NBR(-FD6[d][0], -FD6[d][1], -FD6[d][2], si_, sj, sk)) continue;
-        const size_t sid = CID(sk, sj, si_);
-        if (a.si[sid]) in += a.smv[6 * sid + d];
+    if (id == 0) {  // the next redistribution round starts without donors
+        a.icnt[IC_REDIST_DONORS] = 0;
+        a.icnt[IC_REDIST_DONORS + 1] = 0;
     }
-    T dm = out + in;
-    if (fabs(dm) > T(1.0e-12)) a.mass[id] = a.mass[id] + dm;
+    clear_donor_rows(a, 0, i, j, k);
+    clear_donor_rows(a, 1, i, j, k);
+    // phase A moved something (a cell counted): movers are marked and their
+    // receivers are face neighbours
+    const bool moved = mode >= 0 && a.icnt[mode == 0 ? IC_VOF_ADVECT : IC_VOF_COMPRESS] != 0;
+    // a settled cell with no mover near: its excess (in place) is already zero
+    if (!band_row(a, j, k, moved)) return;
+    T m = a.mass[id];
+    if (moved) {
+        // the shares of a cell that moved nothing are all zero (not stored)
+        T out = T(0);
+        if (a.si[id])
+            for (int d = 0; d < 6; d++) out -= a.smv[6 * id + d];
+        T in = T(0);
+        for (int d = 0; d < 6; d++) {
+            int si_, sj, sk;
+            if (!NBR(-FD6[d][0], -FD6[d][1], -FD6[d][2], si_, sj, sk)) continue;
+            const size_t sid = CID(sk, sj, si_);
+            if (a.si[sid]) in += a.smv[6 * sid + d];
+        }
+        T dm = out + in;
+        if (fabs(dm) > T(1.0e-12)) {
+            m = m + dm;
+            band_mark(a, j, k);
+        }
+    }
+    finalize_cell(a, id, j, k, m, empty_thr, a.excess);
 }
 
 // vof_gas_fraction_at (CODEX:1996-2009)
@@ -1847,8 +1959,9 @@ __global__ __launch_bounds__(RED_THREADS) void k_collapse_decide(const double* p
     }
 }
 template <class T>
-__global__ __launch_bounds__(NT) void k_collapse_apply(VofArgs<T> a) {
+__global__ __launch_bounds__(NT) void k_collapse_apply(VofArgs<T> a, int nflags) {
     VOF_PROLOGUE
+    if (id < (size_t)nflags) a.si[id] = 0;  // the flood fill's sweep flags (kept in si)
     T scale = (T)a.dcnt[DC_REMOVAL_SCALE];
     if (!(scale > T(0))) return;
     T g = gas_fraction(a, id);
@@ -1860,6 +1973,7 @@ __global__ __launch_bounds__(NT) void k_collapse_apply(VofArgs<T> a) {
         a.fill[id] = T(1.0);
         a.type[id] = LBM_FS_LIQUID;
         atomicAdd(&a.icnt[IC_VOF_COLLAPSE_CELLS], 1);
+        band_mark(a, j, k);
     } else if (a.type[id] != LBM_FS_GAS) {
         T m = a.mass[id] * (T(1.0) - scale);
         if (m < T(0)) m = T(0);
@@ -1877,6 +1991,7 @@ __global__ __launch_bounds__(NT) void k_collapse_apply(VofArgs<T> a) {
         a.mass[id] = m;
         a.fill[id] = fl;
         a.type[id] = ct;
+        band_mark(a, j, k);
     }
 }
 
@@ -1895,7 +2010,7 @@ __global__ __launch_bounds__(NT) void k_vof_height(Consts<T> c, Slot<T> s, const
         // the bed alone sets the centre fraction: cell_center_fraction with
         // the column's bed height read once
         const T eta_c = s.eta9[4 * plane + col];
-#pragma unroll 16
+#pragma unroll 32
         for (int k = 0; k < c.nz; k++) {
             const size_t id = (size_t)k * plane + col;
             const bool open = flags[id] & 1u;
@@ -2041,6 +2156,7 @@ __global__ __launch_bounds__(NT) void k_fs_advect(Consts<T> c, const T* qx, cons
 template <class S, class T>
 __global__ __launch_bounds__(NT) void k_vof_refill(VofArgs<T> a, S* f, uint32_t step) {
     VOF_PROLOGUE
+    if (!band_row(a, j, k, false)) return;  // the cell's type changed this step: marked
     if (!(a.type_prev[id] == LBM_FS_GAS && a.type[id] != LBM_FS_GAS && (a.flags[id] & 1u))) return;
     const Consts<T>& c = a.c;
     T ws = 0, fws = 0, ra = 0, xa = 0, ya = 0, za = 0, rf = 0, xf = 0, yf = 0, zf = 0;
@@ -2163,6 +2279,7 @@ __global__ __launch_bounds__(NT) void k_vof_repair(VofArgs<T> a, S* f) {
     // a fresh source is a cell the refill just filled (the refill's own
     // condition), so a step that refilled nothing repairs nothing
     if (a.icnt[IC_FS_REFILL] == 0) return;
+    if (!band_row(a, j, k, true)) return;  // next to a refilled (marked) cell
     if (!((a.flags[id] & 1u) && a.type[id] != LBM_FS_GAS && a.type_prev[id] != LBM_FS_GAS)) return;
     bool rep = false;
     int cnt = 0;
@@ -3530,6 +3647,10 @@ static VofArgs<T> make_vof(const lbm_config_t& cfg, DevBufs& b) {
     a.si = b.si;
     a.mdirty = b.mdirty;
     a.rowdon = b.rowdon;
+    a.act = b.rowact;
+    a.act_prev = b.rowact + (size_t)cfg.nz * cfg.ny;
+    a.iso = b.rowact + 2 * (size_t)cfg.nz * cfg.ny;
+    a.band = 0;
     a.flags = b.cellflags;
     a.icnt = b.icnt;
     a.fs_h = (const T*)b.fs_h;
@@ -3617,31 +3738,26 @@ void launch_init_populations(const lbm_config_t& c, DevBufs& b, int slot, hipStr
     }
 }
 
-// the per-step counters of the VOF pipeline, cleared by one launch at its
-// start (each is only added to after its clear point in the step)
-__global__ void k_vof_counters_reset(int* icnt, double* dcnt) {
-    const int t = (int)threadIdx.x;
-    if (t == 0) {
-        icnt[IC_VOF_EXCESS] = 0;
-        icnt[IC_VOF_ORPHAN] = 0;
-        icnt[IC_VOF_ADVECT] = 0;
-        icnt[IC_VOF_COMPRESS] = 0;
-        icnt[IC_VOF_COLLAPSE_CELLS] = 0;
-        icnt[IC_FS_REFILL] = 0;
-        icnt[IC_FS_REPAIR] = 0;
-    }
-    if (t < 2) dcnt[DC_ADVECT_MASS + t] = 0.0;  // advected and compressed mass
-    if (t < 5) dcnt[DC_CAND_MASS + t] = 0.0;    // the collapse sums
-}
-
+// band: the passes skip the rows the narrow band shows settled (see
+// band_mark); the marks of this step go to row-flag slot st.act_slot
 template <class S, class T>
-static void vof_pipeline(const lbm_config_t& cfg, DevBufs& b, S* f, int slot, int64_t step, hipStream_t s) {
+static void vof_pipeline(const lbm_config_t& cfg, DevBufs& b, S* f, int slot, int64_t step, bool band, StagedState& st,
+                         hipStream_t s) {
     const size_t n = b.n;
     const unsigned nb = nblocks(n);
-    auto A = [&]() { return make_vof<T>(cfg, b); };
-    auto swap_mass = [&]() { std::swap(b.mass, b.mass2); };
+    const size_t rows = (size_t)cfg.nz * cfg.ny;
+    int* act = b.rowact + (size_t)st.act_slot * rows;
+    const int* act_prev = b.rowact + (size_t)(st.act_slot ^ 1) * rows;
+    st.act_slot ^= 1;
+    st.band_ok = 1;  // this step's marks will cover its end state
+    auto A = [&]() {
+        VofArgs<T> a = make_vof<T>(cfg, b);
+        a.act = act;
+        a.act_prev = act_prev;
+        a.band = band ? 1 : 0;
+        return a;
+    };
     T empty_thr = (T)cfg.vof_empty_fill_threshold;
-    k_vof_counters_reset<<<1, 64, 0, s>>>(b.icnt, b.dcnt);
     k_vof_exchange<S, T><<<nb, NT, 0, s>>>(A(), f);
     k_vof_preclassify<T><<<nb, NT, 0, s>>>(A(), empty_thr);
     // a redistribution round and the classification that follows it; the
@@ -3654,13 +3770,11 @@ static void vof_pipeline(const lbm_config_t& cfg, DevBufs& b, S* f, int slot, in
         std::swap(b.excess, b.excess2);
         rslot ^= 1;
     };
-    auto finalize = [&]() { k_vof_finalize<T><<<nb, NT, 0, s>>>(A(), empty_thr); };
     for (int r = 0; r < 3; r++) redistribute_finalize();
     k_vof_enforce<T><<<nb, NT, 0, s>>>(A());
     T othr = (T)cfg.vof_orphan_empty_threshold;
     k_vof_orphan_a<T><<<nb, NT, 0, s>>>(A(), empty_thr, othr, cfg.vof_orphan_search_radius, cfg.vof_orphan_max_weak_neighbors);
     k_vof_orphan_b<T><<<nb, NT, 0, s>>>(A(), othr, cfg.vof_orphan_search_radius);
-    swap_mass();
     k_vof_orphan_c<T><<<nb, NT, 0, s>>>(A());
     for (int mode = 0; mode < 2; mode++) {
         bool on = cfg.vof_detached_advection && (mode == 0 || cfg.vof_detached_residual_fill_threshold > 0.0f);
@@ -3668,9 +3782,8 @@ static void vof_pipeline(const lbm_config_t& cfg, DevBufs& b, S* f, int slot, in
             k_vof_detached_a<T><<<nb, NT, 0, s>>>(A(), mode, othr, cfg.vof_detached_max_wet_neighbors,
                                                   cfg.vof_detached_max_resolved_neighbors,
                                                   (T)cfg.vof_detached_residual_fill_threshold);
-            k_vof_detached_b<T><<<nb, NT, 0, s>>>(A(), mode);
         }
-        finalize();
+        k_vof_detached_fin<T><<<nb, NT, 0, s>>>(A(), empty_thr, on ? mode : -1);
     }
     redistribute_finalize();
     k_vof_enforce<T><<<nb, NT, 0, s>>>(A());
@@ -3692,9 +3805,11 @@ static void vof_pipeline(const lbm_config_t& cfg, DevBufs& b, S* f, int slot, in
         const unsigned nch = (unsigned)reduce_chunks(n);
         k_collapse_sum<T><<<nch, RED_THREADS, 0, s>>>(A(), b.rpart);
         k_collapse_decide<<<1, RED_THREADS, 0, s>>>(b.rpart, (int)nch, b.dcnt, (double)cfg.vof_collapse_max_volume_m3);
-        k_collapse_apply<T><<<nb, NT, 0, s>>>(A());
+        k_collapse_apply<T><<<nb, NT, 0, s>>>(A(), sweeps + 1);
+        // (without a collapse the layer enforced just above holds: no LIQUID
+        // cell touches a GAS one, so a second pass would change nothing)
+        k_vof_enforce<T><<<nb, NT, 0, s>>>(A());
     }
-    k_vof_enforce<T><<<nb, NT, 0, s>>>(A());
     {
         Consts<T> c = make_consts<T>(cfg, -1);
         k_vof_height<T><<<nblocks(b.n2, 64), 64, 0, s>>>(c, make_slot<T>(b, slot), (const T*)b.fill, b.cellflags, (T*)b.fs_h);
@@ -3755,7 +3870,11 @@ static void staged_t(const lbm_config_t& cfg, DevBufs& b, int& cur, int slot, in
     if (cfg.fs_enabled) {
         S* f2 = (S*)b.f[cur];
         if (cfg.fs_tracking == LBM_FST_VOF) {
-            vof_pipeline<S, T>(cfg, b, f2, slot, step, s);
+            // the band needs the marks of a whole previous pipeline and a
+            // static geometry (cells that open or close are not marked)
+            // (and a LIQUID cell, fill 1, counts as resolved in the orphan test)
+            const bool band = st.band_allowed && st.band_ok && static_geo && cfg.vof_orphan_empty_threshold <= 1.0f;
+            vof_pipeline<S, T>(cfg, b, f2, slot, step, band, st, s);
             st.macro_ready = vof_macro;
         } else {
             // height-function tracking (CODEX:3917-3921)

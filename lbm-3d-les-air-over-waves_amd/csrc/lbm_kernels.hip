@@ -1242,10 +1242,10 @@ __device__ __forceinline__ void wet_resolved(const VofArgs<T>& a, int i, int j, 
     int i, j, k;                                                          \
     cell_ijk(a.c, id, i, j, k);
 
-// Narrow band.  A cell is settled when it is LIQUID or GAS, holds what
-// the classification of its own mass gives (mass, fill, type), with no
-// excess and no excess direction, and no pass of this step has changed it
-// otherwise.  Every pass that leaves a cell unsettled, changes its type, or
+// Narrow band.  A cell is settled when it is LIQUID or GAS (with no mass),
+// holds what the classification of its own mass gives (mass, fill, type),
+// with no excess and no excess direction, and no pass of this step has
+// changed it otherwise.  Every pass that leaves a cell unsettled, changes its type, or
 // changes its mass, fill or type outside a classification marks the cell's
 // row (k, j) in act; the preclassification, which classifies every cell,
 // marks from scratch each step.  For a settled cell the passes below are
@@ -1355,12 +1355,23 @@ __global__ __launch_bounds__(NT) void k_vof_preclassify(VofArgs<T> a, T empty_th
     clear_donor_rows(a, 0, i, j, k);
     clear_donor_rows(a, 1, i, j, k);
     int old = a.type[id];
+    // a row the last step left unmarked holds settled cells only, LIQUID or
+    // GAS: type_prev, the excess and its direction, si, a GAS cell's mass and
+    // fill and a LIQUID cell's fill already hold what this pass writes (the
+    // type did not change since the last preclassification, and no excess
+    // was left); a LIQUID cell's mass follows its density
+    if (a.band && a.act_prev[(size_t)k * a.c.ny + j] == 0) {
+        if (old == LBM_FS_LIQUID) {
+            T rr = a.rho[id];
+            if (rr < T(1.0e-8)) rr = a.c.rho0;
+            a.mass[id] = rr;
+        }
+        return;
+    }
     a.type_prev[id] = old;
     a.excess[id] = T(0);
     a.excess_dir[id] = 0;
-    // si is nonzero only at cells the last step marked (its flood-fill flags
-    // are cleared by the collapse)
-    if (!a.band || a.act_prev[(size_t)k * a.c.ny + j] != 0) a.si[id] = 0;
+    a.si[id] = 0;  // nonzero only at cells the last step marked (its flood-fill flags are cleared by the collapse)
     if (!(a.flags[id] & 1u)) {
         a.mass[id] = T(0);
         a.fill[id] = T(0);
@@ -1593,7 +1604,9 @@ __device__ __forceinline__ void finalize_cell(const VofArgs<T>& a, size_t id, in
     a.fill[id] = fl;
     a.type[id] = ct;
     if (cnt) atomicAdd(&a.icnt[IC_VOF_EXCESS], cnt);
-    if (ct == LBM_FS_INTERFACE || ct != old || cnt) band_mark(a, j, k);
+    // (a GAS cell that kept a sliver of mass is not settled: the next
+    // preclassification zeroes it)
+    if (ct == LBM_FS_INTERFACE || ct != old || cnt || (ct == LBM_FS_GAS && m != T(0))) band_mark(a, j, k);
 }
 
 // one redistribution round's phase B followed by the classification
@@ -1890,7 +1903,16 @@ __global__ __launch_bounds__(NT) void k_flood_seed(VofArgs<T> a) {
 template <class T>
 __global__ __launch_bounds__(NT) void k_flood_sweep(VofArgs<T> a, int* changed_prev, int* changed_this) {
     if (*changed_prev == 0) return;
-    VOF_PROLOGUE
+    // cells in decreasing index order: the seeds are the top plane, and the
+    // marks of the blocks that finished earlier in the sweep (higher planes)
+    // carry on down within the same sweep (the fixed point does not depend
+    // on the order)
+    const size_t n = (size_t)a.c.nx * a.c.ny * a.c.nz;
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const size_t id = n - 1 - t;
+    int i, j, k;
+    cell_ijk(a.c, id, i, j, k);
     if (a.ext_gas[id] != 0 || !(gas_fraction(a, id) > T(1.0e-6))) return;
     bool conn = false;
 #pragma unroll
@@ -3800,7 +3822,20 @@ static void vof_pipeline(const lbm_config_t& cfg, DevBufs& b, S* f, int slot, in
         int one = 1;
         HIP_OK(hipMemcpyAsync(flagbuf, &one, sizeof(int), hipMemcpyHostToDevice, s));
         HIP_OK(hipMemsetAsync(flagbuf + 1, 0, sweeps * sizeof(int), s));
-        for (int w = 0; w < sweeps; w++) k_flood_sweep<T><<<nb, NT, 0, s>>>(A(), flagbuf + w, flagbuf + w + 1);
+        // in batches of 16 sweeps; after each, the host reads the last
+        // sweep's flag and stops at the fixed point (a sweep that finds
+        // nothing to change returns at once, but thousands of launches
+        // would still cost milliseconds)
+        for (int w = 0; w < sweeps;) {
+            const int w1 = std::min(sweeps, w + 16);
+            for (; w < w1; w++) k_flood_sweep<T><<<nb, NT, 0, s>>>(A(), flagbuf + w, flagbuf + w + 1);
+            if (w < sweeps) {
+                int changed = 1;
+                HIP_OK(hipMemcpyAsync(&changed, flagbuf + w, sizeof(int), hipMemcpyDeviceToHost, s));
+                HIP_OK(hipStreamSynchronize(s));
+                if (!changed) break;
+            }
+        }
         HIP_OK(hipMemcpyAsync(b.icnt + IC_FLOOD_UNCONVERGED, flagbuf + sweeps, sizeof(int), hipMemcpyDeviceToDevice, s));
         const unsigned nch = (unsigned)reduce_chunks(n);
         k_collapse_sum<T><<<nch, RED_THREADS, 0, s>>>(A(), b.rpart);

@@ -1326,19 +1326,31 @@ __global__ __launch_bounds__(NT) void k_vof_exchange(VofArgs<T> a, const S* f) {
     if (!((a.flags[id] & 1u) && a.type[id] == LBM_FS_INTERFACE)) return;
     T dm = T(0);
     const T fill_c = a.fill[id];
+    // links in two halves of 9: the loads of a half are all issued before
+    // its sums (in q order)
 #pragma unroll
-    for (int q = 1; q < 19; q++) {
-        bool v;
-        const size_t nid = NBR_ID(cxq(q), cyq(q), czq(q), v);
-        const bool open = (a.flags[nid] & 1u) != 0;
-        const int ty = a.type[nid];
-        const T fn = a.fill[nid];
-        const T fa = Store<S, T>::load(f, (size_t)oppq(q) * n + id, oppq(q), a.c.store);
-        const T fb = Store<S, T>::load(f, (size_t)q * n + nid, q, a.c.store);
-        if (v & open & (ty != LBM_FS_GAS)) {
-            T lw = T(1.0);
-            if (ty == LBM_FS_INTERFACE) lw = T(0.5) * (fill_c + fn);
-            if (lw > T(0)) dm += lw * (fa - fb);
+    for (int h = 0; h < 2; h++) {
+        bool wet[9];
+        int ty[9];
+        T fn[9], fa[9], fb[9];
+#pragma unroll
+        for (int e = 0; e < 9; e++) {
+            const int q = 1 + 9 * h + e;
+            bool v;
+            const size_t nid = NBR_ID(cxq(q), cyq(q), czq(q), v);
+            wet[e] = v & ((a.flags[nid] & 1u) != 0);
+            ty[e] = a.type[nid];
+            fn[e] = a.fill[nid];
+            fa[e] = Store<S, T>::load(f, (size_t)oppq(q) * n + id, oppq(q), a.c.store);
+            fb[e] = Store<S, T>::load(f, (size_t)q * n + nid, q, a.c.store);
+        }
+#pragma unroll
+        for (int e = 0; e < 9; e++) {
+            if (wet[e] & (ty[e] != LBM_FS_GAS)) {
+                T lw = T(1.0);
+                if (ty[e] == LBM_FS_INTERFACE) lw = T(0.5) * (fill_c + fn[e]);
+                if (lw > T(0)) dm += lw * (fa[e] - fb[e]);
+            }
         }
     }
     a.mdelta[id] = dm;
@@ -2017,45 +2029,78 @@ __global__ __launch_bounds__(NT) void k_collapse_apply(VofArgs<T> a, int nflags)
     }
 }
 
-// update_free_surface_height_from_vof (CODEX:2146-2161), one thread per column
+// update_free_surface_height_from_vof (CODEX:2146-2161): the height of a
+// column is the bed height plus the sum over k, in k order, of
+// fill * centre fraction * dx over its open cells.  A block takes HC
+// columns: its threads stage the terms of HK planes at a time in LDS (every
+// load of the chunk in flight at once, and the centre fractions computed in
+// parallel; a closed cell's term is +0, which leaves the sum unchanged: it
+// starts at +0 and every term is >= 0), then one thread per column adds
+// them in k order, 16 LDS reads ahead of the adds.
+constexpr int HC = 32;    // columns per block
+constexpr int HK = 192;   // planes per staged chunk
+constexpr int HT = 256;   // threads per block
 template <class T>
-__global__ __launch_bounds__(NT) void k_vof_height(Consts<T> c, Slot<T> s, const T* fill, const uint8_t* flags, T* fs_h) {
-    size_t col = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    size_t plane = (size_t)c.nx * c.ny;
-    if (col >= plane) return;
-    int i = (int)(col % (size_t)c.nx), j = (int)(col / (size_t)c.nx);
-    // loads of every plane are unconditional (a closed cell adds nothing:
-    // the select keeps depth unchanged), so the unrolled loop keeps several
-    // planes in flight instead of one dependent load pair per plane
+__global__ __launch_bounds__(HT) void k_vof_height(Consts<T> c, Slot<T> s, const T* fill, const uint8_t* flags, T* fs_h) {
+    __shared__ T add_s[HK][HC];
+    const size_t plane = (size_t)c.nx * c.ny;
+    const int lc = (int)threadIdx.x % HC, lr = (int)threadIdx.x / HC;
+    const size_t col = (size_t)blockIdx.x * HC + lc;
+    const bool colok = col < plane;
+    const int i = colok ? (int)(col % (size_t)c.nx) : 0, j = colok ? (int)(col / (size_t)c.nx) : 0;
+    // the bed alone sets the centre fraction: cell_center_fraction with the
+    // column's bed height read once
+    const bool bed_only = !c.include_cube && !c.phi_fraction;
+    const T eta_c = (bed_only && colok) ? s.eta9[4 * plane + col] : T(0);
     T depth = T(0);
-    if (!c.include_cube && !c.phi_fraction) {
-        // the bed alone sets the centre fraction: cell_center_fraction with
-        // the column's bed height read once
-        const T eta_c = s.eta9[4 * plane + col];
-#pragma unroll 32
-        for (int k = 0; k < c.nz; k++) {
-            const size_t id = (size_t)k * plane + col;
-            const bool open = flags[id] & 1u;
-            const T fl = fill[id];
-            const T zc = ((T)k + T(0.5)) * c.dx;
-            const T gap = c.include_bed ? zc - eta_c : T(1.0e6);
-            T frac = T(0.5) + div_dx(c, gap);
-            if (frac < T(0)) frac = T(0);
-            if (frac > T(1)) frac = T(1);
-            const T add = fl * frac * c.dx;
-            depth = open ? depth + add : depth;
+    for (int k0 = 0; k0 < c.nz; k0 += HK) {
+        const int kn = c.nz - k0 < HK ? c.nz - k0 : HK;
+        // every load of the chunk issued before any term (planes beyond the
+        // grid and columns beyond the plane read a clamped in-grid cell;
+        // their terms are not read back)
+        constexpr int E = HK / (HT / HC);
+        T fl[E];
+        bool op[E];
+#pragma unroll
+        for (int e = 0; e < E; e++) {
+            const int kk = lr + e * (HT / HC);
+            const int k = k0 + kk < c.nz ? k0 + kk : c.nz - 1;
+            const size_t id = (size_t)k * plane + (colok ? col : 0);
+            op[e] = flags[id] & 1u;
+            fl[e] = fill[id];
         }
-    } else {
-        for (int k = 0; k < c.nz; k++) {
-            const size_t id = (size_t)k * plane + col;
-            const bool open = flags[id] & 1u;
-            const T fl = fill[id];
-            bool by_cube;
-            const T add = fl * cell_center_fraction(c, s, i, j, k, by_cube) * c.dx;
-            depth = open ? depth + add : depth;
+#pragma unroll
+        for (int e = 0; e < E; e++) {
+            const int kk = lr + e * (HT / HC);
+            const int k = k0 + kk < c.nz ? k0 + kk : c.nz - 1;
+            T frac;
+            if (bed_only) {
+                const T zc = ((T)k + T(0.5)) * c.dx;
+                const T gap = c.include_bed ? zc - eta_c : T(1.0e6);
+                frac = T(0.5) + div_dx(c, gap);
+                if (frac < T(0)) frac = T(0);
+                if (frac > T(1)) frac = T(1);
+            } else {
+                bool by_cube;
+                frac = cell_center_fraction(c, s, i, j, k, by_cube);
+            }
+            add_s[kk][lc] = op[e] ? fl[e] * frac * c.dx : T(0);
         }
+        __syncthreads();
+        if (lr == 0) {
+            int kk = 0;
+            for (; kk + 16 <= kn; kk += 16) {
+                T v[16];
+#pragma unroll
+                for (int e = 0; e < 16; e++) v[e] = add_s[kk + e][lc];
+#pragma unroll
+                for (int e = 0; e < 16; e++) depth = depth + v[e];
+            }
+            for (; kk < kn; kk++) depth = depth + add_s[kk][lc];
+        }
+        __syncthreads();
     }
-    fs_h[col] = s.h[col] + depth;
+    if (lr == 0 && colok) fs_h[col] = s.h[col] + depth;
 }
 
 // ---- height-function free-surface tracking (CODEX:1209-1232, 2376-2429) ----
@@ -3747,7 +3792,7 @@ static void init_t(const lbm_config_t& cfg, DevBufs& b, int slot, hipStream_t s)
         VofArgs<T> a = make_vof<T>(cfg, b);
         k_vof_init_mass<T><<<nblocks(b.n), NT, 0, s>>>(a);
         k_vof_enforce<T><<<nblocks(b.n), NT, 0, s>>>(a);
-        k_vof_height<T><<<nblocks(b.n2, 64), 64, 0, s>>>(c, sl, (const T*)b.fill, b.cellflags, (T*)b.fs_h);
+        k_vof_height<T><<<(unsigned)((b.n2 + HC - 1) / HC), HT, 0, s>>>(c, sl, (const T*)b.fill, b.cellflags, (T*)b.fs_h);
     }
     HIP_OK(hipGetLastError());
 }
@@ -3847,7 +3892,7 @@ static void vof_pipeline(const lbm_config_t& cfg, DevBufs& b, S* f, int slot, in
     }
     {
         Consts<T> c = make_consts<T>(cfg, -1);
-        k_vof_height<T><<<nblocks(b.n2, 64), 64, 0, s>>>(c, make_slot<T>(b, slot), (const T*)b.fill, b.cellflags, (T*)b.fs_h);
+        k_vof_height<T><<<(unsigned)((b.n2 + HC - 1) / HC), HT, 0, s>>>(c, make_slot<T>(b, slot), (const T*)b.fill, b.cellflags, (T*)b.fs_h);
     }
     k_vof_refill<S, T><<<nb, NT, 0, s>>>(A(), f, (uint32_t)step);
     k_vof_repair<S, T><<<nb, NT, 0, s>>>(A(), f);

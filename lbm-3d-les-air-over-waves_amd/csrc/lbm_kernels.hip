@@ -717,6 +717,14 @@ __global__ __launch_bounds__(NT) void k_macro_fix(Consts<T> c, const S* f, T* rh
     uz[id] = w;
 }
 
+// waves per SIMD the staged stream is compiled for: 4 with FP32 arithmetic
+// (128 VGPRs; 129 would leave 3), no bound with FP64 (which would spill).
+// (The staged collision at 4 spills and runs slower.)
+template <class T>
+struct StagedOcc {
+    static constexpr int v = sizeof(T) == 4 ? 4 : 1;
+};
+
 // compute_LES (CODEX:3011-3141) of one active cell: omega, nu_t and the
 // Smagorinsky term from the velocities of its six neighbours
 template <class T>
@@ -1105,7 +1113,7 @@ __device__ __forceinline__ void stream_bc_cell(const Consts<T>& c, const Slot<T>
 // or the storage type (the staged step, whose 16-bit stores round with the
 // dither of step ^ SALT_STREAM; the macro step then sees the stored values)
 template <class S, class D, class T, bool COUNT, bool MACRO>
-__global__ __launch_bounds__(NT) void k_stream_bc(Consts<T> c, Slot<T> s, const S* fin, D* fout, const T* ux, const T* uy, const T* uz,
+__global__ __launch_bounds__(NT, StagedOcc<T>::v) void k_stream_bc(Consts<T> c, Slot<T> s, const S* fin, D* fout, const T* ux, const T* uy, const T* uz,
                             const uint8_t* tstate, const int* type, const T* fill, const T* u_inlet, const T* inflow, const T* fs_h,
                             const uint8_t* flags, int* icnt, T* mrho, T* mux, T* muy, T* muz, uint32_t step) {
     size_t id = (size_t)blockIdx.x * blockDim.x + threadIdx.x;

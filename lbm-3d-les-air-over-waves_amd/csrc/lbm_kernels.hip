@@ -781,12 +781,21 @@ __global__ __launch_bounds__(NT) void k_collide(Consts<T> c, Slot<T> s, S* f, co
     if (id >= n) return;
     int i, j, k;
     cell_ijk(c, id, i, j, k);
-    T r = rho[id], u = ux[id], v = uy[id], w = uz[id];
-    bool open0 = flags[id] & 1u;
-    bool gas = false;
+    const uint8_t fl0 = flags[id];
+    bool open0 = fl0 & 1u;
     int ty = LBM_FS_LIQUID;
     if (c.fs_water) {
         ty = type[id];
+        // an open GAS cell with every link open: the stream that follows
+        // gives it the rest equilibrium and takes the free-surface closure
+        // for the links from it, and reads its own post-collision values
+        // only for the wall correction of a cell next to a wall (near, bit 1),
+        // so nothing reads what its collision would store: skipped
+        if (ty == LBM_FS_GAS && open0 && !(fl0 & 2u)) return;
+    }
+    T r = rho[id], u = ux[id], v = uy[id], w = uz[id];
+    bool gas = false;
+    if (c.fs_water) {
         if (ty == LBM_FS_GAS) {
             gas = true;
             r = c.rho0;

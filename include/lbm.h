@@ -348,12 +348,18 @@ int lbm_device_bytes(const lbm_config_t *cfg, size_t *out);
    "graph" (0/1: on the fused path, an lbm_step call of several steps is
    captured into a HIP graph and launched as one graph; the executable
    graph is kept and updated by the next call; ignored with "timing"),
-   "wavefront", "wave_lag", "xcd_strips", "resident" (the fused step's
-   schedule),
+   "wavefront", "wave_lag", "xcd_strips" (the fused step's wavefront
+   schedule), "resident" (0: the wavefront schedule, 1: the tile-resident
+   grid, 2, the default: the faster of the two, timed on the first
+   lbm_step call of at least 18 steps, which then waits for the device),
    "vof_band" (0/1, default 1: the VOF passes of the staged step skip the
    rows of the grid the step's own change marks show settled; exact, the
    results are bit-identical with 0). */
 int lbm_set_option(lbm_handle_t *h, const char *key, int64_t value);
+
+/* Current value of an lbm_set_option knob ("resident": 2 while the choice
+   by timing is pending, then the schedule chosen; no counterpart). */
+int lbm_get_option(lbm_handle_t *h, const char *key, int64_t *out);
 
 /* Resolved execution path of the handle: LBM_PATH_STAGED or LBM_PATH_FUSED. */
 int lbm_get_path(lbm_handle_t *h, int32_t *out);

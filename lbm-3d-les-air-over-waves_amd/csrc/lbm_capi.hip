@@ -49,6 +49,10 @@ struct lbm_handle {
     FusedSched sched;  // fused path: wavefront (one launch per step) or three launches
     StagedState staged;  // staged path: what a step may reuse from the previous one
     int vof_band = 1;    // option vof_band: VOF passes skip the rows settled far from the interface
+    // option "resident" 2 (the default): the schedule of the fused step is
+    // chosen by timing both on the handle's first long enough lbm_step call
+    // (1 pending, 0 chosen or set explicitly)
+    int sched_auto = 1;
     // fused path, option "graph": the launches of an lbm_step call are
     // captured into a HIP graph and launched as one; the executable graph is
     // kept and updated with the next call's launches (same kernels, new
@@ -572,11 +576,56 @@ static void graph_steps(lbm_handle* h, int32_t nsteps) {
     if (e != hipSuccess) throw std::runtime_error(std::string("graph launch: ") + hipGetErrorString(e));
 }
 
+// The fused step's two schedules, the wavefront (both passes in one
+// launch, tiles in block order) and the tile-resident grid, give the same
+// state bit for bit; which is faster depends on the grid and the collision
+// model (neither wins everywhere).  On the first lbm_step call of at least
+// AUTO_MIN steps, after one untimed step of each, AUTO_RUN steps of each are
+// timed twice with HIP events on the handle's stream, and the faster is kept
+// (the call waits for the device meanwhile).
+constexpr int AUTO_RUN = 4;
+constexpr int AUTO_MIN = 2 + 4 * AUTO_RUN;
+static void auto_schedule(lbm_handle* h, int32_t& nsteps) {
+    h->sched_auto = 0;
+    hipEvent_t e[2] = {nullptr, nullptr};
+    float best[2] = {1e30f, 1e30f};
+    try {
+        CHECK(hipEventCreate(&e[0]));
+        CHECK(hipEventCreate(&e[1]));
+        auto run = [&](int res, int ns) {
+            h->sched.resident = res;
+            CHECK(hipEventRecord(e[0], h->stream));
+            enqueue_steps(h, ns);
+            CHECK(hipEventRecord(e[1], h->stream));
+            CHECK(hipEventSynchronize(e[1]));
+            float ms = 0.0f;
+            CHECK(hipEventElapsedTime(&ms, e[0], e[1]));
+            return ms;
+        };
+        run(0, 1);
+        run(1, 1);
+        for (int r = 0; r < 2; r++)
+            for (int res = 0; res < 2; res++) best[res] = std::min(best[res], run(res, AUTO_RUN));
+    } catch (...) {
+        h->sched.resident = 0;
+        for (hipEvent_t x : e)
+            if (x) (void)hipEventDestroy(x);
+        throw;
+    }
+    for (hipEvent_t x : e) (void)hipEventDestroy(x);
+    h->sched.resident = best[1] < best[0] ? 1 : 0;
+    nsteps -= AUTO_MIN;
+}
+
 int lbm_step(lbm_handle_t* h, int32_t nsteps) {
     API_BEGIN(h)
     if (!h->initialized) return fail(h, LBM_E_STATE, "lbm_init_populations (or an upload of F) must run first");
     if (nsteps < 0) return fail(h, LBM_E_ARG, "nsteps must be >= 0");
     if (nsteps == 0) return LBM_OK;
+    if (h->sched_auto && h->path == LBM_PATH_FUSED && h->cfg.storage != LBM_STORE_FP64 && !h->timing && nsteps >= AUTO_MIN) {
+        auto_schedule(h, nsteps);
+        if (nsteps == 0) return LBM_OK;
+    }
     size_t need = h->timing ? (size_t)2 * nsteps + 2 : 0;
     while (h->ev.size() < need) {
         hipEvent_t e;
@@ -1094,7 +1143,9 @@ int lbm_set_option(lbm_handle_t* h, const char* key, int64_t value) {
         return LBM_OK;
     }
     if (!strcmp(key, "resident")) {
-        h->sched.resident = value ? 1 : 0;
+        if (value < 0 || value > 2) return fail(h, LBM_E_ARG, "resident must be 0, 1 or 2 (chosen by timing)");
+        h->sched.resident = value == 1 ? 1 : 0;
+        h->sched_auto = value == 2 ? 1 : 0;
         return LBM_OK;
     }
     if (!strcmp(key, "wave_lag")) {
@@ -1116,6 +1167,20 @@ int lbm_set_option(lbm_handle_t* h, const char* key, int64_t value) {
         return LBM_OK;
     }
     return fail(h, LBM_E_ARG, std::string("unknown option ") + key);
+}
+
+int lbm_get_option(lbm_handle_t* h, const char* key, int64_t* out) {
+    if (!h || !key || !out) return LBM_E_ARG;
+    if (!strcmp(key, "resident")) *out = h->sched_auto ? 2 : h->sched.resident;
+    else if (!strcmp(key, "timing")) *out = h->timing;
+    else if (!strcmp(key, "graph")) *out = h->graph;
+    else if (!strcmp(key, "wavefront")) *out = h->sched.wavefront;
+    else if (!strcmp(key, "wave_lag")) *out = h->sched.lag;
+    else if (!strcmp(key, "xcd_strips")) *out = h->sched.strips;
+    else if (!strcmp(key, "block_x")) *out = h->block_x;
+    else if (!strcmp(key, "vof_band")) *out = h->vof_band;
+    else return fail(h, LBM_E_ARG, std::string("unknown option ") + key);
+    return LBM_OK;
 }
 
 int lbm_get_path(lbm_handle_t* h, int32_t* out) {

@@ -39,6 +39,7 @@ EXPORTS = [
     "lbm_last_step_timing",
     "lbm_device_bytes",
     "lbm_set_option",
+    "lbm_get_option",
     "lbm_get_path",
     "lbm_query",
     "lbm_server_create",
@@ -84,6 +85,7 @@ def load_library(path: Optional[os.PathLike] = None) -> ctypes.CDLL:
     lib.lbm_last_step_timing.argtypes = [H, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]
     lib.lbm_device_bytes.argtypes = [ctypes.POINTER(abi.LbmConfig), ctypes.POINTER(ctypes.c_size_t)]
     lib.lbm_set_option.argtypes = [H, ctypes.c_char_p, ctypes.c_int64]
+    lib.lbm_get_option.argtypes = [H, ctypes.c_char_p, ctypes.POINTER(ctypes.c_int64)]
     lib.lbm_get_path.argtypes = [H, ctypes.POINTER(ctypes.c_int32)]
     lib.lbm_query.argtypes = [H, ctypes.POINTER(ctypes.c_int32)]
     S = ctypes.c_void_p
@@ -255,6 +257,11 @@ class Case:
 
     def set_option(self, key: str, value: int):
         self._check(self.lib.lbm_set_option(self.h, key.encode(), int(value)), f"lbm_set_option({key})")
+
+    def get_option(self, key: str) -> int:
+        out = ctypes.c_int64(0)
+        self._check(self.lib.lbm_get_option(self.h, key.encode(), ctypes.byref(out)), f"lbm_get_option({key})")
+        return int(out.value)
 
     def path(self) -> int:
         out = ctypes.c_int32(0)

@@ -128,3 +128,29 @@ def test_resident_tile_shapes_and_calls(block_x):
         got = c.download(fid)
         assert np.array_equal(got, a), (block_x, fid, int(np.sum(got != a)))
     c.close()
+
+
+@pytest.mark.parametrize("name", ["waves_kbc", "waves_fp16"])
+def test_schedule_chosen_by_timing(name):
+    """Option resident 2 (the default): the first call of at least 18 steps
+    times both schedules and keeps the faster; the state is the one of the
+    wavefront schedule (resident 0) bit for bit, and a later call keeps
+    the choice."""
+    from lbm_native import Case
+
+    print("case", name, flush=True)
+    env = CASES[name]
+    ref = run(env, 40, resident=0)
+    cfg = parse_env(dict(env, LBM_KERNEL_PATH="fused"))
+    c = Case(cfg.to_struct())
+    assert c.get_option("resident") == 2
+    setup_case(c, cfg)
+    c.step(30)
+    chosen = c.get_option("resident")
+    assert chosen in (0, 1)
+    c.step(10)
+    assert c.get_option("resident") == chosen
+    for fid, a in ref.items():
+        got = c.download(fid)
+        assert np.array_equal(got, a), (name, chosen, fid, int(np.sum(got != a)))
+    c.close()

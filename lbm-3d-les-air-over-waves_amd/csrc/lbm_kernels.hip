@@ -3047,6 +3047,10 @@ __global__ __launch_bounds__(FB_THREADS, (FusedOcc<T, BAND>::k1)) void k_moments
 // collision of a launch: any model (chosen per cell from the constants), or
 // one fixed model (no sponge, no backscatter) so the kernel holds only its code
 enum { CM_ANY = 0, CM_KBC = 1, CM_REG = 2 };
+// blocks of 256 threads per CU the wavefront step is compiled for (6: 80
+// VGPRs and 24 B per lane of scratch; 5, 84 VGPRs without scratch, ran 3 %
+// slower on the 256x128x128 waves step)
+constexpr int WAVE_OCC = 6;
 
 template <int CM, class S, class T, class NB>
 __device__ __forceinline__ T les_collide(const FusedArgs<S, T>& a, int i, int j, int k, uint32_t id, T f[19], bool open_cur, bool open_below,
@@ -3290,7 +3294,7 @@ __device__ __forceinline__ void wave_tile(const FusedArgs<S, T>& a, const WaveQ&
 }
 
 template <class S, class T, int CM>
-__global__ __launch_bounds__(FB_THREADS, 6) void k_step_wave(FusedArgs<S, T> a, WaveQ w) {
+__global__ __launch_bounds__(FB_THREADS, WAVE_OCC) void k_step_wave(FusedArgs<S, T> a, WaveQ w) {
     wave_tile<S, T, CM>(a, w, (int)blockIdx.x);
 }
 
@@ -4036,7 +4040,7 @@ static WaveQ make_waveq(const lbm_config_t& cfg, DevBufs& b, const FusedSched& f
     w.err = b.waveq;
     w.epoch = fs.epoch;
     w.bpp = gx * gy;
-    const int lag = fs.lag > 0 ? fs.lag : (6 * fs.cus + w.bpp - 1) / w.bpp;
+    const int lag = fs.lag > 0 ? fs.lag : (WAVE_OCC * fs.cus + w.bpp - 1) / w.bpp;
     w.lag = std::max(1, std::min(lag, cfg.nz));
     w.bx = bx;
     w.gx = gx;

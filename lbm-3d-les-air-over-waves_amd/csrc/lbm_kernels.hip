@@ -717,9 +717,17 @@ __global__ __launch_bounds__(NT) void k_macro_fix(Consts<T> c, const S* f, T* rh
     uz[id] = w;
 }
 
+// collision model a kernel is compiled for: KBC without top sponge and
+// backscatter, regularized, or any (chosen per cell at run time)
+enum { CM_ANY = 0, CM_KBC = 1, CM_REG = 2 };
+static int collision_model(const lbm_config_t& cfg) {
+    return cfg.collision == LBM_COLL_REGULARIZED ? CM_REG : (!cfg.enable_top_sponge && !(cfg.c_backscatter > 0.0f)) ? CM_KBC : CM_ANY;
+}
+
 // waves per SIMD the staged stream is compiled for: 4 with FP32 arithmetic
 // (128 VGPRs; 129 would leave 3), no bound with FP64 (which would spill).
-// (The staged collision at 4 spills and runs slower.)
+// (The staged regularized collision at 4, 128 VGPRs with 24 B of scratch,
+// ran 116 against 100 us on the VOF workload; the KBC one needs 105.)
 template <class T>
 struct StagedOcc {
     static constexpr int v = sizeof(T) == 4 ? 4 : 1;
@@ -773,7 +781,7 @@ __global__ __launch_bounds__(NT) void k_les(Consts<T> c, Slot<T> s, const T* ux,
 // collide_regularized / collide_KBC (CODEX:3143-3416)
 // LES: compute_LES of the cell first (as k_les: active cells only; the
 // others keep their omega), then the collision with that omega
-template <class S, class T, bool LES>
+template <class S, class T, bool LES, int CM>
 __global__ __launch_bounds__(NT) void k_collide(Consts<T> c, Slot<T> s, S* f, const T* rho, const T* ux, const T* uy, const T* uz,
                                                 T* omega, T* nu_t, T* smag, const int* type, const uint8_t* flags, uint32_t step) {
     size_t id = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -815,7 +823,7 @@ __global__ __launch_bounds__(NT) void k_collide(Consts<T> c, Slot<T> s, S* f, co
     } else {
         om = omega[id];
     }
-    if (c.collision_reg) {
+    if (CM == CM_REG || (CM == CM_ANY && c.collision_reg)) {
         if (gas) {
 #pragma unroll
             for (int q = 0; q < 19; q++) fl[q] = feq(r, u, v, w, q);
@@ -826,10 +834,10 @@ __global__ __launch_bounds__(NT) void k_collide(Consts<T> c, Slot<T> s, S* f, co
             collide_regularized<T, false>(fl, r, u, v, w, om, T(0));
         }
     } else {
-        if (c.top_sponge && k > c.nz - 3) {
+        if (CM == CM_ANY && c.top_sponge && k > c.nz - 3) {
             sponge_relax(fl, r, u, v, w);
         } else {
-            bool bs = c.c_backscatter > T(0) && open0;
+            bool bs = CM == CM_ANY && c.c_backscatter > T(0) && open0;
             float xi = bs ? bs_uniform(c.seed, step, (uint32_t)id) : 0.0f;
             if (c.fs_water && !gas) {
                 T force_z = -r * local_g(c, k, open0, ty);
@@ -3046,7 +3054,6 @@ __global__ __launch_bounds__(FB_THREADS, (FusedOcc<T, BAND>::k1)) void k_moments
 // z+, z- (x and y wrap periodically, CODEX:3047-3053; z clamps at the ends).
 // collision of a launch: any model (chosen per cell from the constants), or
 // one fixed model (no sponge, no backscatter) so the kernel holds only its code
-enum { CM_ANY = 0, CM_KBC = 1, CM_REG = 2 };
 // blocks of 256 threads per CU the wavefront step is compiled for (6: 80
 // VGPRs and 24 B per lane of scratch; 5, 84 VGPRs without scratch, ran 3 %
 // slower on the 256x128x128 waves step)
@@ -3952,9 +3959,24 @@ static void staged_t(const lbm_config_t& cfg, DevBufs& b, int& cur, int slot, in
                                             b.tstate);
     else
         k_macro<S, T><<<nb, NT, 0, s>>>(c, f, (T*)b.rho, (T*)b.ux, (T*)b.uy, (T*)b.uz, b.type, b.cellflags, c.fs_water ? b.tstate : nullptr);
-    auto collide = cfg.use_les ? k_collide<S, T, true> : k_collide<S, T, false>;
-    collide<<<nb, NT, 0, s>>>(c, sl, f, (const T*)b.rho, (const T*)b.ux, (const T*)b.uy, (const T*)b.uz, (T*)b.omega, (T*)b.nu_t,
-                              (T*)b.smag, b.type, b.cellflags, (uint32_t)step);
+    {
+        // compiled per LES flag and collision model (fewer registers than
+        // the kernel that holds every model)
+        const int cm = collision_model(cfg);
+#define LBM_COLLIDE(LES, CM)                                                                                                   \
+    k_collide<S, T, LES, CM><<<nb, NT, 0, s>>>(c, sl, f, (const T*)b.rho, (const T*)b.ux, (const T*)b.uy, (const T*)b.uz, (T*)b.omega, \
+                                                (T*)b.nu_t, (T*)b.smag, b.type, b.cellflags, (uint32_t)step)
+        if (cfg.use_les) {
+            if (cm == CM_REG) LBM_COLLIDE(true, CM_REG);
+            else if (cm == CM_KBC) LBM_COLLIDE(true, CM_KBC);
+            else LBM_COLLIDE(true, CM_ANY);
+        } else {
+            if (cm == CM_REG) LBM_COLLIDE(false, CM_REG);
+            else if (cm == CM_KBC) LBM_COLLIDE(false, CM_KBC);
+            else LBM_COLLIDE(false, CM_ANY);
+        }
+#undef LBM_COLLIDE
+    }
     // the link counters of stream describe one step and are overwritten by
     // the next: they are counted on the last step of a call, the one a
     // caller can observe
@@ -4028,9 +4050,6 @@ static FusedArgs<S, T> make_fused_args(const lbm_config_t& cfg, DevBufs& b, int 
 
 // the collision model is fixed at compile time unless the top sponge or
 // backscatter can select another one per cell
-static int collision_model(const lbm_config_t& cfg) {
-    return cfg.collision == LBM_COLL_REGULARIZED ? CM_REG : (!cfg.enable_top_sponge && !(cfg.c_backscatter > 0.0f)) ? CM_KBC : CM_ANY;
-}
 
 // the tile flags of a case's wavefront step (gx x gy tiles per plane of
 // bx x 256/bx cells)

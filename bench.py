@@ -195,6 +195,12 @@ def serving(n_cases: int, storage: str, seed: int = 7):
             progress(f"serving {storage}: {c + 1}/{n_cases} sequential cases done")
     seq_wall = time.perf_counter() - t0
     budget = 200 << 30
+    errors = []
+
+    def note_errors(srv):
+        for r in srv.finished:
+            if r.status == "failed" and len(errors) < 4:
+                errors.append(r.error[:300])
 
     def run_burst(graphs=False, batch=False):
         srv = EnsembleServer(budget, max_active=64, chunk_steps=16, n_workers=4, graphs=graphs, batch=batch)
@@ -203,6 +209,7 @@ def serving(n_cases: int, storage: str, seed: int = 7):
             srv.submit(env, steps)
         srv.run_until_idle()
         st = srv.stats(t0)
+        note_errors(srv)
         srv.close()
         return st
 
@@ -230,6 +237,7 @@ def serving(n_cases: int, storage: str, seed: int = 7):
         srv.pump()
         time.sleep(2e-4)
     poisson = srv.stats(t0)
+    note_errors(srv)
     srv.close()
     progress(f"serving {storage}: Poisson stream done")
     return {
@@ -249,6 +257,7 @@ def serving(n_cases: int, storage: str, seed: int = 7):
         "latency_p99_s": round(poisson["latency_p99_s"], 4),
         "peak_active": max(burst["peak_active"], poisson["peak_active"]),
         "failed": burst_streams["failed"] + burst_graphs["failed"] + burst_batch["failed"] + poisson["failed"],
+        "failure_messages": errors,
     }
 
 

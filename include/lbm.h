@@ -354,7 +354,15 @@ int lbm_device_bytes(const lbm_config_t *cfg, size_t *out);
    lbm_step call of at least 18 steps, which then waits for the device),
    "vof_band" (0/1, default 1: the VOF passes of the staged step skip the
    rows of the grid the step's own change marks show settled; exact, the
-   results are bit-identical with 0). */
+   results are bit-identical with 0),
+   "flood_sync" (0/1, default 1: the trapped-gas flood fill is stopped at
+   its fixed point by the host, which reads its flag every 16 sweeps; 0
+   launches every sweep and never waits).
+   The wavefront and tile-resident schedules let a block wait for other
+   blocks of its launch; while several handles exist in the process, those
+   launches are chained so that one runs at a time.  Handles with "graph"
+   and separate processes stepping the same GPU at the same time should use
+   "wavefront" 0 and "resident" 0. */
 int lbm_set_option(lbm_handle_t *h, const char *key, int64_t value);
 
 /* Current value of an lbm_set_option knob ("resident": 2 while the choice

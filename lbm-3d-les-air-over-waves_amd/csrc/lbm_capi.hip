@@ -10,9 +10,11 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -25,6 +27,7 @@ using namespace lbm;
 namespace {
 
 std::string g_create_error;
+std::atomic<int> g_live_handles{0};  // handles that exist in the process
 
 struct Buf {
     void* p = nullptr;
@@ -49,6 +52,8 @@ struct lbm_handle {
     FusedSched sched;  // fused path: wavefront (one launch per step) or three launches
     StagedState staged;  // staged path: what a step may reuse from the previous one
     int vof_band = 1;    // option vof_band: VOF passes skip the rows settled far from the interface
+    int flood_sync = 1;  // option flood_sync: the host reads the flood fill's flag every 16 sweeps
+    int dev = 0;         // the device the handle was created on
     // option "resident" 2 (the default): the schedule of the fused step is
     // chosen by timing both on the handle's first long enough lbm_step call
     // (1 pending, 0 chosen or set explicitly)
@@ -365,6 +370,33 @@ int batch_key(lbm_handle_t* h) {
 }
 }  // namespace lbm
 
+// Launches whose blocks wait for other blocks of the same launch (the
+// fused step's wavefront and tile-resident schedules, the merged grid of
+// lbm_step_batch): a block waits only for blocks the dispatcher started
+// before it, which is safe while no other such launch runs on the device;
+// two running at once can each fill the compute units that the other's
+// unstarted blocks need.  While more than one handle exists in the process,
+// these launches are chained on one event per device: each starts after the
+// previous one ended (other kernels still run beside it).  Handles with the
+// option graph and separate processes stepping on one GPU are not chained:
+// they should use the three-launch schedule (options wavefront 0, resident
+// 0) when they step at the same time as others.
+static std::mutex g_wait_m;
+static hipEvent_t g_wait_ev[16];
+template <class F>
+static void chain_waiting(int dev, hipStream_t s, bool waits, F&& launch) {
+    if (!waits || g_live_handles.load() < 2) {
+        launch();
+        return;
+    }
+    std::lock_guard<std::mutex> g(g_wait_m);
+    hipEvent_t& ev = g_wait_ev[dev & 15];
+    if (!ev) CHECK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+    CHECK(hipStreamWaitEvent(s, ev, 0));
+    launch();
+    CHECK(hipEventRecord(ev, s));
+}
+
 extern "C" {
 
 int lbm_create(const lbm_config_t* cfg, lbm_handle_t** out) {
@@ -392,6 +424,7 @@ int lbm_create(const lbm_config_t* cfg, lbm_handle_t** out) {
         CHECK(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
         int dev = 0, cus = 0;
         CHECK(hipGetDevice(&dev));
+        h->dev = dev;
         CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
         h->sched.cus = cus > 0 ? cus : 256;
         allocate(h);
@@ -407,11 +440,13 @@ int lbm_create(const lbm_config_t* cfg, lbm_handle_t** out) {
         return LBM_E_NOMEM;
     }
     *out = h;
+    g_live_handles.fetch_add(1);
     return LBM_OK;
 }
 
 void lbm_destroy(lbm_handle_t* h) {
     if (!h) return;
+    g_live_handles.fetch_sub(1);
     if (h->stream) (void)hipStreamSynchronize(h->stream);
     if (h->gexec) (void)hipGraphExecDestroy(h->gexec);
     batch_buf_free(h->batch);
@@ -510,6 +545,7 @@ static void enqueue_steps(lbm_handle* h, int32_t nsteps) {
         if (h->path == LBM_PATH_STAGED) {
             if (!h->post_stream) throw std::runtime_error("internal: staged step on a post-collision state");
             h->staged.band_allowed = h->vof_band;
+            h->staged.flood_sync = h->flood_sync;
             launch_staged_step(c, h->b, h->cur, slot, step, h->omega_band_top, s == nsteps - 1, h->staged, h->stream);
         } else {
             int last = s == nsteps - 1;
@@ -520,8 +556,13 @@ static void enqueue_steps(lbm_handle* h, int32_t nsteps) {
                 h->sched.epoch = 0;
             }
             h->sched.epoch++;
-            launch_fused_step(c, h->b, h->cur, h->post_stream ? 0 : 1, slot_of(h, step - 1), slot, step, last, h->omega_band_top,
-                              h->wall_top, h->block_x, h->sched, h->stream);
+            // (a handle with the option graph is not chained: its stream is
+            // captured, and an event recorded on a stream that begins a
+            // capture cannot be waited on from another)
+            chain_waiting(h->dev, h->stream, (h->sched.wavefront || h->sched.resident) && !h->graph, [&] {
+                launch_fused_step(c, h->b, h->cur, h->post_stream ? 0 : 1, slot_of(h, step - 1), slot, step, last, h->omega_band_top,
+                                  h->wall_top, h->block_x, h->sched, h->stream);
+            });
             h->post_stream = 0;
         }
         if (h->timing) CHECK(hipEventRecord(h->ev[3 + 2 * s], h->stream));
@@ -795,7 +836,7 @@ int step_batch(lbm_handle* const* hs, const int32_t* nsteps, int n, hipStream_t 
                 h->slot_valid[slot] = 1;
             }
         }
-        launch_wave_batch(storage, cm, (const char*)buf.dev + off[s] * E, cnt[s], blocks[s], bs);
+        chain_waiting(h0->dev, bs, true, [&] { launch_wave_batch(storage, cm, (const char*)buf.dev + off[s] * E, cnt[s], blocks[s], bs); });
         for (int i = 0; i < n; i++) {
             lbm_handle* h = hs[i];
             if (s >= nsteps[i]) continue;
@@ -1157,6 +1198,10 @@ int lbm_set_option(lbm_handle_t* h, const char* key, int64_t value) {
         h->vof_band = value ? 1 : 0;
         return LBM_OK;
     }
+    if (!strcmp(key, "flood_sync")) {
+        h->flood_sync = value ? 1 : 0;
+        return LBM_OK;
+    }
     if (!strcmp(key, "xcd_strips")) {
         h->sched.strips = value ? 1 : 0;
         return LBM_OK;
@@ -1179,6 +1224,7 @@ int lbm_get_option(lbm_handle_t* h, const char* key, int64_t* out) {
     else if (!strcmp(key, "xcd_strips")) *out = h->sched.strips;
     else if (!strcmp(key, "block_x")) *out = h->block_x;
     else if (!strcmp(key, "vof_band")) *out = h->vof_band;
+    else if (!strcmp(key, "flood_sync")) *out = h->flood_sync;
     else return fail(h, LBM_E_ARG, std::string("unknown option ") + key);
     return LBM_OK;
 }

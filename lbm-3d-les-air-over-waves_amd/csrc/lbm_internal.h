@@ -120,6 +120,7 @@ struct StagedState {
     int band_ok = 0;      // the VOF row marks of the last step cover its end state (narrow band usable)
     int act_slot = 0;     // row-mark slot of the next step
     int band_allowed = 1;  // option vof_band (set by the C ABI before each step)
+    int flood_sync = 1;    // option flood_sync (the same): the host stops the flood fill at its fixed point
 };
 
 // host-side launchers (lbm_kernels.hip)

@@ -3898,11 +3898,12 @@ static void vof_pipeline(const lbm_config_t& cfg, DevBufs& b, S* f, int slot, in
         // in batches of 16 sweeps; after each, the host reads the last
         // sweep's flag and stops at the fixed point (a sweep that finds
         // nothing to change returns at once, but thousands of launches
-        // would still cost milliseconds)
+        // would still cost milliseconds).  Without flood_sync every sweep
+        // is launched and the host never waits.
         for (int w = 0; w < sweeps;) {
             const int w1 = std::min(sweeps, w + 16);
             for (; w < w1; w++) k_flood_sweep<T><<<nb, NT, 0, s>>>(A(), flagbuf + w, flagbuf + w + 1);
-            if (w < sweeps) {
+            if (w < sweeps && st.flood_sync) {
                 int changed = 1;
                 HIP_OK(hipMemcpyAsync(&changed, flagbuf + w, sizeof(int), hipMemcpyDeviceToHost, s));
                 HIP_OK(hipStreamSynchronize(s));

@@ -151,13 +151,8 @@ class EnsembleServer:
                 solver.initialize()
                 if req.prepare is not None:
                     req.prepare(solver)
-                if solver.case.path() == abi.PATH_FUSED:
-                    # the wavefront schedule, not chosen by timing: the timing
-                    # waits for the device, which would stall a worker's
-                    # other cases
-                    solver.case.set_option("resident", 0)
-                    if self.graphs:
-                        solver.case.set_option("graph", 1)
+                if self.graphs and solver.case.path() == abi.PATH_FUSED:
+                    solver.case.set_option("graph", 1)
                 self.used += req.footprint
                 self.active[req.ticket] = (req, solver)
                 rc = self.lib.lbm_server_add(self.srv, solver.case.h, req.steps, req.ticket)

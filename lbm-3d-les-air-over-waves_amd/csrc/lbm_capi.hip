@@ -559,7 +559,10 @@ static void enqueue_steps(lbm_handle* h, int32_t nsteps) {
             // (a handle with the option graph is not chained: its stream is
             // captured, and an event recorded on a stream that begins a
             // capture cannot be waited on from another)
-            chain_waiting(h->dev, h->stream, (h->sched.wavefront || h->sched.resident) && !h->graph, [&] {
+            // (16-bit storage waits only on the tile-resident grid, FP64 never)
+            const bool waits = c.storage == LBM_STORE_FP32 ? (h->sched.wavefront || h->sched.resident)
+                                                           : (c.storage != LBM_STORE_FP64 && h->sched.resident);
+            chain_waiting(h->dev, h->stream, waits && !h->graph, [&] {
                 launch_fused_step(c, h->b, h->cur, h->post_stream ? 0 : 1, slot_of(h, step - 1), slot, step, last, h->omega_band_top,
                                   h->wall_top, h->block_x, h->sched, h->stream);
             });

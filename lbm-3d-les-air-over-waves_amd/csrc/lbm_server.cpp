@@ -278,16 +278,15 @@ int lbm_server_add(lbm_server_t* s, lbm_handle_t* h, int32_t nsteps, int64_t tic
         std::lock_guard<std::mutex> g(s->fin_m);
         if (!s->inflight.insert(h).second) return LBM_E_STATE;
     }
-    // the fused step's schedule is not chosen by timing here (the timing
-    // waits for the device, which would stall the worker's other cases); the
-    // library chains the launches whose blocks wait for one another across
-    // the handles of the process, so the cases' streams stay safe together,
-    // except for handles whose steps are captured into graphs: those run the
-    // three-launch schedule (no block waits for another)
+    // the workers step their cases on concurrent streams with the fused
+    // step's three-launch schedule, in which no block waits for another (the
+    // library would otherwise chain the waiting launches of the cases, one
+    // at a time: measured slower here); the merged grid of the batching
+    // thread is the one waiting launch.  Nothing a worker runs waits for the
+    // device: no choice of schedule by timing, no flood-fill host reads.
     (void)lbm_set_option(h, "resident", 0);
-    (void)lbm_set_option(h, "flood_sync", 0);  // (nor does the flood fill wait)
-    int64_t graph = 0;
-    if (lbm_get_option(h, "graph", &graph) == LBM_OK && graph) (void)lbm_set_option(h, "wavefront", 0);
+    (void)lbm_set_option(h, "wavefront", 0);
+    (void)lbm_set_option(h, "flood_sync", 0);
     Worker* best = s->workers[0];
     for (Worker* w : s->workers)
         if (w->n_active.load() < best->n_active.load()) best = w;

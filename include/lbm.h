@@ -399,7 +399,9 @@ int lbm_server_batch(lbm_server_t *s, int32_t on);
 
 /* Hand over an initialised case to advance by nsteps; ticket is the caller's
    identifier, reported back by lbm_server_poll.  LBM_E_STATE if the handle
-   is already held by the server (a job on it has not finished). */
+   is already held by the server (a job on it has not finished).  Sets the
+   handle's options "wavefront", "resident" and "flood_sync" to 0 (cases
+   stepped on concurrent streams: no waiting launches, no host waits). */
 int lbm_server_add(lbm_server_t *s, lbm_handle_t *h, int32_t nsteps, int64_t ticket);
 
 /* Move up to max finished tickets into tickets[] (*n_done of them) together

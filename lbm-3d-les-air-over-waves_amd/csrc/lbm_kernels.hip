@@ -700,7 +700,11 @@ __global__ __launch_bounds__(NT) void k_macro_fix(Consts<T> c, const S* f, T* rh
     if (id >= n) return;
     const int ty = type[id];
     const uint8_t fl0 = flags[id];
-    tstate[id] = (uint8_t)(ty * 2 + (int)(fl0 & 1u));  // the stream's source state of this cell
+    // the stream's source state of this cell (bits 0-2), and in bits 3-4 the
+    // step ends in a row, up to 3, at which it was an open GAS cell
+    const int was = tstate[id] >> 3;
+    const int streak = (ty == LBM_FS_GAS && (fl0 & 1u)) ? (was < 3 ? was + 1 : 3) : 0;
+    tstate[id] = (uint8_t)(ty * 2 + (int)(fl0 & 1u) + (streak << 3));
     const bool d = dirty[id] != 0;
     if ((ty == LBM_FS_GAS) == (type_prev[id] == LBM_FS_GAS) && !d) return;
     if (d) dirty[id] = 0;
@@ -994,7 +998,7 @@ __device__ __forceinline__ void stream_bc_cell(const Consts<T>& c, const Slot<T>
             const int sic = si < 0 ? 0 : (si >= nx ? nx - 1 : si);  // outflow: copy of the last layer
             const size_t sidx = src_ok ? ((size_t)sk * ny + sj) * nx + sic : id;
             const size_t tid = (src_ok && inx) ? sidx : id;
-            st_bits |= (uint64_t)tstate[tid] << (3 * q);  // 2 type + open of the source
+            st_bits |= (uint64_t)(tstate[tid] & 7u) << (3 * q);  // 2 type + open of the source
             sv_a[q] = Store<S, T>::load(fin, (size_t)q * n + sidx, q, inv);
         }
         // pass 2: the rules of stream (CODEX:3438-3551) for a liquid or
@@ -1132,13 +1136,21 @@ __device__ __forceinline__ void stream_bc_cell(const Consts<T>& c, const Slot<T>
 template <class S, class D, class T, bool COUNT, bool MACRO>
 __global__ __launch_bounds__(NT, StagedOcc<T>::v) void k_stream_bc(Consts<T> c, Slot<T> s, const S* fin, D* fout, const T* ux, const T* uy, const T* uz,
                             const uint8_t* tstate, const int* type, const T* fill, const T* u_inlet, const T* inflow, const T* fs_h,
-                            const uint8_t* flags, int* icnt, T* mrho, T* mux, T* muy, T* muz, uint32_t step) {
+                            const uint8_t* flags, int* icnt, T* mrho, T* mux, T* muy, T* muz, uint32_t step, int gas_skip) {
     size_t id = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     size_t n = (size_t)c.nx * c.ny * c.nz;
     if (id >= n) return;
     T fo[19];
     int ty, k, n_cut, n_fs, n_bridge;
     stream_bc_cell<S, T>(c, s, fin, ux, uy, uz, tstate, type, fill, u_inlet, inflow, fs_h, flags, id, fo, ty, k, n_cut, n_fs, n_bridge);
+    // gas_skip (the staged VOF step over a static geometry, storage in the
+    // arithmetic type): an open GAS cell away from walls that was one at the
+    // last three step ends already holds what this store would write.  Its
+    // populations in fout are the rest equilibrium that the stream of two
+    // steps ago stored there (the collision in between skips such a cell),
+    // and rho and u the rest state the last stream wrote; a gas cell counts
+    // no links.
+    if (gas_skip && ty == LBM_FS_GAS && (flags[id] & 3u) == 1u && (tstate[id] >> 3) >= 3) return;
     if constexpr (std::is_same<D, T>::value) {
 #pragma unroll
         for (int q = 0; q < 19; q++) fout[(size_t)q * n + id] = fo[q];
@@ -3986,9 +3998,13 @@ static void staged_t(const lbm_config_t& cfg, DevBufs& b, int& cur, int slot, in
     // inside the stream kernel
     auto stream = cfg.fs_enabled ? (count_links ? k_stream_bc<S, S, T, true, true> : k_stream_bc<S, S, T, false, true>)
                                  : (count_links ? k_stream_bc<S, S, T, true, false> : k_stream_bc<S, S, T, false, false>);
+    // the stores the stream may leave out for long-standing gas cells (their
+    // streak counts only where the macro fix runs; 16-bit stores differ from
+    // step to step by their dither)
+    const int gas_skip = (std::is_same<S, T>::value && static_geo && vof_macro && st.macro_ready) ? 1 : 0;
     stream<<<nb, NT, 0, s>>>(c, sl, f, fn, (const T*)b.ux, (const T*)b.uy, (const T*)b.uz, b.tstate, b.type, (const T*)b.fill,
                              (const T*)b.u_inlet, (const T*)b.inflow, (const T*)b.fs_h, b.cellflags, b.icnt, (T*)b.rho, (T*)b.ux,
-                             (T*)b.uy, (T*)b.uz, (uint32_t)step);
+                             (T*)b.uy, (T*)b.uz, (uint32_t)step, gas_skip);
     cur ^= 1;
     HIP_OK(hipGetLastError());
     if (cfg.fs_enabled) {
@@ -4271,7 +4287,7 @@ static void materialize_t(const lbm_config_t& cfg, DevBufs& b, int cur, int slot
     k_stream_bc<S, T, T, false, false><<<nblocks(b.n), NT, 0, s>>>(c, sl, (const S*)b.f[cur], (T*)dst, (const T*)b.ux, (const T*)b.uy,
                                                                    (const T*)b.uz, b.tstate, b.type, (const T*)b.fill, (const T*)b.u_inlet,
                                                                    (const T*)b.inflow, (const T*)b.fs_h, b.cellflags, b.icnt,
-                                                                   nullptr, nullptr, nullptr, nullptr, 0u);
+                                                                   nullptr, nullptr, nullptr, nullptr, 0u, 0);
     HIP_OK(hipGetLastError());
 }
 void launch_materialize(const lbm_config_t& c, DevBufs& b, int cur, int slot, void* dst, hipStream_t s) {
